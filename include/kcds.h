@@ -1,0 +1,137 @@
+/* kcds.h -- C ABI of the MI355X k-clique densest subgraph (k-CDS) solver.
+ *
+ * One kcds_ctx owns one HIP stream on one device and every device buffer of a
+ * solve (graph CSR, DAG, clique list, incidence index, peeling state, flow
+ * networks).  All entry points return KCDS_OK (0) or a negative kcds_status;
+ * none calls exit().  kcds_last_error() describes the last failure.  Contexts
+ * are independent; use one per host thread.
+ *
+ * Mapping to the reference (wajidmanzoor/test) path each entry point replaces:
+ *   kcds_upload_csr        memoryAllocationGraph   src/gpuMemoryAllocation.cu:5-27
+ *   kcds_orient            generateDAG             main.cu:22-91 (+ src/graph.cpp:71-81
+ *                                                  listing order when computed on the GPU)
+ *   kcds_count             listAllCliques + countCliques (T, cliqueDegree)
+ *                                                  main.cu:93-244, src/helpers.cu:1074-1113
+ *   kcds_list              listAllCliques + writeFinalCliques (trie, status)
+ *                                                  main.cu:246-327, src/helpers.cu:489-550
+ *   kcds_peel              cliqueCoreDecompose     main.cu:329-420
+ *   kcds_components        generateDensestCore + prune + componentDecompose
+ *                                                  main.cu:422-728
+ *   kcds_maxflow           one max-flow of dynamicExactAlgo  main.cu:1037-1237
+ *   kcds_solve             main's stage sequence   main.cu:1348-1519
+ *   kcds_create/destroy    the buffer lifecycle of src/gpuMemoryAllocation.cu (alloc/free pairs)
+ */
+#ifndef KCDS_H
+#define KCDS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kcds_ctx kcds_ctx;
+
+typedef enum kcds_status {
+  KCDS_OK = 0,
+  KCDS_E_INVALID = -1, /* bad argument: NULL pointer, k < 3, not a permutation, ... */
+  KCDS_E_HIP = -2,     /* HIP runtime error (message in kcds_last_error) */
+  KCDS_E_NOMEM = -3,   /* HBM budget of the context exceeded or allocation failed */
+  KCDS_E_STATE = -4,   /* call out of order, e.g. kcds_count before kcds_orient */
+  KCDS_E_LIMIT = -5    /* size beyond a supported limit (message names it) */
+} kcds_status;
+
+typedef struct kcds_peel_stats {
+  uint64_t max_num, max_den; /* maxDensity of the densest clique-core, exact (main.cu:398-413) */
+  uint32_t k_prime;          /* ceil(maxDensity) (main.cu:1476) */
+  uint32_t core_size;        /* |{v : core(v) >= k_prime}| (main.cu:1482) */
+  uint64_t core_cliques;     /* cliques with every member in the k'-core */
+  int32_t max_core;          /* largest clique-core number */
+  uint32_t levels;           /* non-empty peeling levels processed */
+} kcds_peel_stats;
+
+typedef struct kcds_result {
+  uint64_t T;            /* k-cliques in the graph */
+  uint64_t rho_num;      /* cliques inside the answer set S */
+  uint64_t rho_den;      /* |S|: density = rho_num / rho_den exactly */
+  uint32_t k_prime;
+  uint32_t core_vertices;
+  uint32_t ncomp;        /* components of the pruned k'-core */
+  uint32_t flow_calls;   /* max-flow solves */
+  kcds_peel_stats peel;
+  float ms_orient, ms_list, ms_peel, ms_components, ms_flow, ms_total; /* stage times (hipEvents) */
+} kcds_result;
+
+/* Creates a context on `device` with an HBM budget (0 = 90% of free memory). */
+int kcds_create(kcds_ctx **ctx, int device, uint64_t hbm_budget_bytes);
+void kcds_destroy(kcds_ctx *ctx);
+const char *kcds_last_error(const kcds_ctx *ctx);
+
+/* Copies an undirected CSR (host arrays; offset[n+1], nbr[offset[n]], no
+ * self-loops, each edge listed in both directions) to the device.  Resets all
+ * later stages. */
+int kcds_upload_csr(kcds_ctx *ctx, uint32_t n, const uint32_t *offset, const uint32_t *nbr);
+
+/* Builds the DAG used for listing.  listing_order == NULL: degeneracy order
+ * computed on the GPU.  Otherwise listing_order[v] in 1..n is the reference's
+ * listing order (src/graph.cpp:78-80: 1 + position in corePeelSequence); arcs
+ * point from the vertex peeled earlier to the one peeled later.  Every
+ * acyclic orientation yields the same cliques. */
+int kcds_orient(kcds_ctx *ctx, const uint32_t *listing_order);
+
+/* Counts k-cliques: *T and, if per_vertex != NULL, the number of k-cliques
+ * containing each vertex (int64, original ids).  3 <= k <= 8. */
+int kcds_count(kcds_ctx *ctx, uint32_t k, uint64_t *T, int64_t *per_vertex);
+
+/* Materialises every k-clique on the device (clique list + vertex->clique
+ * incidence index); *T receives the count. */
+int kcds_list(kcds_ctx *ctx, uint32_t k, uint64_t *T);
+/* Copies up to cap cliques (k ascending original ids each, rows sorted
+ * lexicographically) to the host; *T receives the total. */
+int kcds_download_cliques(kcds_ctx *ctx, uint32_t *out, uint64_t cap, uint64_t *T);
+
+/* k-clique-core peeling over the materialised cliques.  clique_core[n]
+ * (original ids) may be NULL. */
+int kcds_peel(kcds_ctx *ctx, int32_t *clique_core, kcds_peel_stats *stats);
+
+/* Densest core, edge pruning and connected components.  comp[n] receives the
+ * canonical component id of each vertex (smallest original vertex id in its
+ * component; UINT32_MAX outside the k'-core).  comp may be NULL. */
+int kcds_components(kcds_ctx *ctx, uint32_t *comp, uint32_t *ncomp);
+
+/* Max-flow value of the clique flow network of component `comp_id` (its
+ * canonical id) at alpha = p/q, capacities scaled by q:
+ * s->v q*deg(v), v->t k*p, v->c q, c->v q*(k-1).  side[n] (original ids, may
+ * be NULL) receives 1 for vertices on the source side of the maximal min cut. */
+int kcds_maxflow(kcds_ctx *ctx, uint32_t comp_id, uint64_t p, uint64_t q, int64_t *flow_value,
+                 uint8_t *side);
+
+/* Whole pipeline on the uploaded graph: orient (GPU degeneracy order), list,
+ * peel, components, exact density search.  in_set[n] (may be NULL) receives
+ * 1 for members of the maximal densest set S. */
+int kcds_solve(kcds_ctx *ctx, uint32_t k, kcds_result *res, uint8_t *in_set);
+
+/* Blocks until all work queued on the context's stream has finished. */
+int kcds_sync(kcds_ctx *ctx);
+
+/* Listing-kernel launch shape: waves per CU for the persistent listing
+ * kernels (0 = library default).  Used by the autotuner. */
+int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
+
+/* Per-stage facts of the last kcds_count/kcds_list call, for the roofline:
+ * n, m (DAG arcs), W2 = sum over arcs (v,u) of outdeg(u), max out-degree,
+ * and the average duration of the dominant listing kernel (ms, hipEvents). */
+typedef struct kcds_list_info {
+  uint32_t n;
+  uint64_t arcs;
+  uint64_t w2;
+  uint32_t max_outdeg;
+  float kernel_ms;
+  uint32_t kernel_launches;
+} kcds_list_info;
+int kcds_get_list_info(kcds_ctx *ctx, kcds_list_info *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KCDS_H */

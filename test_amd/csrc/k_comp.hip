@@ -1,0 +1,212 @@
+// Densest core, edge pruning and connected components.
+//
+// Reference: generateDensestCore (main.cu:422-527, src/helpers.cu:797-872)
+// keeps {v : cliqueCore[v] >= k'} with k' = ceil(maxDensity); prune
+// (main.cu:528-613, src/helpers.cu:874-1017) keeps an edge iff it lies in a
+// clique whose status >= k' (all members in the core); componentDecompose
+// (main.cu:615-728, src/helpers.cu:913-954) propagates minimum labels to a
+// fixpoint and renumbers components in label order.
+//
+// Here the surviving edges are exactly the member pairs of core cliques, so
+// components are formed directly over core cliques by lock-free union-find:
+// hooking always attaches the root with the larger key (original vertex id)
+// under the one with the smaller key, and finds halve paths (pointer
+// jumping).  The root of each component is therefore its smallest original
+// id, which is the canonical component id.  Components are then numbered
+// densely in order of that id, and the core vertices and core cliques are
+// grouped per component (local vertex ids) for the flow stage.
+#include "kcds_device.hpp"
+#include "kcds_internal.hpp"
+
+namespace kcds {
+
+namespace {
+
+__device__ __forceinline__ uint32_t ld(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ uint32_t uf_find(uint32_t *par, uint32_t x) {
+  while (true) {
+    const uint32_t p = ld(&par[x]);
+    if (p == x) return x;
+    const uint32_t gp = ld(&par[p]);
+    if (gp != p) atomicCAS(&par[x], p, gp);  // path halving: gp is an ancestor of x
+    x = gp;
+  }
+}
+
+__device__ void uf_unite(uint32_t *par, const uint32_t *key, uint32_t a, uint32_t b) {
+  while (true) {
+    a = uf_find(par, a);
+    b = uf_find(par, b);
+    if (a == b) return;
+    if (key[a] < key[b]) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    // a has the larger key: hook it under b (keys strictly decrease upward)
+    if (atomicCAS(&par[a], a, b) == a) return;
+  }
+}
+
+__global__ void k_uf_init(uint32_t n, uint32_t *par) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) par[v] = v;
+}
+
+__global__ void k_uf_hook(uint64_t T, uint32_t k, const uint32_t *cl, const int32_t *status, int32_t kp,
+                          uint32_t *par, const uint32_t *key) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
+       c += (uint64_t)gridDim.x * blockDim.x) {
+    if (status[c] < kp) continue;
+    const uint32_t a = cl[c * k];
+    for (uint32_t t = 1; t < k; ++t) uf_unite(par, key, a, cl[c * k + t]);
+  }
+}
+
+// comp[r] = key of the root for core vertices, ~0 elsewhere; roots flagged in
+// original-id order for dense numbering
+__global__ void k_uf_label(uint32_t n, const int32_t *core, int32_t kp, uint32_t *par, const uint32_t *key,
+                           uint32_t *comp, uint32_t *isroot_orig) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const bool in = core[r] >= kp;
+    const uint32_t root = uf_find(par, r);
+    comp[r] = in ? key[root] : 0xFFFFFFFFu;
+    isroot_orig[key[r]] = (in && root == r) ? 1u : 0u;
+  }
+}
+
+// dense component index, vertex counts and local vertex ids
+__global__ void k_group_vertices(uint32_t n, const uint32_t *comp, const uint32_t *dense_of_orig,
+                                 const uint32_t *rank, uint32_t *cid, uint32_t *vcount, uint32_t *local) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const uint32_t lab = comp[r];
+    if (lab == 0xFFFFFFFFu) {
+      cid[r] = 0xFFFFFFFFu;
+      continue;
+    }
+    const uint32_t d = dense_of_orig[lab];
+    cid[r] = d;
+    local[r] = atomicAdd(&vcount[d], 1u);
+  }
+}
+
+__global__ void k_scatter_vertices(uint32_t n, const uint32_t *cid, const uint32_t *local, const uint32_t *voff,
+                                   uint32_t *verts) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const uint32_t d = cid[r];
+    if (d != 0xFFFFFFFFu) verts[voff[d] + local[r]] = r;
+  }
+}
+
+__global__ void k_count_comp_cliques(uint64_t T, uint32_t k, const uint32_t *cl, const int32_t *status, int32_t kp,
+                                     const uint32_t *cid, ull *ccount) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
+       c += (uint64_t)gridDim.x * blockDim.x)
+    if (status[c] >= kp) atomicAdd(&ccount[cid[cl[c * k]]], 1ull);
+}
+
+// core cliques grouped by component, members as local vertex ids
+__global__ void k_scatter_cliques(uint64_t T, uint32_t k, const uint32_t *cl, const int32_t *status, int32_t kp,
+                                  const uint32_t *cid, const uint32_t *local, const ull *coff, ull *cfill,
+                                  uint32_t *ccl) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
+       c += (uint64_t)gridDim.x * blockDim.x) {
+    if (status[c] < kp) continue;
+    const uint32_t d = cid[cl[c * k]];
+    const ull at = coff[d] + atomicAdd(&cfill[d], 1ull);
+    if (at >= coff[d + 1]) continue;  // never taken: counts and scatter agree
+    for (uint32_t t = 0; t < k; ++t) ccl[at * k + t] = local[cl[c * k + t]];
+  }
+}
+
+uint32_t grid1d(uint64_t len) {
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((len + 255) / 256, 65535ull * 16));
+}
+
+}  // namespace
+
+void components_run(Ctx &c) {
+  if (!c.peeled) fail(KCDS_E_STATE, "components before peel");
+  const uint32_t n = c.n, k = c.k;
+  const uint64_t T = c.T;
+  const int32_t kp = (int32_t)c.peel.k_prime;
+  CompData &g = c.cd;
+  buf_reserve(c, c.comp, sizeof(uint32_t) * ((size_t)n + 1));
+  buf_reserve(c, g.par, sizeof(uint32_t) * ((size_t)n + 1));
+  buf_reserve(c, g.flag, sizeof(uint32_t) * ((size_t)n + 1));
+  buf_reserve(c, g.dense, sizeof(uint32_t) * ((size_t)n + 1));
+  uint32_t *par = g.par.as<uint32_t>();
+  const uint32_t *key = c.inv.as<uint32_t>();
+  k_uf_init<<<grid1d(n), 256, 0, c.stream>>>(n, par);
+  KCDS_LAUNCH_CHECK();
+  if (T) {
+    k_uf_hook<<<grid1d(T), 256, 0, c.stream>>>(T, k, c.cliques.as<uint32_t>(), c.status.as<int32_t>(), kp, par, key);
+    KCDS_LAUNCH_CHECK();
+  }
+  k_uf_label<<<grid1d(n), 256, 0, c.stream>>>(n, c.core.as<int32_t>(), kp, par, key, c.comp.as<uint32_t>(),
+                                              g.flag.as<uint32_t>());
+  KCDS_LAUNCH_CHECK();
+  // dense[orig] = number of component roots with a smaller original id
+  scan_u32_to_u32(c, g.flag.as<uint32_t>(), g.dense.as<uint32_t>(), n);
+  uint32_t *h = (uint32_t *)c.host_scratch(16);
+  HIP_CHECK(hipMemcpyAsync(h, g.dense.as<uint32_t>() + n, 4, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  const uint32_t C = h[0];
+  c.ncomp = C;
+  g.ncomp = C;
+  // vertices per component
+  buf_reserve(c, g.cid, sizeof(uint32_t) * ((size_t)n + 1));
+  buf_reserve(c, g.local, sizeof(uint32_t) * ((size_t)n + 1));
+  buf_reserve(c, g.vcount, sizeof(uint32_t) * ((size_t)C + 2));
+  buf_reserve(c, g.voff, sizeof(uint32_t) * ((size_t)C + 2));
+  buf_reserve(c, g.verts, sizeof(uint32_t) * ((size_t)c.peel.core_size + 1));
+  HIP_CHECK(hipMemsetAsync(g.vcount.p, 0, sizeof(uint32_t) * ((size_t)C + 2), c.stream));
+  k_group_vertices<<<grid1d(n), 256, 0, c.stream>>>(n, c.comp.as<uint32_t>(), g.dense.as<uint32_t>(),
+                                                    c.rank.as<uint32_t>(), g.cid.as<uint32_t>(),
+                                                    g.vcount.as<uint32_t>(), g.local.as<uint32_t>());
+  KCDS_LAUNCH_CHECK();
+  scan_u32_to_u32(c, g.vcount.as<uint32_t>(), g.voff.as<uint32_t>(), C);
+  k_scatter_vertices<<<grid1d(n), 256, 0, c.stream>>>(n, g.cid.as<uint32_t>(), g.local.as<uint32_t>(),
+                                                      g.voff.as<uint32_t>(), g.verts.as<uint32_t>());
+  KCDS_LAUNCH_CHECK();
+  // cliques per component
+  buf_reserve(c, g.ccount, sizeof(ull) * ((size_t)C + 2));
+  buf_reserve(c, g.coff, sizeof(ull) * ((size_t)C + 2));
+  buf_reserve(c, g.cfill, sizeof(ull) * ((size_t)C + 2));
+  HIP_CHECK(hipMemsetAsync(g.ccount.p, 0, sizeof(ull) * ((size_t)C + 2), c.stream));
+  HIP_CHECK(hipMemsetAsync(g.cfill.p, 0, sizeof(ull) * ((size_t)C + 2), c.stream));
+  if (T) {
+    k_count_comp_cliques<<<grid1d(T), 256, 0, c.stream>>>(T, k, c.cliques.as<uint32_t>(), c.status.as<int32_t>(),
+                                                          kp, g.cid.as<uint32_t>(), g.ccount.as<ull>());
+    KCDS_LAUNCH_CHECK();
+  }
+  scan_u64_to_u64(c, g.ccount.as<ull>(), g.coff.as<ull>(), C);
+  buf_reserve(c, g.ccl, sizeof(uint32_t) * (c.peel.core_cliques * k + 1));
+  if (T) {
+    k_scatter_cliques<<<grid1d(T), 256, 0, c.stream>>>(T, k, c.cliques.as<uint32_t>(), c.status.as<int32_t>(), kp,
+                                                       g.cid.as<uint32_t>(), g.local.as<uint32_t>(),
+                                                       g.coff.as<ull>(), g.cfill.as<ull>(), g.ccl.as<uint32_t>());
+    KCDS_LAUNCH_CHECK();
+  }
+  // host copies of the per-component sizes drive the flow stage
+  g.h_voff.resize((size_t)C + 1);
+  g.h_coff.resize((size_t)C + 1);
+  g.h_label.resize(C);
+  HIP_CHECK(hipMemcpyAsync(g.h_voff.data(), g.voff.p, sizeof(uint32_t) * ((size_t)C + 1), hipMemcpyDeviceToHost,
+                           c.stream));
+  HIP_CHECK(hipMemcpyAsync(g.h_coff.data(), g.coff.p, sizeof(ull) * ((size_t)C + 1), hipMemcpyDeviceToHost,
+                           c.stream));
+  c.sync();
+  // canonical labels: the original id of each component's first-listed root,
+  // recovered from the dense numbering (k_uf_label flags roots by original id)
+  std::vector<uint32_t> flags(n);
+  HIP_CHECK(hipMemcpyAsync(flags.data(), g.flag.p, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  for (uint32_t v = 0, d = 0; v < n; ++v)
+    if (flags[v]) g.h_label[d++] = v;
+  c.have_comp = true;
+}
+
+}  // namespace kcds

@@ -1,0 +1,116 @@
+// Device-side helpers shared by the k-CDS kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define KCDS_WAVE 64
+
+#ifndef KCDS_ULL_DEFINED
+#define KCDS_ULL_DEFINED
+typedef unsigned long long ull;
+#endif
+
+// Orders this wave's LDS accesses: LDS instructions of one wave execute in
+// issue order, so a compiler-level fence plus a wave barrier makes lane A's
+// LDS write visible to lane B's later read.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t wave_bcast0(uint32_t v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ ull wave_sum_u64(ull v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o, 64));
+  return v;
+}
+
+// Inclusive prefix sum across the wave.
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(v, o, 64);
+    if (lane >= (uint32_t)o) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ ull wave_incl_scan_u64(ull v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    ull t = __shfl_up(v, o, 64);
+    if (lane >= (uint32_t)o) v += t;
+  }
+  return v;
+}
+
+// Number of lanes below this one whose predicate is set.
+__device__ __forceinline__ uint32_t lanes_below(ull ballot) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(ballot >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)ballot, 0u));
+}
+
+// Index of the first element of a[0..len) that is >= key (a sorted ascending).
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t len, uint32_t key) {
+  uint32_t lo = 0, hi = len;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Index of the first element of a[0..len) that is > key.
+__device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *a, uint32_t len, uint32_t key) {
+  uint32_t lo = 0, hi = len;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] <= key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Software grid barrier for cooperative launches: every thread releases its
+// writes at agent scope, one thread per block arrives on a monotonically
+// increasing counter and waits for all blocks, then every thread acquires.
+__device__ __forceinline__ void grid_barrier(uint32_t *bar, uint32_t nblocks, uint32_t &epoch) {
+  __threadfence();
+  __syncthreads();
+  epoch += 1;
+  if (threadIdx.x == 0) {
+    atomicAdd(bar, 1u);
+    const uint32_t target = epoch * nblocks;
+    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
+      __builtin_amdgcn_s_sleep(1);
+  }
+  __syncthreads();
+  __threadfence();
+}

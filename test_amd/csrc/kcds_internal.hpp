@@ -1,0 +1,184 @@
+// Host-side internals of libkcds: the context, device buffers and the launch
+// wrappers each kernel file exports.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/kcds.h"
+
+#ifndef KCDS_ULL_DEFINED
+#define KCDS_ULL_DEFINED
+typedef unsigned long long ull;
+#endif
+
+namespace kcds {
+
+// Thrown inside the library, turned into a kcds_status at the C boundary.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const char *fmt, ...);
+
+#define HIP_CHECK(expr)                                                          \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      ::kcds::fail(KCDS_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                   __FILE__, __LINE__);                                          \
+  } while (0)
+
+#define KCDS_LAUNCH_CHECK(name) HIP_CHECK(hipGetLastError())
+
+struct Arena;
+
+// A growable device buffer charged against the context's HBM budget.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  template <typename T> T *as() const { return (T *)p; }
+};
+
+struct Ctx;
+void buf_reserve(Ctx &c, DevBuf &b, size_t bytes);  // grows, never shrinks; contents lost on growth
+void buf_free(Ctx &c, DevBuf &b);
+
+// Per-stage timing with hipEvents on the context stream.
+struct StageTimer {
+  hipEvent_t a = nullptr, b = nullptr;
+};
+
+// Core vertices and core cliques grouped by component (dense index d in
+// order of the canonical label = smallest original id of the component).
+struct CompData {
+  DevBuf par, flag, dense, cid, local, vcount, voff, verts, ccount, coff, cfill, ccl;
+  uint32_t ncomp = 0;
+  std::vector<uint32_t> h_voff;  // C+1 vertex offsets
+  std::vector<ull> h_coff;       // C+1 clique offsets
+  std::vector<uint32_t> h_label; // canonical label of each component
+};
+
+// Buffers of one clique flow network (reused across max-flow calls).
+struct FlowBufs {
+  DevBuf off, head, rev, res, excess, height, deg, fill, front, side, scal;
+  uint64_t last_p = 0, last_q = 0;
+  int64_t last_comp = -1;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t budget = 0, used = 0;
+  int num_cus = 0;
+  std::string err;
+  uint32_t list_waves_per_cu = 0;
+  uint32_t tune_flags = 0;
+
+  // pinned host staging for small readbacks
+  void *pinned = nullptr;
+  size_t pinned_bytes = 0;
+
+  // input graph (original ids)
+  uint32_t n = 0;
+  uint64_t nnz = 0;
+  DevBuf off, nbr;
+  bool have_graph = false;
+
+  // orientation: rank[v] = position of v in the listing order (new id),
+  // inv[r] = original id; DAG over new ids, lists sorted ascending
+  DevBuf rank, inv, doff, dnbr;
+  uint32_t dmax = 0;
+  uint64_t darcs = 0;
+  uint64_t w2 = 0;
+  bool oriented = false;
+
+  // root buckets by bitmask width class
+  DevBuf roots, qctr;
+  uint32_t class_count[8] = {0};
+  uint32_t class_start[8] = {0};
+
+  // listing outputs
+  uint32_t k = 0;
+  DevBuf cnt;     // ull[n] per-vertex counts in new-id space
+  DevBuf scal;    // ull scratch scalars (T, counters)
+  DevBuf rootT;   // ull[n+1] per-root clique counts -> offsets
+  DevBuf cliques; // uint32[T*k] new ids
+  DevBuf incoff;  // ull[n+1]
+  DevBuf inc;     // uint32[T*k] clique ids per vertex
+  uint64_t T = 0;
+  bool counted = false, listed = false;
+
+  // peeling
+  DevBuf core;    // int32[n] new-id space
+  DevBuf status;  // int32[T]
+  kcds_peel_stats peel{};
+  bool peeled = false;
+
+  // components
+  DevBuf comp;    // uint32[n] new-id space: canonical component label (min original id) or ~0
+  uint32_t ncomp = 0;
+  bool have_comp = false;
+
+  CompData cd;
+  FlowBufs fb;
+  DevBuf inset;   // uint8[n] new-id space: members of the answer set
+
+  // generic temporaries
+  DevBuf tmp0, tmp1, tmp2, tmp3;
+
+  // last listing kernel timing
+  float list_kernel_ms = 0;
+  uint32_t list_kernel_launches = 0;
+
+  void *host_scratch(size_t bytes);
+  void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+};
+
+// ---- kernel launchers (k_util.hip) ----
+// out[i] = exclusive prefix sum of in[0..i), out[len] = total; returns total.
+void scan_u32_to_u32(Ctx &c, const uint32_t *in, uint32_t *out, uint64_t len);
+void scan_u64_to_u64(Ctx &c, const ull *in, ull *out, uint64_t len);
+void fill_u32(Ctx &c, uint32_t *p, uint32_t v, uint64_t len);
+void fill_u64(Ctx &c, ull *p, ull v, uint64_t len);
+void max_u32(Ctx &c, const uint32_t *in, uint64_t len, uint32_t *out_dev);
+ull sum_u64_host(Ctx &c, const ull *in, uint64_t len);
+ull max_u64_host(Ctx &c, const ull *in, uint64_t len);
+
+// ---- orientation (k_orient.hip) ----
+void orient_from_rank(Ctx &c);       // rank/inv set -> doff, dnbr, dmax, darcs, w2
+void degeneracy_rank_gpu(Ctx &c);    // fills rank/inv from a GPU k-core peel
+
+// ---- listing (k_list.hip) ----
+void list_count(Ctx &c, uint32_t k);     // cnt, T
+void list_materialize(Ctx &c, uint32_t k);  // cliques, incoff, inc (requires list_count)
+
+// ---- peeling (k_peel.hip) ----
+void peel_run(Ctx &c);
+
+// ---- components (k_comp.hip) ----
+void components_run(Ctx &c);
+
+// ---- flow (k_flow.hip) ----
+struct FlowOut {
+  int64_t value;     // max-flow value
+  uint64_t side_n;   // vertices on the source side of the maximal min cut
+  uint64_t side_T;   // component cliques with every member on that side
+  uint32_t phases;   // global relabels performed
+};
+// Max-flow of component d (dense index) at alpha = p/q; the maximal min-cut
+// source side stays in c.fb.side (uint8 per local vertex).
+FlowOut flow_component(Ctx &c, uint32_t d, uint64_t p, uint64_t q);
+// Marks the current c.fb.side of component d in c.inset (new-id space).
+void flow_mark_side(Ctx &c, uint32_t d);
+
+}  // namespace kcds
+
+struct kcds_ctx {
+  kcds::Ctx c;
+};

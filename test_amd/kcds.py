@@ -1,0 +1,181 @@
+"""Python front end of libkcds.so (the C ABI in include/kcds.h).
+
+Mirrors the reference pipeline's stage functions (main.cu): upload the graph
+(memoryAllocationGraph), orient (generateDAG), count / list cliques
+(listAllCliques), peel (cliqueCoreDecompose), components (generateDensestCore
++ prune + componentDecompose), max-flow and the whole solve (main).  Every
+call goes to the HIP library; there is no CPU fallback: if libkcds.so is
+missing or no GPU is visible the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkcds.so")
+_LIB = None
+
+U32P = ctypes.POINTER(ctypes.c_uint32)
+I64P = ctypes.POINTER(ctypes.c_int64)
+I32P = ctypes.POINTER(ctypes.c_int32)
+U8P = ctypes.POINTER(ctypes.c_uint8)
+U64P = ctypes.POINTER(ctypes.c_uint64)
+
+# every entry point include/kcds.h declares
+EXPORTS = ("kcds_create", "kcds_destroy", "kcds_last_error", "kcds_upload_csr", "kcds_orient", "kcds_count",
+           "kcds_list", "kcds_download_cliques", "kcds_peel", "kcds_components", "kcds_maxflow", "kcds_solve",
+           "kcds_sync", "kcds_set_tuning", "kcds_get_list_info")
+
+
+class PeelStats(ctypes.Structure):
+    _fields_ = [("max_num", ctypes.c_uint64), ("max_den", ctypes.c_uint64), ("k_prime", ctypes.c_uint32),
+                ("core_size", ctypes.c_uint32), ("core_cliques", ctypes.c_uint64), ("max_core", ctypes.c_int32),
+                ("levels", ctypes.c_uint32)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("T", ctypes.c_uint64), ("rho_num", ctypes.c_uint64), ("rho_den", ctypes.c_uint64),
+                ("k_prime", ctypes.c_uint32), ("core_vertices", ctypes.c_uint32), ("ncomp", ctypes.c_uint32),
+                ("flow_calls", ctypes.c_uint32), ("peel", PeelStats),
+                ("ms_orient", ctypes.c_float), ("ms_list", ctypes.c_float), ("ms_peel", ctypes.c_float),
+                ("ms_components", ctypes.c_float), ("ms_flow", ctypes.c_float), ("ms_total", ctypes.c_float)]
+
+
+class ListInfo(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("arcs", ctypes.c_uint64), ("w2", ctypes.c_uint64),
+                ("max_outdeg", ctypes.c_uint32), ("kernel_ms", ctypes.c_float),
+                ("kernel_launches", ctypes.c_uint32)]
+
+
+class KcdsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"kcds error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        V = ctypes.c_void_p
+        L.kcds_create.argtypes = [ctypes.POINTER(V), ctypes.c_int, ctypes.c_uint64]
+        L.kcds_destroy.argtypes = [V]
+        L.kcds_destroy.restype = None
+        L.kcds_last_error.argtypes = [V]
+        L.kcds_last_error.restype = ctypes.c_char_p
+        L.kcds_upload_csr.argtypes = [V, ctypes.c_uint32, U32P, U32P]
+        L.kcds_orient.argtypes = [V, U32P]
+        L.kcds_count.argtypes = [V, ctypes.c_uint32, U64P, I64P]
+        L.kcds_list.argtypes = [V, ctypes.c_uint32, U64P]
+        L.kcds_download_cliques.argtypes = [V, U32P, ctypes.c_uint64, U64P]
+        L.kcds_peel.argtypes = [V, I32P, ctypes.POINTER(PeelStats)]
+        L.kcds_components.argtypes = [V, U32P, U32P]
+        L.kcds_maxflow.argtypes = [V, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, I64P, U8P]
+        L.kcds_solve.argtypes = [V, ctypes.c_uint32, ctypes.POINTER(Result), U8P]
+        L.kcds_sync.argtypes = [V]
+        L.kcds_set_tuning.argtypes = [V, ctypes.c_uint32, ctypes.c_uint32]
+        L.kcds_get_list_info.argtypes = [V, ctypes.POINTER(ListInfo)]
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class Solver:
+    """One device context (one HIP stream, its own buffers)."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        self._h = ctypes.c_void_p()
+        rc = lib().kcds_create(ctypes.byref(self._h), device, hbm_budget)
+        if rc != 0:
+            raise KcdsError(rc, "kcds_create failed (no GPU visible?)")
+        self.n = 0
+
+    def close(self):
+        if self._h:
+            lib().kcds_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise KcdsError(rc, lib().kcds_last_error(self._h).decode())
+
+    def upload(self, offset, nbr):
+        self._off = np.ascontiguousarray(offset, dtype=np.uint32)
+        self._nbr = np.ascontiguousarray(nbr, dtype=np.uint32)
+        self.n = len(self._off) - 1
+        self._chk(lib().kcds_upload_csr(self._h, self.n, _p(self._off, U32P), _p(self._nbr, U32P)))
+
+    def orient(self, listing_order=None):
+        lo = None if listing_order is None else np.ascontiguousarray(listing_order, dtype=np.uint32)
+        self._chk(lib().kcds_orient(self._h, _p(lo, U32P)))
+
+    def count(self, k, per_vertex=True):
+        T = ctypes.c_uint64()
+        pv = np.zeros(self.n, np.int64) if per_vertex else None
+        self._chk(lib().kcds_count(self._h, k, ctypes.byref(T), _p(pv, I64P)))
+        return T.value, pv
+
+    def list(self, k):
+        T = ctypes.c_uint64()
+        self._chk(lib().kcds_list(self._h, k, ctypes.byref(T)))
+        return T.value
+
+    def download_cliques(self, k):
+        T = ctypes.c_uint64()
+        self._chk(lib().kcds_download_cliques(self._h, None, 0, ctypes.byref(T)))
+        out = np.zeros((max(T.value, 1), k), np.uint32)
+        self._chk(lib().kcds_download_cliques(self._h, _p(out, U32P), T.value, ctypes.byref(T)))
+        return out[: T.value]
+
+    def peel(self):
+        core = np.zeros(self.n, np.int32)
+        st = PeelStats()
+        self._chk(lib().kcds_peel(self._h, _p(core, I32P), ctypes.byref(st)))
+        return core, st
+
+    def components(self):
+        comp = np.zeros(self.n, np.uint32)
+        nc = ctypes.c_uint32()
+        self._chk(lib().kcds_components(self._h, _p(comp, U32P), ctypes.byref(nc)))
+        return comp, nc.value
+
+    def maxflow(self, comp_id, p, q, want_side=False):
+        f = ctypes.c_int64()
+        side = np.zeros(self.n, np.uint8) if want_side else None
+        self._chk(lib().kcds_maxflow(self._h, comp_id, p, q, ctypes.byref(f), _p(side, U8P)))
+        return (f.value, side.astype(bool)) if want_side else f.value
+
+    def solve(self, k, want_set=True):
+        res = Result()
+        ins = np.zeros(max(self.n, 1), np.uint8) if want_set else None
+        self._chk(lib().kcds_solve(self._h, k, ctypes.byref(res), _p(ins, U8P)))
+        return res, (ins[: self.n].astype(bool) if want_set else None)
+
+    def sync(self):
+        self._chk(lib().kcds_sync(self._h))
+
+    def set_tuning(self, waves_per_cu=0, flags=0):
+        self._chk(lib().kcds_set_tuning(self._h, waves_per_cu, flags))
+
+    def list_info(self):
+        info = ListInfo()
+        self._chk(lib().kcds_get_list_info(self._h, ctypes.byref(info)))
+        return info

@@ -1,0 +1,165 @@
+"""GPU parity of the stages after listing: peeling, densest core + components,
+max-flow, and the whole exact solve, against the CPU oracle.
+
+Bit-exact: clique-core numbers, the peel statistics (maxDensity as a
+rational, k', core size), canonical component labels, max-flow values at
+rational alpha, maximal min-cut sides, and the densest density as an exact
+rational with the unique maximal densest vertex set.
+"""
+import itertools
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from test_amd import gen
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def disjoint_cliques(sizes, extra_edges=()):
+    n = sum(sizes)
+    adj = [set() for _ in range(n)]
+    at = 0
+    for s in sizes:
+        for i in range(at, at + s):
+            for j in range(at, at + s):
+                if i != j:
+                    adj[i].add(j)
+        at += s
+    for u, v in extra_edges:
+        adj[u].add(v)
+        adj[v].add(u)
+    off = np.zeros(n + 1, np.uint32)
+    for v in range(n):
+        off[v + 1] = off[v] + len(adj[v])
+    nbr = np.array([u for v in range(n) for u in sorted(adj[v])], np.uint32)
+    return off, nbr
+
+
+def graphs():
+    off, nbr, _ = O.load_adjlist(os.path.join(GOLD, "ref_data_test.txt"))
+    yield "ref_data_test", off, nbr, (3, 4)
+    g = gen.er(400, 0.04, 18, seed=21)
+    yield "er400", g.offset, g.nbr, (3, 4, 5)
+    g = gen.rmat(10, 8, (12,), seed=9)
+    yield "rmat10", g.offset, g.nbr, (3, 4)
+    off, nbr = disjoint_cliques([6, 6, 5], extra_edges=[(0, 6), (11, 12)])
+    yield "tied_cliques", off, nbr, (3, 4)
+
+
+GRAPHS = list(graphs())
+
+
+@pytest.mark.parametrize("name,off,nbr,ks", GRAPHS, ids=[g[0] for g in GRAPHS])
+def test_peel_and_components(solver, name, off, nbr, ks):
+    n = len(off) - 1
+    for k in ks:
+        solver.upload(off, nbr)
+        solver.orient(None)
+        T = solver.list(k)
+        cl = O.list_cliques(off, nbr, k)
+        assert T == len(cl)
+        if T == 0:
+            continue
+        core, st = solver.peel()
+        core_r, status_r, st_r = O.peel(n, k, cl)
+        print(f"  {name} k={k} T={T} maxDensity={st.max_num}/{st.max_den} k'={st.k_prime}", flush=True)
+        np.testing.assert_array_equal(core, core_r)
+        assert (st.max_num, st.max_den, st.k_prime, st.core_size, st.core_cliques, st.max_core) == \
+            (st_r.max_num, st_r.max_den, st_r.k_prime, st_r.core_size, st_r.core_cliques, st_r.max_core)
+        comp, ncomp = solver.components()
+        _, _, _, comp_r, _ = O.solve(off, nbr, k)
+        np.testing.assert_array_equal(comp, comp_r)
+        assert ncomp == len(set(int(x) for x in comp_r if x != 0xFFFFFFFF))
+
+
+@pytest.mark.parametrize("name,off,nbr,ks", GRAPHS, ids=[g[0] for g in GRAPHS])
+def test_maxflow_values(solver, name, off, nbr, ks):
+    n = len(off) - 1
+    k = ks[0]
+    solver.upload(off, nbr)
+    solver.orient(None)
+    if solver.list(k) == 0:
+        return
+    solver.peel()
+    comp, ncomp = solver.components()
+    cl = O.list_cliques(off, nbr, k)
+    core_r, status_r, st_r = O.peel(n, k, cl)
+    for cid in sorted(set(int(x) for x in comp if x != 0xFFFFFFFF)):
+        verts = np.flatnonzero(comp == cid)
+        local = {int(v): i for i, v in enumerate(verts)}
+        ccl = np.array([[local[int(u)] for u in row] for row, s in zip(cl, status_r)
+                        if s >= st_r.k_prime and int(row[0]) in local], np.uint32).reshape(-1, k)
+        Tc, nc = len(ccl), len(verts)
+        for (p, q) in [(0, 1), (Tc, nc), (Tc + 1, nc), (2 * Tc, 3 * nc + 1), (st_r.max_core, 1), (1, 7)]:
+            f, side = solver.maxflow(cid, p, q, want_side=True)
+            fr, side_r = O.clique_maxflow(nc, k, ccl, p, q, want_side=True)
+            assert f == fr, (cid, p, q)
+            got = set(np.flatnonzero(side).tolist())
+            exp = set(int(verts[i]) for i in np.flatnonzero(side_r))
+            assert got == exp, (cid, p, q)
+
+
+@pytest.mark.parametrize("name,off,nbr,ks", GRAPHS, ids=[g[0] for g in GRAPHS])
+def test_solve_matches_oracle(solver, name, off, nbr, ks):
+    for k in ks:
+        solver.upload(off, nbr)
+        res, ins = solver.solve(k)
+        r, pv, core, comp, ins_r = O.solve(off, nbr, k)
+        print(f"  {name} k={k} rho={res.rho_num}/{res.rho_den} flows={res.flow_calls}", flush=True)
+        assert res.T == r.T
+        assert (res.rho_num, res.rho_den) == (r.rho_num, r.rho_den)
+        np.testing.assert_array_equal(ins, ins_r)
+
+
+def test_tied_cliques_union(solver):
+    # two disjoint K6 and a K5: both K6 reach the optimum C(6,3)/6 for k=3,
+    # so the maximal densest set is their union
+    off, nbr = disjoint_cliques([6, 6, 5])
+    solver.upload(off, nbr)
+    res, ins = solver.solve(3)
+    assert (res.rho_num, res.rho_den) == (2 * math.comb(6, 3), 12)
+    assert set(np.flatnonzero(ins).tolist()) == set(range(12))
+
+
+def test_brute_force_small(solver):
+    rng = np.random.default_rng(7)
+    for trial in range(12):
+        n = int(rng.integers(6, 12))
+        p = float(rng.uniform(0.35, 0.8))
+        g = gen.er(n, p, 0, seed=100 + trial)
+        adj = [set(g.nbr[g.offset[v]:g.offset[v + 1]].tolist()) for v in range(n)]
+        for k in (3, 4):
+            cl = [c for c in itertools.combinations(range(n), k)
+                  if all(b in adj[a] for a, b in itertools.combinations(c, 2))]
+            solver.upload(g.offset, g.nbr)
+            res, ins = solver.solve(k)
+            assert res.T == len(cl)
+            if not cl:
+                continue
+            best = (0, 1)
+            sets = []
+            for mask in range(1, 1 << n):
+                S = [v for v in range(n) if mask >> v & 1]
+                t = sum(1 for c in cl if all(mask >> v & 1 for v in c))
+                if t * best[1] > best[0] * len(S):
+                    best = (t, len(S))
+                sets.append((mask, t, len(S)))
+            union = 0
+            for mask, t, s in sets:
+                if t * best[1] == best[0] * s and t > 0:
+                    union |= mask
+            assert res.rho_num * best[1] == best[0] * res.rho_den
+            assert int(sum(1 << int(v) for v in np.flatnonzero(ins))) == union
+
+
+def test_config1_closed_form(solver):
+    g, k = gen.config(1)
+    solver.upload(g.offset, g.nbr)
+    res, ins = solver.solve(k)
+    assert (res.rho_num, res.rho_den) == (math.comb(40, 3), 40)
+    assert set(np.flatnonzero(ins).tolist()) == set(g.planted.tolist())
