@@ -1,0 +1,123 @@
+"""Benchmark: k-cliques listed per second on one MI355X.
+
+Workload (one step = one pass of the hot path over the graph already resident
+in HBM): config C2 -- R-MAT scale 18 (n = 262,144), edge factor 16,
+(a,b,c,d) = (.57,.19,.19,.05), symmetrised and deduplicated, plus a planted
+60-clique and five 30-cliques; k = 4.  A step is GPU degeneracy ordering +
+DAG build (generateDAG) followed by the wave64 listing pass that counts every
+4-clique and every per-vertex 4-clique count (listAllCliques' T and
+cliqueDegree).  value = T / seconds per step.
+
+Also reported on the same JSON line:
+  roofline      the listing kernels' algorithmic HBM bytes per pass
+                B = 4(n+1) + 4*arcs + 4*W2 + 8n (DAG CSR read once, every
+                root's out-neighbour rows staged, int64 counts written) over
+                their hipEvent-timed duration, against 8 TB/s;
+  cpu_baseline  the CPU oracle's count of the same graph (all host threads);
+  extras        exact k-CDS solves/s on config C1 (ER 2000 + K40, k = 3).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the listing shards by root ranges with one all-reduce of int64 counts; not built here)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from test_amd import gen, kcds
+
+    g, k = gen.config(2)
+    s = kcds.Solver(0)
+    s.upload(g.offset, g.nbr)
+
+    def step():
+        s.orient(None)
+        T, _ = s.count(k, per_vertex=False)
+        return T
+
+    for _ in range(args.warmup):
+        step()
+    s.sync()
+    kern_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        T = step()
+        kern_ms.append(s.list_info().kernel_ms)
+    s.sync()
+    dt = time.perf_counter() - t0
+    ms = dt * 1e3 / args.steps
+    info = s.list_info()
+    value = T / (ms / 1e3)
+
+    n, arcs, w2 = info.n, info.arcs, info.w2
+    bytes_list = 4 * (n + 1) + 4 * arcs + 4 * w2 + 8 * n
+    kms = float(np.mean(kern_ms))
+    achieved = bytes_list / (kms / 1e3)
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None,
+                "kernel": "k_list_count<W,4> (4 width-class launches per pass)",
+                "kernel_ms": kms, "bytes_per_pass": bytes_list}
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        threads = os.cpu_count() or 1
+        t1 = time.perf_counter()
+        Tc, _ = O.count(g.offset, g.nbr, k, per_vertex=True, threads=threads)
+        tc = time.perf_counter() - t1
+        assert Tc == T, (Tc, T)
+        cpu = {"value": Tc / tc, "unit": "cliques/s", "cores": threads, "kind": "port",
+               "sample": f"full C2 graph, k={k}, per-vertex counts (oracle kClist-style listing, OpenMP)",
+               "seconds": tc}
+
+    extras = {}
+    if not args.no_extras:
+        g1, k1 = gen.config(1)
+        s1 = kcds.Solver(0)
+        s1.upload(g1.offset, g1.nbr)
+        s1.solve(k1, want_set=False)
+        reps = 5
+        t2 = time.perf_counter()
+        for _ in range(reps):
+            res, _ = s1.solve(k1, want_set=False)
+        t_solve = (time.perf_counter() - t2) / reps
+        extras["C1_exact_solves_per_s"] = 1.0 / t_solve
+        extras["C1_solve_ms"] = t_solve * 1e3
+        extras["C1_rho"] = f"{res.rho_num}/{res.rho_den}"
+        s1.close()
+
+    line = {
+        "metric": "k-cliques listed/s", "value": value, "unit": "cliques/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "C2 R-MAT scale 18 ef16 +K60+5xK30, k=4: GPU degeneracy order + DAG + "
+                               "per-vertex 4-clique counts", "n": g.n, "m": g.m, "k": k, "T": T,
+                   "dag_arcs": arcs, "w2": w2, "max_outdeg": info.max_outdeg},
+        "roofline": roofline, "cpu_baseline": cpu, "extras": extras,
+    }
+    print(json.dumps(line))
+    s.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -111,6 +111,28 @@ int kcds_maxflow(kcds_ctx *ctx, uint32_t comp_id, uint64_t p, uint64_t q, int64_
  * 1 for members of the maximal densest set S. */
 int kcds_solve(kcds_ctx *ctx, uint32_t k, kcds_result *res, uint8_t *in_set);
 
+/* Per-graph result of a batched solve. */
+typedef struct kcds_graph_result {
+  uint64_t T;          /* k-cliques of the graph */
+  uint64_t rho_num;    /* cliques inside the graph's maximal densest set */
+  uint64_t rho_den;    /* its size (0 when the graph has no k-clique) */
+  uint32_t k_prime;
+  uint32_t core_vertices;
+  uint32_t ncomp;
+  uint32_t flow_calls;
+} kcds_graph_result;
+
+/* Batched solve: the uploaded CSR is the disjoint union of ngraphs graphs,
+ * graph g owning vertex ids [graph_voff[g], graph_voff[g+1]) (graph_voff[0] =
+ * 0, graph_voff[ngraphs] = n, no edge between graphs).  Every stage runs once
+ * over the union; peeling keeps per-graph density bookkeeping and every
+ * component's exact search runs in one persistent launch.  results[ngraphs]
+ * get each graph's answer (identical to solving it alone); in_set[n] (may be
+ * NULL) marks the members of each graph's maximal densest set.  No
+ * counterpart in the reference, which solves one graph per process. */
+int kcds_solve_batch(kcds_ctx *ctx, uint32_t k, uint32_t ngraphs, const uint32_t *graph_voff,
+                     kcds_graph_result *results, uint8_t *in_set);
+
 /* Blocks until all work queued on the context's stream has finished. */
 int kcds_sync(kcds_ctx *ctx);
 

@@ -53,6 +53,17 @@ def build_lib(jobs=8):
     return out
 
 
+def build_cli():
+    """The drop-in CLI: Graph loader (graph.cpp) + main.cpp linked to libkcds.so."""
+    os.makedirs(OBJ, exist_ok=True)
+    out = os.path.join(HERE, "kcds")
+    srcs = [os.path.join(SRC, "graph.cpp"), os.path.join(SRC, "main.cpp")]
+    _run([HIPCC, "-O3", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", out + ".tmp"] + srcs +
+         ["-L", HERE, "-lkcds", "-Wl,-rpath,$ORIGIN"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_gen():
     out = os.path.join(HERE, "libkcdsgen.so")
     _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", out + ".tmp",
@@ -62,7 +73,7 @@ def build_gen():
 
 
 def build_all(jobs=8):
-    return [build_gen(), build_lib(jobs)]
+    return [build_gen(), build_lib(jobs), build_cli()]
 
 
 if __name__ == "__main__":

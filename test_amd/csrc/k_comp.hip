@@ -55,11 +55,17 @@ __global__ void k_uf_init(uint32_t n, uint32_t *par) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) par[v] = v;
 }
 
-__global__ void k_uf_hook(uint64_t T, uint32_t k, const uint32_t *cl, const int32_t *status, int32_t kp,
-                          uint32_t *par, const uint32_t *key) {
+// a clique is in its graph's k'-core iff its status (min member core) >= k'
+__device__ __forceinline__ bool core_clique(uint64_t c, uint32_t k, const uint32_t *cl, const int32_t *status,
+                                            const GraphPeel *gp, const uint32_t *gid) {
+  return status[c] >= (int32_t)gp[gid[cl[c * k]]].k_prime;
+}
+
+__global__ void k_uf_hook(uint64_t T, uint32_t k, const uint32_t *cl, const int32_t *status, const GraphPeel *gp,
+                          const uint32_t *gid, uint32_t *par, const uint32_t *key) {
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
        c += (uint64_t)gridDim.x * blockDim.x) {
-    if (status[c] < kp) continue;
+    if (!core_clique(c, k, cl, status, gp, gid)) continue;
     const uint32_t a = cl[c * k];
     for (uint32_t t = 1; t < k; ++t) uf_unite(par, key, a, cl[c * k + t]);
   }
@@ -67,10 +73,10 @@ __global__ void k_uf_hook(uint64_t T, uint32_t k, const uint32_t *cl, const int3
 
 // comp[r] = key of the root for core vertices, ~0 elsewhere; roots flagged in
 // original-id order for dense numbering
-__global__ void k_uf_label(uint32_t n, const int32_t *core, int32_t kp, uint32_t *par, const uint32_t *key,
-                           uint32_t *comp, uint32_t *isroot_orig) {
+__global__ void k_uf_label(uint32_t n, const int32_t *core, const GraphPeel *gp, const uint32_t *gid, uint32_t *par,
+                           const uint32_t *key, uint32_t *comp, uint32_t *isroot_orig) {
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-    const bool in = core[r] >= kp;
+    const bool in = core[r] >= (int32_t)gp[gid[r]].k_prime;
     const uint32_t root = uf_find(par, r);
     comp[r] = in ? key[root] : 0xFFFFFFFFu;
     isroot_orig[key[r]] = (in && root == r) ? 1u : 0u;
@@ -100,24 +106,38 @@ __global__ void k_scatter_vertices(uint32_t n, const uint32_t *cid, const uint32
   }
 }
 
-__global__ void k_count_comp_cliques(uint64_t T, uint32_t k, const uint32_t *cl, const int32_t *status, int32_t kp,
-                                     const uint32_t *cid, ull *ccount) {
+__global__ void k_count_comp_cliques(uint64_t T, uint32_t k, const uint32_t *cl, const int32_t *status,
+                                     const GraphPeel *gp, const uint32_t *gid, const uint32_t *cid, ull *ccount) {
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
        c += (uint64_t)gridDim.x * blockDim.x)
-    if (status[c] >= kp) atomicAdd(&ccount[cid[cl[c * k]]], 1ull);
+    if (core_clique(c, k, cl, status, gp, gid)) atomicAdd(&ccount[cid[cl[c * k]]], 1ull);
 }
 
 // core cliques grouped by component, members as local vertex ids
-__global__ void k_scatter_cliques(uint64_t T, uint32_t k, const uint32_t *cl, const int32_t *status, int32_t kp,
-                                  const uint32_t *cid, const uint32_t *local, const ull *coff, ull *cfill,
-                                  uint32_t *ccl) {
+__global__ void k_scatter_cliques(uint64_t T, uint32_t k, const uint32_t *cl, const int32_t *status,
+                                  const GraphPeel *gp, const uint32_t *gid, const uint32_t *cid, const uint32_t *local,
+                                  const ull *coff, ull *cfill, uint32_t *ccl) {
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
        c += (uint64_t)gridDim.x * blockDim.x) {
-    if (status[c] < kp) continue;
+    if (!core_clique(c, k, cl, status, gp, gid)) continue;
     const uint32_t d = cid[cl[c * k]];
     const ull at = coff[d] + atomicAdd(&cfill[d], 1ull);
     if (at >= coff[d + 1]) continue;  // never taken: counts and scatter agree
     for (uint32_t t = 0; t < k; ++t) ccl[at * k + t] = local[cl[c * k + t]];
+  }
+}
+
+// counts edges whose endpoints lie in different graphs of the batch
+__global__ void k_check_partition(uint32_t n, const uint32_t *off, const uint32_t *nbr, const uint32_t *gvoff,
+                                  uint32_t G, uint32_t *bad) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const uint32_t g = upper_bound_u32(gvoff, G + 1, v) - 1;
+    const uint32_t lo = gvoff[g], hi = gvoff[g + 1];
+    for (uint32_t i = off[v]; i < off[v + 1]; ++i)
+      if (nbr[i] < lo || nbr[i] >= hi) {
+        atomicAdd(bad, 1u);
+        break;
+      }
   }
 }
 
@@ -127,11 +147,36 @@ uint32_t grid1d(uint64_t len) {
 
 }  // namespace
 
+void check_partition(Ctx &c) {
+  const uint32_t n = c.n, G = c.ngraphs;
+  buf_reserve(c, c.gvoff, sizeof(uint32_t) * ((size_t)G + 1));
+  buf_reserve(c, c.scal, 64 * sizeof(ull));
+  uint32_t *bad = (uint32_t *)(c.scal.as<ull>() + 40);
+  HIP_CHECK(hipMemcpyAsync(c.gvoff.p, c.h_gvoff.data(), sizeof(uint32_t) * ((size_t)G + 1), hipMemcpyHostToDevice,
+                           c.stream));
+  HIP_CHECK(hipMemsetAsync(bad, 0, sizeof(uint32_t), c.stream));
+  if (n) {
+    k_check_partition<<<grid1d(n), 256, 0, c.stream>>>(n, c.off.as<uint32_t>(), c.nbr.as<uint32_t>(),
+                                                       c.gvoff.as<uint32_t>(), G, bad);
+    KCDS_LAUNCH_CHECK();
+  }
+  uint32_t *h = (uint32_t *)c.host_scratch(8);
+  HIP_CHECK(hipMemcpyAsync(h, bad, 4, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  if (*h) fail(KCDS_E_INVALID, "%u vertices have edges into another graph of the batch", *h);
+}
+
 void components_run(Ctx &c) {
   if (!c.peeled) fail(KCDS_E_STATE, "components before peel");
   const uint32_t n = c.n, k = c.k;
   const uint64_t T = c.T;
-  const int32_t kp = (int32_t)c.peel.k_prime;
+  const GraphPeel *gp = c.gpeel.as<GraphPeel>();
+  const uint32_t *gidp = c.gid.as<uint32_t>();
+  uint64_t core_vertices = 0, core_cliques = 0;
+  for (const GraphPeel &P : c.h_gpeel) {
+    core_vertices += P.core_size;
+    core_cliques += P.core_cliques;
+  }
   CompData &g = c.cd;
   buf_reserve(c, c.comp, sizeof(uint32_t) * ((size_t)n + 1));
   buf_reserve(c, g.par, sizeof(uint32_t) * ((size_t)n + 1));
@@ -142,10 +187,11 @@ void components_run(Ctx &c) {
   k_uf_init<<<grid1d(n), 256, 0, c.stream>>>(n, par);
   KCDS_LAUNCH_CHECK();
   if (T) {
-    k_uf_hook<<<grid1d(T), 256, 0, c.stream>>>(T, k, c.cliques.as<uint32_t>(), c.status.as<int32_t>(), kp, par, key);
+    k_uf_hook<<<grid1d(T), 256, 0, c.stream>>>(T, k, c.cliques.as<uint32_t>(), c.status.as<int32_t>(), gp, gidp, par,
+                                               key);
     KCDS_LAUNCH_CHECK();
   }
-  k_uf_label<<<grid1d(n), 256, 0, c.stream>>>(n, c.core.as<int32_t>(), kp, par, key, c.comp.as<uint32_t>(),
+  k_uf_label<<<grid1d(n), 256, 0, c.stream>>>(n, c.core.as<int32_t>(), gp, gidp, par, key, c.comp.as<uint32_t>(),
                                               g.flag.as<uint32_t>());
   KCDS_LAUNCH_CHECK();
   // dense[orig] = number of component roots with a smaller original id
@@ -161,7 +207,7 @@ void components_run(Ctx &c) {
   buf_reserve(c, g.local, sizeof(uint32_t) * ((size_t)n + 1));
   buf_reserve(c, g.vcount, sizeof(uint32_t) * ((size_t)C + 2));
   buf_reserve(c, g.voff, sizeof(uint32_t) * ((size_t)C + 2));
-  buf_reserve(c, g.verts, sizeof(uint32_t) * ((size_t)c.peel.core_size + 1));
+  buf_reserve(c, g.verts, sizeof(uint32_t) * ((size_t)core_vertices + 1));
   HIP_CHECK(hipMemsetAsync(g.vcount.p, 0, sizeof(uint32_t) * ((size_t)C + 2), c.stream));
   k_group_vertices<<<grid1d(n), 256, 0, c.stream>>>(n, c.comp.as<uint32_t>(), g.dense.as<uint32_t>(),
                                                     c.rank.as<uint32_t>(), g.cid.as<uint32_t>(),
@@ -179,14 +225,14 @@ void components_run(Ctx &c) {
   HIP_CHECK(hipMemsetAsync(g.cfill.p, 0, sizeof(ull) * ((size_t)C + 2), c.stream));
   if (T) {
     k_count_comp_cliques<<<grid1d(T), 256, 0, c.stream>>>(T, k, c.cliques.as<uint32_t>(), c.status.as<int32_t>(),
-                                                          kp, g.cid.as<uint32_t>(), g.ccount.as<ull>());
+                                                          gp, gidp, g.cid.as<uint32_t>(), g.ccount.as<ull>());
     KCDS_LAUNCH_CHECK();
   }
   scan_u64_to_u64(c, g.ccount.as<ull>(), g.coff.as<ull>(), C);
-  buf_reserve(c, g.ccl, sizeof(uint32_t) * (c.peel.core_cliques * k + 1));
+  buf_reserve(c, g.ccl, sizeof(uint32_t) * (core_cliques * k + 1));
   if (T) {
-    k_scatter_cliques<<<grid1d(T), 256, 0, c.stream>>>(T, k, c.cliques.as<uint32_t>(), c.status.as<int32_t>(), kp,
-                                                       g.cid.as<uint32_t>(), g.local.as<uint32_t>(),
+    k_scatter_cliques<<<grid1d(T), 256, 0, c.stream>>>(T, k, c.cliques.as<uint32_t>(), c.status.as<int32_t>(), gp,
+                                                       gidp, g.cid.as<uint32_t>(), g.local.as<uint32_t>(),
                                                        g.coff.as<ull>(), g.cfill.as<ull>(), g.ccl.as<uint32_t>());
     KCDS_LAUNCH_CHECK();
   }

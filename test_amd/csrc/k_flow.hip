@@ -145,16 +145,24 @@ __device__ __forceinline__ bool discharge_step(uint32_t u, ull best, const uint3
   return true;
 }
 
-__global__ __launch_bounds__(1024) void k_push_relabel(uint32_t nc, uint64_t Tc, const uint32_t *off,
-                                                       const uint32_t *head, const uint32_t *rev, i64 *res,
-                                                       i64 *excess, uint32_t *height, uint32_t *fr0, uint32_t *fr1,
-                                                       FlowState *st) {
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t nthreads = gridDim.x * blockDim.x;
+// Max-preflow phase of push-relabel over threads [tid, nthreads) of either
+// the whole grid (GRID: cooperative launch, software grid barrier) or one
+// workgroup (block barrier).  On return st->status = 1 (done) or 2 (phase cap)
+// and heights hold exact distances to t (N for nodes that cannot reach t).
+template <bool GRID>
+__device__ void max_preflow(uint32_t nc, uint64_t Tc, const uint32_t *off, const uint32_t *head, const uint32_t *rev,
+                            i64 *res, i64 *excess, uint32_t *height, uint32_t *fr0, uint32_t *fr1, FlowState *st,
+                            const uint32_t tid, const uint32_t nthreads, uint32_t &epoch) {
   const uint32_t lane = lane_id();
   const uint32_t wid = tid / KCDS_WAVE, nwaves = nthreads / KCDS_WAVE;
   const uint32_t N = nc + (uint32_t)Tc + 2, s = N - 2, t = N - 1;
-  uint32_t epoch = 0;
+  auto sync = [&]() {
+    if constexpr (GRID) grid_barrier(&st->barrier, gridDim.x, epoch);
+    else {
+      __threadfence_block();
+      __syncthreads();
+    }
+  };
   for (uint32_t phase = 0;; ++phase) {
     // ---- global relabel: exact distances to t in the residual graph ----
     for (uint32_t x = tid; x < N; x += nthreads) height[x] = (x == t) ? 0u : N;
@@ -166,7 +174,7 @@ __global__ __launch_bounds__(1024) void k_push_relabel(uint32_t nc, uint64_t Tc,
       st->changed[0] = 0;
       st->changed[1] = 0;
     }
-    grid_barrier(&st->barrier, gridDim.x, epoch);
+    sync();
     int cur = 0;
     for (uint32_t dist = 0;; ++dist) {
       const uint32_t fs = ld32(&st->fsize[cur]);
@@ -181,9 +189,9 @@ __global__ __launch_bounds__(1024) void k_push_relabel(uint32_t nc, uint64_t Tc,
           if (atomicCAS(&height[x], N, dist + 1) == N) nx[atomicAdd(&st->fsize[cur ^ 1], 1u)] = x;
         }
       }
-      grid_barrier(&st->barrier, gridDim.x, epoch);
+      sync();
       if (tid == 0) st->fsize[cur] = 0;
-      grid_barrier(&st->barrier, gridDim.x, epoch);
+      sync();
       cur ^= 1;
     }
     // ---- any node with excess that can still reach t? ----
@@ -191,13 +199,14 @@ __global__ __launch_bounds__(1024) void k_push_relabel(uint32_t nc, uint64_t Tc,
     for (uint32_t x = tid; x < s; x += nthreads) act += (ld64(&excess[x]) > 0 && height[x] < N);
     act = wave_sum_u32(act);
     if (lane == 0 && act) atomicAdd(&st->active, act);
-    grid_barrier(&st->barrier, gridDim.x, epoch);
+    sync();
     const uint32_t active = ld32(&st->active);
     if (active == 0 || phase + 1 >= MAX_PHASES) {
       if (tid == 0) {
         st->status = active == 0 ? 1u : 2u;
         st->phases = phase + 1;
       }
+      sync();
       break;
     }
     // ---- push / relabel sweeps ----
@@ -235,13 +244,237 @@ __global__ __launch_bounds__(1024) void k_push_relabel(uint32_t nc, uint64_t Tc,
       }
       changed = wave_sum_u32(changed);
       if (lane == 0 && changed) atomicAdd(&st->changed[slot], changed);
-      grid_barrier(&st->barrier, gridDim.x, epoch);
+      sync();
       // every thread reads this sweep's slot after the barrier; the other slot
       // was last read before it, so it can be cleared for the next sweep now
       const uint32_t any = ld32(&st->changed[slot]);
       if (tid == 0) st->changed[slot ^ 1] = 0;
       if (any == 0) break;
     }
+    sync();  // the next phase's resets follow every thread's last read
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_push_relabel(uint32_t nc, uint64_t Tc, const uint32_t *off,
+                                                       const uint32_t *head, const uint32_t *rev, i64 *res,
+                                                       i64 *excess, uint32_t *height, uint32_t *fr0, uint32_t *fr1,
+                                                       FlowState *st) {
+  uint32_t epoch = 0;
+  max_preflow<true>(nc, Tc, off, head, rev, res, excess, height, fr0, fr1, st,
+                    blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, epoch);
+}
+
+// ---------------------------------------------------------------------------
+// Block-per-component exact search (batched solves).  A persistent grid of
+// 1024-thread workgroups pops components from a queue; each workgroup builds
+// its component's network in a private slice of the scratch arrays and runs
+// the whole Dinkelbach iteration on the device:
+//   alpha = p0/q0; repeat { max-flow at alpha; if flow < q*k*Tc then
+//   alpha := density of the maximal min-cut source side } until no set is
+//   denser than alpha.
+// The result is the component's best density reachable from alpha0 and the
+// maximal min-cut side at that alpha (empty when no set reaches alpha0).
+
+struct CompTask {
+  ull Tc;         // cliques of the component
+  ull coff;       // first clique row in ccl
+  ull node_base;  // slice of the node arrays
+  ull arc_base;   // slice of the arc arrays
+  ull p0, q0;     // starting alpha
+  uint32_t nc;    // vertices of the component
+  uint32_t voff;  // first vertex in verts / cside
+};
+
+struct CompOut {
+  ull p, q;            // final alpha, reduced
+  ull side_n, side_T;  // maximal min-cut side at the final alpha
+  uint32_t flows;
+  uint32_t status;     // 1 ok, 2 push-relabel phase cap, 3 iteration cap
+};
+
+constexpr uint32_t MAX_DINKELBACH = 64;
+
+__device__ ull gcd_u64(ull a, ull b) {
+  while (b) {
+    const ull t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// exclusive scan of one value per thread over a 1024-thread block
+__device__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t *total, uint32_t *wsum) {
+  const uint32_t lane = threadIdx.x % KCDS_WAVE, w = threadIdx.x / KCDS_WAVE;
+  const uint32_t incl = wave_incl_scan_u32(v);
+  if (lane == KCDS_WAVE - 1) wsum[w] = incl;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (uint32_t i = 0; i < blockDim.x / KCDS_WAVE; ++i) {
+    if (i < w) before += wsum[i];
+    all += wsum[i];
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
+__global__ __launch_bounds__(1024) void k_dinkelbach_block(const CompTask *tasks, uint32_t ntasks, uint32_t *qctr,
+                                                           uint32_t k, const uint32_t *ccl_all, uint32_t *noff_all,
+                                                           i64 *excess_all, uint32_t *height_all, uint32_t *fr0_all,
+                                                           uint32_t *fr1_all, uint32_t *head_all, uint32_t *rev_all,
+                                                           i64 *res_all, uint8_t *cside, CompOut *outs) {
+  __shared__ FlowState st;
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t task_id;
+  __shared__ ull red[2];
+  __shared__ ull alpha[2];
+  __shared__ uint32_t done;
+  const uint32_t tid = threadIdx.x, nthreads = blockDim.x;
+  uint32_t epoch = 0;
+  while (true) {
+    if (tid == 0) task_id = atomicAdd(qctr, 1u);
+    __syncthreads();
+    const uint32_t ti = task_id;
+    __syncthreads();
+    if (ti >= ntasks) break;
+    const CompTask tk = tasks[ti];
+    const uint32_t nc = tk.nc;
+    const ull Tc = tk.Tc;
+    const uint32_t N = nc + (uint32_t)Tc + 2, s = N - 2, t = N - 1;
+    const uint32_t *ccl = ccl_all + tk.coff * k;
+    uint32_t *off = noff_all + tk.node_base, *height = height_all + tk.node_base;
+    uint32_t *fr0 = fr0_all + tk.node_base, *fr1 = fr1_all + tk.node_base;
+    i64 *excess = excess_all + tk.node_base;
+    uint32_t *head = head_all + tk.arc_base, *rev = rev_all + tk.arc_base;
+    i64 *res = res_all + tk.arc_base;
+    // ---- network topology ----
+    for (uint32_t v = tid; v < nc; v += nthreads) fr0[v] = 0;
+    __syncthreads();
+    for (ull i = tid; i < Tc * k; i += nthreads) atomicAdd(&fr0[ccl[i]], 1u);
+    __syncthreads();
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nc; b0 += nthreads) {
+      const uint32_t v = b0 + tid;
+      const uint32_t cntv = v < nc ? 2 + fr0[v] : 0;
+      uint32_t tot;
+      const uint32_t ex = block_excl_scan_1024(cntv, &tot, wsum);
+      if (v < nc) off[v] = carry + ex;
+      carry += tot;
+    }
+    const uint32_t CA = carry;  // = 2*nc + k*Tc
+    const uint32_t SA = CA + (uint32_t)(Tc * k), TA = SA + nc;
+    for (ull c = tid; c <= Tc; c += nthreads) off[nc + c] = CA + (uint32_t)(c * k);
+    if (tid == 0) {
+      off[s] = SA;
+      off[t] = TA;
+      off[t + 1] = TA + nc;
+    }
+    for (uint32_t v = tid; v < nc; v += nthreads) {
+      const uint32_t a = off[v];
+      head[a] = t;
+      rev[a] = TA + v;
+      head[TA + v] = v;
+      rev[TA + v] = a;
+      head[a + 1] = s;
+      rev[a + 1] = SA + v;
+      head[SA + v] = v;
+      rev[SA + v] = a + 1;
+      fr1[v] = 0;
+    }
+    __syncthreads();
+    for (ull i = tid; i < Tc * k; i += nthreads) {
+      const uint32_t v = ccl[i];
+      const uint32_t slot = atomicAdd(&fr1[v], 1u);
+      const uint32_t a = off[v] + 2 + slot, b = CA + (uint32_t)i;
+      head[a] = nc + (uint32_t)(i / k);
+      rev[a] = b;
+      head[b] = v;
+      rev[b] = a;
+    }
+    if (tid == 0) {
+      alpha[0] = tk.p0;
+      alpha[1] = tk.q0;
+      done = 0;
+    }
+    __syncthreads();
+    uint32_t flows = 0, status = 1;
+    for (uint32_t it = 0;; ++it) {
+      const i64 p = (i64)alpha[0], q = (i64)alpha[1];
+      // residual capacities for alpha = p/q after the preflow saturates s->v
+      for (uint32_t v = tid; v < nc; v += nthreads) {
+        const uint32_t a = off[v];
+        const i64 deg = (i64)(off[v + 1] - a - 2);
+        res[a] = (i64)k * p;
+        res[TA + v] = 0;
+        res[a + 1] = q * deg;
+        res[SA + v] = 0;
+        excess[v] = q * deg;
+      }
+      for (ull i = tid; i < Tc * k; i += nthreads) {
+        const uint32_t b = CA + (uint32_t)i;
+        res[b] = q * (i64)(k - 1);
+        res[rev[b]] = q;
+      }
+      for (ull c = nc + tid; c < N; c += nthreads) excess[c] = 0;
+      if (tid == 0) {
+        red[0] = 0;
+        red[1] = 0;
+        st.status = 0;
+      }
+      __syncthreads();
+      max_preflow<false>(nc, Tc, off, head, rev, res, excess, height, fr0, fr1, &st, tid, nthreads, epoch);
+      ++flows;
+      if (st.status != 1) {
+        status = 2;
+        break;
+      }
+      // maximal min-cut side: vertices that cannot reach t
+      ull sn = 0, sT = 0;
+      for (uint32_t v = tid; v < nc; v += nthreads) sn += height[v] >= N;
+      for (ull c = tid; c < Tc; c += nthreads) {
+        bool all = true;
+        for (uint32_t j = 0; j < k; ++j) all = all && height[ccl[c * k + j]] >= N;
+        sT += all;
+      }
+      sn = wave_sum_u64(sn);
+      sT = wave_sum_u64(sT);
+      if (lane_id() == 0) {
+        if (sn) atomicAdd(&red[0], sn);
+        if (sT) atomicAdd(&red[1], sT);
+      }
+      __syncthreads();
+      if (tid == 0) {
+        const i64 flow = excess[t];
+        const ull full = (ull)q * k * Tc;
+        if ((ull)flow >= full) {
+          done = 1;  // no set denser than alpha: this side is the maximal tight set
+        } else if (red[0] == 0 || it + 1 >= MAX_DINKELBACH) {
+          done = 2;
+        } else {
+          const ull g = gcd_u64(red[1], red[0]);
+          alpha[0] = red[1] / g;
+          alpha[1] = red[0] / g;
+        }
+      }
+      __syncthreads();
+      if (done) {
+        if (done == 2) status = 3;
+        break;
+      }
+    }
+    for (uint32_t v = tid; v < nc; v += nthreads) cside[tk.voff + v] = (status == 1 && height[v] >= N) ? 1 : 0;
+    if (tid == 0) {
+      CompOut o;
+      o.p = alpha[0];
+      o.q = alpha[1];
+      o.side_n = status == 1 ? red[0] : 0;
+      o.side_T = status == 1 ? red[1] : 0;
+      o.flows = flows;
+      o.status = status;
+      outs[ti] = o;
+    }
+    __syncthreads();
   }
 }
 
@@ -379,6 +612,97 @@ void flow_mark_side(Ctx &c, uint32_t d) {
   k_flow_mark<<<grid1d(nc), 256, 0, c.stream>>>(nc, g.verts.as<uint32_t>() + g.h_voff[d], c.fb.side.as<uint8_t>(),
                                                 c.inset.as<uint8_t>());
   KCDS_LAUNCH_CHECK();
+}
+
+
+namespace {
+__global__ void k_copy_side(uint32_t nc, const uint8_t *side, uint8_t *dst) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nc; v += gridDim.x * blockDim.x) dst[v] = side[v];
+}
+__global__ void k_mark_cside(uint32_t nc, const uint32_t *verts, const uint8_t *cside, uint8_t *inset) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nc; v += gridDim.x * blockDim.x)
+    if (cside[v]) inset[verts[v]] = 1;
+}
+}  // namespace
+
+void flow_store_side(Ctx &c, uint32_t d) {
+  CompData &g = c.cd;
+  const uint32_t nc = g.h_voff[d + 1] - g.h_voff[d];
+  if (!nc) return;
+  k_copy_side<<<grid1d(nc), 256, 0, c.stream>>>(nc, c.fb.side.as<uint8_t>(), c.fb.cside.as<uint8_t>() + g.h_voff[d]);
+  KCDS_LAUNCH_CHECK();
+}
+
+void mark_component_side(Ctx &c, uint32_t d) {
+  CompData &g = c.cd;
+  const uint32_t nc = g.h_voff[d + 1] - g.h_voff[d];
+  if (!nc) return;
+  k_mark_cside<<<grid1d(nc), 256, 0, c.stream>>>(nc, g.verts.as<uint32_t>() + g.h_voff[d],
+                                                 c.fb.cside.as<uint8_t>() + g.h_voff[d], c.inset.as<uint8_t>());
+  KCDS_LAUNCH_CHECK();
+}
+
+void dinkelbach_blocks(Ctx &c, const std::vector<BlockTask> &in, std::vector<BlockResult> &out) {
+  out.assign(in.size(), BlockResult{});
+  if (in.empty()) return;
+  CompData &g = c.cd;
+  FlowBufs &f = c.fb;
+  const uint32_t k = c.k;
+  std::vector<CompTask> tasks(in.size());
+  ull nodes = 0, arcs = 0;
+  for (size_t i = 0; i < in.size(); ++i) {
+    const uint32_t d = in[i].d;
+    CompTask &t = tasks[i];
+    t.nc = g.h_voff[d + 1] - g.h_voff[d];
+    t.voff = g.h_voff[d];
+    t.Tc = g.h_coff[d + 1] - g.h_coff[d];
+    t.coff = g.h_coff[d];
+    t.p0 = in[i].p0;
+    t.q0 = in[i].q0;
+    t.node_base = nodes;
+    t.arc_base = arcs;
+    nodes += (ull)t.nc + t.Tc + 4;
+    arcs += 4ull * t.nc + 2ull * k * t.Tc;
+    if ((ull)t.nc + t.Tc + 2 >= 0x7FFFFFFFull || 4ull * t.nc + 2ull * k * t.Tc >= 0xFFFFFFFFull)
+      fail(KCDS_E_LIMIT, "component %u too large for the block solver", d);
+  }
+  buf_reserve(c, f.b_off, sizeof(uint32_t) * (nodes + 1));
+  buf_reserve(c, f.b_excess, sizeof(i64) * (nodes + 1));
+  buf_reserve(c, f.b_height, sizeof(uint32_t) * (nodes + 1));
+  buf_reserve(c, f.b_fr0, sizeof(uint32_t) * (nodes + 1));
+  buf_reserve(c, f.b_fr1, sizeof(uint32_t) * (nodes + 1));
+  buf_reserve(c, f.b_head, sizeof(uint32_t) * (arcs + 1));
+  buf_reserve(c, f.b_rev, sizeof(uint32_t) * (arcs + 1));
+  buf_reserve(c, f.b_res, sizeof(i64) * (arcs + 1));
+  buf_reserve(c, f.b_tasks, sizeof(CompTask) * tasks.size() + sizeof(CompOut) * tasks.size() + 256);
+  CompTask *dt = f.b_tasks.as<CompTask>();
+  CompOut *dout = (CompOut *)(((uintptr_t)(dt + tasks.size()) + 63) & ~(uintptr_t)63);
+  HIP_CHECK(hipMemcpyAsync(dt, tasks.data(), sizeof(CompTask) * tasks.size(), hipMemcpyHostToDevice, c.stream));
+  buf_reserve(c, f.scal, 256);
+  uint32_t *qctr = (uint32_t *)((char *)f.scal.p + 192);
+  HIP_CHECK(hipMemsetAsync(qctr, 0, sizeof(uint32_t), c.stream));
+  int per_cu = 0;
+  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_dinkelbach_block, 1024, 0));
+  if (per_cu < 1) fail(KCDS_E_HIP, "k_dinkelbach_block cannot be resident");
+  const uint32_t grid =
+      (uint32_t)std::min<uint64_t>(tasks.size(), (uint64_t)c.num_cus * (uint64_t)std::min(per_cu, 2));
+  k_dinkelbach_block<<<grid, 1024, 0, c.stream>>>(
+      dt, (uint32_t)tasks.size(), qctr, k, g.ccl.as<uint32_t>(), f.b_off.as<uint32_t>(), f.b_excess.as<i64>(),
+      f.b_height.as<uint32_t>(), f.b_fr0.as<uint32_t>(), f.b_fr1.as<uint32_t>(), f.b_head.as<uint32_t>(),
+      f.b_rev.as<uint32_t>(), f.b_res.as<i64>(), f.cside.as<uint8_t>(), dout);
+  KCDS_LAUNCH_CHECK();
+  std::vector<CompOut> h(tasks.size());
+  HIP_CHECK(hipMemcpyAsync(h.data(), dout, sizeof(CompOut) * tasks.size(), hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  for (size_t i = 0; i < h.size(); ++i) {
+    if (h[i].status != 1)
+      fail(KCDS_E_LIMIT, "block solver did not converge on component %u (status %u)", in[i].d, h[i].status);
+    out[i].p = h[i].p;
+    out[i].q = h[i].q;
+    out[i].side_n = h[i].side_n;
+    out[i].side_T = h[i].side_T;
+    out[i].flows = h[i].flows;
+  }
 }
 
 }  // namespace kcds

@@ -67,7 +67,6 @@ void gather_by_rank_u64(Ctx &c, const ull *src_new, ull *dst_orig);
 void gather_by_rank_i32(Ctx &c, const int32_t *src_new, int32_t *dst_orig);
 void gather_by_rank_u32(Ctx &c, const uint32_t *src_new, uint32_t *dst_orig);
 void rank_from_listing_order(Ctx &c, const uint32_t *lo_dev);
-void solve_pipeline(Ctx &c, uint32_t k, kcds_result *res, uint8_t *in_set);
 void maxflow_component(Ctx &c, uint32_t comp_id, uint64_t p, uint64_t q, int64_t *flow, uint8_t *side);
 
 static void reset_after_graph(Ctx &c) {
@@ -132,9 +131,15 @@ void kcds_destroy(kcds_ctx *ctx) {
   Ctx &c = ctx->c;
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
+  CompData &g = c.cd;
+  FlowBufs &f = c.fb;
   DevBuf *bufs[] = {&c.off, &c.nbr, &c.rank, &c.inv, &c.doff, &c.dnbr, &c.roots, &c.qctr, &c.cnt, &c.scal,
                     &c.rootT, &c.cliques, &c.incoff, &c.inc, &c.core, &c.status, &c.comp, &c.tmp0,
-                    &c.tmp1, &c.tmp2, &c.tmp3};
+                    &c.tmp1, &c.tmp2, &c.tmp3, &c.gvoff, &c.gid, &c.gpeel, &c.tlist, &c.inset,
+                    &g.par, &g.flag, &g.dense, &g.cid, &g.local, &g.vcount, &g.voff, &g.verts, &g.ccount,
+                    &g.coff, &g.cfill, &g.ccl, &f.off, &f.head, &f.rev, &f.res, &f.excess, &f.height, &f.deg,
+                    &f.fill, &f.front, &f.side, &f.scal, &f.cside, &f.b_off, &f.b_excess, &f.b_height,
+                    &f.b_fr0, &f.b_fr1, &f.b_head, &f.b_rev, &f.b_res, &f.b_tasks};
   for (DevBuf *b : bufs) buf_free(c, *b);
   if (c.pinned) (void)hipHostFree(c.pinned);
   if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -171,6 +176,8 @@ int kcds_upload_csr(kcds_ctx *ctx, uint32_t n, const uint32_t *offset, const uin
     c.sync();
     c.n = n;
     c.nnz = nnz;
+    c.ngraphs = 1;
+    c.h_gvoff.assign({0u, n});
     c.have_graph = true;
   });
 }
@@ -283,7 +290,32 @@ int kcds_solve(kcds_ctx *ctx, uint32_t k, kcds_result *res, uint8_t *in_set) {
   return guarded(ctx, [&](Ctx &c) {
     if (!c.have_graph) fail(KCDS_E_STATE, "kcds_solve before kcds_upload_csr");
     if (k < 3 || k > 8) fail(KCDS_E_INVALID, "k = %u outside 3..8", k);
-    solve_pipeline(c, k, res, in_set);
+    c.ngraphs = 1;
+    c.h_gvoff.assign({0u, c.n});
+    solve_graphs(c, k, res, nullptr, in_set);
+  });
+}
+
+int kcds_solve_batch(kcds_ctx *ctx, uint32_t k, uint32_t ngraphs, const uint32_t *graph_voff,
+                     kcds_graph_result *results, uint8_t *in_set) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!c.have_graph) fail(KCDS_E_STATE, "kcds_solve_batch before kcds_upload_csr");
+    if (k < 3 || k > 8) fail(KCDS_E_INVALID, "k = %u outside 3..8", k);
+    if (!graph_voff || !results || ngraphs == 0) fail(KCDS_E_INVALID, "null batch partition or results");
+    if (graph_voff[0] != 0 || graph_voff[ngraphs] != c.n)
+      fail(KCDS_E_INVALID, "graph_voff must run from 0 to n");
+    for (uint32_t g = 0; g < ngraphs; ++g)
+      if (graph_voff[g + 1] < graph_voff[g]) fail(KCDS_E_INVALID, "graph_voff not monotone at %u", g);
+    c.ngraphs = ngraphs;
+    c.h_gvoff.assign(graph_voff, graph_voff + ngraphs + 1);
+    struct Restore {
+      Ctx &c;
+      ~Restore() {
+        c.ngraphs = 1;
+        c.h_gvoff.assign({0u, c.n});
+      }
+    } restore{c};
+    solve_graphs(c, k, nullptr, results, in_set);
   });
 }
 

@@ -54,6 +54,20 @@ struct StageTimer {
   hipEvent_t a = nullptr, b = nullptr;
 };
 
+// Per-graph peeling bookkeeping (a batch peels many disjoint graphs at once).
+struct GraphPeel {
+  ull live;            // live cliques during the peel; T of the graph before it
+  ull max_num, max_den;  // maxDensity of the graph's clique-cores
+  ull T;
+  ull core_cliques;    // cliques with every member in the k'-core
+  uint32_t rem;        // remaining vertices during the peel
+  uint32_t touch;      // touched in the current level
+  uint32_t n;
+  uint32_t k_prime;    // ceil(maxDensity); 0x7FFFFFFF for a graph without cliques
+  uint32_t core_size;
+  uint32_t max_core;
+};
+
 // Core vertices and core cliques grouped by component (dense index d in
 // order of the canonical label = smallest original id of the component).
 struct CompData {
@@ -67,6 +81,8 @@ struct CompData {
 // Buffers of one clique flow network (reused across max-flow calls).
 struct FlowBufs {
   DevBuf off, head, rev, res, excess, height, deg, fill, front, side, scal;
+  DevBuf cside;  // uint8 per core vertex (component-grouped): min-cut side of each solved component
+  DevBuf b_off, b_excess, b_height, b_fr0, b_fr1, b_head, b_rev, b_res, b_tasks;  // block solver slices
   uint64_t last_p = 0, last_q = 0;
   int64_t last_comp = -1;
 };
@@ -125,6 +141,12 @@ struct Ctx {
   uint32_t ncomp = 0;
   bool have_comp = false;
 
+  // batch partition: graph g owns original ids [h_gvoff[g], h_gvoff[g+1])
+  uint32_t ngraphs = 1;
+  std::vector<uint32_t> h_gvoff;
+  DevBuf gvoff, gid, gpeel, tlist;  // gid[r]: graph of new id r
+  std::vector<GraphPeel> h_gpeel;
+
   CompData cd;
   FlowBufs fb;
   DevBuf inset;   // uint8[n] new-id space: members of the answer set
@@ -176,6 +198,24 @@ struct FlowOut {
 FlowOut flow_component(Ctx &c, uint32_t d, uint64_t p, uint64_t q);
 // Marks the current c.fb.side of component d in c.inset (new-id space).
 void flow_mark_side(Ctx &c, uint32_t d);
+// Copies c.fb.side of component d into its slice of c.fb.cside.
+void flow_store_side(Ctx &c, uint32_t d);
+// Marks component d's slice of c.fb.cside in c.inset.
+void mark_component_side(Ctx &c, uint32_t d);
+
+// Whole exact search of many components at once, one workgroup each.
+struct BlockTask {
+  uint32_t d;     // component dense index
+  ull p0, q0;     // starting alpha
+};
+struct BlockResult {
+  ull p = 0, q = 1;         // final alpha (reduced)
+  ull side_n = 0, side_T = 0;  // maximal min-cut side at the final alpha
+  uint32_t flows = 0;
+};
+void dinkelbach_blocks(Ctx &c, const std::vector<BlockTask> &in, std::vector<BlockResult> &out);
+void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_graph, uint8_t *in_set);
+void graph_ids(Ctx &c);
 
 }  // namespace kcds
 

@@ -1,4 +1,5 @@
-// Stage sequence of a whole solve and the exact density search.
+// Stage sequence of a solve (one graph or a batch of disjoint graphs) and the
+// exact density search.
 //
 // Reference: main (main.cu:1348-1519) runs generateDAG, listAllCliques,
 // cliqueCoreDecompose, generateDensestCore, prune, componentDecompose and
@@ -6,16 +7,19 @@
 // component (getLbUbandSize, src/helpers.cu:1115-1146: lb = Tc/nc,
 // ub = min(max clique-core, Tc^((k-1)/k) / (k!)^(1/k))), skips components with
 // ub <= max lb (main.cu:915) and bisects alpha with a double-precision
-// push-relabel per component.
+// push-relabel, one component after another on the host.
 //
-// Here the search is exact and rational.  alpha starts at the best whole
-// component density (max lb); for each component (in canonical order) whose
-// upper bound reaches alpha, max-flow at alpha = p/q either certifies that
-// no set is denser (flow == q*k*Tc) or yields a strictly denser set A (the
-// source side of the maximal minimum cut), and alpha := density(A).  When no
-// component admits a denser set, alpha = rho* exactly; the answer set is the
-// union over components of the maximal minimum-cut source sides at rho*,
-// i.e. the unique maximal densest set.
+// Here the search is exact and rational, and components run concurrently.
+// For every graph, alpha0 = its best whole-component density (max lb); every
+// component whose upper bound reaches alpha0 runs a Dinkelbach iteration
+// from alpha0: max-flow at alpha = p/q either certifies that no set is denser
+// (flow == q*k*Tc) or yields a strictly denser set A (the source side of the
+// maximal minimum cut) and alpha := density(A).  A graph's rho* is the largest
+// final alpha of its components, and its answer set is the union of the
+// maximal minimum-cut sides of the components that end at rho* -- the unique
+// maximal densest set.  Components up to BLOCK_MAX_NODES flow nodes run the
+// whole iteration inside one workgroup each (one launch for all of them);
+// larger ones use the grid-wide push-relabel kernel per max-flow.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -24,12 +28,13 @@
 
 namespace kcds {
 
-void gather_by_rank_u32(Ctx &c, const uint32_t *src_new, uint32_t *dst_orig);
-void rank_from_listing_order(Ctx &c, const uint32_t *lo_dev);
+void check_partition(Ctx &c);
 
 namespace {
 
 typedef unsigned __int128 u128;
+
+constexpr uint64_t BLOCK_MAX_NODES = 1u << 17;
 
 ull gcd(ull a, ull b) {
   while (b) {
@@ -61,15 +66,13 @@ struct Timer {
   }
 };
 
-// upper bound of any set density inside component d (helpers.cu:1126-1135);
-// conservative by a relative 1e-9 so rounding never skips a component that
-// could hold a set of density >= num/den
-bool may_reach(const Ctx &c, uint32_t d, ull num, ull den) {
-  const CompData &g = c.cd;
-  const double tc = (double)(g.h_coff[d + 1] - g.h_coff[d]);
+// Upper bound of any set density inside a component with Tc cliques
+// (helpers.cu:1126-1135), widened by a relative 1e-9 so that rounding never
+// skips a component that could hold a set of density >= num/den.
+bool may_reach(uint32_t k, uint32_t max_core, ull Tc, ull num, ull den) {
   double kf = 1;
-  for (uint32_t i = 2; i <= c.k; ++i) kf *= i;
-  const double ub = std::min((double)c.peel.max_core, std::pow(tc, (c.k - 1.0) / c.k) / std::pow(kf, 1.0 / c.k));
+  for (uint32_t i = 2; i <= k; ++i) kf *= i;
+  const double ub = std::min((double)max_core, std::pow((double)Tc, (k - 1.0) / k) / std::pow(kf, 1.0 / k));
   return ub * (1 + 1e-9) + 1e-9 >= (double)num / (double)den;
 }
 
@@ -95,9 +98,13 @@ void maxflow_component(Ctx &c, uint32_t comp_id, uint64_t p, uint64_t q, int64_t
   }
 }
 
-void solve_pipeline(Ctx &c, uint32_t k, kcds_result *res, uint8_t *in_set) {
+// Solves every graph of the current partition (c.ngraphs, c.h_gvoff).
+void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_graph, uint8_t *in_set) {
+  const uint32_t G = c.ngraphs;
   kcds_result R;
   memset(&R, 0, sizeof R);
+  std::vector<kcds_graph_result> out(G);
+  memset(out.data(), 0, sizeof(kcds_graph_result) * G);
   Timer total(c.stream);
   {
     Timer t(c.stream);
@@ -106,6 +113,7 @@ void solve_pipeline(Ctx &c, uint32_t k, kcds_result *res, uint8_t *in_set) {
     c.oriented = true;
     R.ms_orient = t.stop();
   }
+  if (G > 1) check_partition(c);
   {
     Timer t(c.stream);
     list_count(c, k);
@@ -114,9 +122,10 @@ void solve_pipeline(Ctx &c, uint32_t k, kcds_result *res, uint8_t *in_set) {
     R.ms_list = t.stop();
   }
   if (in_set && c.n) memset(in_set, 0, c.n);
-  if (c.T == 0) {  // main.cu:1431-1438: "Max Density 0 Size 0"
+  if (c.T == 0) {  // main.cu:1431-1438: "Max Density 0 Size 0" for every graph
     R.ms_total = total.stop();
-    if (res) *res = R;
+    if (agg) *agg = R;
+    if (per_graph) memcpy(per_graph, out.data(), sizeof(kcds_graph_result) * G);
     return;
   }
   {
@@ -136,56 +145,112 @@ void solve_pipeline(Ctx &c, uint32_t k, kcds_result *res, uint8_t *in_set) {
   Timer tf(c.stream);
   CompData &g = c.cd;
   const uint32_t C = g.ncomp;
-  ull an = 0, ad = 1;
+  // graph of each component: labels are original ids, graphs own id ranges
+  std::vector<uint32_t> cg(C);
+  for (uint32_t d = 0; d < C; ++d)
+    cg[d] = (uint32_t)(std::upper_bound(c.h_gvoff.begin(), c.h_gvoff.end(), g.h_label[d]) - c.h_gvoff.begin()) - 1;
+  // alpha0 per graph: the best whole-component density
+  std::vector<ull> an(G, 0), ad(G, 1);
   for (uint32_t d = 0; d < C; ++d) {
     const ull tc = g.h_coff[d + 1] - g.h_coff[d], nc = g.h_voff[d + 1] - g.h_voff[d];
-    if (nc && ratio_gt(tc, nc, an, ad)) {
-      an = tc;
-      ad = nc;
+    const uint32_t gi = cg[d];
+    out[gi].ncomp++;
+    if (nc && ratio_gt(tc, nc, an[gi], ad[gi])) {
+      an[gi] = tc;
+      ad[gi] = nc;
     }
   }
-  {
-    ull gg = gcd(an, ad);
+  for (uint32_t gi = 0; gi < G; ++gi) {
+    ull gg = gcd(an[gi], ad[gi]);
     if (gg) {
-      an /= gg;
-      ad /= gg;
+      an[gi] /= gg;
+      ad[gi] /= gg;
     }
   }
-  // alpha at which each component's last flow ran (its side is then reusable)
-  std::vector<std::pair<ull, ull>> last(C, {0, 0});
+  uint64_t core_vertices = g.h_voff[C];
+  buf_reserve(c, c.fb.cside, core_vertices + 8);
+  HIP_CHECK(hipMemsetAsync(c.fb.cside.p, 0, core_vertices + 8, c.stream));
+  std::vector<BlockTask> small;
+  std::vector<uint32_t> large;
   for (uint32_t d = 0; d < C; ++d) {
+    const ull tc = g.h_coff[d + 1] - g.h_coff[d], nc = g.h_voff[d + 1] - g.h_voff[d];
+    const uint32_t gi = cg[d];
+    if (tc == 0 || !may_reach(k, c.h_gpeel[gi].max_core, tc, an[gi], ad[gi])) continue;
+    if (nc + tc + 2 <= BLOCK_MAX_NODES) small.push_back(BlockTask{d, an[gi], ad[gi]});
+    else large.push_back(d);
+  }
+  // final alpha and side size of every solved component
+  std::vector<BlockResult> res(C);
+  std::vector<uint8_t> solved(C, 0);
+  {
+    std::vector<BlockResult> r;
+    dinkelbach_blocks(c, small, r);
+    for (size_t i = 0; i < small.size(); ++i) {
+      res[small[i].d] = r[i];
+      solved[small[i].d] = 1;
+      out[cg[small[i].d]].flow_calls += r[i].flows;
+    }
+  }
+  for (uint32_t d : large) {
+    const uint32_t gi = cg[d];
     const ull tc = g.h_coff[d + 1] - g.h_coff[d];
-    if (tc == 0) continue;
-    while (may_reach(c, d, an, ad)) {
-      FlowOut o = flow_component(c, d, an, ad);
-      R.flow_calls++;
-      if ((u128)o.value >= (u128)ad * k * tc) break;  // no set denser than alpha
-      if (o.side_n == 0 || !ratio_gt(o.side_T, o.side_n, an, ad))
+    ull p = an[gi], q = ad[gi];
+    BlockResult br;
+    for (uint32_t it = 0;; ++it) {
+      FlowOut o = flow_component(c, d, p, q);
+      br.flows++;
+      if ((u128)o.value >= (u128)q * k * tc) {  // no set denser than p/q: keep this side
+        br.side_n = o.side_n;
+        br.side_T = o.side_T;
+        flow_store_side(c, d);
+        break;
+      }
+      if (o.side_n == 0 || !ratio_gt(o.side_T, o.side_n, p, q) || it > 64)
         fail(KCDS_E_HIP, "min cut below q*k*Tc without a denser source side (component %u)", d);
       ull gg = gcd(o.side_T, o.side_n);
-      an = o.side_T / gg;
-      ad = o.side_n / gg;
+      p = o.side_T / gg;
+      q = o.side_n / gg;
+    }
+    br.p = p;
+    br.q = q;
+    res[d] = br;
+    solved[d] = 1;
+    out[gi].flow_calls += br.flows;
+  }
+  // per graph: rho* = largest final alpha among components with a non-empty
+  // side; the answer is the union of those components' sides at rho*
+  std::vector<ull> bn(G, 0), bd(G, 1);
+  std::vector<uint8_t> have(G, 0);
+  for (uint32_t d = 0; d < C; ++d) {
+    if (!solved[d] || res[d].side_n == 0) continue;
+    const uint32_t gi = cg[d];
+    if (!have[gi] || ratio_gt(res[d].p, res[d].q, bn[gi], bd[gi])) {
+      bn[gi] = res[d].p;
+      bd[gi] = res[d].q;
+      have[gi] = 1;
     }
   }
-  // maximal densest set at alpha = rho*
   buf_reserve(c, c.inset, (size_t)c.n + 8);
   HIP_CHECK(hipMemsetAsync(c.inset.p, 0, c.n, c.stream));
-  ull snum = 0, sden = 0;
   for (uint32_t d = 0; d < C; ++d) {
-    const ull tc = g.h_coff[d + 1] - g.h_coff[d];
-    if (tc == 0 || !may_reach(c, d, an, ad)) continue;
-    FlowOut o = flow_component(c, d, an, ad);
-    R.flow_calls++;
-    if (o.side_n == 0) continue;
-    flow_mark_side(c, d);
-    snum += o.side_T;
-    sden += o.side_n;
+    if (!solved[d] || res[d].side_n == 0) continue;
+    const uint32_t gi = cg[d];
+    if ((u128)res[d].p * bd[gi] != (u128)bn[gi] * res[d].q) continue;
+    mark_component_side(c, d);
+    out[gi].rho_num += res[d].side_T;
+    out[gi].rho_den += res[d].side_n;
   }
-  if (sden == 0 || (u128)snum * ad != (u128)an * sden)
-    fail(KCDS_E_HIP, "densest set check failed: %llu/%llu vs %llu/%llu", (unsigned long long)snum,
-         (unsigned long long)sden, (unsigned long long)an, (unsigned long long)ad);
-  R.rho_num = snum;
-  R.rho_den = sden;
+  for (uint32_t gi = 0; gi < G; ++gi) {
+    const GraphPeel &P = c.h_gpeel[gi];
+    out[gi].T = P.T;
+    out[gi].k_prime = P.T ? P.k_prime : 0;
+    out[gi].core_vertices = P.T ? P.core_size : 0;
+    if (P.T && (out[gi].rho_den == 0 || (u128)out[gi].rho_num * bd[gi] != (u128)bn[gi] * out[gi].rho_den))
+      fail(KCDS_E_HIP, "densest set check failed for graph %u", gi);
+  }
+  R.rho_num = out[0].rho_num;
+  R.rho_den = out[0].rho_den;
+  for (uint32_t gi = 0; gi < G; ++gi) R.flow_calls += out[gi].flow_calls;
   R.ms_flow = tf.stop();
   if (in_set && c.n) {
     std::vector<uint8_t> tmp(c.n);
@@ -196,7 +261,8 @@ void solve_pipeline(Ctx &c, uint32_t k, kcds_result *res, uint8_t *in_set) {
     for (uint32_t v = 0; v < c.n; ++v) in_set[v] = tmp[rank[v]];
   }
   R.ms_total = total.stop();
-  if (res) *res = R;
+  if (agg) *agg = R;
+  if (per_graph) memcpy(per_graph, out.data(), sizeof(kcds_graph_result) * G);
 }
 
 }  // namespace kcds

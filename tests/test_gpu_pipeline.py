@@ -126,12 +126,27 @@ def test_tied_cliques_union(solver):
     assert set(np.flatnonzero(ins).tolist()) == set(range(12))
 
 
+def brute_densest(n, cl):
+    """(cliques, vertices) of the optimum and the union of all optimal sets."""
+    masks = np.arange(1, 1 << n, dtype=np.int64)
+    sizes = np.array([bin(int(m)).count("1") for m in masks])
+    cm = np.array([sum(1 << v for v in c) for c in cl], dtype=np.int64)
+    tin = ((masks[:, None] & cm[None, :]) == cm[None, :]).sum(1)
+    best = (0, 1)
+    for i in range(len(masks)):
+        if tin[i] * best[1] > best[0] * sizes[i]:
+            best = (int(tin[i]), int(sizes[i]))
+    union = 0
+    for i in np.flatnonzero((tin > 0) & (tin * best[1] == best[0] * sizes)):
+        union |= int(masks[i])
+    return best, union
+
+
 def test_brute_force_small(solver):
     rng = np.random.default_rng(7)
-    for trial in range(12):
+    for trial in range(10):
         n = int(rng.integers(6, 12))
-        p = float(rng.uniform(0.35, 0.8))
-        g = gen.er(n, p, 0, seed=100 + trial)
+        g = gen.er(n, float(rng.uniform(0.35, 0.8)), 0, seed=100 + trial)
         adj = [set(g.nbr[g.offset[v]:g.offset[v + 1]].tolist()) for v in range(n)]
         for k in (3, 4):
             cl = [c for c in itertools.combinations(range(n), k)
@@ -141,18 +156,7 @@ def test_brute_force_small(solver):
             assert res.T == len(cl)
             if not cl:
                 continue
-            best = (0, 1)
-            sets = []
-            for mask in range(1, 1 << n):
-                S = [v for v in range(n) if mask >> v & 1]
-                t = sum(1 for c in cl if all(mask >> v & 1 for v in c))
-                if t * best[1] > best[0] * len(S):
-                    best = (t, len(S))
-                sets.append((mask, t, len(S)))
-            union = 0
-            for mask, t, s in sets:
-                if t * best[1] == best[0] * s and t > 0:
-                    union |= mask
+            best, union = brute_densest(n, cl)
             assert res.rho_num * best[1] == best[0] * res.rho_den
             assert int(sum(1 << int(v) for v in np.flatnonzero(ins))) == union
 
