@@ -93,88 +93,99 @@ __global__ void k_w2(const uint32_t *doff, const uint32_t *dnbr, uint64_t arcs, 
 }
 
 // ---------------------------------------------------------------------------
-// GPU degeneracy order: level-synchronous k-core peel in one persistent
-// cooperative kernel.  Level L removes every live vertex of degree <= L in
-// sub-rounds; each sub-round's vertices take the next ranks (in frontier
-// order), then decrement their live neighbours; a neighbour whose degree
-// falls from L+1 to L joins the next sub-round.  Empty levels are skipped by
-// a min-reduction over live degrees.
+// GPU degeneracy order: level-synchronous k-core peel as a fixed sequence of
+// step kernels (min / open / collect / rank / decrement / finish) driven by
+// device-side control state, enqueued by the host in batches.  Level L
+// removes every live vertex of degree <= L in sub-rounds; each sub-round's
+// vertices take the next ranks (in frontier order), then decrement their live
+// neighbours; a neighbour whose degree falls from L+1 to L joins the next
+// sub-round.  Empty levels are skipped by a min-reduction over live degrees.
+// Kernel boundaries order the steps; no grid-wide barrier is needed.
 
-struct PeelState {
+struct OrderCtl {
   uint32_t level;
+  uint32_t minval;
   uint32_t next_rank;
   uint32_t fsize[2];
-  uint32_t minval;
-  uint32_t barrier;
+  uint32_t cur;
+  uint32_t done;
+  uint32_t opening;
 };
 
-__global__ __launch_bounds__(1024) void k_kcore_order(const uint32_t *off, const uint32_t *nbr, uint32_t n,
-                                                      uint32_t *deg, uint32_t *rank, uint32_t *inv,
-                                                      uint32_t *fr0, uint32_t *fr1, PeelState *st) {
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t nthreads = gridDim.x * blockDim.x;
-  const uint32_t lane = lane_id();
-  const uint32_t wid = tid / KCDS_WAVE, nwaves = nthreads / KCDS_WAVE;
-  const uint32_t DEAD = 0xFFFFFFFFu;
-  uint32_t epoch = 0;
-  int cur = 0;
-  while (true) {
-    // all ranks assigned?
-    uint32_t done = __hip_atomic_load(&st->next_rank, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (done >= n) break;
-    uint32_t fs = __hip_atomic_load(&st->fsize[cur], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (fs == 0) {
-      // new level: L = min live degree (never below the current level)
-      for (uint32_t v = tid; v < n; v += nthreads) {
-        uint32_t d = deg[v];
-        if (d != DEAD) atomicMin(&st->minval, d);
-      }
-      grid_barrier(&st->barrier, gridDim.x, epoch);
-      uint32_t L = max(__hip_atomic_load(&st->minval, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                       __hip_atomic_load(&st->level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      uint32_t *fr = cur ? fr1 : fr0;
-      for (uint32_t v = tid; v < n; v += nthreads) {
-        uint32_t d = deg[v];
-        if (d != DEAD && d <= L) fr[atomicAdd(&st->fsize[cur], 1u)] = v;
-      }
-      grid_barrier(&st->barrier, gridDim.x, epoch);
-      if (tid == 0) {
-        st->level = L;
-        st->minval = DEAD;
-      }
-      fs = __hip_atomic_load(&st->fsize[cur], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      grid_barrier(&st->barrier, gridDim.x, epoch);
-    }
-    const uint32_t L = __hip_atomic_load(&st->level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t base = __hip_atomic_load(&st->next_rank, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t *fr = cur ? fr1 : fr0;
-    uint32_t *nx = cur ? fr0 : fr1;
-    // take ranks and leave the graph
-    for (uint32_t i = tid; i < fs; i += nthreads) {
-      uint32_t v = fr[i];
-      rank[v] = base + i;
-      inv[base + i] = v;
-      deg[v] = DEAD;
-    }
-    grid_barrier(&st->barrier, gridDim.x, epoch);
-    // decrement live neighbours (wave per removed vertex)
-    for (uint32_t i = wid; i < fs; i += nwaves) {
-      uint32_t v = fr[i];
-      for (uint32_t j = off[v] + lane; j < off[v + 1]; j += KCDS_WAVE) {
-        uint32_t u = nbr[j];
-        if (__hip_atomic_load(&deg[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == DEAD) continue;
-        uint32_t old = atomicSub(&deg[u], 1u);
-        if (old == L + 1) nx[atomicAdd(&st->fsize[cur ^ 1], 1u)] = u;
-      }
-    }
-    grid_barrier(&st->barrier, gridDim.x, epoch);
-    if (tid == 0) {
-      st->next_rank = base + fs;
-      st->fsize[cur] = 0;
-    }
-    grid_barrier(&st->barrier, gridDim.x, epoch);
-    cur ^= 1;
+constexpr uint32_t DEAD = 0xFFFFFFFFu;
+
+__global__ void k_ko_min(OrderCtl *ctl, uint32_t n, const uint32_t *deg) {
+  if (ctl->done || ctl->fsize[ctl->cur] != 0) return;
+  uint32_t m = DEAD;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) m = min(m, deg[v]);
+  m = wave_min_u32(m);
+  if (lane_id() == 0 && m != DEAD) atomicMin(&ctl->minval, m);
+}
+
+__global__ void k_ko_open(OrderCtl *ctl) {
+  if (ctl->done || ctl->fsize[ctl->cur] != 0) return;
+  if (ctl->minval == DEAD) {
+    ctl->done = 1;
+    return;
   }
+  ctl->level = max(ctl->level, ctl->minval);
+  ctl->minval = DEAD;
+  ctl->opening = 1;
+}
+
+__global__ void k_ko_collect(OrderCtl *ctl, uint32_t n, const uint32_t *deg, uint32_t *fr0, uint32_t *fr1) {
+  if (ctl->done || !ctl->opening) return;
+  const uint32_t L = ctl->level, cur = ctl->cur;
+  uint32_t *fr = cur ? fr1 : fr0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const uint32_t d = deg[v];
+    if (d != DEAD && d <= L) fr[atomicAdd(&ctl->fsize[cur], 1u)] = v;
+  }
+}
+
+// the frontier takes the next ranks and leaves the graph
+__global__ void k_ko_rank(OrderCtl *ctl, const uint32_t *fr0, const uint32_t *fr1, uint32_t *deg, uint32_t *rank,
+                          uint32_t *inv) {
+  if (ctl->done) return;
+  const uint32_t cur = ctl->cur, fs = ctl->fsize[cur], base = ctl->next_rank;
+  const uint32_t *fr = cur ? fr1 : fr0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < fs; i += gridDim.x * blockDim.x) {
+    const uint32_t v = fr[i];
+    rank[v] = base + i;
+    inv[base + i] = v;
+    deg[v] = DEAD;
+  }
+}
+
+// wave per removed vertex: decrement live neighbours
+__global__ void k_ko_decrement(OrderCtl *ctl, const uint32_t *off, const uint32_t *nbr, uint32_t *deg,
+                               uint32_t *fr0, uint32_t *fr1) {
+  if (ctl->done) return;
+  const uint32_t cur = ctl->cur, fs = ctl->fsize[cur], L = ctl->level;
+  const uint32_t *fr = cur ? fr1 : fr0;
+  uint32_t *nx = cur ? fr0 : fr1;
+  const uint32_t lane = lane_id();
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KCDS_WAVE;
+  const uint32_t nw = gridDim.x * blockDim.x / KCDS_WAVE;
+  for (uint32_t i = wid; i < fs; i += nw) {
+    const uint32_t v = fr[i];
+    for (uint32_t j = off[v] + lane; j < off[v + 1]; j += KCDS_WAVE) {
+      const uint32_t u = nbr[j];
+      if (__hip_atomic_load(&deg[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == DEAD) continue;
+      const uint32_t old = atomicSub(&deg[u], 1u);
+      if (old == L + 1) nx[atomicAdd(&ctl->fsize[cur ^ 1], 1u)] = u;
+    }
+  }
+}
+
+__global__ void k_ko_finish(OrderCtl *ctl, uint32_t n) {
+  if (ctl->done) return;
+  const uint32_t cur = ctl->cur;
+  ctl->next_rank += ctl->fsize[cur];
+  ctl->fsize[cur] = 0;
+  ctl->opening = 0;
+  ctl->cur = cur ^ 1;
+  if (ctl->next_rank >= n) ctl->done = 1;
 }
 
 __global__ void k_init_deg(const uint32_t *off, uint32_t n, uint32_t *deg) {
@@ -189,23 +200,40 @@ void degeneracy_rank_gpu(Ctx &c) {
   buf_reserve(c, c.rank, sizeof(uint32_t) * (size_t)n + 4);
   buf_reserve(c, c.inv, sizeof(uint32_t) * (size_t)n + 4);
   if (n == 0) return;
-  buf_reserve(c, c.tmp0, sizeof(uint32_t) * (size_t)n * 3 + sizeof(PeelState) + 64);
+  buf_reserve(c, c.tmp0, sizeof(uint32_t) * (size_t)n * 3 + sizeof(OrderCtl) + 256);
   uint32_t *deg = c.tmp0.as<uint32_t>();
   uint32_t *fr0 = deg + n, *fr1 = fr0 + n;
-  PeelState *st = (PeelState *)(((uintptr_t)(fr1 + n) + 63) & ~(uintptr_t)63);
+  OrderCtl *ctl = (OrderCtl *)(((uintptr_t)(fr1 + n) + 255) & ~(uintptr_t)255);
+  const uint32_t vgrid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, c.num_cus * 8ull));
   k_init_deg<<<(n + 255) / 256, 256, 0, c.stream>>>(c.off.as<uint32_t>(), n, deg);
   KCDS_LAUNCH_CHECK();
-  HIP_CHECK(hipMemsetAsync(st, 0, sizeof(PeelState), c.stream));
-  fill_u32(c, &st->minval, 0xFFFFFFFFu, 1);
-  int per_cu = 0;
-  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_kcore_order, 1024, 0));
-  if (per_cu < 1) fail(KCDS_E_HIP, "k_kcore_order cannot be resident");
-  uint32_t grid = (uint32_t)c.num_cus * (uint32_t)std::min(per_cu, 2);
+  OrderCtl h0{};
+  h0.minval = DEAD;
+  HIP_CHECK(hipMemcpyAsync(ctl, &h0, sizeof h0, hipMemcpyHostToDevice, c.stream));
   const uint32_t *off = c.off.as<uint32_t>(), *nbr = c.nbr.as<uint32_t>();
   uint32_t *rank = c.rank.as<uint32_t>(), *inv = c.inv.as<uint32_t>();
-  void *args[] = {(void *)&off, (void *)&nbr, (void *)&n, (void *)&deg, (void *)&rank,
-                  (void *)&inv, (void *)&fr0, (void *)&fr1, (void *)&st};
-  HIP_CHECK(hipLaunchCooperativeKernel((const void *)k_kcore_order, dim3(grid), dim3(1024), args, 0, c.stream));
+  uint32_t *hdone = (uint32_t *)c.host_scratch(64);
+  uint32_t batch = 8;
+  uint64_t rounds = 0;
+  while (true) {
+    for (uint32_t r = 0; r < batch; ++r) {
+      k_ko_min<<<vgrid, 256, 0, c.stream>>>(ctl, n, deg);
+      k_ko_open<<<1, 1, 0, c.stream>>>(ctl);
+      k_ko_collect<<<vgrid, 256, 0, c.stream>>>(ctl, n, deg, fr0, fr1);
+      k_ko_rank<<<vgrid, 256, 0, c.stream>>>(ctl, fr0, fr1, deg, rank, inv);
+      k_ko_decrement<<<vgrid, 256, 0, c.stream>>>(ctl, off, nbr, deg, fr0, fr1);
+      k_ko_finish<<<1, 1, 0, c.stream>>>(ctl, n);
+      KCDS_LAUNCH_CHECK();
+    }
+    rounds += batch;
+    HIP_CHECK(hipMemcpyAsync(hdone, &ctl->done, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    if (*hdone) break;
+    // every sub-round ranks at least one vertex
+    if (rounds > (uint64_t)n + 1) fail(KCDS_E_HIP, "ordering did not finish within %llu sub-rounds",
+                                       (unsigned long long)rounds);
+    batch = std::min<uint32_t>(batch * 2, 512);
+  }
 }
 
 void orient_from_rank(Ctx &c) {

@@ -8,18 +8,27 @@
 // remaining vertices; the largest density (ties keep the later core,
 // main.cu:407) is maxDensity.
 //
-// Here the whole peel is one persistent cooperative kernel over the
-// materialised cliques and the vertex->clique incidence index:
-//   * a new level starts only when the frontier is empty; its value is the
+// Here one peeling sub-round is a fixed sequence of step kernels whose work
+// is decided by device-side control state, so the host enqueues many
+// sub-rounds back to back and only reads the "done" flag now and then:
+//   touch / min / open / collect   (no-ops unless the frontier is empty)
+//     a new level starts only when the frontier is empty; its value is the
 //     minimum live count (empty levels change neither density nor cores, so
-//     they are skipped by a min-reduction);
-//   * sub-rounds remove the frontier, claim each live clique of a removed
-//     vertex exactly once with atomicCAS(status, -1, L) and decrement the
-//     surviving members; a member whose count falls from L+1 to L joins the
-//     next sub-round (each vertex crosses that value once);
-//   * one thread keeps the exact rational maximum of live/remaining.
-// Frontier vertices with long incidence lists are processed by the whole
-// grid, the rest wave-per-vertex.
+//     they are skipped by a min-reduction); the density of the level just
+//     completed is recorded first, as an exact rational;
+//   mark      frontier vertices take the level as their core number;
+//   process   each live clique of a removed vertex is claimed exactly once
+//             (atomicCAS(status, -1, L)) and its surviving members are
+//             decremented; a member whose count falls from L+1 to L joins the
+//             next sub-round (each vertex crosses that value once);
+//   finish    advance to the next frontier.
+// Kernel boundaries order every step, so no step needs a grid-wide barrier.
+// Frontier vertices with long incidence lists are processed grid-wide, the
+// rest wave-per-vertex.  With several graphs in one batch, every graph keeps
+// its own live cliques / remaining vertices and records its own densities at
+// the end of each level that touched it -- exactly what its own peel records,
+// because a graph's vertices leave at their core numbers whatever else is
+// peeled alongside.
 #include "kcds_device.hpp"
 #include "kcds_internal.hpp"
 
@@ -29,17 +38,20 @@ namespace {
 
 constexpr uint32_t BIG_LIST = 1u << 14;
 
-struct CorePeelState {
-  ull level;
-  ull minval;
-  ull live;        // live cliques
+struct PeelCtl {
+  ull level;      // current level
+  ull minval;     // min live count over remaining vertices (reduction target)
+  ull live;       // live cliques (one-graph bookkeeping)
   ull max_num, max_den;
+  ull destroyed;  // cliques destroyed in the current sub-round
   uint32_t removed;
   uint32_t fsize[2];
+  uint32_t cur;
   uint32_t nbig;
   uint32_t levels;
-  uint32_t barrier;
   uint32_t ntouch;
+  uint32_t done;
+  uint32_t opening;  // a level was opened in this sub-round: collect its frontier
 };
 
 // a/b >= c/d for 64-bit non-negative integers, b, d > 0 (exact, 128-bit products)
@@ -47,6 +59,85 @@ __device__ __forceinline__ bool ratio_ge(ull a, ull b, ull c, ull d) {
   ull hi1 = __umul64hi(a, d), lo1 = a * d;
   ull hi2 = __umul64hi(c, b), lo2 = c * b;
   return hi1 > hi2 || (hi1 == hi2 && lo1 >= lo2);
+}
+
+__device__ __forceinline__ bool level_end(const PeelCtl *ctl) {
+  return !ctl->done && ctl->fsize[ctl->cur] == 0;
+}
+
+// multi-graph: record the density of every graph touched by the level that
+// just completed
+__global__ void k_pl_touch(PeelCtl *ctl, GraphPeel *gp, uint32_t *tlist) {
+  if (!level_end(ctl)) return;
+  const uint32_t nt = ctl->ntouch;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x) {
+    GraphPeel &G = gp[tlist[i]];
+    const ull rem = G.rem;
+    if (rem > 0 && ratio_ge(G.live, rem, G.max_num, G.max_den)) {
+      G.max_num = G.live;
+      G.max_den = rem;
+    }
+    G.touch = 0;
+  }
+}
+
+__global__ void k_pl_min(PeelCtl *ctl, uint32_t n, const ull *cnt, const int32_t *core) {
+  if (!level_end(ctl)) return;
+  ull m = ~0ull;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    if (core[v] == -1) m = min(m, cnt[v]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = min(m, (ull)__shfl_xor(m, o, 64));
+  if (lane_id() == 0 && m != ~0ull) atomicMin(&ctl->minval, m);
+}
+
+// one thread: close the completed level, open the next one
+__global__ void k_pl_open(PeelCtl *ctl, uint32_t n, uint32_t multi) {
+  if (!level_end(ctl)) return;
+  if (!multi) {
+    const ull rem = (ull)(n - ctl->removed);
+    if (rem > 0 && ratio_ge(ctl->live, rem, ctl->max_num, ctl->max_den)) {
+      ctl->max_num = ctl->live;
+      ctl->max_den = rem;
+    }
+  }
+  ctl->ntouch = 0;
+  if (ctl->minval == ~0ull) {  // nothing left (cannot happen while removed < n)
+    ctl->done = 1;
+    return;
+  }
+  ctl->level = max(ctl->level, ctl->minval);
+  ctl->minval = ~0ull;
+  ctl->levels += 1;
+  ctl->opening = 1;
+}
+
+__global__ void k_pl_collect(PeelCtl *ctl, uint32_t n, const ull *cnt, const int32_t *core, uint32_t *fr0,
+                             uint32_t *fr1) {
+  if (ctl->done || !ctl->opening) return;
+  const ull L = ctl->level;
+  const uint32_t cur = ctl->cur;
+  uint32_t *fr = cur ? fr1 : fr0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    if (core[v] == -1 && cnt[v] <= L) fr[atomicAdd(&ctl->fsize[cur], 1u)] = v;
+}
+
+__global__ void k_pl_mark(PeelCtl *ctl, const uint32_t *fr0, const uint32_t *fr1, const ull *incoff, int32_t *core,
+                          uint32_t *big, GraphPeel *gp, const uint32_t *gid, uint32_t *tlist) {
+  if (ctl->done) return;
+  const uint32_t cur = ctl->cur, fs = ctl->fsize[cur];
+  const uint32_t *fr = cur ? fr1 : fr0;
+  const int32_t L = (int32_t)ctl->level;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < fs; i += gridDim.x * blockDim.x) {
+    const uint32_t v = fr[i];
+    core[v] = L;
+    if (incoff[v + 1] - incoff[v] >= BIG_LIST) big[atomicAdd(&ctl->nbig, 1u)] = v;
+    if (gp) {
+      const uint32_t g = gid[v];
+      atomicSub(&gp[g].rem, 1u);
+      if (atomicExch(&gp[g].touch, 1u) == 0u) tlist[atomicAdd(&ctl->ntouch, 1u)] = g;
+    }
+  }
 }
 
 __device__ __forceinline__ void claim_clique(uint32_t c, uint32_t v, uint32_t k, ull L, const uint32_t *cl,
@@ -60,118 +151,69 @@ __device__ __forceinline__ void claim_clique(uint32_t c, uint32_t v, uint32_t k,
   for (uint32_t t = 0; t < k; ++t) {
     const uint32_t u = cl[(ull)c * k + t];
     if (u == v) continue;
-    if (__hip_atomic_load(&core[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != -1) continue;  // removed
+    if (core[u] != -1) continue;  // removed (core numbers were set by the mark step)
     const ull old = atomicSub(&cnt[u], 1ull);
     if (old == L + 1) nx[atomicAdd(nx_size, 1u)] = u;
   }
 }
 
-// gp == nullptr: one graph, bookkeeping on the global live count.  Otherwise
-// the vertices form several graphs (gid[r]); each graph keeps its own live
-// cliques / remaining vertices and, at the end of every level that touched
-// it, its own maximum density -- exactly what that graph's own peel records,
-// because a graph's vertices leave at their core numbers whatever else is
-// peeled alongside.
-__global__ __launch_bounds__(1024) void k_clique_peel(uint32_t n, uint32_t k, const uint32_t *cl,
-                                                      const ull *incoff, const uint32_t *inc, ull *cnt,
-                                                      int32_t *core, int32_t *status, uint32_t *fr0,
-                                                      uint32_t *fr1, uint32_t *big, CorePeelState *st,
-                                                      GraphPeel *gp, const uint32_t *gid, uint32_t *tlist) {
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t nthreads = gridDim.x * blockDim.x;
+// wave per frontier vertex with a short incidence list
+__global__ void k_pl_proc(PeelCtl *ctl, uint32_t k, const uint32_t *cl, const ull *incoff, const uint32_t *inc,
+                          ull *cnt, const int32_t *core, int32_t *status, uint32_t *fr0, uint32_t *fr1,
+                          GraphPeel *gp, const uint32_t *gid) {
+  if (ctl->done) return;
+  const uint32_t cur = ctl->cur, fs = ctl->fsize[cur];
+  const uint32_t *fr = cur ? fr1 : fr0;
+  uint32_t *nx = cur ? fr0 : fr1;
+  const ull L = ctl->level;
   const uint32_t lane = lane_id();
-  const uint32_t wid = tid / KCDS_WAVE, nwaves = nthreads / KCDS_WAVE;
-  uint32_t epoch = 0;
-  int cur = 0;
-  while (true) {
-    const uint32_t removed = __hip_atomic_load(&st->removed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (removed >= n) break;
-    uint32_t fs = __hip_atomic_load(&st->fsize[cur], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (fs == 0) {
-      // previous level complete: record its density, then open the next one
-      if (!gp) {
-        if (tid == 0) {
-          const ull live = st->live, rem = (ull)(n - removed);
-          if (rem > 0 && ratio_ge(live, rem, st->max_num, st->max_den)) {
-            st->max_num = live;
-            st->max_den = rem;
-          }
-        }
-      } else {
-        const uint32_t nt = __hip_atomic_load(&st->ntouch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (uint32_t i = tid; i < nt; i += nthreads) {
-          GraphPeel &G = gp[tlist[i]];
-          if (G.rem > 0 && ratio_ge(G.live, G.rem, G.max_num, G.max_den)) {
-            G.max_num = G.live;
-            G.max_den = G.rem;
-          }
-          G.touch = 0;
-        }
-      }
-      ull m = ~0ull;
-      for (uint32_t v = tid; v < n; v += nthreads)
-        if (core[v] == -1) m = min(m, cnt[v]);
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) m = min(m, (ull)__shfl_xor(m, o, 64));
-      if (lane == 0 && m != ~0ull) atomicMin(&st->minval, m);
-      grid_barrier(&st->barrier, gridDim.x, epoch);
-      const ull L = max(__hip_atomic_load(&st->minval, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                        __hip_atomic_load(&st->level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      uint32_t *fr = cur ? fr1 : fr0;
-      for (uint32_t v = tid; v < n; v += nthreads)
-        if (core[v] == -1 && cnt[v] <= L) fr[atomicAdd(&st->fsize[cur], 1u)] = v;
-      grid_barrier(&st->barrier, gridDim.x, epoch);
-      if (tid == 0) {
-        st->level = L;
-        st->minval = ~0ull;
-        st->levels += 1;
-        st->ntouch = 0;  // every thread read it before the first barrier of this branch
-      }
-      fs = __hip_atomic_load(&st->fsize[cur], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      grid_barrier(&st->barrier, gridDim.x, epoch);
-    }
-    const ull L = __hip_atomic_load(&st->level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t *fr = cur ? fr1 : fr0;
-    uint32_t *nx = cur ? fr0 : fr1;
-    uint32_t *nx_size = &st->fsize[cur ^ 1];
-    // remove the frontier; long incidence lists go to the grid-wide list
-    for (uint32_t i = tid; i < fs; i += nthreads) {
-      const uint32_t v = fr[i];
-      core[v] = (int32_t)L;
-      if (incoff[v + 1] - incoff[v] >= BIG_LIST) big[atomicAdd(&st->nbig, 1u)] = v;
-      if (gp) {
-        const uint32_t g = gid[v];
-        atomicSub(&gp[g].rem, 1u);
-        if (atomicExch(&gp[g].touch, 1u) == 0u) tlist[atomicAdd(&st->ntouch, 1u)] = g;
-      }
-    }
-    grid_barrier(&st->barrier, gridDim.x, epoch);
-    ull destroyed = 0;
-    for (uint32_t i = wid; i < fs; i += nwaves) {
-      const uint32_t v = fr[i];
-      const ull b = incoff[v], e = incoff[v + 1];
-      if (e - b >= BIG_LIST) continue;
-      for (ull j = b + lane; j < e; j += KCDS_WAVE)
-        claim_clique(inc[j], v, k, L, cl, status, core, cnt, nx, nx_size, destroyed, gp, gid);
-    }
-    const uint32_t nbig = __hip_atomic_load(&st->nbig, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (uint32_t bi = 0; bi < nbig; ++bi) {
-      const uint32_t v = big[bi];
-      const ull b = incoff[v], e = incoff[v + 1];
-      for (ull j = b + tid; j < e; j += nthreads)
-        claim_clique(inc[j], v, k, L, cl, status, core, cnt, nx, nx_size, destroyed, gp, gid);
-    }
-    destroyed = wave_sum_u64(destroyed);
-    if (lane == 0 && destroyed) atomicSub(&st->live, destroyed);
-    grid_barrier(&st->barrier, gridDim.x, epoch);
-    if (tid == 0) {
-      st->removed = removed + fs;
-      st->fsize[cur] = 0;
-      st->nbig = 0;
-    }
-    grid_barrier(&st->barrier, gridDim.x, epoch);
-    cur ^= 1;
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KCDS_WAVE;
+  const uint32_t nw = gridDim.x * blockDim.x / KCDS_WAVE;
+  ull destroyed = 0;
+  for (uint32_t i = wid; i < fs; i += nw) {
+    const uint32_t v = fr[i];
+    const ull b = incoff[v], e = incoff[v + 1];
+    if (e - b >= BIG_LIST) continue;
+    for (ull j = b + lane; j < e; j += KCDS_WAVE)
+      claim_clique(inc[j], v, k, L, cl, status, core, cnt, nx, &ctl->fsize[cur ^ 1], destroyed, gp, gid);
   }
+  destroyed = wave_sum_u64(destroyed);
+  if (lane == 0 && destroyed) atomicAdd(&ctl->destroyed, destroyed);
+}
+
+// the whole grid over each frontier vertex with a long incidence list
+__global__ void k_pl_proc_big(PeelCtl *ctl, uint32_t k, const uint32_t *cl, const ull *incoff, const uint32_t *inc,
+                              ull *cnt, const int32_t *core, int32_t *status, uint32_t *fr0, uint32_t *fr1,
+                              const uint32_t *big, GraphPeel *gp, const uint32_t *gid) {
+  if (ctl->done) return;
+  const uint32_t nbig = ctl->nbig;
+  if (nbig == 0) return;
+  const uint32_t cur = ctl->cur;
+  uint32_t *nx = cur ? fr0 : fr1;
+  const ull L = ctl->level;
+  const ull tid = (ull)blockIdx.x * blockDim.x + threadIdx.x, nt = (ull)gridDim.x * blockDim.x;
+  ull destroyed = 0;
+  for (uint32_t bi = 0; bi < nbig; ++bi) {
+    const uint32_t v = big[bi];
+    for (ull j = incoff[v] + tid; j < incoff[v + 1]; j += nt)
+      claim_clique(inc[j], v, k, L, cl, status, core, cnt, nx, &ctl->fsize[cur ^ 1], destroyed, gp, gid);
+  }
+  destroyed = wave_sum_u64(destroyed);
+  if (lane_id() == 0 && destroyed) atomicAdd(&ctl->destroyed, destroyed);
+}
+
+// one thread: advance to the next frontier
+__global__ void k_pl_finish(PeelCtl *ctl, uint32_t n) {
+  if (ctl->done) return;
+  const uint32_t cur = ctl->cur;
+  ctl->removed += ctl->fsize[cur];
+  ctl->live -= ctl->destroyed;
+  ctl->destroyed = 0;
+  ctl->fsize[cur] = 0;
+  ctl->nbig = 0;
+  ctl->opening = 0;
+  ctl->cur = cur ^ 1;
+  if (ctl->removed >= n) ctl->done = 1;
 }
 
 __global__ void k_peel_init(uint32_t n, uint64_t T, const ull *cnt0, ull *cnt, int32_t *core, int32_t *status) {
@@ -208,9 +250,9 @@ __global__ void k_graph_init(uint32_t G, uint32_t k, GraphPeel *gp) {
   }
 }
 
-__global__ void k_graph_single(const CorePeelState *st, GraphPeel *gp) {
-  gp[0].max_num = st->max_num;
-  gp[0].max_den = st->max_den;
+__global__ void k_graph_single(const PeelCtl *ctl, GraphPeel *gp) {
+  gp[0].max_num = ctl->max_num;
+  gp[0].max_den = ctl->max_den;
 }
 
 // k' = ceil(maxDensity) per graph (main.cu:1476)
@@ -240,6 +282,10 @@ __global__ void k_graph_summary(uint32_t n, uint64_t T, uint32_t k, const uint32
   }
 }
 
+uint32_t grid1d(uint64_t len, uint32_t cap = 65535u * 16) {
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((len + 255) / 256, cap));
+}
+
 }  // namespace
 
 void graph_ids(Ctx &c) {
@@ -249,8 +295,7 @@ void graph_ids(Ctx &c) {
   HIP_CHECK(hipMemcpyAsync(c.gvoff.p, c.h_gvoff.data(), sizeof(uint32_t) * ((size_t)G + 1), hipMemcpyHostToDevice,
                            c.stream));
   if (n) {
-    k_gid<<<(n + 255) / 256, 256, 0, c.stream>>>(n, c.inv.as<uint32_t>(), c.gvoff.as<uint32_t>(), G,
-                                                 c.gid.as<uint32_t>());
+    k_gid<<<grid1d(n), 256, 0, c.stream>>>(n, c.inv.as<uint32_t>(), c.gvoff.as<uint32_t>(), G, c.gid.as<uint32_t>());
     KCDS_LAUNCH_CHECK();
   }
   c.sync();  // h_gvoff may change after return
@@ -271,59 +316,72 @@ void peel_run(Ctx &c) {
   buf_reserve(c, c.tlist, sizeof(uint32_t) * ((size_t)G + 1));
   GraphPeel *gp = c.gpeel.as<GraphPeel>();
   HIP_CHECK(hipMemsetAsync(gp, 0, sizeof(GraphPeel) * G, c.stream));
-  // working counts, two frontiers, the big-vertex list and the state
-  size_t words = (size_t)n * 2 /*cnt ull*/ + (size_t)n * 3 + 64;
-  buf_reserve(c, c.tmp0, words * sizeof(uint32_t) + sizeof(CorePeelState) + 256);
+  // working counts, two frontiers, the big-vertex list and the control block
+  const size_t words = (size_t)n * 2 /*cnt ull*/ + (size_t)n * 3 + 64;
+  buf_reserve(c, c.tmp0, words * sizeof(uint32_t) + sizeof(PeelCtl) + 256);
   ull *cnt = c.tmp0.as<ull>();
   uint32_t *fr0 = (uint32_t *)(cnt + n), *fr1 = fr0 + n, *big = fr1 + n;
-  CorePeelState *st = (CorePeelState *)(((uintptr_t)(big + n) + 255) & ~(uintptr_t)255);
-  CorePeelState h0{};
-  h0.level = 0;
+  PeelCtl *ctl = (PeelCtl *)(((uintptr_t)(big + n) + 255) & ~(uintptr_t)255);
+  PeelCtl h0{};
   h0.minval = ~0ull;
   h0.live = T;
   h0.max_num = T;
   h0.max_den = n ? n : 1;  // main.cu:361-366: the whole graph's density T/n
-  HIP_CHECK(hipMemcpyAsync(st, &h0, sizeof h0, hipMemcpyHostToDevice, c.stream));
-  const uint64_t mx = std::max<uint64_t>(n, T);
-  uint32_t grid = (uint32_t)std::min<uint64_t>((mx + 255) / 256, 65535ull * 16);
-  grid = std::max<uint32_t>(grid, 1);
-  k_peel_init<<<grid, 256, 0, c.stream>>>(n, T, c.cnt.as<ull>(), cnt, c.core.as<int32_t>(),
-                                          c.status.as<int32_t>());
+  h0.done = n == 0;
+  HIP_CHECK(hipMemcpyAsync(ctl, &h0, sizeof h0, hipMemcpyHostToDevice, c.stream));
+  const uint32_t grid = grid1d(std::max<uint64_t>(n, T));
+  k_peel_init<<<grid, 256, 0, c.stream>>>(n, T, c.cnt.as<ull>(), cnt, c.core.as<int32_t>(), c.status.as<int32_t>());
   KCDS_LAUNCH_CHECK();
   const uint32_t *gidp = c.gid.as<uint32_t>();
   if (n) {
-    k_graph_accum<<<(n + 255) / 256, 256, 0, c.stream>>>(n, gidp, c.cnt.as<ull>(), gp);
+    k_graph_accum<<<grid1d(n), 256, 0, c.stream>>>(n, gidp, c.cnt.as<ull>(), gp);
     KCDS_LAUNCH_CHECK();
   }
-  k_graph_init<<<(G + 255) / 256, 256, 0, c.stream>>>(G, k, gp);
+  k_graph_init<<<grid1d(G), 256, 0, c.stream>>>(G, k, gp);
   KCDS_LAUNCH_CHECK();
-  if (n) {
-    int per_cu = 0;
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_clique_peel, 1024, 0));
-    if (per_cu < 1) fail(KCDS_E_HIP, "k_clique_peel cannot be resident");
-    uint32_t pgrid = (uint32_t)c.num_cus * (uint32_t)std::min(per_cu, 2);
-    const uint32_t *cl = c.cliques.as<uint32_t>(), *inc = c.inc.as<uint32_t>();
-    const ull *incoff = c.incoff.as<ull>();
-    int32_t *core = c.core.as<int32_t>(), *status = c.status.as<int32_t>();
-    uint32_t nn = n, kk = k;
-    GraphPeel *gpa = multi ? gp : nullptr;
-    uint32_t *tl = c.tlist.as<uint32_t>();
-    void *args[] = {(void *)&nn, (void *)&kk, (void *)&cl, (void *)&incoff, (void *)&inc, (void *)&cnt,
-                    (void *)&core, (void *)&status, (void *)&fr0, (void *)&fr1, (void *)&big, (void *)&st,
-                    (void *)&gpa, (void *)&gidp, (void *)&tl};
-    HIP_CHECK(hipLaunchCooperativeKernel((const void *)k_clique_peel, dim3(pgrid), dim3(1024), args, 0, c.stream));
+  const uint32_t *cl = c.cliques.as<uint32_t>(), *inc = c.inc.as<uint32_t>();
+  const ull *incoff = c.incoff.as<ull>();
+  int32_t *core = c.core.as<int32_t>(), *status = c.status.as<int32_t>();
+  GraphPeel *gpa = multi ? gp : nullptr;
+  uint32_t *tl = c.tlist.as<uint32_t>();
+  const uint32_t vgrid = grid1d(n, (uint32_t)c.num_cus * 8);
+  const uint32_t pgrid = (uint32_t)c.num_cus * 8;  // 256-thread blocks, 4 waves each
+  uint32_t *hdone = (uint32_t *)c.host_scratch(64);
+  // enqueue sub-rounds in growing batches; every step is a no-op once done
+  uint32_t batch = 4;
+  uint64_t rounds = 0;
+  while (n) {
+    for (uint32_t r = 0; r < batch; ++r) {
+      if (multi) k_pl_touch<<<grid1d(G, 1024), 256, 0, c.stream>>>(ctl, gp, tl);
+      k_pl_min<<<vgrid, 256, 0, c.stream>>>(ctl, n, cnt, core);
+      k_pl_open<<<1, 1, 0, c.stream>>>(ctl, n, multi ? 1u : 0u);
+      k_pl_collect<<<vgrid, 256, 0, c.stream>>>(ctl, n, cnt, core, fr0, fr1);
+      k_pl_mark<<<vgrid, 256, 0, c.stream>>>(ctl, fr0, fr1, incoff, core, big, gpa, gidp, tl);
+      k_pl_proc<<<pgrid, 256, 0, c.stream>>>(ctl, k, cl, incoff, inc, cnt, core, status, fr0, fr1, gpa, gidp);
+      k_pl_proc_big<<<pgrid, 256, 0, c.stream>>>(ctl, k, cl, incoff, inc, cnt, core, status, fr0, fr1, big, gpa,
+                                                 gidp);
+      k_pl_finish<<<1, 1, 0, c.stream>>>(ctl, n);
+      KCDS_LAUNCH_CHECK();
+    }
+    rounds += batch;
+    HIP_CHECK(hipMemcpyAsync(hdone, &ctl->done, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    if (*hdone) break;
+    // every sub-round removes at least one vertex, so n + 1 sub-rounds suffice
+    if (rounds > (uint64_t)n + 1)
+      fail(KCDS_E_HIP, "peeling did not finish within %llu sub-rounds", (unsigned long long)rounds);
+    batch = std::min<uint32_t>(batch * 2, 256);
   }
   if (!multi) {
-    k_graph_single<<<1, 1, 0, c.stream>>>(st, gp);
+    k_graph_single<<<1, 1, 0, c.stream>>>(ctl, gp);
     KCDS_LAUNCH_CHECK();
   }
-  k_graph_kprime<<<(G + 255) / 256, 256, 0, c.stream>>>(G, gp);
+  k_graph_kprime<<<grid1d(G), 256, 0, c.stream>>>(G, gp);
   KCDS_LAUNCH_CHECK();
-  k_graph_summary<<<grid, 256, 0, c.stream>>>(n, T, k, gidp, c.core.as<int32_t>(), c.status.as<int32_t>(),
-                                              c.cliques.as<uint32_t>(), gp);
+  k_graph_summary<<<grid, 256, 0, c.stream>>>(n, T, k, gidp, core, status, cl, gp);
   KCDS_LAUNCH_CHECK();
-  CorePeelState *hs = (CorePeelState *)c.host_scratch(sizeof(CorePeelState) + 64);
-  HIP_CHECK(hipMemcpyAsync(hs, st, sizeof(CorePeelState), hipMemcpyDeviceToHost, c.stream));
+  PeelCtl *hs = (PeelCtl *)c.host_scratch(sizeof(PeelCtl) + 64);
+  HIP_CHECK(hipMemcpyAsync(hs, ctl, sizeof(PeelCtl), hipMemcpyDeviceToHost, c.stream));
   c.h_gpeel.resize(G);
   HIP_CHECK(hipMemcpyAsync(c.h_gpeel.data(), gp, sizeof(GraphPeel) * G, hipMemcpyDeviceToHost, c.stream));
   c.sync();

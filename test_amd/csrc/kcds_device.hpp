@@ -97,20 +97,3 @@ __device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *a, uint32_t 
   }
   return lo;
 }
-
-// Software grid barrier for cooperative launches: every thread releases its
-// writes at agent scope, one thread per block arrives on a monotonically
-// increasing counter and waits for all blocks, then every thread acquires.
-__device__ __forceinline__ void grid_barrier(uint32_t *bar, uint32_t nblocks, uint32_t &epoch) {
-  __threadfence();
-  __syncthreads();
-  epoch += 1;
-  if (threadIdx.x == 0) {
-    atomicAdd(bar, 1u);
-    const uint32_t target = epoch * nblocks;
-    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
-      __builtin_amdgcn_s_sleep(1);
-  }
-  __syncthreads();
-  __threadfence();
-}

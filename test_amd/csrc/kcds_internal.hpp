@@ -80,11 +80,9 @@ struct CompData {
 
 // Buffers of one clique flow network (reused across max-flow calls).
 struct FlowBufs {
-  DevBuf off, head, rev, res, excess, height, deg, fill, front, side, scal;
+  DevBuf scal;
   DevBuf cside;  // uint8 per core vertex (component-grouped): min-cut side of each solved component
   DevBuf b_off, b_excess, b_height, b_fr0, b_fr1, b_head, b_rev, b_res, b_tasks;  // block solver slices
-  uint64_t last_p = 0, last_q = 0;
-  int64_t last_comp = -1;
 };
 
 struct Ctx {
@@ -187,33 +185,23 @@ void peel_run(Ctx &c);
 void components_run(Ctx &c);
 
 // ---- flow (k_flow.hip) ----
-struct FlowOut {
-  int64_t value;     // max-flow value
-  uint64_t side_n;   // vertices on the source side of the maximal min cut
-  uint64_t side_T;   // component cliques with every member on that side
-  uint32_t phases;   // global relabels performed
-};
-// Max-flow of component d (dense index) at alpha = p/q; the maximal min-cut
-// source side stays in c.fb.side (uint8 per local vertex).
-FlowOut flow_component(Ctx &c, uint32_t d, uint64_t p, uint64_t q);
-// Marks the current c.fb.side of component d in c.inset (new-id space).
-void flow_mark_side(Ctx &c, uint32_t d);
-// Copies c.fb.side of component d into its slice of c.fb.cside.
-void flow_store_side(Ctx &c, uint32_t d);
-// Marks component d's slice of c.fb.cside in c.inset.
+// Marks component d's slice of c.fb.cside (its last min-cut side) in c.inset.
 void mark_component_side(Ctx &c, uint32_t d);
 
-// Whole exact search of many components at once, one workgroup each.
+// Exact search (or one max-flow when max_iters == 1) of many components at
+// once, one workgroup each.
 struct BlockTask {
   uint32_t d;     // component dense index
   ull p0, q0;     // starting alpha
 };
 struct BlockResult {
-  ull p = 0, q = 1;         // final alpha (reduced)
+  ull p = 0, q = 1;            // final alpha (reduced)
   ull side_n = 0, side_T = 0;  // maximal min-cut side at the final alpha
+  long long flow = 0;          // value of the last max-flow
   uint32_t flows = 0;
 };
-void dinkelbach_blocks(Ctx &c, const std::vector<BlockTask> &in, std::vector<BlockResult> &out);
+void dinkelbach_blocks(Ctx &c, const std::vector<BlockTask> &in, std::vector<BlockResult> &out, uint32_t max_iters);
+void graph_ids(Ctx &c);
 void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_graph, uint8_t *in_set);
 void graph_ids(Ctx &c);
 

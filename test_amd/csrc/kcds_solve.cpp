@@ -17,9 +17,8 @@
 // maximal minimum cut) and alpha := density(A).  A graph's rho* is the largest
 // final alpha of its components, and its answer set is the union of the
 // maximal minimum-cut sides of the components that end at rho* -- the unique
-// maximal densest set.  Components up to BLOCK_MAX_NODES flow nodes run the
-// whole iteration inside one workgroup each (one launch for all of them);
-// larger ones use the grid-wide push-relabel kernel per max-flow.
+// maximal densest set.  Every component runs its whole iteration inside one
+// workgroup (one launch for all components of all graphs).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -34,7 +33,7 @@ namespace {
 
 typedef unsigned __int128 u128;
 
-constexpr uint64_t BLOCK_MAX_NODES = 1u << 17;
+constexpr uint32_t ALL_ITERATIONS = 0xFFFFFFFFu;
 
 ull gcd(ull a, ull b) {
   while (b) {
@@ -83,12 +82,13 @@ void maxflow_component(Ctx &c, uint32_t comp_id, uint64_t p, uint64_t q, int64_t
   auto it = std::lower_bound(g.h_label.begin(), g.h_label.end(), comp_id);
   if (it == g.h_label.end() || *it != comp_id) fail(KCDS_E_INVALID, "no component with id %u", comp_id);
   const uint32_t d = (uint32_t)(it - g.h_label.begin());
-  FlowOut o = flow_component(c, d, p, q);
-  if (flow) *flow = o.value;
+  std::vector<BlockResult> r;
+  dinkelbach_blocks(c, {BlockTask{d, p, q}}, r, 1);
+  if (flow) *flow = r[0].flow;
   if (side && c.n) {
     buf_reserve(c, c.inset, (size_t)c.n + 8);
     HIP_CHECK(hipMemsetAsync(c.inset.p, 0, c.n, c.stream));
-    flow_mark_side(c, d);
+    mark_component_side(c, d);
     std::vector<uint8_t> tmp(c.n);
     std::vector<uint32_t> rank(c.n);
     HIP_CHECK(hipMemcpyAsync(tmp.data(), c.inset.p, c.n, hipMemcpyDeviceToHost, c.stream));
@@ -170,52 +170,24 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
   uint64_t core_vertices = g.h_voff[C];
   buf_reserve(c, c.fb.cside, core_vertices + 8);
   HIP_CHECK(hipMemsetAsync(c.fb.cside.p, 0, core_vertices + 8, c.stream));
-  std::vector<BlockTask> small;
-  std::vector<uint32_t> large;
+  std::vector<BlockTask> tasks;
   for (uint32_t d = 0; d < C; ++d) {
-    const ull tc = g.h_coff[d + 1] - g.h_coff[d], nc = g.h_voff[d + 1] - g.h_voff[d];
+    const ull tc = g.h_coff[d + 1] - g.h_coff[d];
     const uint32_t gi = cg[d];
     if (tc == 0 || !may_reach(k, c.h_gpeel[gi].max_core, tc, an[gi], ad[gi])) continue;
-    if (nc + tc + 2 <= BLOCK_MAX_NODES) small.push_back(BlockTask{d, an[gi], ad[gi]});
-    else large.push_back(d);
+    tasks.push_back(BlockTask{d, an[gi], ad[gi]});
   }
   // final alpha and side size of every solved component
   std::vector<BlockResult> res(C);
   std::vector<uint8_t> solved(C, 0);
   {
     std::vector<BlockResult> r;
-    dinkelbach_blocks(c, small, r);
-    for (size_t i = 0; i < small.size(); ++i) {
-      res[small[i].d] = r[i];
-      solved[small[i].d] = 1;
-      out[cg[small[i].d]].flow_calls += r[i].flows;
+    dinkelbach_blocks(c, tasks, r, ALL_ITERATIONS);
+    for (size_t i = 0; i < tasks.size(); ++i) {
+      res[tasks[i].d] = r[i];
+      solved[tasks[i].d] = 1;
+      out[cg[tasks[i].d]].flow_calls += r[i].flows;
     }
-  }
-  for (uint32_t d : large) {
-    const uint32_t gi = cg[d];
-    const ull tc = g.h_coff[d + 1] - g.h_coff[d];
-    ull p = an[gi], q = ad[gi];
-    BlockResult br;
-    for (uint32_t it = 0;; ++it) {
-      FlowOut o = flow_component(c, d, p, q);
-      br.flows++;
-      if ((u128)o.value >= (u128)q * k * tc) {  // no set denser than p/q: keep this side
-        br.side_n = o.side_n;
-        br.side_T = o.side_T;
-        flow_store_side(c, d);
-        break;
-      }
-      if (o.side_n == 0 || !ratio_gt(o.side_T, o.side_n, p, q) || it > 64)
-        fail(KCDS_E_HIP, "min cut below q*k*Tc without a denser source side (component %u)", d);
-      ull gg = gcd(o.side_T, o.side_n);
-      p = o.side_T / gg;
-      q = o.side_n / gg;
-    }
-    br.p = p;
-    br.q = q;
-    res[d] = br;
-    solved[d] = 1;
-    out[gi].flow_calls += br.flows;
   }
   // per graph: rho* = largest final alpha among components with a non-empty
   // side; the answer is the union of those components' sides at rho*

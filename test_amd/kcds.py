@@ -25,7 +25,7 @@ U64P = ctypes.POINTER(ctypes.c_uint64)
 # every entry point include/kcds.h declares
 EXPORTS = ("kcds_create", "kcds_destroy", "kcds_last_error", "kcds_upload_csr", "kcds_orient", "kcds_count",
            "kcds_list", "kcds_download_cliques", "kcds_peel", "kcds_components", "kcds_maxflow", "kcds_solve",
-           "kcds_sync", "kcds_set_tuning", "kcds_get_list_info")
+           "kcds_sync", "kcds_set_tuning", "kcds_get_list_info", "kcds_solve_batch")
 
 
 class PeelStats(ctypes.Structure):
@@ -40,6 +40,12 @@ class Result(ctypes.Structure):
                 ("flow_calls", ctypes.c_uint32), ("peel", PeelStats),
                 ("ms_orient", ctypes.c_float), ("ms_list", ctypes.c_float), ("ms_peel", ctypes.c_float),
                 ("ms_components", ctypes.c_float), ("ms_flow", ctypes.c_float), ("ms_total", ctypes.c_float)]
+
+
+class GraphResult(ctypes.Structure):
+    _fields_ = [("T", ctypes.c_uint64), ("rho_num", ctypes.c_uint64), ("rho_den", ctypes.c_uint64),
+                ("k_prime", ctypes.c_uint32), ("core_vertices", ctypes.c_uint32), ("ncomp", ctypes.c_uint32),
+                ("flow_calls", ctypes.c_uint32)]
 
 
 class ListInfo(ctypes.Structure):
@@ -75,6 +81,7 @@ def lib():
         L.kcds_components.argtypes = [V, U32P, U32P]
         L.kcds_maxflow.argtypes = [V, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, I64P, U8P]
         L.kcds_solve.argtypes = [V, ctypes.c_uint32, ctypes.POINTER(Result), U8P]
+        L.kcds_solve_batch.argtypes = [V, ctypes.c_uint32, ctypes.c_uint32, U32P, ctypes.POINTER(GraphResult), U8P]
         L.kcds_sync.argtypes = [V]
         L.kcds_set_tuning.argtypes = [V, ctypes.c_uint32, ctypes.c_uint32]
         L.kcds_get_list_info.argtypes = [V, ctypes.POINTER(ListInfo)]
@@ -169,6 +176,19 @@ class Solver:
         self._chk(lib().kcds_solve(self._h, k, ctypes.byref(res), _p(ins, U8P)))
         return res, (ins[: self.n].astype(bool) if want_set else None)
 
+    def solve_batch(self, graphs, k, want_set=True):
+        """Solves many disjoint graphs [(offset, nbr), ...] in one batched pass.
+
+        Returns (list of GraphResult, list of per-graph boolean in_set arrays)."""
+        off, nbr, voff = concat_graphs(graphs)
+        self.upload(off, nbr)
+        G = len(graphs)
+        res = (GraphResult * G)()
+        ins = np.zeros(max(self.n, 1), np.uint8) if want_set else None
+        self._chk(lib().kcds_solve_batch(self._h, k, G, _p(voff, U32P), res, _p(ins, U8P)))
+        sets = [ins[voff[i]:voff[i + 1]].astype(bool) for i in range(G)] if want_set else None
+        return list(res), sets
+
     def sync(self):
         self._chk(lib().kcds_sync(self._h))
 
@@ -179,3 +199,20 @@ class Solver:
         info = ListInfo()
         self._chk(lib().kcds_get_list_info(self._h, ctypes.byref(info)))
         return info
+
+
+def concat_graphs(graphs):
+    """Disjoint union of CSR graphs: (offset, nbr, graph_voff)."""
+    ns = [len(o) - 1 for o, _ in graphs]
+    voff = np.zeros(len(graphs) + 1, np.uint32)
+    voff[1:] = np.cumsum(ns, dtype=np.uint64).astype(np.uint32)
+    offs, nbrs = [np.zeros(1, np.uint64)], []
+    base_e = 0
+    for (o, b), v0 in zip(graphs, voff[:-1]):
+        o = np.asarray(o, np.uint64)
+        offs.append(o[1:] + base_e)
+        nbrs.append(np.asarray(b, np.uint32) + np.uint32(v0))
+        base_e += int(o[-1])
+    off = np.concatenate(offs).astype(np.uint32)
+    nbr = np.concatenate(nbrs) if nbrs else np.zeros(0, np.uint32)
+    return off, nbr.astype(np.uint32), voff

@@ -1,0 +1,79 @@
+"""GPU parity of the batched solve: many disjoint graphs solved in one pass
+(one orientation, one listing, one peel with per-graph bookkeeping, one
+launch for every component's exact search) must give each graph exactly the
+answer it gets alone -- and the oracle's.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from test_amd import gen
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_batch():
+    gs = []
+    for i in range(14):
+        g = gen.er(60 + 17 * i, 0.08, 6 + i % 9, seed=300 + i)
+        gs.append((g.offset, g.nbr))
+    g = gen.rmat(9, 8, (10,), seed=41)
+    gs.append((g.offset, g.nbr))
+    gs.append((np.zeros(6, np.uint32), np.zeros(0, np.uint32)))           # 5 isolated vertices
+    gs.append((np.array([0, 1, 2], np.uint32), np.array([1, 0], np.uint32)))  # one edge, no clique
+    # two disjoint K6 (a tie) inside one graph
+    n = 12
+    adj = [[u for u in range(6 * (v // 6), 6 * (v // 6) + 6) if u != v] for v in range(n)]
+    off = np.zeros(n + 1, np.uint32)
+    off[1:] = np.cumsum([len(a) for a in adj])
+    gs.append((off, np.array([u for a in adj for u in a], np.uint32)))
+    return gs
+
+
+@pytest.mark.parametrize("k", [3, 4])
+def test_batch_matches_oracle_and_solo(solver, k):
+    gs = mixed_batch()
+    res, sets = solver.solve_batch(gs, k)
+    for i, (off, nbr) in enumerate(gs):
+        r, pv, core, comp, ins_r = O.solve(off, nbr, k)
+        got = res[i]
+        print(f"  graph {i}: n={len(off)-1} T={got.T} rho={got.rho_num}/{got.rho_den}", flush=True)
+        assert got.T == r.T
+        if r.T == 0:
+            assert got.rho_den == 0 and not sets[i].any()
+            continue
+        assert (got.rho_num, got.rho_den) == (r.rho_num, r.rho_den)
+        assert got.k_prime == r.k_prime
+        np.testing.assert_array_equal(sets[i], ins_r)
+    # solo solves of a few graphs give the same answers
+    for i in (0, 5, 14, len(gs) - 1):
+        solver.upload(*gs[i])
+        solo, ins = solver.solve(k)
+        assert (solo.T, solo.rho_num, solo.rho_den) == (res[i].T, res[i].rho_num, res[i].rho_den)
+        np.testing.assert_array_equal(ins, sets[i])
+
+
+def test_batch_config4_subset(solver):
+    specs = gen.batch_specs()[:96]
+    gs = [gen.batch_graph(s) for s in specs]
+    res, sets = solver.solve_batch([(g.offset, g.nbr) for g in gs], 4)
+    for i, g in enumerate(gs):
+        r, pv, core, comp, ins_r = O.solve(g.offset, g.nbr, 4)
+        assert res[i].T == r.T
+        assert (res[i].rho_num, res[i].rho_den) == (r.rho_num, r.rho_den)
+        np.testing.assert_array_equal(sets[i], ins_r)
+        s = len(g.planted)
+        assert res[i].rho_num * s >= math.comb(s, 4) * res[i].rho_den  # at least the planted clique
+
+
+def test_batch_rejects_cross_edges(solver):
+    from test_amd import kcds
+    g = gen.er(50, 0.2, 5, seed=1)
+    solver.upload(g.offset, g.nbr)
+    import ctypes
+    voff = np.array([0, 25, 50], np.uint32)
+    res = (kcds.GraphResult * 2)()
+    rc = kcds.lib().kcds_solve_batch(solver._h, 3, 2, voff.ctypes.data_as(kcds.U32P), res, None)
+    assert rc == -1  # KCDS_E_INVALID: edges between the two declared graphs

@@ -167,3 +167,16 @@ def test_config1_closed_form(solver):
     res, ins = solver.solve(k)
     assert (res.rho_num, res.rho_den) == (math.comb(40, 3), 40)
     assert set(np.flatnonzero(ins).tolist()) == set(g.planted.tolist())
+
+
+def test_cli_matches_reference_output_lines():
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "test_amd", "kcds")
+    out = subprocess.run([exe, os.path.join(GOLD, "ref_data_test.txt"), "3", "1000", "1000", "100", "10"],
+                         stdout=subprocess.PIPE, text=True, check=True).stdout.splitlines()
+    assert out[-2] == "Max Density 1 Size 4"
+    assert out[-1].startswith("Time taken: ") and out[-1].endswith(" ms")
+    assert "Exact Density 4/4" in out
+    out = subprocess.run([exe, os.path.join(GOLD, "ref_data_test.txt"), "4"],
+                         stdout=subprocess.PIPE, text=True, check=True).stdout.splitlines()
+    assert out[-2] == "Max Density 0.25 Size 4"
