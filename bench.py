@@ -35,6 +35,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default waves per root (skip the autotuner)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -53,6 +54,11 @@ def main():
         s.orient(None)
         T, _ = s.count(k, per_vertex=False)
         return T
+
+    tuned = None
+    if not args.no_tune:  # setup, outside the timed region: pick waves per root for this graph
+        s.orient(None)
+        tuned = s.autotune_list(k)
 
     for _ in range(args.warmup):
         step()
@@ -74,7 +80,7 @@ def main():
     achieved = bytes_list / (kms / 1e3)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": None,
-                "kernel": "k_list_count<W,4> (4 width-class launches per pass)",
+                "kernel": "k_list_count<W,4> (one launch per width class W=1,2,4,8, back to back)",
                 "kernel_ms": kms, "bytes_per_pass": bytes_list}
 
     cpu = None
@@ -112,7 +118,9 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": "C2 R-MAT scale 18 ef16 +K60+5xK30, k=4: GPU degeneracy order + DAG + "
                                "per-vertex 4-clique counts", "n": g.n, "m": g.m, "k": k, "T": T,
-                   "dag_arcs": arcs, "w2": w2, "max_outdeg": info.max_outdeg},
+                   "dag_arcs": arcs, "w2": w2, "max_outdeg": info.max_outdeg,
+                   "class_roots": list(info.class_roots), "waves_per_root": list(info.waves_per_root),
+                   "autotune_class_ms": tuned[1] if tuned else None},
         "roofline": roofline, "cpu_baseline": cpu, "extras": extras,
     }
     print(json.dumps(line))

@@ -140,9 +140,25 @@ int kcds_sync(kcds_ctx *ctx);
  * kernels (0 = library default).  Used by the autotuner. */
 int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
 
+/* Listing launch shape: waves per root (workgroup size / 64, 1..8) of each
+ * width class W = 1, 2, 4, 8 words (roots of out-degree <= 64, 128, 256,
+ * 512); NULL restores the defaults {1, 2, 4, 8}.  Replaces the reference's
+ * hand-set sizing arguments pSize, cpSize, glBufferSize, partitionSize
+ * (main.cu:1273-1293), which size its per-warp partitions. */
+int kcds_set_list_shape(kcds_ctx *ctx, const uint32_t *waves_per_root /* 4, or NULL */);
+
+/* Autotuner: on the oriented graph, times every waves-per-root choice of
+ * every width class at clique size k, keeps the fastest and returns it in
+ * waves_per_root[4] and the class times (ms) in class_ms[4] (either may be
+ * NULL).  No counterpart in the reference (its launch shape is fixed). */
+int kcds_autotune_list(kcds_ctx *ctx, uint32_t k, uint32_t *waves_per_root, float *class_ms);
+
 /* Per-stage facts of the last kcds_count/kcds_list call, for the roofline:
- * n, m (DAG arcs), W2 = sum over arcs (v,u) of outdeg(u), max out-degree,
- * and the average duration of the dominant listing kernel (ms, hipEvents). */
+ * n, m (DAG arcs), W2 = sum over the roots the pass stages (out-degree >=
+ * k-1) of the out-degrees of their out-neighbours, max out-degree, the
+ * duration of the listing pass (ms, hipEvents around its launches), the
+ * number of launches, the roots of each width class and their waves per
+ * root. */
 typedef struct kcds_list_info {
   uint32_t n;
   uint64_t arcs;
@@ -150,6 +166,8 @@ typedef struct kcds_list_info {
   uint32_t max_outdeg;
   float kernel_ms;
   uint32_t kernel_launches;
+  uint32_t class_roots[4];
+  uint32_t waves_per_root[4];
 } kcds_list_info;
 int kcds_get_list_info(kcds_ctx *ctx, kcds_list_info *info);
 

@@ -229,11 +229,6 @@ __device__ ull gcd_u64(ull a, ull b) {
 }
 
 // workgroup barrier after this wave's global stores and atomics completed
-__device__ __forceinline__ void block_sync() {
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-}
-
 // exclusive scan of one value per thread over a 1024-thread block
 __device__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t *total, uint32_t *wsum) {
   const uint32_t lane = threadIdx.x % KCDS_WAVE, w = threadIdx.x / KCDS_WAVE;

@@ -6,22 +6,32 @@
 // (countCliques, src/helpers.cu:1074-1113) and writes them with per-vertex
 // clique degrees (writeFinalCliques, src/helpers.cu:489-550).
 //
-// Here one wavefront owns one root r of the DAG (new ids, lists ascending):
-//   1. stage N+(r) = L[0..d) in LDS and build the symmetric adjacency of the
-//      induced subgraph as d rows of W 64-bit words, A[i] bit j = {L[i],L[j]}
-//      is an edge.  Every element of N+(L[i]) is looked up in L by binary
-//      search (the out-wedge sum W2 of the roofline is these reads).
+// Here one workgroup of nw wavefronts owns one root r of the DAG (new ids,
+// lists ascending) at a time:
+//   1. Stage N+(r) = L[0..d) in LDS with an open-addressing table
+//      id -> local index, and the out-lists N+(L[i]) as one concatenated
+//      sequence of non-empty segments.  The waves split that sequence into
+//      64-element chunks; a lane finds its segment from a ballot of the
+//      segment starts inside its chunk, reads one element w (the out-wedge
+//      sum W2 of the roofline is these reads) and looks w up in the table.
+//      A hit j sets bit j of row i and bit i of row j: rows are the
+//      symmetric adjacency of the subgraph induced by N+(r), d rows of W
+//      64-bit words stored word-major (A[w*D + j]) so that 64 lanes reading
+//      64 consecutive rows hit 64 distinct banks.
 //   2. Because ids increase along every DAG arc, the out-neighbours of L[c]
-//      inside N+(r) are A[c] masked to bits above c, so recursion on
+//      inside N+(r) are row c masked to bits above c, so recursion on
 //      candidate sets S (W words, wave-uniform) needs no further global
-//      reads: S' = S & A[c] & above(c).
+//      reads: S' = S & A[c] & above(c).  The waves split the first level.
 //   3. At two vertices left, the cliques extending the prefix are the edges
 //      inside S: lane j (owner of rows j, j+64, ...) adds popc(A[j] & S) to
 //      its own counter; the wave sum / 2 is credited to every prefix vertex.
 //      Per-vertex counts are exact without touching individual cliques.
-//   4. Counters flush to HBM with one 64-bit atomic per (root, member).
-// The materialising variant enumerates the same cliques in the same order and
-// writes them row-major (k new ids per clique) at the root's offset.
+//   4. Counters flush to HBM with one 64-bit atomic per (wave, root, member).
+// The materialising variant (one wave per root) enumerates the same cliques
+// in a fixed order and writes them row-major (k new ids per clique) at the
+// root's offset.
+#include <algorithm>
+
 #include "kcds_device.hpp"
 #include "kcds_internal.hpp"
 
@@ -31,13 +41,25 @@ namespace {
 
 constexpr int NCLASS = 4;  // W = 1, 2, 4, 8 words per row (out-degree <= 64, 128, 256, 512)
 constexpr uint32_t MAX_LDS_DEG = 512;
+constexpr uint32_t MAX_WAVES = 8;  // waves per workgroup (launch bound 512)
+constexpr uint32_t EMPTY_KEY = 0xFFFFFFFFu;
 
 template <int W> struct Frame {
-  uint32_t L[64 * W];        // N+(r)
-  uint32_t pre[64 * W + 1];  // prefix sums of outdeg(L[i])
-  uint32_t st[64 * W];       // doff[L[i]] - pre[i]
-  ull A[64 * W * W];         // symmetric adjacency rows
+  static constexpr uint32_t D = 64u * W;  // row capacity
+  static constexpr uint32_t HB = W == 1 ? 8 : W == 2 ? 9 : W == 4 ? 10 : 11;
+  static constexpr uint32_t H = 1u << HB;  // table slots (load <= 1/4)
+  uint32_t L[D];                 // N+(r), ascending
+  uint32_t spre[D + 64];         // start of non-empty segment s; ~0 beyond the last
+  ull sinfo[D];                  // (i << 32) | (doff[L[i]] - spre[s]) mod 2^32
+  uint32_t hkey[H];              // L[i], EMPTY_KEY when free
+  uint16_t hval[H];              // i
+  uint8_t flag[MAX_WAVES][64];   // per-wave segment-start marks
+  ull A[W * D];                  // A[w*D + j]: bits 64w..64w+63 of row j
+  ull rt;                        // cliques of the current root (summed over waves)
+  uint32_t root, nseg, E, raw;
 };
+
+__device__ __forceinline__ uint32_t hslot(uint32_t key, uint32_t hb) { return (key * 0x9E3779B1u) >> (32 - hb); }
 
 __device__ __forceinline__ ull above_mask(int q, uint32_t c) {
   const int cq = (int)(c >> 6), cb = (int)(c & 63);
@@ -46,47 +68,110 @@ __device__ __forceinline__ ull above_mask(int q, uint32_t c) {
   return cb == 63 ? 0ull : (~0ull << (cb + 1));
 }
 
+template <int W> __device__ __forceinline__ ull row(const Frame<W> &f, uint32_t j, int w) {
+  return f.A[(uint32_t)w * Frame<W>::D + j];
+}
+
+// Builds the frame of the root whose out-list is dnbr[base, base + d), d >= 2.
+// Every thread of the workgroup calls it; returns sum over u in N+(r) of
+// outdeg(u) (in every thread).
 template <int W>
-__device__ void build_rows(Frame<W> &f, const uint32_t *__restrict__ doff, const uint32_t *__restrict__ dnbr,
-                           uint32_t base, uint32_t d) {
-  const uint32_t lane = lane_id();
-  uint32_t carry = 0;
-  for (uint32_t t0 = 0; t0 < d; t0 += KCDS_WAVE) {
-    const uint32_t t = t0 + lane;
-    uint32_t du = 0, ou = 0;
-    if (t < d) {
-      uint32_t u = dnbr[base + t];
-      ou = doff[u];
-      du = doff[u + 1] - ou;
-      f.L[t] = u;
-    }
-    uint32_t incl = wave_incl_scan_u32(du);
-    if (t < d) {
-      f.pre[t + 1] = carry + incl;
-      f.st[t] = ou - (carry + incl - du);
-    }
-    carry += __shfl(incl, 63, 64);
+__device__ uint32_t build_rows(Frame<W> &f, const uint32_t *__restrict__ doff, const uint32_t *__restrict__ dnbr,
+                               uint32_t base, uint32_t d) {
+  constexpr uint32_t D = Frame<W>::D, H = Frame<W>::H, HB = Frame<W>::HB;
+  const uint32_t tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+  for (uint32_t t = tid; t < H; t += nt) f.hkey[t] = EMPTY_KEY;
+  for (uint32_t t = tid; t < D + 64; t += nt) f.spre[t] = 0xFFFFFFFFu;
+  // LDS starts with whatever the previous kernel left: every chunk relies on
+  // all marks being clear before it sets its own (it clears what it set)
+  for (uint32_t t = tid; t < MAX_WAVES * 64; t += nt) (&f.flag[0][0])[t] = 0;
+  for (uint32_t j = tid; j < d; j += nt) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) f.A[(uint32_t)w * D + j] = 0ull;
   }
-  if (lane == 0) f.pre[0] = 0;
-  for (uint32_t i = lane; i < d * W; i += KCDS_WAVE) f.A[i] = 0ull;
-  wave_sync();
-  const uint32_t E = carry;
-  const uint32_t last = f.L[d - 1];
-  for (uint32_t e0 = 0; e0 < E; e0 += KCDS_WAVE) {
-    const uint32_t e = e0 + lane;
-    if (e < E) {
-      const uint32_t i = upper_bound_u32(f.pre, d + 1, e) - 1;
-      const uint32_t w = dnbr[f.st[i] + e];
-      if (w <= last) {
-        const uint32_t j = i + 1 + lower_bound_u32(f.L + i + 1, d - i - 1, w);
-        if (j < d && f.L[j] == w) {
-          atomicOr(&f.A[i * W + (j >> 6)], 1ull << (j & 63));
-          atomicOr(&f.A[j * W + (i >> 6)], 1ull << (i & 63));
+  block_sync();
+  const uint32_t last = dnbr[base + d - 1];
+  if (wv == 0) {
+    uint32_t carry = 0, ns = 0, raw = 0;
+    for (uint32_t t0 = 0; t0 < d; t0 += KCDS_WAVE) {
+      const uint32_t t = t0 + lane;
+      uint32_t ou = 0, du = 0;
+      if (t < d) {
+        const uint32_t u = dnbr[base + t];
+        ou = doff[u];
+        du = doff[u + 1] - ou;
+        f.L[t] = u;
+        raw += du;
+        uint32_t h = hslot(u, HB);
+        while (atomicCAS(&f.hkey[h], EMPTY_KEY, u) != EMPTY_KEY) h = (h + 1) & (H - 1);
+        f.hval[h] = (uint16_t)t;
+      }
+      const uint32_t len = (t + 1 < d) ? du : 0u;  // L[d-1] has no later candidate
+      const ull ne = __ballot(len > 0);
+      const uint32_t incl = wave_incl_scan_u32(len);
+      if (len) {
+        const uint32_t s = ns + lanes_below(ne);
+        const uint32_t start = carry + incl - len;
+        f.spre[s] = start;
+        f.sinfo[s] = ((ull)t << 32) | (ull)(uint32_t)(ou - start);
+      }
+      ns += (uint32_t)__popcll(ne);
+      carry += __shfl(incl, 63, 64);
+    }
+    raw = wave_sum_u32(raw);
+    if (lane == 0) {
+      f.nseg = ns;
+      f.E = carry;
+      f.raw = raw;
+    }
+  }
+  block_sync();
+  const uint32_t E = f.E, ns = f.nseg, raw = f.raw;
+  // contiguous range of the concatenated sequence per wave, in whole chunks
+  const uint32_t per = ((E + nw * 64 - 1) / (nw * 64)) * 64;
+  const uint32_t eb = wv * per, ee = min(E, eb + per);
+  if (eb < ee) {
+    uint32_t s0 = upper_bound_u32(f.spre, ns, eb) - 1;  // spre[0] = 0 <= eb
+    for (uint32_t e0 = eb; e0 < ee; e0 += KCDS_WAVE) {
+      uint32_t x;
+      while (true) {  // advance s0 to the segment holding e0 (segments are non-empty)
+        x = f.spre[s0 + 1 + lane];
+        const ull b = __ballot(x <= e0);
+        s0 += (uint32_t)__popcll(b);
+        if (b != ~0ull) break;
+      }
+      x = f.spre[s0 + 1 + lane];  // starts after e0, ascending
+      if (x < e0 + 64) f.flag[wv][x - e0] = 1;
+      wave_sync();
+      const bool fl = f.flag[wv][lane] != 0;
+      const ull M = __ballot(fl);
+      if (fl) f.flag[wv][lane] = 0;
+      const uint32_t e = e0 + lane;
+      if (e < ee) {
+        const uint32_t s = s0 + (uint32_t)__popcll(M & ((2ull << lane) - 1ull));
+        const ull info = f.sinfo[s];
+        const uint32_t i = (uint32_t)(info >> 32);
+        const uint32_t w = dnbr[(uint32_t)info + e];
+        if (w <= last) {
+          uint32_t h = hslot(w, HB);
+          while (true) {
+            const uint32_t key = f.hkey[h];
+            if (key == w) {
+              const uint32_t j = f.hval[h];  // j > i: ids ascend along arcs and L is ascending
+              atomicOr(&f.A[(j >> 6) * D + i], 1ull << (j & 63));
+              atomicOr(&f.A[(i >> 6) * D + j], 1ull << (i & 63));
+              break;
+            }
+            if (key == EMPTY_KEY) break;
+            h = (h + 1) & (H - 1);
+          }
         }
       }
+      wave_sync();
     }
   }
-  wave_sync();
+  block_sync();
+  return raw;
 }
 
 // Credits E to local vertex c (owner lane c & 63, slot c >> 6).
@@ -114,7 +199,7 @@ template <int W, int REM> struct Count {
         uint32_t pc = 0;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-          S2[w] = S[w] & f.A[c * W + w] & above_mask(w, c);
+          S2[w] = S[w] & row(f, c, w) & above_mask(w, c);
           pc += (uint32_t)__popcll(S2[w]);
         }
         if (pc < (uint32_t)(REM - 1)) continue;
@@ -127,22 +212,31 @@ template <int W, int REM> struct Count {
   }
 };
 
+// Edges inside S, credited to both endpoints; rows q of S handled by this wave
+// are those with q % nw == wv (all rows when nw == 1).  Returns the wave's
+// share of 2 * edges.
+template <int W>
+__device__ __forceinline__ uint32_t edges_in(const Frame<W> &f, const ull *S, ull *cnt, uint32_t wv, uint32_t nw) {
+  const uint32_t lane = lane_id();
+  uint32_t xs = 0;
+#pragma unroll
+  for (int q = 0; q < W; ++q) {
+    if ((uint32_t)q % nw != wv) continue;
+    if ((S[q] >> lane) & 1ull) {
+      const uint32_t j = (uint32_t)q * 64 + lane;
+      uint32_t x = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) x += (uint32_t)__popcll(row(f, j, w) & S[w]);
+      cnt[q] += x;
+      xs += x;
+    }
+  }
+  return wave_sum_u32(xs);
+}
+
 template <int W> struct Count<W, 2> {
   static __device__ ull run(const Frame<W> &f, const ull *S, ull *cnt) {
-    const uint32_t lane = lane_id();
-    uint32_t xs = 0;
-#pragma unroll
-    for (int q = 0; q < W; ++q) {
-      if ((S[q] >> lane) & 1ull) {
-        const uint32_t j = (uint32_t)q * 64 + lane;
-        uint32_t x = 0;
-#pragma unroll
-        for (int w = 0; w < W; ++w) x += (uint32_t)__popcll(f.A[j * W + w] & S[w]);
-        cnt[q] += x;
-        xs += x;
-      }
-    }
-    return (ull)(wave_sum_u32(xs) >> 1);
+    return (ull)(edges_in<W>(f, S, cnt, 0, 1) >> 1);
   }
 };
 
@@ -161,7 +255,7 @@ template <int W, int K, int REM> struct Emit {
         uint32_t pc = 0;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-          S2[w] = S[w] & f.A[c * W + w] & above_mask(w, c);
+          S2[w] = S[w] & row(f, c, w) & above_mask(w, c);
           pc += (uint32_t)__popcll(S2[w]);
         }
         if (pc < (uint32_t)(REM - 1)) continue;
@@ -184,7 +278,7 @@ template <int W, int K> struct Emit<W, K, 2> {
       const bool in = (S[q] >> lane) & 1ull;
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        H[w] = in ? (f.A[j * W + w] & S[w] & above_mask(w, j)) : 0ull;
+        H[w] = in ? (row(f, j, w) & S[w] & above_mask(w, j)) : 0ull;
         x += (uint32_t)__popcll(H[w]);
       }
       const uint32_t incl = wave_incl_scan_u32(x);
@@ -199,11 +293,11 @@ template <int W, int K> struct Emit<W, K, 2> {
             const int bb = __builtin_ctzll(h);
             h &= h - 1;
             if (at >= lim) break;  // never taken when count and emit agree
-            uint32_t *row = out + at * K;
+            uint32_t *rw = out + at * K;
 #pragma unroll
-            for (int t = 0; t < K - 2; ++t) row[t] = pfx[t];
-            row[K - 2] = uj;
-            row[K - 1] = f.L[w * 64 + bb];
+            for (int t = 0; t < K - 2; ++t) rw[t] = pfx[t];
+            rw[K - 2] = uj;
+            rw[K - 1] = f.L[w * 64 + bb];
             ++at;
           }
         }
@@ -221,41 +315,74 @@ template <int W> __device__ __forceinline__ void all_bits(ull *S, uint32_t d) {
   }
 }
 
+// Takes the next root index from the queue (thread 0) and hands it to the
+// workgroup.
+template <int W> __device__ __forceinline__ uint32_t next_root(Frame<W> &f, uint32_t *qctr) {
+  if (threadIdx.x == 0) f.root = atomicAdd(qctr, 1u);
+  block_sync();
+  const uint32_t idx = f.root;
+  block_sync();  // every thread has read f.root before it is reused
+  return idx;
+}
+
 template <int W, int K>
-__global__ __launch_bounds__(64) void k_list_count(const uint32_t *__restrict__ doff,
-                                                   const uint32_t *__restrict__ dnbr,
-                                                   const uint32_t *__restrict__ roots, uint32_t nroots,
-                                                   uint32_t *qctr, ull *gcnt, ull *rootT, ull *gtotal) {
+__global__ __launch_bounds__(64 * MAX_WAVES) void k_list_count(const uint32_t *__restrict__ doff,
+                                                               const uint32_t *__restrict__ dnbr,
+                                                               const uint32_t *__restrict__ roots, uint32_t nroots,
+                                                               uint32_t *qctr, ull *gcnt, ull *rootT, ull *gtotal,
+                                                               ull *gw2) {
   __shared__ Frame<W> f;
-  const uint32_t lane = lane_id();
-  ull tot = 0;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  ull tot = 0, w2 = 0;
+  if (tid == 0) f.rt = 0;
   while (true) {
-    uint32_t idx = 0;
-    if (lane == 0) idx = atomicAdd(qctr, 1u);
-    idx = wave_bcast0(__shfl(idx, 0, 64));
+    const uint32_t idx = next_root(f, qctr);
     if (idx >= nroots) break;
     const uint32_t r = roots[idx];
     const uint32_t base = doff[r], d = doff[r + 1] - base;
-    build_rows<W>(f, doff, dnbr, base, d);
+    w2 += build_rows<W>(f, doff, dnbr, base, d);
     ull cnt[W];
 #pragma unroll
     for (int q = 0; q < W; ++q) cnt[q] = 0;
     ull S[W];
     all_bits<W>(S, d);
-    const ull rt = Count<W, K - 1>::run(f, S, cnt);
+    ull rt = 0;
+    if constexpr (K == 3) {
+      rt = edges_in<W>(f, S, cnt, wv, nw);  // twice the wave's share of edges
+    } else {
+      for (uint32_t c = wv; c < d; c += nw) {
+        ull S2[W];
+        uint32_t pc = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          S2[w] = row(f, c, w) & above_mask(w, c);
+          pc += (uint32_t)__popcll(S2[w]);
+        }
+        if (pc < (uint32_t)(K - 2)) continue;
+        const ull E = Count<W, K - 2>::run(f, S2, cnt);
+        credit<W>(cnt, c, E);
+        rt += E;
+      }
+    }
 #pragma unroll
     for (int q = 0; q < W; ++q) {
       const uint32_t j = (uint32_t)q * 64 + lane;
       if (j < d && cnt[q]) atomicAdd(&gcnt[f.L[j]], cnt[q]);
     }
-    if (lane == 0) {
-      if (rt) atomicAdd(&gcnt[r], rt);
-      rootT[r] = rt;
+    if (lane == 0 && rt) atomicAdd(&f.rt, rt);
+    block_sync();
+    if (tid == 0) {
+      const ull all = K == 3 ? (f.rt >> 1) : f.rt;
+      f.rt = 0;
+      if (all) atomicAdd(&gcnt[r], all);
+      rootT[r] = all;
+      tot += all;
     }
-    tot += rt;
-    wave_sync();
   }
-  if (lane == 0 && tot) atomicAdd(gtotal, tot);
+  if (tid == 0) {
+    if (tot) atomicAdd(gtotal, tot);
+    if (w2) atomicAdd(gw2, w2);
+  }
 }
 
 template <int W, int K>
@@ -264,11 +391,8 @@ __global__ __launch_bounds__(64) void k_list_emit(const uint32_t *__restrict__ d
                                                   const uint32_t *__restrict__ roots, uint32_t nroots,
                                                   uint32_t *qctr, const ull *rootOff, uint32_t *out) {
   __shared__ Frame<W> f;
-  const uint32_t lane = lane_id();
   while (true) {
-    uint32_t idx = 0;
-    if (lane == 0) idx = atomicAdd(qctr, 1u);
-    idx = wave_bcast0(__shfl(idx, 0, 64));
+    const uint32_t idx = next_root(f, qctr);
     if (idx >= nroots) break;
     const uint32_t r = roots[idx];
     ull pos = rootOff[r];
@@ -281,31 +405,69 @@ __global__ __launch_bounds__(64) void k_list_emit(const uint32_t *__restrict__ d
     uint32_t pfx[K];
     pfx[0] = r;
     Emit<W, K, K - 1>::run(f, S, pfx, out, pos, lim);
-    wave_sync();
   }
 }
 
 // Root classification: class c holds roots with 64*2^(c-1) < outdeg <= 64*2^c
-// and outdeg >= k-1 (smaller roots have no k-clique).
+// and outdeg >= k-1 (smaller roots have no k-clique); class 4 = over the limit.
 __device__ __forceinline__ int width_class(uint32_t d) {
   return d <= 64 ? 0 : d <= 128 ? 1 : d <= 256 ? 2 : d <= 512 ? 3 : 4;
 }
 
-__global__ void k_class_count(const uint32_t *doff, uint32_t n, uint32_t k, uint32_t *ccount) {
+__global__ __launch_bounds__(256) void k_class_count(const uint32_t *doff, uint32_t n, uint32_t k,
+                                                     uint32_t *ccount) {
+  __shared__ uint32_t sc[8];
+  if (threadIdx.x < 8) sc[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t mine[5] = {0, 0, 0, 0, 0};
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-    uint32_t d = doff[r + 1] - doff[r];
+    const uint32_t d = doff[r + 1] - doff[r];
     if (d + 1 < k) continue;
-    atomicAdd(&ccount[width_class(d)], 1u);
+    const int c = width_class(d);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) mine[i] += (c == i);
   }
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    const uint32_t v = wave_sum_u32(mine[i]);
+    if (lane_id() == 0 && v) atomicAdd(&sc[i], v);
+  }
+  __syncthreads();
+  if (threadIdx.x < 5 && sc[threadIdx.x]) atomicAdd(&ccount[threadIdx.x], sc[threadIdx.x]);
 }
 
-__global__ void k_class_fill(const uint32_t *doff, uint32_t n, uint32_t k, const uint32_t *cstart,
-                             uint32_t *cfill, uint32_t *roots) {
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-    uint32_t d = doff[r + 1] - doff[r];
-    if (d + 1 < k) continue;
-    int c = width_class(d);
-    roots[cstart[c] + atomicAdd(&cfill[c], 1u)] = r;
+// Writes each root into its class's range: one global atomic per class per
+// 256 roots.
+__global__ __launch_bounds__(256) void k_class_fill(const uint32_t *doff, uint32_t n, uint32_t k,
+                                                    const uint32_t *cstart, uint32_t *cfill, uint32_t *roots) {
+  __shared__ uint32_t wc[4][NCLASS], bbase[NCLASS];
+  const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+  for (uint32_t r0 = blockIdx.x * 256; r0 < n; r0 += gridDim.x * 256) {
+    const uint32_t r = r0 + threadIdx.x;
+    int c = -1;
+    if (r < n) {
+      const uint32_t d = doff[r + 1] - doff[r];
+      if (d + 1 >= k) c = width_class(d);
+    }
+    ull bal[NCLASS];
+#pragma unroll
+    for (int i = 0; i < NCLASS; ++i) {
+      bal[i] = __ballot(c == i);
+      if (lane == 0) wc[wv][i] = (uint32_t)__popcll(bal[i]);
+    }
+    __syncthreads();
+    if (threadIdx.x < NCLASS) {
+      const uint32_t i = threadIdx.x;
+      const uint32_t tot = wc[0][i] + wc[1][i] + wc[2][i] + wc[3][i];
+      bbase[i] = tot ? cstart[i] + atomicAdd(&cfill[i], tot) : 0u;
+    }
+    __syncthreads();
+    if (c >= 0 && c < NCLASS) {
+      uint32_t at = bbase[c] + lanes_below(bal[c]);
+      for (uint32_t v = 0; v < wv; ++v) at += wc[v][c];
+      roots[at] = r;
+    }
+    __syncthreads();
   }
 }
 
@@ -323,7 +485,7 @@ __global__ void k_incidence(const uint32_t *cl, uint64_t T, uint32_t k, const ul
 }
 
 typedef void (*CountFn)(const uint32_t *, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, ull *, ull *,
-                        ull *);
+                        ull *, ull *);
 typedef void (*EmitFn)(const uint32_t *, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, const ull *,
                        uint32_t *);
 
@@ -351,7 +513,13 @@ template <int W> EmitFn emit_fn(uint32_t k) {
   return nullptr;
 }
 
-const uint32_t kWaveDefault[NCLASS] = {16, 12, 8, 4};
+CountFn class_count_fn(int cls, uint32_t k) {
+  return cls == 0 ? count_fn<1>(k) : cls == 1 ? count_fn<2>(k) : cls == 2 ? count_fn<4>(k) : count_fn<8>(k);
+}
+
+// Waves per root by class when not tuned: heavy classes split each root.
+const uint32_t kWavesPerRoot[NCLASS] = {1, 2, 4, 8};
+constexpr uint32_t WAVE_SLOTS_PER_CU = 32;
 
 uint32_t lds_bytes(int cls) {
   switch (cls) {
@@ -362,6 +530,14 @@ uint32_t lds_bytes(int cls) {
   }
 }
 
+
+uint32_t wg_grid(const Ctx &c, int cls, uint32_t nroots, uint32_t nw) {
+  uint32_t slots = c.list_waves_per_cu ? c.list_waves_per_cu : WAVE_SLOTS_PER_CU;
+  uint32_t per_cu = std::max<uint32_t>(1, slots / nw);
+  per_cu = std::min<uint32_t>(per_cu, std::max<uint32_t>(1, (160u * 1024u) / lds_bytes(cls)));
+  return std::max<uint32_t>(1, std::min<uint32_t>(nroots, (uint32_t)c.num_cus * per_cu));
+}
+
 void classify_roots(Ctx &c, uint32_t k) {
   const uint32_t n = c.n;
   buf_reserve(c, c.roots, sizeof(uint32_t) * ((size_t)n + 1));
@@ -369,7 +545,7 @@ void classify_roots(Ctx &c, uint32_t k) {
   uint32_t *q = c.qctr.as<uint32_t>();
   HIP_CHECK(hipMemsetAsync(q, 0, 64 * sizeof(uint32_t), c.stream));
   uint32_t *ccount = q + 16, *cstart = q + 24, *cfill = q + 32;
-  uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((n + 255) / 256, 4096));
+  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((n + 255) / 256, 256));
   if (n) {
     k_class_count<<<grid, 256, 0, c.stream>>>(c.doff.as<uint32_t>(), n, k, ccount);
     KCDS_LAUNCH_CHECK();
@@ -394,14 +570,69 @@ void classify_roots(Ctx &c, uint32_t k) {
   c.sync();  // hs lives on the host stack
 }
 
-uint32_t wave_grid(Ctx &c, int cls, uint32_t nroots) {
-  uint32_t per_cu = c.list_waves_per_cu ? c.list_waves_per_cu : kWaveDefault[cls];
-  uint32_t lds_per_cu = 160u * 1024u;
-  per_cu = std::min<uint32_t>(per_cu, std::max<uint32_t>(1, lds_per_cu / lds_bytes(cls)));
-  return std::max<uint32_t>(1, std::min<uint32_t>(nroots, (uint32_t)c.num_cus * per_cu));
+// One launch of class cls over its roots; counters accumulate into gcnt,
+// rootT, total, w2.
+void launch_class(Ctx &c, int cls, uint32_t k, uint32_t nw, uint32_t *qslot, ull *gcnt, ull *rootT, ull *total,
+                  ull *w2) {
+  const uint32_t nr = c.class_count[cls];
+  const uint32_t *roots = c.roots.as<uint32_t>() + c.class_start[cls];
+  hipLaunchKernelGGL(class_count_fn(cls, k), dim3(wg_grid(c, cls, nr, nw)), dim3(64 * nw), 0, c.stream,
+                     c.doff.as<uint32_t>(), c.dnbr.as<uint32_t>(), roots, nr, qslot, gcnt, rootT, total, w2);
+  KCDS_LAUNCH_CHECK();
 }
 
 }  // namespace
+
+uint32_t list_waves_per_root(const Ctx &c, int cls, uint32_t k) {
+  uint32_t nw = c.list_waves_per_root[cls] ? c.list_waves_per_root[cls] : kWavesPerRoot[cls];
+  nw = std::min<uint32_t>(std::max<uint32_t>(nw, 1), MAX_WAVES);
+  if (k == 3) nw = std::min<uint32_t>(nw, 1u << cls);  // K = 3 splits rows by 64-row word
+  return nw;
+}
+
+// Picks the waves per root of every class by timing each candidate on the
+// current graph (results of the trial runs go to scratch buffers).
+void list_autotune(Ctx &c, uint32_t k) {
+  if (!c.oriented) fail(KCDS_E_STATE, "autotune before kcds_orient");
+  if (k < 3 || k > 8) fail(KCDS_E_INVALID, "k = %u outside 3..8", k);
+  const uint32_t n = c.n;
+  classify_roots(c, k);
+  buf_reserve(c, c.tmp2, sizeof(ull) * ((size_t)n + 2));
+  buf_reserve(c, c.tmp3, sizeof(ull) * ((size_t)n + 2));
+  buf_reserve(c, c.scal, 64 * sizeof(ull));
+  ull *gcnt = c.tmp2.as<ull>(), *rootT = c.tmp3.as<ull>(), *sc = c.scal.as<ull>() + 40;
+  uint32_t *q = c.qctr.as<uint32_t>() + 48;
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  for (int cls = 0; cls < NCLASS; ++cls) {
+    if (!c.class_count[cls]) continue;
+    float best = 1e30f;
+    uint32_t best_nw = kWavesPerRoot[cls];
+    for (uint32_t nw = 1; nw <= MAX_WAVES; nw *= 2) {
+      if (k == 3 && nw > (1u << cls)) break;
+      float tsum = 0;
+      for (int rep = 0; rep < 2; ++rep) {
+        HIP_CHECK(hipMemsetAsync(q, 0, sizeof(uint32_t), c.stream));
+        HIP_CHECK(hipEventRecord(e0, c.stream));
+        launch_class(c, cls, k, nw, q, gcnt, rootT, sc, sc + 1);
+        HIP_CHECK(hipEventRecord(e1, c.stream));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (rep) tsum += ms;  // first run warms caches
+      }
+      if (tsum < best) {
+        best = tsum;
+        best_nw = nw;
+      }
+    }
+    c.list_waves_per_root[cls] = best_nw;
+    c.tune_ms[cls] = best;
+  }
+  HIP_CHECK(hipEventDestroy(e0));
+  HIP_CHECK(hipEventDestroy(e1));
+}
 
 void list_count(Ctx &c, uint32_t k) {
   if (!c.oriented) fail(KCDS_E_STATE, "kcds_count/kcds_list before kcds_orient");
@@ -414,7 +645,7 @@ void list_count(Ctx &c, uint32_t k) {
   HIP_CHECK(hipMemsetAsync(c.cnt.p, 0, sizeof(ull) * ((size_t)n + 1), c.stream));
   HIP_CHECK(hipMemsetAsync(c.rootT.p, 0, sizeof(ull) * ((size_t)n + 2), c.stream));
   ull *total = c.scal.as<ull>();
-  HIP_CHECK(hipMemsetAsync(total, 0, sizeof(ull), c.stream));
+  HIP_CHECK(hipMemsetAsync(total, 0, 2 * sizeof(ull), c.stream));
   uint32_t *q = c.qctr.as<uint32_t>();
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
@@ -422,18 +653,14 @@ void list_count(Ctx &c, uint32_t k) {
   HIP_CHECK(hipEventRecord(e0, c.stream));
   uint32_t launches = 0;
   for (int cls = NCLASS - 1; cls >= 0; --cls) {
-    const uint32_t nr = c.class_count[cls];
-    if (!nr) continue;
-    CountFn fn = cls == 0 ? count_fn<1>(k) : cls == 1 ? count_fn<2>(k) : cls == 2 ? count_fn<4>(k) : count_fn<8>(k);
-    const uint32_t *roots = c.roots.as<uint32_t>() + c.class_start[cls];
-    hipLaunchKernelGGL(fn, dim3(wave_grid(c, cls, nr)), dim3(64), 0, c.stream, c.doff.as<uint32_t>(),
-                       c.dnbr.as<uint32_t>(), roots, nr, q + cls, c.cnt.as<ull>(), c.rootT.as<ull>(), total);
-    KCDS_LAUNCH_CHECK();
+    if (!c.class_count[cls]) continue;
+    launch_class(c, cls, k, list_waves_per_root(c, cls, k), q + cls, c.cnt.as<ull>(), c.rootT.as<ull>(), total,
+                 total + 1);
     ++launches;
   }
   HIP_CHECK(hipEventRecord(e1, c.stream));
-  ull *h = (ull *)c.host_scratch(8);
-  HIP_CHECK(hipMemcpyAsync(h, total, sizeof(ull), hipMemcpyDeviceToHost, c.stream));
+  ull *h = (ull *)c.host_scratch(16);
+  HIP_CHECK(hipMemcpyAsync(h, total, 2 * sizeof(ull), hipMemcpyDeviceToHost, c.stream));
   c.sync();
   float ms = 0;
   HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
@@ -441,7 +668,8 @@ void list_count(Ctx &c, uint32_t k) {
   HIP_CHECK(hipEventDestroy(e1));
   c.list_kernel_ms = ms;
   c.list_kernel_launches = launches;
-  c.T = *h;
+  c.T = h[0];
+  c.w2 = h[1];
   c.k = k;
   c.counted = true;
   c.listed = false;
@@ -470,7 +698,7 @@ void list_materialize(Ctx &c, uint32_t k) {
     if (!nr) continue;
     EmitFn fn = cls == 0 ? emit_fn<1>(k) : cls == 1 ? emit_fn<2>(k) : cls == 2 ? emit_fn<4>(k) : emit_fn<8>(k);
     const uint32_t *roots = c.roots.as<uint32_t>() + c.class_start[cls];
-    hipLaunchKernelGGL(fn, dim3(wave_grid(c, cls, nr)), dim3(64), 0, c.stream, c.doff.as<uint32_t>(),
+    hipLaunchKernelGGL(fn, dim3(wg_grid(c, cls, nr, 1)), dim3(64), 0, c.stream, c.doff.as<uint32_t>(),
                        c.dnbr.as<uint32_t>(), roots, nr, q + cls, rootOff, c.cliques.as<uint32_t>());
     KCDS_LAUNCH_CHECK();
   }

@@ -21,7 +21,7 @@ namespace kcds {
 
 namespace {
 
-constexpr uint32_t SORT_LDS_MAX = 8192;  // out-lists sorted in LDS (32 KB)
+constexpr uint32_t SORT_WAVE = 1024;  // longest out-list a wave sorts in LDS
 
 __global__ void k_rank_from_lo(const uint32_t *lo, uint32_t n, uint32_t *rank, uint32_t *inv) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
@@ -31,161 +31,169 @@ __global__ void k_rank_from_lo(const uint32_t *lo, uint32_t n, uint32_t *rank, u
   }
 }
 
-// wave per vertex: out-degree in the DAG, indexed by new id
-__global__ __launch_bounds__(256) void k_outdeg(const uint32_t *off, const uint32_t *nbr,
-                                                const uint32_t *rank, uint32_t n, uint32_t *dout) {
-  const uint32_t lane = lane_id();
-  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KCDS_WAVE;
-  const uint32_t nw = gridDim.x * blockDim.x / KCDS_WAVE;
-  for (uint32_t v = wid; v < n; v += nw) {
-    uint32_t b = off[v], e = off[v + 1], r = rank[v];
-    uint32_t c = 0;
-    for (uint32_t i = b + lane; i < e; i += KCDS_WAVE) c += rank[nbr[i]] > r;
-    c = wave_sum_u32(c);
-    if (lane == 0) dout[r] = c;
-  }
-}
-
-// wave per vertex: gather the out-list into LDS, sort it by rank counting
-// (position = number of smaller entries; entries are distinct), write it.
-__global__ __launch_bounds__(64) void k_dag_fill(const uint32_t *off, const uint32_t *nbr,
-                                                 const uint32_t *rank, uint32_t n, const uint32_t *doff,
-                                                 uint32_t *dnbr) {
-  __shared__ uint32_t buf[SORT_LDS_MAX];
-  const uint32_t lane = lane_id();
-  for (uint32_t v = blockIdx.x; v < n; v += gridDim.x) {
-    uint32_t b = off[v], e = off[v + 1], r = rank[v];
-    uint32_t ob = doff[r], d = doff[r + 1] - ob;
-    if (d == 0) continue;
+// Wave per vertex v (4 waves per block, 4 KB of LDS each): gathers the new
+// ids of its later neighbours, sorts them by rank counting (position = number
+// of smaller entries; entries are distinct) and writes them to
+// tmp[off[v] ..), the start of v's undirected list; dout[rank[v]] = count.
+// A list longer than SORT_WAVE is counted but not written (the caller fails).
+__global__ __launch_bounds__(256) void k_dag_gather(const uint32_t *__restrict__ off,
+                                                    const uint32_t *__restrict__ nbr,
+                                                    const uint32_t *__restrict__ rank, uint32_t n,
+                                                    uint32_t *__restrict__ dout, uint32_t *__restrict__ tmp) {
+  __shared__ uint32_t sbuf[4][SORT_WAVE];
+  const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+  uint32_t *buf = sbuf[wv];
+  const uint32_t gw = blockIdx.x * 4 + wv, nwv = gridDim.x * 4;
+  for (uint32_t v = gw; v < n; v += nwv) {
+    const uint32_t b = off[v], e = off[v + 1], r = rank[v];
     uint32_t cnt = 0;
     for (uint32_t i0 = b; i0 < e; i0 += KCDS_WAVE) {
-      uint32_t i = i0 + lane;
-      uint32_t x = (i < e) ? rank[nbr[i]] : 0u;
-      bool keep = (i < e) && x > r;
-      ull bal = __ballot(keep);
-      if (keep) buf[cnt + lanes_below(bal)] = x;
+      const uint32_t i = i0 + lane;
+      const uint32_t x = (i < e) ? rank[nbr[i]] : 0u;
+      const bool keep = (i < e) && x > r;
+      const ull bal = __ballot(keep);
+      if (keep && cnt + lanes_below(bal) < SORT_WAVE) buf[cnt + lanes_below(bal)] = x;
       cnt += (uint32_t)__popcll(bal);
     }
+    if (lane == 0) dout[r] = cnt;
     wave_sync();
-    for (uint32_t i = lane; i < d; i += KCDS_WAVE) {
-      uint32_t x = buf[i], pos = 0;
-      for (uint32_t j = 0; j < d; ++j) {
-        uint32_t y = buf[j];
-        pos += (y < x) || (y == x && j < i);  // stable: equal entries keep their order
+    if (cnt <= SORT_WAVE) {
+      for (uint32_t i = lane; i < cnt; i += KCDS_WAVE) {
+        const uint32_t x = buf[i];
+        uint32_t pos = 0;
+        for (uint32_t j = 0; j < cnt; ++j) pos += buf[j] < x;
+        tmp[b + pos] = x;
       }
-      dnbr[ob + pos] = x;
     }
     wave_sync();
   }
 }
 
-// out-wedge sum W2 = sum over arcs (r,u) of outdeg(u), and a per-root work
-// estimate (used by the listing kernels to order roots)
-__global__ void k_w2(const uint32_t *doff, const uint32_t *dnbr, uint64_t arcs, ull *w2) {
-  ull s = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < arcs;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t u = dnbr[i];
-    s += doff[u + 1] - doff[u];
+// Wave per new id r: copies the sorted out-list of v = inv[r] into the DAG.
+__global__ __launch_bounds__(256) void k_dag_copy(const uint32_t *__restrict__ off, const uint32_t *__restrict__ inv,
+                                                  uint32_t n, const uint32_t *__restrict__ doff,
+                                                  const uint32_t *__restrict__ tmp, uint32_t *__restrict__ dnbr) {
+  const uint32_t lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwv = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t r = gw; r < n; r += nwv) {
+    const uint32_t src = off[inv[r]], dst = doff[r], d = doff[r + 1] - dst;
+    for (uint32_t i = lane; i < d; i += KCDS_WAVE) dnbr[dst + i] = tmp[src + i];
   }
-  s = wave_sum_u64(s);
-  if (lane_id() == 0) atomicAdd(w2, s);
 }
 
 // ---------------------------------------------------------------------------
-// GPU degeneracy order: level-synchronous k-core peel as a fixed sequence of
-// step kernels (min / open / collect / rank / decrement / finish) driven by
-// device-side control state, enqueued by the host in batches.  Level L
-// removes every live vertex of degree <= L in sub-rounds; each sub-round's
-// vertices take the next ranks (in frontier order), then decrement their live
-// neighbours; a neighbour whose degree falls from L+1 to L joins the next
-// sub-round.  Empty levels are skipped by a min-reduction over live degrees.
-// Kernel boundaries order the steps; no grid-wide barrier is needed.
+// GPU degeneracy order: level-synchronous k-core peel as a fixed pair of
+// kernels per sub-round (work / control) driven by device-side control
+// state and enqueued by the host in batches.
+//   OPEN    collect every live vertex of degree <= target into the frontier
+//           and take the minimum degree of the others; the control step
+//           either starts level = target or, when nothing was collected,
+//           retries with target = that minimum (empty levels cost one pass).
+//   PROCESS the frontier takes the next ranks (in frontier order) and leaves
+//           the graph, decrementing its live neighbours; a neighbour whose
+//           degree falls from L+1 to L joins the next sub-round of level L.
+// A vertex leaves by having its degree replaced with DEAD; decrements that
+// race with that replacement keep the value >= DEAD_MIN, which every reader
+// treats as dead.  Kernel boundaries order the steps; no grid-wide barrier
+// is needed.
 
 struct OrderCtl {
   uint32_t level;
+  uint32_t target;
   uint32_t minval;
   uint32_t next_rank;
   uint32_t fsize[2];
   uint32_t cur;
   uint32_t done;
-  uint32_t opening;
+  uint32_t phase;  // 0 = OPEN, 1 = PROCESS
 };
 
 constexpr uint32_t DEAD = 0xFFFFFFFFu;
+constexpr uint32_t DEAD_MIN = 0x80000000u;
 
-__global__ void k_ko_min(OrderCtl *ctl, uint32_t n, const uint32_t *deg) {
-  if (ctl->done || ctl->fsize[ctl->cur] != 0) return;
-  uint32_t m = DEAD;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) m = min(m, deg[v]);
-  m = wave_min_u32(m);
-  if (lane_id() == 0 && m != DEAD) atomicMin(&ctl->minval, m);
+// Appends v (where want) to q with one counter atomic per wave.
+__device__ __forceinline__ void wave_append(bool want, uint32_t v, uint32_t *q, uint32_t *qsize) {
+  const ull bal = __ballot(want);
+  if (!bal) return;
+  uint32_t base = 0;
+  if (lane_id() == (uint32_t)__builtin_ctzll(bal)) base = atomicAdd(qsize, (uint32_t)__popcll(bal));
+  base = __shfl(base, __builtin_ctzll(bal), 64);
+  if (want) q[base + lanes_below(bal)] = v;
 }
 
-__global__ void k_ko_open(OrderCtl *ctl) {
-  if (ctl->done || ctl->fsize[ctl->cur] != 0) return;
-  if (ctl->minval == DEAD) {
-    ctl->done = 1;
-    return;
-  }
-  ctl->level = max(ctl->level, ctl->minval);
-  ctl->minval = DEAD;
-  ctl->opening = 1;
-}
-
-__global__ void k_ko_collect(OrderCtl *ctl, uint32_t n, const uint32_t *deg, uint32_t *fr0, uint32_t *fr1) {
-  if (ctl->done || !ctl->opening) return;
-  const uint32_t L = ctl->level, cur = ctl->cur;
+__global__ __launch_bounds__(256) void k_ko_work(OrderCtl *ctl, uint32_t n, const uint32_t *__restrict__ off,
+                                                 const uint32_t *__restrict__ nbr, uint32_t *deg, uint32_t *fr0,
+                                                 uint32_t *fr1, uint32_t *rank, uint32_t *inv) {
+  if (ctl->done) return;
+  const uint32_t cur = ctl->cur;
   uint32_t *fr = cur ? fr1 : fr0;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    const uint32_t d = deg[v];
-    if (d != DEAD && d <= L) fr[atomicAdd(&ctl->fsize[cur], 1u)] = v;
-  }
-}
-
-// the frontier takes the next ranks and leaves the graph
-__global__ void k_ko_rank(OrderCtl *ctl, const uint32_t *fr0, const uint32_t *fr1, uint32_t *deg, uint32_t *rank,
-                          uint32_t *inv) {
-  if (ctl->done) return;
-  const uint32_t cur = ctl->cur, fs = ctl->fsize[cur], base = ctl->next_rank;
-  const uint32_t *fr = cur ? fr1 : fr0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < fs; i += gridDim.x * blockDim.x) {
-    const uint32_t v = fr[i];
-    rank[v] = base + i;
-    inv[base + i] = v;
-    deg[v] = DEAD;
-  }
-}
-
-// wave per removed vertex: decrement live neighbours
-__global__ void k_ko_decrement(OrderCtl *ctl, const uint32_t *off, const uint32_t *nbr, uint32_t *deg,
-                               uint32_t *fr0, uint32_t *fr1) {
-  if (ctl->done) return;
-  const uint32_t cur = ctl->cur, fs = ctl->fsize[cur], L = ctl->level;
-  const uint32_t *fr = cur ? fr1 : fr0;
   uint32_t *nx = cur ? fr0 : fr1;
   const uint32_t lane = lane_id();
+  if (ctl->phase == 0) {
+    const uint32_t target = ctl->target;
+    uint32_t m = DEAD;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < n; v0 += stride) {
+      const uint32_t v = v0 + threadIdx.x;
+      const uint32_t d = v < n ? deg[v] : DEAD;
+      const bool live = d < DEAD_MIN;
+      wave_append(live && d <= target, v, fr, &ctl->fsize[cur]);
+      if (live && d > target) m = min(m, d);
+    }
+    m = wave_min_u32(m);
+    if (lane == 0 && m != DEAD) atomicMin(&ctl->minval, m);
+    return;
+  }
+  const uint32_t fs = ctl->fsize[cur], L = ctl->level, base = ctl->next_rank;
   const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KCDS_WAVE;
   const uint32_t nw = gridDim.x * blockDim.x / KCDS_WAVE;
   for (uint32_t i = wid; i < fs; i += nw) {
     const uint32_t v = fr[i];
-    for (uint32_t j = off[v] + lane; j < off[v + 1]; j += KCDS_WAVE) {
-      const uint32_t u = nbr[j];
-      if (__hip_atomic_load(&deg[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == DEAD) continue;
-      const uint32_t old = atomicSub(&deg[u], 1u);
-      if (old == L + 1) nx[atomicAdd(&ctl->fsize[cur ^ 1], 1u)] = u;
+    if (lane == 0) {
+      rank[v] = base + i;
+      inv[base + i] = v;
+      atomicExch(&deg[v], DEAD);
+    }
+    const uint32_t b = off[v], e = off[v + 1];
+    for (uint32_t j0 = b; j0 < e; j0 += KCDS_WAVE) {
+      const uint32_t j = j0 + lane;
+      bool join = false;
+      uint32_t u = 0;
+      if (j < e) {
+        u = nbr[j];
+        if (__hip_atomic_load(&deg[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < DEAD_MIN)
+          join = atomicSub(&deg[u], 1u) == L + 1;
+      }
+      wave_append(join, u, nx, &ctl->fsize[cur ^ 1]);
     }
   }
 }
 
-__global__ void k_ko_finish(OrderCtl *ctl, uint32_t n) {
+__global__ void k_ko_ctl(OrderCtl *ctl, uint32_t n) {
   if (ctl->done) return;
   const uint32_t cur = ctl->cur;
-  ctl->next_rank += ctl->fsize[cur];
-  ctl->fsize[cur] = 0;
-  ctl->opening = 0;
-  ctl->cur = cur ^ 1;
-  if (ctl->next_rank >= n) ctl->done = 1;
+  if (ctl->phase == 1) {
+    ctl->next_rank += ctl->fsize[cur];
+    ctl->fsize[cur] = 0;
+    ctl->cur = cur ^ 1;
+    if (ctl->next_rank >= n) {
+      ctl->done = 1;
+    } else if (ctl->fsize[cur ^ 1] == 0) {
+      ctl->phase = 0;
+      ctl->target = ctl->level + 1;
+      ctl->minval = DEAD;
+    }
+    return;
+  }
+  if (ctl->fsize[cur] > 0) {
+    ctl->level = ctl->target;
+    ctl->phase = 1;
+  } else if (ctl->minval == DEAD) {
+    ctl->done = 1;
+  } else {
+    ctl->target = ctl->minval;
+    ctl->minval = DEAD;
+  }
 }
 
 __global__ void k_init_deg(const uint32_t *off, uint32_t n, uint32_t *deg) {
@@ -213,38 +221,38 @@ void degeneracy_rank_gpu(Ctx &c) {
   const uint32_t *off = c.off.as<uint32_t>(), *nbr = c.nbr.as<uint32_t>();
   uint32_t *rank = c.rank.as<uint32_t>(), *inv = c.inv.as<uint32_t>();
   uint32_t *hdone = (uint32_t *)c.host_scratch(64);
-  uint32_t batch = 8;
+  // the number of steps is a property of the graph: start from the last count
+  uint32_t batch = c.order_steps ? (uint32_t)std::min<uint64_t>(c.order_steps, 1u << 20) : 32;
   uint64_t rounds = 0;
   while (true) {
     for (uint32_t r = 0; r < batch; ++r) {
-      k_ko_min<<<vgrid, 256, 0, c.stream>>>(ctl, n, deg);
-      k_ko_open<<<1, 1, 0, c.stream>>>(ctl);
-      k_ko_collect<<<vgrid, 256, 0, c.stream>>>(ctl, n, deg, fr0, fr1);
-      k_ko_rank<<<vgrid, 256, 0, c.stream>>>(ctl, fr0, fr1, deg, rank, inv);
-      k_ko_decrement<<<vgrid, 256, 0, c.stream>>>(ctl, off, nbr, deg, fr0, fr1);
-      k_ko_finish<<<1, 1, 0, c.stream>>>(ctl, n);
+      k_ko_work<<<vgrid, 256, 0, c.stream>>>(ctl, n, off, nbr, deg, fr0, fr1, rank, inv);
+      k_ko_ctl<<<1, 1, 0, c.stream>>>(ctl, n);
       KCDS_LAUNCH_CHECK();
     }
     rounds += batch;
     HIP_CHECK(hipMemcpyAsync(hdone, &ctl->done, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
     c.sync();
     if (*hdone) break;
-    // every sub-round ranks at least one vertex
-    if (rounds > (uint64_t)n + 1) fail(KCDS_E_HIP, "ordering did not finish within %llu sub-rounds",
-                                       (unsigned long long)rounds);
-    batch = std::min<uint32_t>(batch * 2, 512);
+    // every PROCESS step ranks at least one vertex; an OPEN step is followed
+    // by a PROCESS step or by one OPEN step that succeeds
+    if (rounds > 3ull * n + 8) fail(KCDS_E_HIP, "ordering did not finish within %llu sub-rounds",
+                                    (unsigned long long)rounds);
+    batch = std::min<uint32_t>(std::max<uint32_t>(batch, 32) * 2, 128);
   }
+  c.order_steps = rounds;
 }
 
 void orient_from_rank(Ctx &c) {
   const uint32_t n = c.n;
   buf_reserve(c, c.doff, sizeof(uint32_t) * ((size_t)n + 1));
   buf_reserve(c, c.tmp1, sizeof(uint32_t) * ((size_t)n + 1));
-  uint32_t *dout = c.tmp1.as<uint32_t>();
+  buf_reserve(c, c.tmp0, sizeof(uint32_t) * ((size_t)c.nnz + 1));
+  uint32_t *dout = c.tmp1.as<uint32_t>(), *tmp = c.tmp0.as<uint32_t>();
   if (n) {
-    uint32_t grid = std::min<uint32_t>((n + 3) / 4, 65535u * 8);
-    k_outdeg<<<grid, 256, 0, c.stream>>>(c.off.as<uint32_t>(), c.nbr.as<uint32_t>(), c.rank.as<uint32_t>(), n,
-                                         dout);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 3) / 4, (uint64_t)c.num_cus * 40);
+    k_dag_gather<<<grid, 256, 0, c.stream>>>(c.off.as<uint32_t>(), c.nbr.as<uint32_t>(), c.rank.as<uint32_t>(), n,
+                                             dout, tmp);
     KCDS_LAUNCH_CHECK();
   }
   scan_u32_to_u32(c, dout, c.doff.as<uint32_t>(), n);
@@ -258,27 +266,16 @@ void orient_from_rank(Ctx &c) {
   c.sync();
   c.darcs = h[0];
   c.dmax = h[1];
-  if (c.dmax > SORT_LDS_MAX)
-    fail(KCDS_E_LIMIT, "DAG out-degree %u exceeds %u (orientation not degeneracy-bounded)", c.dmax,
-         SORT_LDS_MAX);
+  if (c.dmax > SORT_WAVE)
+    fail(KCDS_E_LIMIT, "DAG out-degree %u exceeds %u (orientation not degeneracy-bounded)", c.dmax, SORT_WAVE);
   buf_reserve(c, c.dnbr, sizeof(uint32_t) * (c.darcs + 1));
   if (n) {
-    uint32_t grid = std::min<uint32_t>(n, 65535u * 4);
-    k_dag_fill<<<grid, 64, 0, c.stream>>>(c.off.as<uint32_t>(), c.nbr.as<uint32_t>(), c.rank.as<uint32_t>(), n,
-                                          c.doff.as<uint32_t>(), c.dnbr.as<uint32_t>());
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 3) / 4, (uint64_t)c.num_cus * 40);
+    k_dag_copy<<<grid, 256, 0, c.stream>>>(c.off.as<uint32_t>(), c.inv.as<uint32_t>(), n, c.doff.as<uint32_t>(),
+                                           tmp, c.dnbr.as<uint32_t>());
     KCDS_LAUNCH_CHECK();
   }
-  ull *w2d = c.scal.as<ull>() + 59;
-  fill_u64(c, w2d, 0, 1);
-  if (c.darcs) {
-    uint32_t grid = (uint32_t)std::min<uint64_t>((c.darcs + 255) / 256, 65535u * 4);
-    k_w2<<<grid, 256, 0, c.stream>>>(c.doff.as<uint32_t>(), c.dnbr.as<uint32_t>(), c.darcs, w2d);
-    KCDS_LAUNCH_CHECK();
-  }
-  ull *hw = (ull *)c.host_scratch(8);
-  HIP_CHECK(hipMemcpyAsync(hw, w2d, 8, hipMemcpyDeviceToHost, c.stream));
-  c.sync();
-  c.w2 = *hw;
+  c.w2 = 0;  // the listing pass reports the out-wedge sum it stages
 }
 
 void rank_from_listing_order(Ctx &c, const uint32_t *lo_dev) {

@@ -338,6 +338,30 @@ int kcds_get_list_info(kcds_ctx *ctx, kcds_list_info *info) {
     info->max_outdeg = c.dmax;
     info->kernel_ms = c.list_kernel_ms;
     info->kernel_launches = c.list_kernel_launches;
+    for (int i = 0; i < 4; ++i) {
+      info->class_roots[i] = c.class_count[i];
+      info->waves_per_root[i] = list_waves_per_root(c, i, c.k ? c.k : 4);
+    }
+  });
+}
+
+int kcds_set_list_shape(kcds_ctx *ctx, const uint32_t *waves_per_root) {
+  return guarded(ctx, [&](Ctx &c) {
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t w = waves_per_root ? waves_per_root[i] : 0;
+      if (w > 8) fail(KCDS_E_INVALID, "waves per root %u above 8", w);
+      c.list_waves_per_root[i] = w;
+    }
+  });
+}
+
+int kcds_autotune_list(kcds_ctx *ctx, uint32_t k, uint32_t *waves_per_root, float *class_ms) {
+  return guarded(ctx, [&](Ctx &c) {
+    list_autotune(c, k);
+    for (int i = 0; i < 4; ++i) {
+      if (waves_per_root) waves_per_root[i] = list_waves_per_root(c, i, k);
+      if (class_ms) class_ms[i] = c.tune_ms[i];
+    }
   });
 }
 

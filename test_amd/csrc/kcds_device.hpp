@@ -21,6 +21,13 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Workgroup barrier that also waits for this wave's outstanding memory
+// operations, so LDS and global writes made before it are complete.
+__device__ __forceinline__ void block_sync() {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+}
+
 __device__ __forceinline__ uint32_t wave_bcast0(uint32_t v) {
   return __builtin_amdgcn_readfirstlane(v);
 }

@@ -93,6 +93,9 @@ struct Ctx {
   std::string err;
   uint32_t list_waves_per_cu = 0;
   uint32_t tune_flags = 0;
+  uint32_t list_waves_per_root[4] = {0, 0, 0, 0};  // 0 = default of the class
+  float tune_ms[4] = {0, 0, 0, 0};
+  uint64_t order_steps = 0;  // work/control steps of the last GPU ordering
 
   // pinned host staging for small readbacks
   void *pinned = nullptr;
@@ -176,6 +179,8 @@ void degeneracy_rank_gpu(Ctx &c);    // fills rank/inv from a GPU k-core peel
 
 // ---- listing (k_list.hip) ----
 void list_count(Ctx &c, uint32_t k);     // cnt, T
+void list_autotune(Ctx &c, uint32_t k);  // sets list_waves_per_root, tune_ms
+uint32_t list_waves_per_root(const Ctx &c, int cls, uint32_t k);  // effective waves per root of a class
 void list_materialize(Ctx &c, uint32_t k);  // cliques, incoff, inc (requires list_count)
 
 // ---- peeling (k_peel.hip) ----

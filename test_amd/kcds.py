@@ -25,7 +25,8 @@ U64P = ctypes.POINTER(ctypes.c_uint64)
 # every entry point include/kcds.h declares
 EXPORTS = ("kcds_create", "kcds_destroy", "kcds_last_error", "kcds_upload_csr", "kcds_orient", "kcds_count",
            "kcds_list", "kcds_download_cliques", "kcds_peel", "kcds_components", "kcds_maxflow", "kcds_solve",
-           "kcds_sync", "kcds_set_tuning", "kcds_get_list_info", "kcds_solve_batch")
+           "kcds_sync", "kcds_set_tuning", "kcds_get_list_info", "kcds_solve_batch", "kcds_set_list_shape",
+           "kcds_autotune_list")
 
 
 class PeelStats(ctypes.Structure):
@@ -51,7 +52,8 @@ class GraphResult(ctypes.Structure):
 class ListInfo(ctypes.Structure):
     _fields_ = [("n", ctypes.c_uint32), ("arcs", ctypes.c_uint64), ("w2", ctypes.c_uint64),
                 ("max_outdeg", ctypes.c_uint32), ("kernel_ms", ctypes.c_float),
-                ("kernel_launches", ctypes.c_uint32)]
+                ("kernel_launches", ctypes.c_uint32), ("class_roots", ctypes.c_uint32 * 4),
+                ("waves_per_root", ctypes.c_uint32 * 4)]
 
 
 class KcdsError(RuntimeError):
@@ -85,6 +87,8 @@ def lib():
         L.kcds_sync.argtypes = [V]
         L.kcds_set_tuning.argtypes = [V, ctypes.c_uint32, ctypes.c_uint32]
         L.kcds_get_list_info.argtypes = [V, ctypes.POINTER(ListInfo)]
+        L.kcds_set_list_shape.argtypes = [V, U32P]
+        L.kcds_autotune_list.argtypes = [V, ctypes.c_uint32, U32P, ctypes.POINTER(ctypes.c_float)]
         _LIB = L
     return _LIB
 
@@ -194,6 +198,20 @@ class Solver:
 
     def set_tuning(self, waves_per_cu=0, flags=0):
         self._chk(lib().kcds_set_tuning(self._h, waves_per_cu, flags))
+
+    def set_list_shape(self, waves_per_root=None):
+        """Waves per root of the four width classes (None = library defaults)."""
+        w = None if waves_per_root is None else np.ascontiguousarray(waves_per_root, dtype=np.uint32)
+        if w is not None and len(w) != 4:
+            raise ValueError("waves_per_root needs 4 entries")
+        self._chk(lib().kcds_set_list_shape(self._h, _p(w, U32P)))
+
+    def autotune_list(self, k):
+        """Times every waves-per-root choice per width class; keeps and returns the fastest."""
+        w = np.zeros(4, np.uint32)
+        ms = (ctypes.c_float * 4)()
+        self._chk(lib().kcds_autotune_list(self._h, k, _p(w, U32P), ms))
+        return w.tolist(), list(ms)
 
     def list_info(self):
         info = ListInfo()
