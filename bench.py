@@ -36,6 +36,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default waves per root (skip the autotuner)")
+    ap.add_argument("--shape", default=None, help="waves per root of the 4 width classes, e.g. 2,2,4,8 (skips the autotuner)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy 5 times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -45,6 +48,17 @@ def main():
 
     import numpy as np
     from test_amd import gen, kcds
+
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(1 << 28, dtype=torch.int32, device="cuda")  # 1 GiB
+        b = torch.empty_like(a)
+        a.fill_(1)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "copies": 5}))
+        return
 
     g, k = gen.config(2)
     s = kcds.Solver(0)
@@ -56,7 +70,9 @@ def main():
         return T
 
     tuned = None
-    if not args.no_tune:  # setup, outside the timed region: pick waves per root for this graph
+    if args.shape:
+        s.set_list_shape([int(x) for x in args.shape.split(",")])
+    elif not args.no_tune:  # setup, outside the timed region: pick waves per root for this graph
         s.orient(None)
         tuned = s.autotune_list(k)
 

@@ -9,15 +9,13 @@
 // Here one workgroup of nw wavefronts owns one root r of the DAG (new ids,
 // lists ascending) at a time:
 //   1. Stage N+(r) = L[0..d) in LDS with an open-addressing table
-//      id -> local index, and the out-lists N+(L[i]) as one concatenated
-//      sequence of non-empty segments.  The waves split that sequence into
-//      64-element chunks; a lane finds its segment from a ballot of the
-//      segment starts inside its chunk, reads one element w (the out-wedge
-//      sum W2 of the roofline is these reads) and looks w up in the table.
-//      A hit j sets bit j of row i and bit i of row j: rows are the
-//      symmetric adjacency of the subgraph induced by N+(r), d rows of W
-//      64-bit words stored word-major (A[w*D + j]) so that 64 lanes reading
-//      64 consecutive rows hit 64 distinct banks.
+//      id -> local index.  The waves take the out-lists N+(L[i]) in turn and
+//      read each in 64-element chunks, four chunks in flight per wave (the
+//      out-wedge sum W2 of the roofline is these reads); every element w is
+//      looked up in the table.  A hit j sets bit j of row i and bit i of
+//      row j: rows are the symmetric adjacency of the subgraph induced by
+//      N+(r), d rows of W 64-bit words stored word-major (A[w*D + j]) so
+//      that 64 lanes reading 64 consecutive rows hit 64 distinct banks.
 //   2. Because ids increase along every DAG arc, the out-neighbours of L[c]
 //      inside N+(r) are row c masked to bits above c, so recursion on
 //      candidate sets S (W words, wave-uniform) needs no further global
@@ -48,16 +46,17 @@ template <int W> struct Frame {
   static constexpr uint32_t D = 64u * W;  // row capacity
   static constexpr uint32_t HB = W == 1 ? 8 : W == 2 ? 9 : W == 4 ? 10 : 11;
   static constexpr uint32_t H = 1u << HB;  // table slots (load <= 1/4)
-  uint32_t L[D];                 // N+(r), ascending
-  uint32_t spre[D + 64];         // start of non-empty segment s; ~0 beyond the last
-  ull sinfo[D];                  // (i << 32) | (doff[L[i]] - spre[s]) mod 2^32
-  uint32_t hkey[H];              // L[i], EMPTY_KEY when free
-  uint16_t hval[H];              // i
-  uint8_t flag[MAX_WAVES][64];   // per-wave segment-start marks
-  ull A[W * D];                  // A[w*D + j]: bits 64w..64w+63 of row j
-  ull rt;                        // cliques of the current root (summed over waves)
-  uint32_t root, nseg, E, raw;
+  uint32_t L[D];     // N+(r), ascending
+  uint32_t soff[D];  // doff[L[i]]
+  uint32_t slen[D];  // outdeg(L[i]); 0 for the last (it has no later candidate)
+  uint32_t hkey[H];  // L[i], EMPTY_KEY when free
+  uint16_t hval[H];  // i
+  ull A[W * D];      // A[w*D + j]: bits 64w..64w+63 of row j
+  ull rt;            // cliques of the current root (summed over waves)
+  uint32_t root;
 };
+
+constexpr uint32_t UNROLL = 4;  // 64-element chunks of one out-list in flight per wave
 
 __device__ __forceinline__ uint32_t hslot(uint32_t key, uint32_t hb) { return (key * 0x9E3779B1u) >> (32 - hb); }
 
@@ -73,101 +72,61 @@ template <int W> __device__ __forceinline__ ull row(const Frame<W> &f, uint32_t 
 }
 
 // Builds the frame of the root whose out-list is dnbr[base, base + d), d >= 2.
-// Every thread of the workgroup calls it; returns sum over u in N+(r) of
-// outdeg(u) (in every thread).
+// Every thread of the workgroup calls it.  Returns this lane's share of
+// sum over u in N+(r) of outdeg(u).
 template <int W>
 __device__ uint32_t build_rows(Frame<W> &f, const uint32_t *__restrict__ doff, const uint32_t *__restrict__ dnbr,
                                uint32_t base, uint32_t d) {
   constexpr uint32_t D = Frame<W>::D, H = Frame<W>::H, HB = Frame<W>::HB;
   const uint32_t tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
   for (uint32_t t = tid; t < H; t += nt) f.hkey[t] = EMPTY_KEY;
-  for (uint32_t t = tid; t < D + 64; t += nt) f.spre[t] = 0xFFFFFFFFu;
-  // LDS starts with whatever the previous kernel left: every chunk relies on
-  // all marks being clear before it sets its own (it clears what it set)
-  for (uint32_t t = tid; t < MAX_WAVES * 64; t += nt) (&f.flag[0][0])[t] = 0;
   for (uint32_t j = tid; j < d; j += nt) {
 #pragma unroll
     for (int w = 0; w < W; ++w) f.A[(uint32_t)w * D + j] = 0ull;
   }
   block_sync();
-  const uint32_t last = dnbr[base + d - 1];
-  if (wv == 0) {
-    uint32_t carry = 0, ns = 0, raw = 0;
-    for (uint32_t t0 = 0; t0 < d; t0 += KCDS_WAVE) {
-      const uint32_t t = t0 + lane;
-      uint32_t ou = 0, du = 0;
-      if (t < d) {
-        const uint32_t u = dnbr[base + t];
-        ou = doff[u];
-        du = doff[u + 1] - ou;
-        f.L[t] = u;
-        raw += du;
-        uint32_t h = hslot(u, HB);
-        while (atomicCAS(&f.hkey[h], EMPTY_KEY, u) != EMPTY_KEY) h = (h + 1) & (H - 1);
-        f.hval[h] = (uint16_t)t;
-      }
-      const uint32_t len = (t + 1 < d) ? du : 0u;  // L[d-1] has no later candidate
-      const ull ne = __ballot(len > 0);
-      const uint32_t incl = wave_incl_scan_u32(len);
-      if (len) {
-        const uint32_t s = ns + lanes_below(ne);
-        const uint32_t start = carry + incl - len;
-        f.spre[s] = start;
-        f.sinfo[s] = ((ull)t << 32) | (ull)(uint32_t)(ou - start);
-      }
-      ns += (uint32_t)__popcll(ne);
-      carry += __shfl(incl, 63, 64);
-    }
-    raw = wave_sum_u32(raw);
-    if (lane == 0) {
-      f.nseg = ns;
-      f.E = carry;
-      f.raw = raw;
-    }
+  // stage N+(r), the out-list extents and the id -> index table
+  uint32_t raw = 0;
+  for (uint32_t t = tid; t < d; t += nt) {
+    const uint32_t u = dnbr[base + t];
+    const uint32_t ou = doff[u], du = doff[u + 1] - ou;
+    f.L[t] = u;
+    f.soff[t] = ou;
+    f.slen[t] = t + 1 < d ? du : 0u;
+    raw += du;
+    uint32_t h = hslot(u, HB);
+    while (atomicCAS(&f.hkey[h], EMPTY_KEY, u) != EMPTY_KEY) h = (h + 1) & (H - 1);
+    f.hval[h] = (uint16_t)t;
   }
   block_sync();
-  const uint32_t E = f.E, ns = f.nseg, raw = f.raw;
-  // contiguous range of the concatenated sequence per wave, in whole chunks
-  const uint32_t per = ((E + nw * 64 - 1) / (nw * 64)) * 64;
-  const uint32_t eb = wv * per, ee = min(E, eb + per);
-  if (eb < ee) {
-    uint32_t s0 = upper_bound_u32(f.spre, ns, eb) - 1;  // spre[0] = 0 <= eb
-    for (uint32_t e0 = eb; e0 < ee; e0 += KCDS_WAVE) {
-      uint32_t x;
-      while (true) {  // advance s0 to the segment holding e0 (segments are non-empty)
-        x = f.spre[s0 + 1 + lane];
-        const ull b = __ballot(x <= e0);
-        s0 += (uint32_t)__popcll(b);
-        if (b != ~0ull) break;
+  // row i collects N+(L[i]) inside N+(r): the waves take out-lists in turn,
+  // 64 * UNROLL elements per step with every load issued before the lookups
+  const uint32_t last = f.L[d - 1];
+  for (uint32_t i = wv; i + 1 < d; i += nw) {
+    const uint32_t o = f.soff[i], len = f.slen[i];
+    for (uint32_t t0 = 0; t0 < len; t0 += 64 * UNROLL) {
+      uint32_t w[UNROLL];
+#pragma unroll
+      for (uint32_t x = 0; x < UNROLL; ++x) {
+        const uint32_t t = t0 + x * 64 + lane;
+        w[x] = t < len ? dnbr[o + t] : 0xFFFFFFFFu;
       }
-      x = f.spre[s0 + 1 + lane];  // starts after e0, ascending
-      if (x < e0 + 64) f.flag[wv][x - e0] = 1;
-      wave_sync();
-      const bool fl = f.flag[wv][lane] != 0;
-      const ull M = __ballot(fl);
-      if (fl) f.flag[wv][lane] = 0;
-      const uint32_t e = e0 + lane;
-      if (e < ee) {
-        const uint32_t s = s0 + (uint32_t)__popcll(M & ((2ull << lane) - 1ull));
-        const ull info = f.sinfo[s];
-        const uint32_t i = (uint32_t)(info >> 32);
-        const uint32_t w = dnbr[(uint32_t)info + e];
-        if (w <= last) {
-          uint32_t h = hslot(w, HB);
-          while (true) {
-            const uint32_t key = f.hkey[h];
-            if (key == w) {
-              const uint32_t j = f.hval[h];  // j > i: ids ascend along arcs and L is ascending
-              atomicOr(&f.A[(j >> 6) * D + i], 1ull << (j & 63));
-              atomicOr(&f.A[(i >> 6) * D + j], 1ull << (i & 63));
-              break;
-            }
-            if (key == EMPTY_KEY) break;
-            h = (h + 1) & (H - 1);
+#pragma unroll
+      for (uint32_t x = 0; x < UNROLL; ++x) {
+        if (w[x] > last) continue;  // also the padding
+        uint32_t h = hslot(w[x], HB);
+        while (true) {
+          const uint32_t key = f.hkey[h];
+          if (key == w[x]) {
+            const uint32_t j = f.hval[h];  // j > i: ids ascend along arcs and L is ascending
+            atomicOr(&f.A[(j >> 6) * D + i], 1ull << (j & 63));
+            atomicOr(&f.A[(i >> 6) * D + j], 1ull << (i & 63));
+            break;
           }
+          if (key == EMPTY_KEY) break;
+          h = (h + 1) & (H - 1);
         }
       }
-      wave_sync();
     }
   }
   block_sync();
@@ -315,10 +274,9 @@ template <int W> __device__ __forceinline__ void all_bits(ull *S, uint32_t d) {
   }
 }
 
-// Takes the next root index from the queue (thread 0) and hands it to the
-// workgroup.
-template <int W> __device__ __forceinline__ uint32_t next_root(Frame<W> &f, uint32_t *qctr) {
-  if (threadIdx.x == 0) f.root = atomicAdd(qctr, 1u);
+// Hands the root index held by thread 0 to the workgroup.
+template <int W> __device__ __forceinline__ uint32_t share_root(Frame<W> &f, uint32_t idx0) {
+  if (threadIdx.x == 0) f.root = idx0;
   block_sync();
   const uint32_t idx = f.root;
   block_sync();  // every thread has read f.root before it is reused
@@ -329,15 +287,21 @@ template <int W, int K>
 __global__ __launch_bounds__(64 * MAX_WAVES) void k_list_count(const uint32_t *__restrict__ doff,
                                                                const uint32_t *__restrict__ dnbr,
                                                                const uint32_t *__restrict__ roots, uint32_t nroots,
-                                                               uint32_t *qctr, ull *gcnt, ull *rootT, ull *gtotal,
-                                                               ull *gw2) {
+                                                               uint32_t *queue, ull *gcnt, uint64_t cstride,
+                                                               uint32_t ncopy, ull *rootT, ull *gtotal) {
   __shared__ Frame<W> f;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
   ull tot = 0, w2 = 0;
   if (tid == 0) f.rt = 0;
+  // per-vertex counts go to one of ncopy copies (summed afterwards), so a
+  // hub's counter is not one serialised address for the whole launch
+  gcnt += (uint64_t)(blockIdx.x % ncopy) * cstride;
+  QueueCursor qc = queue_start(blockIdx.x);
+  uint32_t pre = tid == 0 ? queue_next(queue, nroots, qc) : 0u;  // next root (thread 0)
   while (true) {
-    const uint32_t idx = next_root(f, qctr);
+    const uint32_t idx = share_root(f, pre);
     if (idx >= nroots) break;
+    if (tid == 0) pre = queue_next(queue, nroots, qc);  // in flight while this root is processed
     const uint32_t r = roots[idx];
     const uint32_t base = doff[r], d = doff[r + 1] - base;
     w2 += build_rows<W>(f, doff, dnbr, base, d);
@@ -379,9 +343,19 @@ __global__ __launch_bounds__(64 * MAX_WAVES) void k_list_count(const uint32_t *_
       tot += all;
     }
   }
-  if (tid == 0) {
-    if (tot) atomicAdd(gtotal, tot);
-    if (w2) atomicAdd(gw2, w2);
+  // totals per shard: [s*8] = cliques, [s*8+1] = staged out-wedge sum
+  ull *mine = gtotal + (blockIdx.x % QSHARDS) * 8;
+  w2 = wave_sum_u64(w2);
+  if (lane == 0 && w2) atomicAdd(mine + 1, w2);
+  if (tid == 0 && tot) atomicAdd(mine, tot);
+}
+
+// cnt[v] = sum of the ncopy copies (copy 0 is cnt itself)
+__global__ void k_sum_copies(ull *cnt, uint64_t n, uint64_t cstride, uint32_t ncopy) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+    ull s = cnt[v];
+    for (uint32_t c = 1; c < ncopy; ++c) s += cnt[c * cstride + v];
+    cnt[v] = s;
   }
 }
 
@@ -389,11 +363,14 @@ template <int W, int K>
 __global__ __launch_bounds__(64) void k_list_emit(const uint32_t *__restrict__ doff,
                                                   const uint32_t *__restrict__ dnbr,
                                                   const uint32_t *__restrict__ roots, uint32_t nroots,
-                                                  uint32_t *qctr, const ull *rootOff, uint32_t *out) {
+                                                  uint32_t *queue, const ull *rootOff, uint32_t *out) {
   __shared__ Frame<W> f;
+  QueueCursor qc = queue_start(blockIdx.x);
+  uint32_t pre = threadIdx.x == 0 ? queue_next(queue, nroots, qc) : 0u;
   while (true) {
-    const uint32_t idx = next_root(f, qctr);
+    const uint32_t idx = share_root(f, pre);
     if (idx >= nroots) break;
+    if (threadIdx.x == 0) pre = queue_next(queue, nroots, qc);
     const uint32_t r = roots[idx];
     ull pos = rootOff[r];
     const ull lim = rootOff[r + 1];
@@ -484,8 +461,8 @@ __global__ void k_incidence(const uint32_t *cl, uint64_t T, uint32_t k, const ul
   }
 }
 
-typedef void (*CountFn)(const uint32_t *, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, ull *, ull *,
-                        ull *, ull *);
+typedef void (*CountFn)(const uint32_t *, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, ull *, uint64_t,
+                        uint32_t, ull *, ull *);
 typedef void (*EmitFn)(const uint32_t *, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, const ull *,
                        uint32_t *);
 
@@ -538,12 +515,25 @@ uint32_t wg_grid(const Ctx &c, int cls, uint32_t nroots, uint32_t nw) {
   return std::max<uint32_t>(1, std::min<uint32_t>(nroots, (uint32_t)c.num_cus * per_cu));
 }
 
+// c.qctr layout (32-bit words): [0, 64) classification counters, then one
+// sharded queue per class for counting, one per class for materialising and
+// one for the autotuner.
+constexpr uint32_t QBASE = 256;
+constexpr uint32_t QCTR_WORDS = QBASE + (2 * NCLASS + 1) * QUEUE_WORDS;
+uint32_t *count_queue(Ctx &c, int cls) { return c.qctr.as<uint32_t>() + QBASE + cls * QUEUE_WORDS; }
+uint32_t *emit_queue(Ctx &c, int cls) { return c.qctr.as<uint32_t>() + QBASE + (NCLASS + cls) * QUEUE_WORDS; }
+uint32_t *tune_queue(Ctx &c) { return c.qctr.as<uint32_t>() + QBASE + 2 * NCLASS * QUEUE_WORDS; }
+
+// Copies of the per-vertex counters (8 while they fit comfortably).
+uint32_t count_copies(uint32_t n) { return n <= (1u << 26) ? 8u : 1u; }
+uint64_t copy_stride(uint32_t n) { return (((uint64_t)n + 1 + 63) / 64) * 64; }
+
 void classify_roots(Ctx &c, uint32_t k) {
   const uint32_t n = c.n;
   buf_reserve(c, c.roots, sizeof(uint32_t) * ((size_t)n + 1));
-  buf_reserve(c, c.qctr, 64 * sizeof(uint32_t));
+  buf_reserve(c, c.qctr, QCTR_WORDS * sizeof(uint32_t));
   uint32_t *q = c.qctr.as<uint32_t>();
-  HIP_CHECK(hipMemsetAsync(q, 0, 64 * sizeof(uint32_t), c.stream));
+  HIP_CHECK(hipMemsetAsync(q, 0, QCTR_WORDS * sizeof(uint32_t), c.stream));
   uint32_t *ccount = q + 16, *cstart = q + 24, *cfill = q + 32;
   const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((n + 255) / 256, 256));
   if (n) {
@@ -570,15 +560,28 @@ void classify_roots(Ctx &c, uint32_t k) {
   c.sync();  // hs lives on the host stack
 }
 
-// One launch of class cls over its roots; counters accumulate into gcnt,
-// rootT, total, w2.
-void launch_class(Ctx &c, int cls, uint32_t k, uint32_t nw, uint32_t *qslot, ull *gcnt, ull *rootT, ull *total,
-                  ull *w2) {
+// One launch of class cls over its roots; counters accumulate into the
+// ncopy copies of gcnt, rootT and the sharded totals.
+void launch_class(Ctx &c, int cls, uint32_t k, uint32_t nw, uint32_t *queue, ull *gcnt, ull *rootT, ull *totals) {
   const uint32_t nr = c.class_count[cls];
   const uint32_t *roots = c.roots.as<uint32_t>() + c.class_start[cls];
   hipLaunchKernelGGL(class_count_fn(cls, k), dim3(wg_grid(c, cls, nr, nw)), dim3(64 * nw), 0, c.stream,
-                     c.doff.as<uint32_t>(), c.dnbr.as<uint32_t>(), roots, nr, qslot, gcnt, rootT, total, w2);
+                     c.doff.as<uint32_t>(), c.dnbr.as<uint32_t>(), roots, nr, queue, gcnt, copy_stride(c.n),
+                     count_copies(c.n), rootT, totals);
   KCDS_LAUNCH_CHECK();
+}
+
+// Sums the sharded totals on the host: (cliques, staged out-wedge sum).
+void read_totals(Ctx &c, const ull *totals, ull &T, ull &w2) {
+  ull *h = (ull *)c.host_scratch(sizeof(ull) * QSHARDS * 8);
+  HIP_CHECK(hipMemcpyAsync(h, totals, sizeof(ull) * QSHARDS * 8, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  T = 0;
+  w2 = 0;
+  for (uint32_t i = 0; i < QSHARDS; ++i) {
+    T += h[i * 8];
+    w2 += h[i * 8 + 1];
+  }
 }
 
 }  // namespace
@@ -597,11 +600,12 @@ void list_autotune(Ctx &c, uint32_t k) {
   if (k < 3 || k > 8) fail(KCDS_E_INVALID, "k = %u outside 3..8", k);
   const uint32_t n = c.n;
   classify_roots(c, k);
-  buf_reserve(c, c.tmp2, sizeof(ull) * ((size_t)n + 2));
+  const uint64_t cw = copy_stride(n) * count_copies(n);
+  buf_reserve(c, c.tmp2, sizeof(ull) * cw);
   buf_reserve(c, c.tmp3, sizeof(ull) * ((size_t)n + 2));
-  buf_reserve(c, c.scal, 64 * sizeof(ull));
-  ull *gcnt = c.tmp2.as<ull>(), *rootT = c.tmp3.as<ull>(), *sc = c.scal.as<ull>() + 40;
-  uint32_t *q = c.qctr.as<uint32_t>() + 48;
+  buf_reserve(c, c.lsum, sizeof(ull) * QSHARDS * 8 * 2);
+  ull *gcnt = c.tmp2.as<ull>(), *rootT = c.tmp3.as<ull>(), *totals = c.lsum.as<ull>() + QSHARDS * 8;
+  uint32_t *q = tune_queue(c);
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
@@ -613,9 +617,9 @@ void list_autotune(Ctx &c, uint32_t k) {
       if (k == 3 && nw > (1u << cls)) break;
       float tsum = 0;
       for (int rep = 0; rep < 2; ++rep) {
-        HIP_CHECK(hipMemsetAsync(q, 0, sizeof(uint32_t), c.stream));
+        HIP_CHECK(hipMemsetAsync(q, 0, QUEUE_WORDS * sizeof(uint32_t), c.stream));
         HIP_CHECK(hipEventRecord(e0, c.stream));
-        launch_class(c, cls, k, nw, q, gcnt, rootT, sc, sc + 1);
+        launch_class(c, cls, k, nw, q, gcnt, rootT, totals);
         HIP_CHECK(hipEventRecord(e1, c.stream));
         HIP_CHECK(hipEventSynchronize(e1));
         float ms = 0;
@@ -639,14 +643,15 @@ void list_count(Ctx &c, uint32_t k) {
   if (k < 3 || k > 8) fail(KCDS_E_INVALID, "k = %u outside 3..8", k);
   const uint32_t n = c.n;
   classify_roots(c, k);
-  buf_reserve(c, c.cnt, sizeof(ull) * ((size_t)n + 1));
+  const uint32_t ncopy = count_copies(n);
+  const uint64_t cstride = copy_stride(n);
+  buf_reserve(c, c.cnt, sizeof(ull) * cstride * ncopy);
   buf_reserve(c, c.rootT, sizeof(ull) * ((size_t)n + 2));
-  buf_reserve(c, c.scal, 64 * sizeof(ull));
-  HIP_CHECK(hipMemsetAsync(c.cnt.p, 0, sizeof(ull) * ((size_t)n + 1), c.stream));
+  buf_reserve(c, c.lsum, sizeof(ull) * QSHARDS * 8 * 2);
+  HIP_CHECK(hipMemsetAsync(c.cnt.p, 0, sizeof(ull) * cstride * ncopy, c.stream));
   HIP_CHECK(hipMemsetAsync(c.rootT.p, 0, sizeof(ull) * ((size_t)n + 2), c.stream));
-  ull *total = c.scal.as<ull>();
-  HIP_CHECK(hipMemsetAsync(total, 0, 2 * sizeof(ull), c.stream));
-  uint32_t *q = c.qctr.as<uint32_t>();
+  ull *totals = c.lsum.as<ull>();
+  HIP_CHECK(hipMemsetAsync(totals, 0, sizeof(ull) * QSHARDS * 8, c.stream));
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
@@ -654,22 +659,27 @@ void list_count(Ctx &c, uint32_t k) {
   uint32_t launches = 0;
   for (int cls = NCLASS - 1; cls >= 0; --cls) {
     if (!c.class_count[cls]) continue;
-    launch_class(c, cls, k, list_waves_per_root(c, cls, k), q + cls, c.cnt.as<ull>(), c.rootT.as<ull>(), total,
-                 total + 1);
+    launch_class(c, cls, k, list_waves_per_root(c, cls, k), count_queue(c, cls), c.cnt.as<ull>(), c.rootT.as<ull>(),
+                 totals);
+    ++launches;
+  }
+  if (ncopy > 1 && n) {
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)c.num_cus * 8);
+    k_sum_copies<<<grid, 256, 0, c.stream>>>(c.cnt.as<ull>(), n, cstride, ncopy);
+    KCDS_LAUNCH_CHECK();
     ++launches;
   }
   HIP_CHECK(hipEventRecord(e1, c.stream));
-  ull *h = (ull *)c.host_scratch(16);
-  HIP_CHECK(hipMemcpyAsync(h, total, 2 * sizeof(ull), hipMemcpyDeviceToHost, c.stream));
-  c.sync();
+  ull T = 0, w2 = 0;
+  read_totals(c, totals, T, w2);
   float ms = 0;
   HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
   HIP_CHECK(hipEventDestroy(e0));
   HIP_CHECK(hipEventDestroy(e1));
   c.list_kernel_ms = ms;
   c.list_kernel_launches = launches;
-  c.T = h[0];
-  c.w2 = h[1];
+  c.T = T;
+  c.w2 = w2;
   c.k = k;
   c.counted = true;
   c.listed = false;
@@ -691,15 +701,14 @@ void list_materialize(Ctx &c, uint32_t k) {
   scan_u64_to_u64(c, c.cnt.as<ull>(), c.incoff.as<ull>(), n);
   buf_reserve(c, c.cliques, sizeof(uint32_t) * (T * k + 1));
   buf_reserve(c, c.inc, sizeof(uint32_t) * (T * k + 1));
-  uint32_t *q = c.qctr.as<uint32_t>();
-  HIP_CHECK(hipMemsetAsync(q, 0, 8 * sizeof(uint32_t), c.stream));
+  HIP_CHECK(hipMemsetAsync(emit_queue(c, 0), 0, NCLASS * QUEUE_WORDS * sizeof(uint32_t), c.stream));
   for (int cls = NCLASS - 1; cls >= 0; --cls) {
     const uint32_t nr = c.class_count[cls];
     if (!nr) continue;
     EmitFn fn = cls == 0 ? emit_fn<1>(k) : cls == 1 ? emit_fn<2>(k) : cls == 2 ? emit_fn<4>(k) : emit_fn<8>(k);
     const uint32_t *roots = c.roots.as<uint32_t>() + c.class_start[cls];
     hipLaunchKernelGGL(fn, dim3(wg_grid(c, cls, nr, 1)), dim3(64), 0, c.stream, c.doff.as<uint32_t>(),
-                       c.dnbr.as<uint32_t>(), roots, nr, q + cls, rootOff, c.cliques.as<uint32_t>());
+                       c.dnbr.as<uint32_t>(), roots, nr, emit_queue(c, cls), rootOff, c.cliques.as<uint32_t>());
     KCDS_LAUNCH_CHECK();
   }
   buf_reserve(c, c.tmp3, sizeof(uint32_t) * ((size_t)n + 1));

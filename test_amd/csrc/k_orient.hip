@@ -14,6 +14,8 @@
 // Relabelling makes the out-neighbourhood of every vertex an increasing run
 // of ids, which the listing kernel uses to derive directed rows from
 // symmetric bitmask rows.
+#include <cstring>
+
 #include "kcds_device.hpp"
 #include "kcds_internal.hpp"
 
@@ -31,21 +33,32 @@ __global__ void k_rank_from_lo(const uint32_t *lo, uint32_t n, uint32_t *rank, u
   }
 }
 
+constexpr uint32_t HEAVY_DEG = 2048;   // vertices of higher degree are gathered by a whole workgroup
+constexpr uint32_t HEAVY_THREADS = 1024;
+
 // Wave per vertex v (4 waves per block, 4 KB of LDS each): gathers the new
 // ids of its later neighbours, sorts them by rank counting (position = number
 // of smaller entries; entries are distinct) and writes them to
 // tmp[off[v] ..), the start of v's undirected list; dout[rank[v]] = count.
-// A list longer than SORT_WAVE is counted but not written (the caller fails).
+// Vertices of degree above HEAVY_DEG are only listed in heavy[] for
+// k_dag_gather_heavy.  A list longer than SORT_WAVE is counted but not
+// written (the caller fails).
 __global__ __launch_bounds__(256) void k_dag_gather(const uint32_t *__restrict__ off,
                                                     const uint32_t *__restrict__ nbr,
                                                     const uint32_t *__restrict__ rank, uint32_t n,
-                                                    uint32_t *__restrict__ dout, uint32_t *__restrict__ tmp) {
+                                                    uint32_t *__restrict__ dout, uint32_t *__restrict__ tmp,
+                                                    uint32_t *__restrict__ heavy, uint32_t *nheavy) {
   __shared__ uint32_t sbuf[4][SORT_WAVE];
   const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
   uint32_t *buf = sbuf[wv];
   const uint32_t gw = blockIdx.x * 4 + wv, nwv = gridDim.x * 4;
   for (uint32_t v = gw; v < n; v += nwv) {
-    const uint32_t b = off[v], e = off[v + 1], r = rank[v];
+    const uint32_t b = off[v], e = off[v + 1];
+    if (e - b > HEAVY_DEG) {
+      if (lane == 0) heavy[atomicAdd(nheavy, 1u)] = v;
+      continue;
+    }
+    const uint32_t r = rank[v];
     uint32_t cnt = 0;
     for (uint32_t i0 = b; i0 < e; i0 += KCDS_WAVE) {
       const uint32_t i = i0 + lane;
@@ -69,6 +82,49 @@ __global__ __launch_bounds__(256) void k_dag_gather(const uint32_t *__restrict__
   }
 }
 
+// Workgroup per heavy vertex: the same gather with 16 waves sharing one LDS
+// list (slots taken with one LDS atomic per wave and chunk).
+__global__ __launch_bounds__(HEAVY_THREADS) void k_dag_gather_heavy(
+    const uint32_t *__restrict__ off, const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ rank,
+    const uint32_t *__restrict__ heavy, const uint32_t *nheavy, uint32_t *__restrict__ dout,
+    uint32_t *__restrict__ tmp) {
+  __shared__ uint32_t buf[SORT_WAVE];
+  __shared__ uint32_t cnt;
+  const uint32_t tid = threadIdx.x, lane = lane_id();
+  const uint32_t nh = *nheavy;
+  for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+    const uint32_t v = heavy[h];
+    const uint32_t b = off[v], e = off[v + 1], r = rank[v];
+    if (tid == 0) cnt = 0;
+    block_sync();
+    for (uint32_t i0 = b; i0 < e; i0 += HEAVY_THREADS) {
+      const uint32_t i = i0 + tid;
+      const uint32_t x = (i < e) ? rank[nbr[i]] : 0u;
+      const bool keep = (i < e) && x > r;
+      const ull bal = __ballot(keep);
+      if (bal) {
+        const uint32_t lead = (uint32_t)__builtin_ctzll(bal);
+        uint32_t at = 0;
+        if (lane == lead) at = atomicAdd(&cnt, (uint32_t)__popcll(bal));
+        at = __shfl(at, lead, 64) + lanes_below(bal);
+        if (keep && at < SORT_WAVE) buf[at] = x;
+      }
+    }
+    block_sync();
+    const uint32_t c = cnt;
+    if (tid == 0) dout[r] = c;
+    if (c <= SORT_WAVE) {
+      for (uint32_t i = tid; i < c; i += HEAVY_THREADS) {
+        const uint32_t x = buf[i];
+        uint32_t pos = 0;
+        for (uint32_t j = 0; j < c; ++j) pos += buf[j] < x;
+        tmp[b + pos] = x;
+      }
+    }
+    block_sync();
+  }
+}
+
 // Wave per new id r: copies the sorted out-list of v = inv[r] into the DAG.
 __global__ __launch_bounds__(256) void k_dag_copy(const uint32_t *__restrict__ off, const uint32_t *__restrict__ inv,
                                                   uint32_t n, const uint32_t *__restrict__ doff,
@@ -82,9 +138,9 @@ __global__ __launch_bounds__(256) void k_dag_copy(const uint32_t *__restrict__ o
 }
 
 // ---------------------------------------------------------------------------
-// GPU degeneracy order: level-synchronous k-core peel as a fixed pair of
-// kernels per sub-round (work / control) driven by device-side control
-// state and enqueued by the host in batches.
+// GPU degeneracy order: level-synchronous k-core peel as one kernel per
+// step driven by device-side control state and enqueued by the host in
+// batches.
 //   OPEN    collect every live vertex of degree <= target into the frontier
 //           and take the minimum degree of the others; the control step
 //           either starts level = target or, when nothing was collected,
@@ -92,108 +148,205 @@ __global__ __launch_bounds__(256) void k_dag_copy(const uint32_t *__restrict__ o
 //   PROCESS the frontier takes the next ranks (in frontier order) and leaves
 //           the graph, decrementing its live neighbours; a neighbour whose
 //           degree falls from L+1 to L joins the next sub-round of level L.
+// A frontier carries each vertex's adjacency cut into PIECE-neighbour
+// pieces, and the decrements run wave per piece, so a hub of degree 25,000
+// is spread over 50 waves instead of one.
 // A vertex leaves by having its degree replaced with DEAD; decrements that
 // race with that replacement keep the value >= DEAD_MIN, which every reader
-// treats as dead.  Kernel boundaries order the steps; no grid-wide barrier
-// is needed.
+// treats as dead.  Kernel boundaries order the steps (the next step reads
+// the frontiers this one wrote); inside a step workgroups share only atomic
+// counters, and nothing waits on another workgroup.
 
+constexpr uint32_t NSEG = 8;  // frontier segments / counter shards (one per workgroup residue mod 8)
+
+// Counters are 64 words apart so that no two share a cache line (same-address
+// device atomics are serialised).
 struct OrderCtl {
-  uint32_t level;
-  uint32_t target;
-  uint32_t minval;
-  uint32_t next_rank;
-  uint32_t fsize[2];
-  uint32_t cur;
-  uint32_t done;
-  uint32_t phase;  // 0 = OPEN, 1 = PROCESS
+  uint32_t level, target, next_rank, cur, done, phase;
+  uint32_t steps;  // steps taken (sizes the next call's first batch)
+  uint32_t pad0[64 - 7];
+  uint32_t fsize[2][NSEG][64];  // vertex frontier segments: [frontier][segment][0]
+  uint32_t psize[2][NSEG][64];  // adjacency-piece frontier segments
+  uint32_t gmin[NSEG][64];      // minimum live degree above target, per segment (OPEN)
+  uint32_t gblocks[NSEG][64];   // finished workgroups per segment
+  uint32_t blocks[64];          // finished segments
 };
 
 constexpr uint32_t DEAD = 0xFFFFFFFFu;
 constexpr uint32_t DEAD_MIN = 0x80000000u;
+constexpr uint32_t PIECE = 512;   // neighbours per decrement work item
+constexpr uint32_t PUNROLL = PIECE / KCDS_WAVE;
 
-// Appends v (where want) to q with one counter atomic per wave.
-__device__ __forceinline__ void wave_append(bool want, uint32_t v, uint32_t *q, uint32_t *qsize) {
+// Appends v (where want) to a vertex frontier segment fv and its adjacency,
+// cut in PIECE-neighbour pieces (start << 32 | end), to the piece segment
+// fp; one counter atomic of each kind per wave.  Every lane calls it.
+__device__ __forceinline__ void wave_append(bool want, uint32_t v, const uint32_t *__restrict__ off, uint32_t *fv,
+                                            uint32_t *fvsize, ull *fp, uint32_t *fpsize) {
   const ull bal = __ballot(want);
   if (!bal) return;
-  uint32_t base = 0;
-  if (lane_id() == (uint32_t)__builtin_ctzll(bal)) base = atomicAdd(qsize, (uint32_t)__popcll(bal));
-  base = __shfl(base, __builtin_ctzll(bal), 64);
-  if (want) q[base + lanes_below(bal)] = v;
+  const uint32_t lane = lane_id(), lead = (uint32_t)__builtin_ctzll(bal);
+  uint32_t b = 0, e = 0, np = 0;
+  if (want) {
+    b = off[v];
+    e = off[v + 1];
+    np = (e - b + PIECE - 1) / PIECE;
+  }
+  const uint32_t incl = wave_incl_scan_u32(np);
+  const uint32_t tot = __shfl(incl, 63, 64);
+  uint32_t vb = 0, pb = 0;
+  if (lane == lead) {
+    vb = atomicAdd(fvsize, (uint32_t)__popcll(bal));
+    if (tot) pb = atomicAdd(fpsize, tot);
+  }
+  vb = __shfl(vb, lead, 64);
+  pb = __shfl(pb, lead, 64);
+  if (want) {
+    fv[vb + lanes_below(bal)] = v;
+    uint32_t p = pb + incl - np;
+    for (uint32_t st = b; st < e; st += PIECE) fp[p++] = ((ull)st << 32) | min(st + PIECE, e);
+  }
 }
 
-__global__ __launch_bounds__(256) void k_ko_work(OrderCtl *ctl, uint32_t n, const uint32_t *__restrict__ off,
-                                                 const uint32_t *__restrict__ nbr, uint32_t *deg, uint32_t *fr0,
-                                                 uint32_t *fr1, uint32_t *rank, uint32_t *inv) {
-  if (ctl->done) return;
-  const uint32_t cur = ctl->cur;
-  uint32_t *fr = cur ? fr1 : fr0;
-  uint32_t *nx = cur ? fr0 : fr1;
-  const uint32_t lane = lane_id();
-  if (ctl->phase == 0) {
-    const uint32_t target = ctl->target;
+// Control step, run by one thread of the last workgroup to finish a step.
+// Every workgroup's counter atomics completed before it took its ticket; the
+// counters are read back with read-modify-write atomics, which are performed
+// where the other workgroups' atomics were.
+__device__ void order_control(OrderCtl *ctl, uint32_t n) {
+  const uint32_t cur = ctl->cur, phase = ctl->phase;
+  uint32_t fcur = 0, fnext = 0, mn = DEAD;
+  for (uint32_t g = 0; g < NSEG; ++g) {
+    fcur += atomicAdd(&ctl->fsize[cur][g][0], 0u);
+    fnext += atomicAdd(&ctl->fsize[cur ^ 1][g][0], 0u);
+    mn = min(mn, atomicExch(&ctl->gmin[g][0], DEAD));
+    ctl->gblocks[g][0] = 0;
+  }
+  ctl->blocks[0] = 0;
+  ++ctl->steps;
+  if (phase == 1) {
+    ctl->next_rank += fcur;
+    for (uint32_t g = 0; g < NSEG; ++g) {
+      ctl->fsize[cur][g][0] = 0;
+      ctl->psize[cur][g][0] = 0;
+    }
+    ctl->cur = cur ^ 1;
+    if (ctl->next_rank >= n) {
+      ctl->done = 1;
+    } else if (fnext == 0) {
+      ctl->phase = 0;
+      ctl->target = ctl->level + 1;
+    }
+  } else if (fcur > 0) {
+    ctl->level = ctl->target;
+    ctl->phase = 1;
+  } else if (mn == DEAD) {
+    ctl->done = 1;
+  } else {
+    ctl->target = mn;
+  }
+}
+
+// One step (OPEN or PROCESS); the last workgroup to finish runs the control
+// step, so consecutive steps need no separate control launch.  Frontiers are
+// NSEG segments of capacity seg_v (vertices) and seg_p (pieces); workgroup b
+// appends to segment b % NSEG.
+__global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, const uint32_t *__restrict__ off,
+                                                 const uint32_t *__restrict__ nbr, uint32_t *deg, uint32_t *fv0,
+                                                 uint32_t *fv1, ull *fp0, ull *fp1, uint32_t seg_v, uint32_t seg_p,
+                                                 uint32_t *rank, uint32_t *inv) {
+  __shared__ uint32_t s_f[NSEG + 1], s_p[NSEG + 1], s_min, s_last;
+  __shared__ uint32_t s_hdr[6];
+  const uint32_t tid = threadIdx.x, lane = lane_id(), seg = blockIdx.x % NSEG;
+  if (tid < 6) s_hdr[tid] = (&ctl->level)[tid];
+  block_sync();
+  const uint32_t level = s_hdr[0], target = s_hdr[1], base = s_hdr[2], cur = s_hdr[3], done = s_hdr[4],
+                 phase = s_hdr[5];
+  if (done) return;
+  uint32_t *fv = cur ? fv1 : fv0;
+  uint32_t *nv = cur ? fv0 : fv1;
+  ull *fp = cur ? fp1 : fp0;
+  ull *np = cur ? fp0 : fp1;
+  const uint32_t gtid = blockIdx.x * blockDim.x + tid, gstride = gridDim.x * blockDim.x;
+  if (phase == 0) {
+    if (tid == 0) s_min = DEAD;
+    block_sync();
     uint32_t m = DEAD;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < n; v0 += stride) {
-      const uint32_t v = v0 + threadIdx.x;
+    for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < n; v0 += gstride) {
+      const uint32_t v = v0 + tid;
       const uint32_t d = v < n ? deg[v] : DEAD;
       const bool live = d < DEAD_MIN;
-      wave_append(live && d <= target, v, fr, &ctl->fsize[cur]);
+      wave_append(live && d <= target, v, off, fv + (size_t)seg * seg_v, &ctl->fsize[cur][seg][0],
+                  fp + (size_t)seg * seg_p, &ctl->psize[cur][seg][0]);
       if (live && d > target) m = min(m, d);
     }
     m = wave_min_u32(m);
-    if (lane == 0 && m != DEAD) atomicMin(&ctl->minval, m);
-    return;
-  }
-  const uint32_t fs = ctl->fsize[cur], L = ctl->level, base = ctl->next_rank;
-  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KCDS_WAVE;
-  const uint32_t nw = gridDim.x * blockDim.x / KCDS_WAVE;
-  for (uint32_t i = wid; i < fs; i += nw) {
-    const uint32_t v = fr[i];
-    if (lane == 0) {
+    if (lane == 0 && m != DEAD) atomicMin(&s_min, m);
+    block_sync();
+    if (tid == 0 && s_min != DEAD) atomicMin(&ctl->gmin[seg][0], s_min);
+  } else {
+    // segment sizes and prefixes of the current frontier
+    if (tid < NSEG) {
+      s_f[tid + 1] = ctl->fsize[cur][tid][0];
+      s_p[tid + 1] = ctl->psize[cur][tid][0];
+    }
+    if (tid == 0) {
+      s_f[0] = 0;
+      s_p[0] = 0;
+    }
+    block_sync();
+    if (tid == 0) {
+      for (uint32_t g = 0; g < NSEG; ++g) {
+        s_f[g + 1] += s_f[g];
+        s_p[g + 1] += s_p[g];
+      }
+    }
+    block_sync();
+    const uint32_t fs = s_f[NSEG], ps = s_p[NSEG];
+    for (uint32_t i = gtid; i < fs; i += gstride) {  // the frontier takes the next ranks and leaves
+      uint32_t g = 0;
+      while (i >= s_f[g + 1]) ++g;
+      const uint32_t v = fv[(size_t)g * seg_v + (i - s_f[g])];
       rank[v] = base + i;
       inv[base + i] = v;
       atomicExch(&deg[v], DEAD);
     }
-    const uint32_t b = off[v], e = off[v + 1];
-    for (uint32_t j0 = b; j0 < e; j0 += KCDS_WAVE) {
-      const uint32_t j = j0 + lane;
-      bool join = false;
-      uint32_t u = 0;
-      if (j < e) {
-        u = nbr[j];
-        if (__hip_atomic_load(&deg[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < DEAD_MIN)
-          join = atomicSub(&deg[u], 1u) == L + 1;
+    // wave per adjacency piece of the frontier.  Decrementing a vertex that
+    // already left keeps its value >= DEAD_MIN (it loses at most its degree).
+    const uint32_t wid = gtid / KCDS_WAVE, nw = gstride / KCDS_WAVE;
+    for (uint32_t it = wid; it < ps; it += nw) {
+      uint32_t g = 0;
+      while (it >= s_p[g + 1]) ++g;
+      const ull item = fp[(size_t)g * seg_p + (it - s_p[g])];
+      const uint32_t st = (uint32_t)(item >> 32), e = (uint32_t)item;
+      uint32_t u[PUNROLL], dv[PUNROLL];
+#pragma unroll
+      for (uint32_t x = 0; x < PUNROLL; ++x) {
+        const uint32_t j = st + x * KCDS_WAVE + lane;
+        u[x] = j < e ? nbr[j] : 0u;
       }
-      wave_append(join, u, nx, &ctl->fsize[cur ^ 1]);
+#pragma unroll
+      for (uint32_t x = 0; x < PUNROLL; ++x) {
+        const uint32_t j = st + x * KCDS_WAVE + lane;
+        dv[x] = j < e ? atomicSub(&deg[u[x]], 1u) : DEAD;
+      }
+#pragma unroll
+      for (uint32_t x = 0; x < PUNROLL; ++x)
+        wave_append(dv[x] == level + 1, u[x], off, nv + (size_t)seg * seg_v, &ctl->fsize[cur ^ 1][seg][0],
+                    np + (size_t)seg * seg_p, &ctl->psize[cur ^ 1][seg][0]);
     }
   }
-}
-
-__global__ void k_ko_ctl(OrderCtl *ctl, uint32_t n) {
-  if (ctl->done) return;
-  const uint32_t cur = ctl->cur;
-  if (ctl->phase == 1) {
-    ctl->next_rank += ctl->fsize[cur];
-    ctl->fsize[cur] = 0;
-    ctl->cur = cur ^ 1;
-    if (ctl->next_rank >= n) {
-      ctl->done = 1;
-    } else if (ctl->fsize[cur ^ 1] == 0) {
-      ctl->phase = 0;
-      ctl->target = ctl->level + 1;
-      ctl->minval = DEAD;
+  // two-level ticket: the last workgroup of a segment, then the last segment
+  block_sync();  // this workgroup's atomics have completed
+  if (tid == 0) {
+    const uint32_t in_seg = (gridDim.x - seg + NSEG - 1) / NSEG;  // workgroups with this residue
+    uint32_t last = 0;
+    if (atomicAdd(&ctl->gblocks[seg][0], 1u) == in_seg - 1) {
+      const uint32_t segs = min(gridDim.x, NSEG);
+      last = atomicAdd(&ctl->blocks[0], 1u) == segs - 1;
     }
-    return;
+    s_last = last;
   }
-  if (ctl->fsize[cur] > 0) {
-    ctl->level = ctl->target;
-    ctl->phase = 1;
-  } else if (ctl->minval == DEAD) {
-    ctl->done = 1;
-  } else {
-    ctl->target = ctl->minval;
-    ctl->minval = DEAD;
-  }
+  block_sync();
+  if (s_last && tid == 0) order_control(ctl, n);
 }
 
 __global__ void k_init_deg(const uint32_t *off, uint32_t n, uint32_t *deg) {
@@ -208,39 +361,49 @@ void degeneracy_rank_gpu(Ctx &c) {
   buf_reserve(c, c.rank, sizeof(uint32_t) * (size_t)n + 4);
   buf_reserve(c, c.inv, sizeof(uint32_t) * (size_t)n + 4);
   if (n == 0) return;
-  buf_reserve(c, c.tmp0, sizeof(uint32_t) * (size_t)n * 3 + sizeof(OrderCtl) + 256);
+  // every segment can hold any frontier: n vertices, their pieces
+  const uint32_t seg_v = ((n + 63) / 64) * 64;
+  const uint64_t seg_p64 = (((uint64_t)n + c.nnz / PIECE + 64 + 63) / 64) * 64;
+  if (seg_p64 * NSEG >= (1ull << 32)) fail(KCDS_E_LIMIT, "graph too large for the ordering frontiers");
+  const uint32_t seg_p = (uint32_t)seg_p64;
+  const size_t bytes = sizeof(uint32_t) * ((size_t)n + 2 * NSEG * (size_t)seg_v) +
+                       sizeof(ull) * 2 * NSEG * (size_t)seg_p + sizeof(OrderCtl) + 1024;
+  buf_reserve(c, c.tmp0, bytes);
   uint32_t *deg = c.tmp0.as<uint32_t>();
-  uint32_t *fr0 = deg + n, *fr1 = fr0 + n;
-  OrderCtl *ctl = (OrderCtl *)(((uintptr_t)(fr1 + n) + 255) & ~(uintptr_t)255);
-  const uint32_t vgrid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, c.num_cus * 8ull));
+  uint32_t *fv0 = deg + ((n + 63) / 64) * 64, *fv1 = fv0 + (size_t)NSEG * seg_v;
+  ull *fp0 = (ull *)(((uintptr_t)(fv1 + (size_t)NSEG * seg_v) + 255) & ~(uintptr_t)255);
+  ull *fp1 = fp0 + (size_t)NSEG * seg_p;
+  OrderCtl *ctl = (OrderCtl *)(((uintptr_t)(fp1 + (size_t)NSEG * seg_p) + 255) & ~(uintptr_t)255);
+  const uint32_t vgrid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, c.num_cus));
   k_init_deg<<<(n + 255) / 256, 256, 0, c.stream>>>(c.off.as<uint32_t>(), n, deg);
   KCDS_LAUNCH_CHECK();
-  OrderCtl h0{};
-  h0.minval = DEAD;
-  HIP_CHECK(hipMemcpyAsync(ctl, &h0, sizeof h0, hipMemcpyHostToDevice, c.stream));
+  OrderCtl *h0 = (OrderCtl *)c.host_scratch(sizeof(OrderCtl));
+  memset(h0, 0, sizeof(OrderCtl));
+  for (uint32_t g = 0; g < NSEG; ++g) h0->gmin[g][0] = DEAD;
+  HIP_CHECK(hipMemcpyAsync(ctl, h0, sizeof(OrderCtl), hipMemcpyHostToDevice, c.stream));
+  c.sync();  // h0 is reused below
   const uint32_t *off = c.off.as<uint32_t>(), *nbr = c.nbr.as<uint32_t>();
   uint32_t *rank = c.rank.as<uint32_t>(), *inv = c.inv.as<uint32_t>();
-  uint32_t *hdone = (uint32_t *)c.host_scratch(64);
+  uint32_t *hc = (uint32_t *)h0;
   // the number of steps is a property of the graph: start from the last count
   uint32_t batch = c.order_steps ? (uint32_t)std::min<uint64_t>(c.order_steps, 1u << 20) : 32;
   uint64_t rounds = 0;
   while (true) {
     for (uint32_t r = 0; r < batch; ++r) {
-      k_ko_work<<<vgrid, 256, 0, c.stream>>>(ctl, n, off, nbr, deg, fr0, fr1, rank, inv);
-      k_ko_ctl<<<1, 1, 0, c.stream>>>(ctl, n);
+      k_ko_step<<<vgrid, 256, 0, c.stream>>>(ctl, n, off, nbr, deg, fv0, fv1, fp0, fp1, seg_v, seg_p, rank, inv);
       KCDS_LAUNCH_CHECK();
     }
     rounds += batch;
-    HIP_CHECK(hipMemcpyAsync(hdone, &ctl->done, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipMemcpyAsync(hc, ctl, 64 * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
     c.sync();
-    if (*hdone) break;
+    if (((OrderCtl *)hc)->done) break;
     // every PROCESS step ranks at least one vertex; an OPEN step is followed
     // by a PROCESS step or by one OPEN step that succeeds
     if (rounds > 3ull * n + 8) fail(KCDS_E_HIP, "ordering did not finish within %llu sub-rounds",
                                     (unsigned long long)rounds);
     batch = std::min<uint32_t>(std::max<uint32_t>(batch, 32) * 2, 128);
   }
-  c.order_steps = rounds;
+  c.order_steps = ((OrderCtl *)hc)->steps;
 }
 
 void orient_from_rank(Ctx &c) {
@@ -248,11 +411,17 @@ void orient_from_rank(Ctx &c) {
   buf_reserve(c, c.doff, sizeof(uint32_t) * ((size_t)n + 1));
   buf_reserve(c, c.tmp1, sizeof(uint32_t) * ((size_t)n + 1));
   buf_reserve(c, c.tmp0, sizeof(uint32_t) * ((size_t)c.nnz + 1));
+  buf_reserve(c, c.tmp2, sizeof(uint32_t) * ((size_t)n + 64));
   uint32_t *dout = c.tmp1.as<uint32_t>(), *tmp = c.tmp0.as<uint32_t>();
+  uint32_t *nheavy = c.tmp2.as<uint32_t>(), *heavy = nheavy + 64;
   if (n) {
+    fill_u32(c, nheavy, 0, 1);
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 3) / 4, (uint64_t)c.num_cus * 40);
     k_dag_gather<<<grid, 256, 0, c.stream>>>(c.off.as<uint32_t>(), c.nbr.as<uint32_t>(), c.rank.as<uint32_t>(), n,
-                                             dout, tmp);
+                                             dout, tmp, heavy, nheavy);
+    KCDS_LAUNCH_CHECK();
+    k_dag_gather_heavy<<<c.num_cus, HEAVY_THREADS, 0, c.stream>>>(c.off.as<uint32_t>(), c.nbr.as<uint32_t>(),
+                                                                   c.rank.as<uint32_t>(), heavy, nheavy, dout, tmp);
     KCDS_LAUNCH_CHECK();
   }
   scan_u32_to_u32(c, dout, c.doff.as<uint32_t>(), n);

@@ -134,7 +134,7 @@ void kcds_destroy(kcds_ctx *ctx) {
   CompData &g = c.cd;
   FlowBufs &f = c.fb;
   DevBuf *bufs[] = {&c.off, &c.nbr, &c.rank, &c.inv, &c.doff, &c.dnbr, &c.roots, &c.qctr, &c.cnt, &c.scal,
-                    &c.rootT, &c.cliques, &c.incoff, &c.inc, &c.core, &c.status, &c.comp, &c.tmp0,
+                    &c.rootT, &c.lsum, &c.cliques, &c.incoff, &c.inc, &c.core, &c.status, &c.comp, &c.tmp0,
                     &c.tmp1, &c.tmp2, &c.tmp3, &c.gvoff, &c.gid, &c.gpeel, &c.tlist, &c.inset,
                     &g.par, &g.flag, &g.dense, &g.cid, &g.local, &g.vcount, &g.voff, &g.verts, &g.ccount,
                     &g.coff, &g.cfill, &g.ccl, &f.scal, &f.cside, &f.b_off, &f.b_excess, &f.b_height,

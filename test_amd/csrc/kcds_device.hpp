@@ -104,3 +104,32 @@ __device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *a, uint32_t 
   }
   return lo;
 }
+
+// Same-address device atomics are serialised (about 11 ns each on MI355X), so
+// a work queue touched by thousands of workgroups is split into QSHARDS
+// contiguous shards with one counter each, QSTRIDE words apart.  A grabber
+// starts at its own shard and moves on when one runs dry; it returns n when
+// every shard is empty.
+constexpr uint32_t QSHARDS = 32, QSTRIDE = 64;
+constexpr uint32_t QUEUE_WORDS = QSHARDS * QSTRIDE;
+
+struct QueueCursor {
+  uint32_t shard, tries;
+};
+
+__device__ __forceinline__ QueueCursor queue_start(uint32_t id) { return QueueCursor{id % QSHARDS, 0}; }
+
+__device__ __forceinline__ uint32_t queue_next(uint32_t *q, uint32_t n, QueueCursor &cur) {
+  const uint32_t per = (n + QSHARDS - 1) / QSHARDS;
+  while (cur.tries < QSHARDS) {
+    const uint32_t lo = cur.shard * per;
+    if (lo < n) {
+      const uint32_t i = atomicAdd(&q[cur.shard * QSTRIDE], 1u);
+      if (i < min(per, n - lo)) return lo + i;
+    }
+    cur.shard = cur.shard + 1 == QSHARDS ? 0 : cur.shard + 1;
+    ++cur.tries;
+  }
+  return n;
+}
+

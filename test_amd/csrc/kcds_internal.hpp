@@ -122,7 +122,8 @@ struct Ctx {
 
   // listing outputs
   uint32_t k = 0;
-  DevBuf cnt;     // ull[n] per-vertex counts in new-id space
+  DevBuf cnt;     // ull[n] per-vertex counts in new-id space (copy 0 of the listing's copies)
+  DevBuf lsum;    // ull sharded listing totals
   DevBuf scal;    // ull scratch scalars (T, counters)
   DevBuf rootT;   // ull[n+1] per-root clique counts -> offsets
   DevBuf cliques; // uint32[T*k] new ids
