@@ -94,8 +94,16 @@ def main():
     bytes_list = 4 * (n + 1) + 4 * arcs + 4 * w2 + 8 * n
     kms = float(np.mean(kern_ms))
     achieved = bytes_list / (kms / 1e3)
+    # HBM bytes per pass from the committed rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE scaled by a
+    # plain-copy calibration); counters cannot be read from inside this process
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", "list_count_c2_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tj = json.load(fh)
+        traffic, traffic_src = tj.get("traffic_bytes"), tj.get("source")
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None,
+                "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": "k_list_count<W,4> (one launch per width class W=1,2,4,8, back to back)",
                 "kernel_ms": kms, "bytes_per_pass": bytes_list}
 

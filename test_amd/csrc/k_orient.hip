@@ -33,6 +33,25 @@ __global__ void k_rank_from_lo(const uint32_t *lo, uint32_t n, uint32_t *rank, u
   }
 }
 
+// Position of x among the distinct entries a[0..cnt) (number of smaller
+// entries); a is 16-byte aligned and padded with ~0 up to a multiple of 4, so
+// it is read 4 entries per LDS access with two reads in flight.
+__device__ __forceinline__ uint32_t rank_in(const uint32_t *a, uint32_t cnt, uint32_t x) {
+  const uint4 *a4 = reinterpret_cast<const uint4 *>(a);
+  const uint32_t n4 = (cnt + 3) / 4;
+  uint32_t p0 = 0, p1 = 0, j = 0;
+  for (; j + 2 <= n4; j += 2) {
+    const uint4 u = a4[j], w = a4[j + 1];
+    p0 += (u.x < x) + (u.y < x) + (u.z < x) + (u.w < x);
+    p1 += (w.x < x) + (w.y < x) + (w.z < x) + (w.w < x);
+  }
+  if (j < n4) {
+    const uint4 u = a4[j];
+    p0 += (u.x < x) + (u.y < x) + (u.z < x) + (u.w < x);
+  }
+  return p0 + p1;
+}
+
 constexpr uint32_t HEAVY_DEG = 2048;   // vertices of higher degree are gathered by a whole workgroup
 constexpr uint32_t HEAVY_THREADS = 1024;
 
@@ -48,7 +67,7 @@ __global__ __launch_bounds__(256) void k_dag_gather(const uint32_t *__restrict__
                                                     const uint32_t *__restrict__ rank, uint32_t n,
                                                     uint32_t *__restrict__ dout, uint32_t *__restrict__ tmp,
                                                     uint32_t *__restrict__ heavy, uint32_t *nheavy) {
-  __shared__ uint32_t sbuf[4][SORT_WAVE];
+  __shared__ __align__(16) uint32_t sbuf[4][SORT_WAVE + 4];
   const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
   uint32_t *buf = sbuf[wv];
   const uint32_t gw = blockIdx.x * 4 + wv, nwv = gridDim.x * 4;
@@ -69,14 +88,10 @@ __global__ __launch_bounds__(256) void k_dag_gather(const uint32_t *__restrict__
       cnt += (uint32_t)__popcll(bal);
     }
     if (lane == 0) dout[r] = cnt;
+    if (lane < 4 && cnt + lane < SORT_WAVE) buf[cnt + lane] = 0xFFFFFFFFu;  // pad to whole 4-entry reads
     wave_sync();
     if (cnt <= SORT_WAVE) {
-      for (uint32_t i = lane; i < cnt; i += KCDS_WAVE) {
-        const uint32_t x = buf[i];
-        uint32_t pos = 0;
-        for (uint32_t j = 0; j < cnt; ++j) pos += buf[j] < x;
-        tmp[b + pos] = x;
-      }
+      for (uint32_t i = lane; i < cnt; i += KCDS_WAVE) tmp[b + rank_in(buf, cnt, buf[i])] = buf[i];
     }
     wave_sync();
   }
@@ -88,7 +103,7 @@ __global__ __launch_bounds__(HEAVY_THREADS) void k_dag_gather_heavy(
     const uint32_t *__restrict__ off, const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ rank,
     const uint32_t *__restrict__ heavy, const uint32_t *nheavy, uint32_t *__restrict__ dout,
     uint32_t *__restrict__ tmp) {
-  __shared__ uint32_t buf[SORT_WAVE];
+  __shared__ __align__(16) uint32_t buf[SORT_WAVE + 4];
   __shared__ uint32_t cnt;
   const uint32_t tid = threadIdx.x, lane = lane_id();
   const uint32_t nh = *nheavy;
@@ -113,13 +128,10 @@ __global__ __launch_bounds__(HEAVY_THREADS) void k_dag_gather_heavy(
     block_sync();
     const uint32_t c = cnt;
     if (tid == 0) dout[r] = c;
+    if (tid < 4 && c + tid < SORT_WAVE) buf[c + tid] = 0xFFFFFFFFu;  // pad to whole 4-entry reads
+    block_sync();
     if (c <= SORT_WAVE) {
-      for (uint32_t i = tid; i < c; i += HEAVY_THREADS) {
-        const uint32_t x = buf[i];
-        uint32_t pos = 0;
-        for (uint32_t j = 0; j < c; ++j) pos += buf[j] < x;
-        tmp[b + pos] = x;
-      }
+      for (uint32_t i = tid; i < c; i += HEAVY_THREADS) tmp[b + rank_in(buf, c, buf[i])] = buf[i];
     }
     block_sync();
   }

@@ -1,5 +1,7 @@
 // Device-wide primitives: exclusive scan, fill, max and sum reductions.
 // Written directly for wave64: 256-thread blocks, 16 items per thread.
+#include <algorithm>
+
 #include "kcds_device.hpp"
 #include "kcds_internal.hpp"
 
@@ -110,32 +112,57 @@ template <typename T> __global__ void k_fill(T *p, T v, uint64_t len) {
     p[i] = v;
 }
 
-__global__ void k_max_u32(const uint32_t *in, uint64_t len, uint32_t *out) {
+// Reductions: one device atomic per 256-thread block (a block reduces in LDS
+// first); launched with at most REDUCE_BLOCKS blocks, since same-address
+// device atomics are serialised.
+constexpr uint32_t REDUCE_BLOCKS = 256;
+
+__global__ __launch_bounds__(256) void k_max_u32(const uint32_t *in, uint64_t len, uint32_t *out) {
+  __shared__ uint32_t s;
+  if (threadIdx.x == 0) s = 0;
+  __syncthreads();
   uint32_t m = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len;
        i += (uint64_t)gridDim.x * blockDim.x)
     m = max(m, in[i]);
   m = wave_max_u32(m);
-  if (lane_id() == 0) atomicMax(out, m);
+  if (lane_id() == 0) atomicMax(&s, m);
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(out, s);
 }
 
-__global__ void k_max_u64(const ull *in, uint64_t len, ull *out) {
+__global__ __launch_bounds__(256) void k_max_u64(const ull *in, uint64_t len, ull *out) {
+  __shared__ ull s;
+  if (threadIdx.x == 0) s = 0;
+  __syncthreads();
   ull m = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len;
        i += (uint64_t)gridDim.x * blockDim.x)
     m = max(m, in[i]);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = max(m, (ull)__shfl_xor(m, o, 64));
-  if (lane_id() == 0) atomicMax(out, m);
+  if (lane_id() == 0) atomicMax(&s, m);
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(out, s);
 }
 
-__global__ void k_sum_u64(const ull *in, uint64_t len, ull *out) {
-  ull s = 0;
+__global__ __launch_bounds__(256) void k_sum_u64(const ull *in, uint64_t len, ull *out) {
+  __shared__ ull s;
+  if (threadIdx.x == 0) s = 0;
+  __syncthreads();
+  ull t = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len;
        i += (uint64_t)gridDim.x * blockDim.x)
-    s += in[i];
-  s = wave_sum_u64(s);
-  if (lane_id() == 0) atomicAdd(out, s);
+    t += in[i];
+  t = wave_sum_u64(t);
+  if (lane_id() == 0) atomicAdd(&s, t);
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, s);
+}
+
+uint32_t reduce_grid(uint64_t len) {
+  const uint64_t g = (len + 255) / 256;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, REDUCE_BLOCKS));
 }
 
 uint32_t grid_for(uint64_t len, uint32_t threads) {
@@ -190,7 +217,7 @@ void fill_u64(Ctx &c, ull *p, ull v, uint64_t len) {
 void max_u32(Ctx &c, const uint32_t *in, uint64_t len, uint32_t *out_dev) {
   fill_u32(c, out_dev, 0, 1);
   if (!len) return;
-  k_max_u32<<<grid_for(len, 256), 256, 0, c.stream>>>(in, len, out_dev);
+  k_max_u32<<<reduce_grid(len), 256, 0, c.stream>>>(in, len, out_dev);
   KCDS_LAUNCH_CHECK();
 }
 
@@ -199,7 +226,7 @@ ull max_u64_host(Ctx &c, const ull *in, uint64_t len) {
   ull *d = c.scal.as<ull>() + 62;
   fill_u64(c, d, 0, 1);
   if (len) {
-    k_max_u64<<<grid_for(len, 256), 256, 0, c.stream>>>(in, len, d);
+    k_max_u64<<<reduce_grid(len), 256, 0, c.stream>>>(in, len, d);
     KCDS_LAUNCH_CHECK();
   }
   ull *h = (ull *)c.host_scratch(sizeof(ull));
@@ -213,7 +240,7 @@ ull sum_u64_host(Ctx &c, const ull *in, uint64_t len) {
   ull *d = c.scal.as<ull>() + 63;
   fill_u64(c, d, 0, 1);
   if (len) {
-    k_sum_u64<<<grid_for(len, 256), 256, 0, c.stream>>>(in, len, d);
+    k_sum_u64<<<reduce_grid(len), 256, 0, c.stream>>>(in, len, d);
     KCDS_LAUNCH_CHECK();
   }
   ull *h = (ull *)c.host_scratch(sizeof(ull));
