@@ -137,7 +137,11 @@ int kcds_solve_batch(kcds_ctx *ctx, uint32_t k, uint32_t ngraphs, const uint32_t
 int kcds_sync(kcds_ctx *ctx);
 
 /* Listing-kernel launch shape: waves per CU for the persistent listing
- * kernels (0 = library default).  Used by the autotuner. */
+ * kernels (0 = library default), and flags:
+ *   KCDS_TUNE_GRID_PEEL  peel with the grid-wide step kernels even when every
+ *                        graph fits the workgroup-per-graph peel (both give
+ *                        the same results; used to test one against the other). */
+#define KCDS_TUNE_GRID_PEEL 1u
 int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
 
 /* Listing launch shape: waves per root (workgroup size / 64, 1..8) of each

@@ -25,14 +25,14 @@
 //   denser than alpha.
 // The max-preflow phase alternates a global relabel (level-synchronous BFS
 // from t over residual arcs: exact distance labels, height N for nodes that
-// cannot reach t) with synchronous push/relabel sweeps (wave per vertex
-// node, thread per clique node) until a global relabel finds no node with
-// excess that can still reach t.  The excess at t is then the max-flow value
-// and the vertices that cannot reach t form the source side of the maximal
-// minimum cut.  Workgroup barriers wait for each wave's own memory
-// operations first, and arrays written inside the launch are read with
-// device-coherent loads (atomics and other waves' write-through stores do not
-// refresh the CU's L1).
+// cannot reach t) with synchronous discharge sweeps (wave per vertex node,
+// thread per clique node; a node pushes along all of its residual arcs to
+// the lowest neighbour height at once, or relabels) until a global relabel
+// finds no node with excess that can still reach t.  The excess at t is then
+// the max-flow value and the vertices that cannot reach t form the source
+// side of the maximal minimum cut.  A component's arrays belong to one
+// workgroup, so its memory operations use workgroup scope; workgroup
+// barriers wait for each wave's own memory operations first.
 #include "kcds_device.hpp"
 #include "kcds_internal.hpp"
 
@@ -51,44 +51,113 @@ struct FlowState {
 };
 
 constexpr uint32_t SWEEPS = 24;
-constexpr uint32_t MAX_PHASES = 1u << 16;
+constexpr uint32_t MAX_PHASES = 1u << 14;
 
 uint32_t grid1d(uint64_t len) {
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((len + 255) / 256, 65535ull * 16));
 }
 
+// Every array of a component's network is private to the one workgroup that
+// solves it, so memory operations use workgroup scope: all its waves share
+// one CU and its L1, and the atomics stay in the cache hierarchy instead of
+// going to the device-wide coherence point.
 __device__ __forceinline__ uint32_t ld32(const uint32_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 __device__ __forceinline__ i64 ld64(const i64 *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 __device__ __forceinline__ void cst32(uint32_t *p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
-__device__ __forceinline__ void add64(i64 *p, i64 v) { atomicAdd((ull *)p, (ull)v); }
+__device__ __forceinline__ void add64(i64 *p, i64 v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t add32(uint32_t *p, uint32_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ bool cas32(uint32_t *p, uint32_t expect, uint32_t v) {
+  return __hip_atomic_compare_exchange_strong(p, &expect, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
-// Pushes from u along the admissible arc with the lowest head, or relabels.
-// best = (height << 32) | local arc index of the lowest residual neighbour.
-__device__ __forceinline__ bool discharge_step(uint32_t u, ull best, const uint32_t *off, const uint32_t *head,
-                                               const uint32_t *rev, i64 *res, i64 *excess, uint32_t *height) {
+constexpr uint32_t NO_ARC = 0xFFFFFFFFu;
+
+// Discharge of vertex node u by its wave: push the excess along every
+// residual arc whose head has the lowest height, split in arc order, or
+// relabel u when no neighbour is lower.  Only u's wave lowers excess[u] and
+// the residuals of u's arcs, so reading them first and pushing at most that
+// much keeps every value non-negative.  Returns whether anything changed.
+__device__ bool discharge_vertex(uint32_t u, const uint32_t *off, const uint32_t *head, const uint32_t *rev, i64 *res,
+                                 i64 *excess, uint32_t *height) {
+  const uint32_t lane = lane_id();
   const i64 e = ld64(&excess[u]);
-  if (best == ~0ull) return false;
-  const uint32_t hmin = (uint32_t)(best >> 32);
-  const uint32_t a = ld32(&off[u]) + (uint32_t)(best & 0xFFFFFFFFu);
-  const uint32_t hu = ld32(&height[u]);
-  if (hu > hmin) {
+  const uint32_t b = ld32(&off[u]), en = ld32(&off[u + 1]);
+  uint32_t hmin = NO_ARC;
+  for (uint32_t a = b + lane; a < en; a += KCDS_WAVE)
+    if (ld64(&res[a]) > 0) hmin = min(hmin, ld32(&height[ld32(&head[a])]));
+  hmin = wave_min_u32(hmin);
+  if (hmin == NO_ARC) return false;
+  if (ld32(&height[u]) <= hmin) {
+    if (lane == 0) cst32(&height[u], hmin + 1);
+    return true;
+  }
+  i64 left = e;
+  for (uint32_t a0 = b; a0 < en && left > 0; a0 += KCDS_WAVE) {
+    const uint32_t a = a0 + lane;
+    i64 r = 0;
+    uint32_t x = 0;
+    if (a < en) {
+      r = ld64(&res[a]);
+      if (r > 0) {
+        x = ld32(&head[a]);
+        if (ld32(&height[x]) != hmin) r = 0;
+      } else {
+        r = 0;
+      }
+    }
+    const ull incl = wave_incl_scan_u64((ull)r);
+    const i64 before = (i64)incl - r;
+    const i64 d = r > 0 && left > before ? min(r, left - before) : 0;
+    if (d > 0) {
+      add64(&res[a], -d);
+      add64(&res[ld32(&rev[a])], d);
+      add64(&excess[x], d);
+    }
+    const i64 tot = (i64)__shfl(incl, 63, 64);
+    left -= min(tot, left);
+  }
+  if (lane == 0 && e > left) add64(&excess[u], -(e - left));
+  return e > left;
+}
+
+// Discharge of clique node u (k arcs) by one thread, with the same rule.
+__device__ bool discharge_clique(uint32_t u, const uint32_t *off, const uint32_t *head, const uint32_t *rev, i64 *res,
+                                 i64 *excess, uint32_t *height) {
+  const i64 e = ld64(&excess[u]);
+  const uint32_t b = ld32(&off[u]), en = ld32(&off[u + 1]);
+  uint32_t hmin = NO_ARC;
+  for (uint32_t a = b; a < en; ++a)
+    if (ld64(&res[a]) > 0) hmin = min(hmin, ld32(&height[ld32(&head[a])]));
+  if (hmin == NO_ARC) return false;
+  if (ld32(&height[u]) <= hmin) {
+    cst32(&height[u], hmin + 1);
+    return true;
+  }
+  i64 left = e;
+  for (uint32_t a = b; a < en && left > 0; ++a) {
     const i64 r = ld64(&res[a]);
-    const i64 d = e < r ? e : r;
-    if (d <= 0) return false;
+    if (r <= 0) continue;
+    const uint32_t x = ld32(&head[a]);
+    if (ld32(&height[x]) != hmin) continue;
+    const i64 d = min(r, left);
     add64(&res[a], -d);
     add64(&res[ld32(&rev[a])], d);
-    add64(&excess[u], -d);
-    add64(&excess[ld32(&head[a])], d);
-  } else {
-    __hip_atomic_store(&height[u], hmin + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    add64(&excess[x], d);
+    left -= d;
   }
-  return true;
+  if (e > left) add64(&excess[u], -(e - left));
+  return e > left;
 }
 
 // Max-preflow phase of push-relabel by one workgroup.  On return
@@ -129,7 +198,7 @@ __device__ void max_preflow(uint32_t nc, uint64_t Tc, const uint32_t *off, const
           const uint32_t x = ld32(&head[a]);
           if (x == s || ld64(&res[ld32(&rev[a])]) <= 0) continue;  // x -> y has no residual capacity
           if (ld32(&height[x]) != N) continue;
-          if (atomicCAS(&height[x], N, dist + 1) == N) cst32(&nx[atomicAdd(&st->fsize[cur ^ 1], 1u)], x);
+          if (cas32(&height[x], N, dist + 1)) cst32(&nx[atomicAdd(&st->fsize[cur ^ 1], 1u)], x);
         }
       }
       sync();
@@ -163,31 +232,13 @@ __device__ void max_preflow(uint32_t nc, uint64_t Tc, const uint32_t *off, const
       // vertex nodes: one wave each (their arc lists can be long)
       for (uint32_t u = wid; u < nc; u += nwaves) {
         if (ld64(&excess[u]) <= 0 || ld32(&height[u]) >= N) continue;
-        ull best = ~0ull;
-        const uint32_t b = ld32(&off[u]), e = ld32(&off[u + 1]);
-        for (uint32_t a = b + lane; a < e; a += KCDS_WAVE)
-          if (ld64(&res[a]) > 0) {
-            const ull cand = ((ull)ld32(&height[ld32(&head[a])]) << 32) | (ull)(a - b);
-            best = cand < best ? cand : best;
-          }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-          const ull other = __shfl_xor(best, o, 64);
-          best = other < best ? other : best;
-        }
-        if (lane == 0) changed += discharge_step(u, best, off, head, rev, res, excess, height);
+        const bool ch = discharge_vertex(u, off, head, rev, res, excess, height);
+        if (lane == 0) changed += ch;
       }
       // clique nodes: k arcs, one thread each
       for (uint32_t u = nc + tid; u < s; u += nthreads) {
         if (ld64(&excess[u]) <= 0 || ld32(&height[u]) >= N) continue;
-        ull best = ~0ull;
-        const uint32_t b = ld32(&off[u]), e = ld32(&off[u + 1]);
-        for (uint32_t a = b; a < e; ++a)
-          if (ld64(&res[a]) > 0) {
-            const ull cand = ((ull)ld32(&height[ld32(&head[a])]) << 32) | (ull)(a - b);
-            best = cand < best ? cand : best;
-          }
-        changed += discharge_step(u, best, off, head, rev, res, excess, height);
+        changed += discharge_clique(u, off, head, rev, res, excess, height);
       }
       changed = wave_sum_u32(changed);
       if (lane == 0 && changed) atomicAdd(&st->changed[slot], changed);
@@ -228,23 +279,6 @@ __device__ ull gcd_u64(ull a, ull b) {
   return a;
 }
 
-// workgroup barrier after this wave's global stores and atomics completed
-// exclusive scan of one value per thread over a 1024-thread block
-__device__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t *total, uint32_t *wsum) {
-  const uint32_t lane = threadIdx.x % KCDS_WAVE, w = threadIdx.x / KCDS_WAVE;
-  const uint32_t incl = wave_incl_scan_u32(v);
-  if (lane == KCDS_WAVE - 1) wsum[w] = incl;
-  block_sync();
-  uint32_t before = 0, all = 0;
-  for (uint32_t i = 0; i < blockDim.x / KCDS_WAVE; ++i) {
-    if (i < w) before += wsum[i];
-    all += wsum[i];
-  }
-  __syncthreads();
-  *total = all;
-  return before + incl - v;
-}
-
 __global__ __launch_bounds__(1024) void k_dinkelbach_block(const CompTask *tasks, uint32_t ntasks, uint32_t *qctr,
                                                            uint32_t k, uint32_t max_iters, const uint32_t *ccl_all, uint32_t *noff_all,
                                                            i64 *excess_all, uint32_t *height_all, uint32_t *fr0_all,
@@ -258,8 +292,9 @@ __global__ __launch_bounds__(1024) void k_dinkelbach_block(const CompTask *tasks
   __shared__ uint32_t done;
   __shared__ i64 last_flow;
   const uint32_t tid = threadIdx.x, nthreads = blockDim.x;
+  QueueCursor qc = queue_start(blockIdx.x);
   while (true) {
-    if (tid == 0) task_id = atomicAdd(qctr, 1u);
+    if (tid == 0) task_id = queue_next(qctr, ntasks, qc);
     block_sync();
     const uint32_t ti = task_id;
     block_sync();
@@ -277,7 +312,7 @@ __global__ __launch_bounds__(1024) void k_dinkelbach_block(const CompTask *tasks
     // ---- network topology ----
     for (uint32_t v = tid; v < nc; v += nthreads) fr0[v] = 0;
     block_sync();
-    for (ull i = tid; i < Tc * k; i += nthreads) atomicAdd(&fr0[ccl[i]], 1u);
+    for (ull i = tid; i < Tc * k; i += nthreads) add32(&fr0[ccl[i]], 1u);
     block_sync();
     uint32_t carry = 0;
     for (uint32_t b0 = 0; b0 < nc; b0 += nthreads) {
@@ -311,7 +346,7 @@ __global__ __launch_bounds__(1024) void k_dinkelbach_block(const CompTask *tasks
     block_sync();
     for (ull i = tid; i < Tc * k; i += nthreads) {
       const uint32_t v = ccl[i];
-      const uint32_t slot = atomicAdd(&fr1[v], 1u);
+      const uint32_t slot = add32(&fr1[v], 1u);
       const uint32_t a = ld32(&off[v]) + 2 + slot, b = CA + (uint32_t)i;
       head[a] = nc + (uint32_t)(i / k);
       rev[a] = b;
@@ -466,9 +501,9 @@ void dinkelbach_blocks(Ctx &c, const std::vector<BlockTask> &in, std::vector<Blo
   CompTask *dt = f.b_tasks.as<CompTask>();
   CompOut *dout = (CompOut *)(((uintptr_t)(dt + tasks.size()) + 63) & ~(uintptr_t)63);
   HIP_CHECK(hipMemcpyAsync(dt, tasks.data(), sizeof(CompTask) * tasks.size(), hipMemcpyHostToDevice, c.stream));
-  buf_reserve(c, f.scal, 256);
+  buf_reserve(c, f.scal, QUEUE_WORDS * sizeof(uint32_t));
   uint32_t *qctr = (uint32_t *)f.scal.p;
-  HIP_CHECK(hipMemsetAsync(qctr, 0, sizeof(uint32_t), c.stream));
+  HIP_CHECK(hipMemsetAsync(qctr, 0, QUEUE_WORDS * sizeof(uint32_t), c.stream));
   int per_cu = 0;
   HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_dinkelbach_block, 1024, 0));
   if (per_cu < 1) fail(KCDS_E_HIP, "k_dinkelbach_block cannot be resident");

@@ -282,6 +282,189 @@ __global__ void k_graph_summary(uint32_t n, uint64_t T, uint32_t k, const uint32
   }
 }
 
+// ---------------------------------------------------------------------------
+// Workgroup-per-graph peel.  For graphs small enough for one workgroup, a
+// whole peel -- every level and sub-round, the density records and the k'
+// summary -- runs inside one 1024-thread workgroup with workgroup barriers
+// only; the workgroups of a launch take graphs from a sharded queue.  The
+// semantics are those of the step kernels above (levels skip to the minimum
+// live count, a level cascades in sub-rounds, each live clique of a removed
+// vertex is claimed once and its surviving members are decremented, the
+// density live/remaining is recorded after each level with ties kept).
+// Graph g owns original ids [gvoff[g], gvoff[g+1]); its vertex v has new id
+// rank[v]; its frontiers live in fr0/fr1 at the same original-id range.
+constexpr uint32_t BP_THREADS = 1024;
+
+__device__ __forceinline__ ull block_min_u64(ull v, ull *slot) {
+  if (threadIdx.x == 0) *slot = ~0ull;
+  block_sync();
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (ull)__shfl_xor(v, o, 64));
+  if (lane_id() == 0) atomicMin(slot, v);
+  block_sync();
+  const ull r = *slot;
+  block_sync();
+  return r;
+}
+
+__device__ __forceinline__ ull block_sum_u64(ull v, ull *slot) {
+  if (threadIdx.x == 0) *slot = 0;
+  block_sync();
+  v = wave_sum_u64(v);
+  if (lane_id() == 0 && v) atomicAdd(slot, v);
+  block_sync();
+  const ull r = *slot;
+  block_sync();
+  return r;
+}
+
+__global__ __launch_bounds__(BP_THREADS) void k_peel_block(uint32_t G, const uint32_t *__restrict__ gvoff,
+                                                          const uint32_t *__restrict__ rank, uint32_t k,
+                                                          const ull *__restrict__ cnt0, ull *cnt, int32_t *core,
+                                                          int32_t *status, const uint32_t *__restrict__ cl,
+                                                          const ull *__restrict__ incoff,
+                                                          const uint32_t *__restrict__ inc, uint32_t *fr0,
+                                                          uint32_t *fr1, uint32_t *queue, GraphPeel *gp) {
+  __shared__ ull wsum[16];
+  __shared__ ull s_red;
+  __shared__ ull s_pre[BP_THREADS + 1];
+  __shared__ uint32_t s_r[BP_THREADS];
+  __shared__ ull s_b[BP_THREADS];
+  __shared__ uint32_t s_g, s_fsize[2];
+  const uint32_t tid = threadIdx.x, nt = blockDim.x;
+  QueueCursor qc = queue_start(blockIdx.x);
+  while (true) {
+    if (tid == 0) s_g = queue_next(queue, G, qc);
+    block_sync();
+    const uint32_t g = s_g;
+    block_sync();
+    if (g >= G) break;
+    const uint32_t vb = gvoff[g], ng = gvoff[g + 1] - vb;
+    uint32_t *F0 = fr0 + vb, *F1 = fr1 + vb;
+    ull mine = 0;
+    for (uint32_t i = tid; i < ng; i += nt) {
+      const uint32_t r = rank[vb + i];
+      const ull c0 = cnt0[r];
+      stc_u64(&cnt[r], c0);
+      stc_i32(&core[r], -1);
+      mine += c0;
+    }
+    const ull T = block_sum_u64(mine, &s_red) / k;  // sum of per-vertex counts = k T
+    ull live = T, max_num = T, max_den = ng ? ng : 1, L = 0;
+    uint32_t rem = ng, levels = 0;
+    while (rem > 0) {
+      ull m = ~0ull;
+      for (uint32_t i = tid; i < ng; i += nt) {
+        const uint32_t r = rank[vb + i];
+        if (ldc_i32(&core[r]) == -1) m = min(m, ldc_u64(&cnt[r]));
+      }
+      m = block_min_u64(m, &s_red);
+      if (m == ~0ull) break;
+      L = max(L, m);
+      ++levels;
+      if (tid == 0) s_fsize[0] = 0;
+      block_sync();
+      for (uint32_t i0 = 0; i0 < ng; i0 += nt) {  // the level's frontier
+        const uint32_t i = i0 + tid;
+        bool want = false;
+        uint32_t r = 0;
+        if (i < ng) {
+          r = rank[vb + i];
+          want = ldc_i32(&core[r]) == -1 && ldc_u64(&cnt[r]) <= L;
+        }
+        const ull bal = __ballot(want);
+        uint32_t at = 0;
+        if (bal && lane_id() == (uint32_t)__builtin_ctzll(bal)) at = atomicAdd(&s_fsize[0], (uint32_t)__popcll(bal));
+        at = __shfl(at, bal ? __builtin_ctzll(bal) : 0, 64);
+        if (want) stc_u32(&F0[at + lanes_below(bal)], r);
+      }
+      block_sync();
+      uint32_t cur = 0;
+      while (true) {  // sub-rounds of level L
+        const uint32_t fs = s_fsize[cur];
+        if (fs == 0) break;
+        uint32_t *Fc = cur ? F1 : F0, *Fn = cur ? F0 : F1;
+        for (uint32_t i = tid; i < fs; i += nt) stc_i32(&core[ldc_u32(&Fc[i])], (int32_t)L);
+        rem -= fs;
+        if (tid == 0) s_fsize[cur ^ 1] = 0;
+        block_sync();
+        ull destroyed = 0;
+        for (uint32_t f0 = 0; f0 < fs; f0 += nt) {
+          ull len = 0;
+          if (f0 + tid < fs) {
+            const uint32_t v = ldc_u32(&Fc[f0 + tid]);
+            s_r[tid] = v;
+            s_b[tid] = incoff[v];
+            len = incoff[v + 1] - incoff[v];
+          }
+          ull total = 0;
+          const ull ex = block_excl_scan_u64(len, &total, wsum);
+          const uint32_t nv = min(nt, fs - f0);
+          s_pre[tid] = ex;
+          if (tid == 0) s_pre[nv] = total;
+          block_sync();
+          for (ull j = tid; j < total; j += nt) {
+            uint32_t lo = 0, hi = nv;  // owner t: s_pre[t] <= j < s_pre[t+1]
+            while (hi - lo > 1) {
+              const uint32_t mid = (lo + hi) >> 1;
+              if (s_pre[mid] <= j) lo = mid;
+              else hi = mid;
+            }
+            const uint32_t v = s_r[lo];
+            const uint32_t c = inc[s_b[lo] + (j - s_pre[lo])];
+            if (ldc_i32(&status[c]) != -1) continue;
+            if (atomicCAS(&status[c], -1, (int32_t)L) != -1) continue;
+            ++destroyed;
+            for (uint32_t t = 0; t < k; ++t) {
+              const uint32_t u = cl[(ull)c * k + t];
+              if (u == v || ldc_i32(&core[u]) != -1) continue;
+              if (atomicSub(&cnt[u], 1ull) == L + 1) stc_u32(&Fn[atomicAdd(&s_fsize[cur ^ 1], 1u)], u);
+            }
+          }
+          block_sync();  // s_pre, s_r, s_b are reused by the next chunk
+        }
+        live -= block_sum_u64(destroyed, &s_red);
+        cur ^= 1;
+      }
+      // density of the level just completed (ties keep the later core)
+      if (rem > 0 && ratio_ge(live, rem, max_num, max_den)) {
+        max_num = live;
+        max_den = rem;
+      }
+    }
+    // k' = ceil(maxDensity) and the k'-core summary
+    const uint32_t kp = T == 0 ? 0x7FFFFFFFu : (uint32_t)((max_num + max_den - 1) / max_den);
+    ull csize = 0, ccl = 0, mc = 0;
+    for (uint32_t i = tid; i < ng; i += nt) {
+      const uint32_t r = rank[vb + i];
+      const int32_t cv = ldc_i32(&core[r]);
+      if (cv >= (int32_t)kp) csize += 1;
+      if (cv > 0) mc = max(mc, (ull)cv);
+      for (ull j = incoff[r]; j < incoff[r + 1]; ++j) ccl += ldc_i32(&status[inc[j]]) >= (int32_t)kp;
+    }
+    csize = block_sum_u64(csize, &s_red);
+    ccl = block_sum_u64(ccl, &s_red) / k;  // every clique is seen from its k members
+    mc = ~block_min_u64(~mc, &s_red);
+    if (tid == 0) {
+      GraphPeel P;
+      P.live = live;
+      P.max_num = max_num;
+      P.max_den = max_den;
+      P.T = T;
+      P.core_cliques = ccl;
+      P.rem = rem;
+      P.touch = 0;
+      P.n = ng;
+      P.k_prime = kp;
+      P.core_size = (uint32_t)csize;
+      P.max_core = (uint32_t)mc;
+      P.levels = levels;
+      P.pad = 0;
+      gp[g] = P;
+    }
+  }
+}
+
 uint32_t grid1d(uint64_t len, uint32_t cap = 65535u * 16) {
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((len + 255) / 256, cap));
 }
@@ -301,6 +484,38 @@ void graph_ids(Ctx &c) {
   c.sync();  // h_gvoff may change after return
 }
 
+// Graphs up to this size are peeled by one workgroup each.
+constexpr uint32_t BLOCK_PEEL_MAX_N = 1u << 15;
+
+void peel_blocks(Ctx &c) {
+  const uint32_t n = c.n, k = c.k, G = c.ngraphs;
+  const uint64_t T = c.T;
+  buf_reserve(c, c.core, sizeof(int32_t) * ((size_t)n + 1));
+  buf_reserve(c, c.status, sizeof(int32_t) * (T + 1));
+  buf_reserve(c, c.gpeel, sizeof(GraphPeel) * ((size_t)G + 1));
+  buf_reserve(c, c.gvoff, sizeof(uint32_t) * ((size_t)G + 1));
+  HIP_CHECK(hipMemcpyAsync(c.gvoff.p, c.h_gvoff.data(), sizeof(uint32_t) * ((size_t)G + 1), hipMemcpyHostToDevice,
+                           c.stream));
+  const size_t words = (size_t)n * 2 /*cnt*/ + (size_t)n * 2 /*frontiers*/ + QUEUE_WORDS + 64;
+  buf_reserve(c, c.tmp0, words * sizeof(uint32_t) + 256);
+  ull *cnt = c.tmp0.as<ull>();
+  uint32_t *fr0 = (uint32_t *)(cnt + n), *fr1 = fr0 + n, *queue = fr1 + n;
+  HIP_CHECK(hipMemsetAsync(queue, 0, QUEUE_WORDS * sizeof(uint32_t), c.stream));
+  if (T) {
+    k_peel_init<<<grid1d(T), 256, 0, c.stream>>>(0, T, nullptr, nullptr, nullptr, c.status.as<int32_t>());
+    KCDS_LAUNCH_CHECK();
+  }
+  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(G, (uint32_t)c.num_cus * 2));
+  k_peel_block<<<grid, BP_THREADS, 0, c.stream>>>(G, c.gvoff.as<uint32_t>(), c.rank.as<uint32_t>(), k,
+                                                  c.cnt.as<ull>(), cnt, c.core.as<int32_t>(), c.status.as<int32_t>(),
+                                                  c.cliques.as<uint32_t>(), c.incoff.as<ull>(), c.inc.as<uint32_t>(),
+                                                  fr0, fr1, queue, c.gpeel.as<GraphPeel>());
+  KCDS_LAUNCH_CHECK();
+  c.h_gpeel.resize(G);
+  HIP_CHECK(hipMemcpyAsync(c.h_gpeel.data(), c.gpeel.p, sizeof(GraphPeel) * G, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+}
+
 void peel_run(Ctx &c) {
   const uint32_t n = c.n, k = c.k, G = c.ngraphs;
   const uint64_t T = c.T;
@@ -310,6 +525,31 @@ void peel_run(Ctx &c) {
   if (T > 0x7FFFFFFFull && max_u64_host(c, c.cnt.as<ull>(), n) > 0x7FFFFFFFull)
     fail(KCDS_E_LIMIT, "a per-vertex clique count exceeds the int32 clique-core range");
   graph_ids(c);
+  uint32_t largest = 0;
+  for (uint32_t g = 0; g < G; ++g) largest = std::max(largest, c.h_gvoff[g + 1] - c.h_gvoff[g]);
+  if (largest <= BLOCK_PEEL_MAX_N && !(c.tune_flags & KCDS_TUNE_GRID_PEEL)) {
+    peel_blocks(c);
+    const GraphPeel &P = c.h_gpeel[0];
+    kcds_peel_stats ps{};
+    ull g0 = P.max_num, b0 = P.max_den;
+    while (b0) {
+      ull t = g0 % b0;
+      g0 = b0;
+      b0 = t;
+    }
+    if (!g0) g0 = 1;
+    ps.max_num = P.max_num / g0;
+    ps.max_den = P.max_den / g0;
+    ps.k_prime = P.k_prime;
+    ps.levels = P.levels;
+    ps.core_size = P.core_size;
+    ps.core_cliques = P.core_cliques;
+    ps.max_core = (int32_t)P.max_core;
+    c.peel = ps;
+    c.peeled = true;
+    c.have_comp = false;
+    return;
+  }
   buf_reserve(c, c.core, sizeof(int32_t) * ((size_t)n + 1));
   buf_reserve(c, c.status, sizeof(int32_t) * (T + 1));
   buf_reserve(c, c.gpeel, sizeof(GraphPeel) * ((size_t)G + 1));

@@ -133,3 +133,59 @@ __device__ __forceinline__ uint32_t queue_next(uint32_t *q, uint32_t n, QueueCur
   return n;
 }
 
+// Exclusive scan of one value per thread over the workgroup (blockDim a
+// multiple of 64, at most 1024); wsum is 16 words of LDS.  Every thread gets
+// the total.
+__device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t *total, uint32_t *wsum) {
+  const uint32_t lane = threadIdx.x % KCDS_WAVE, w = threadIdx.x / KCDS_WAVE;
+  const uint32_t incl = wave_incl_scan_u32(v);
+  if (lane == KCDS_WAVE - 1) wsum[w] = incl;
+  block_sync();
+  uint32_t before = 0, all = 0;
+  for (uint32_t i = 0; i < blockDim.x / KCDS_WAVE; ++i) {
+    if (i < w) before += wsum[i];
+    all += wsum[i];
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
+__device__ __forceinline__ ull block_excl_scan_u64(ull v, ull *total, ull *wsum) {
+  const uint32_t lane = threadIdx.x % KCDS_WAVE, w = threadIdx.x / KCDS_WAVE;
+  const ull incl = wave_incl_scan_u64(v);
+  if (lane == KCDS_WAVE - 1) wsum[w] = incl;
+  block_sync();
+  ull before = 0, all = 0;
+  for (uint32_t i = 0; i < blockDim.x / KCDS_WAVE; ++i) {
+    if (i < w) before += wsum[i];
+    all += wsum[i];
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
+// Device-coherent loads: read data that other waves of this launch changed
+// with atomics or stores (they bypass this CU's vector L1).
+__device__ __forceinline__ uint32_t ldc_u32(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int32_t ldc_i32(const int32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ ull ldc_u64(const ull *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Device-coherent stores, paired with the loads above for data that other
+// waves of the same launch read or update with atomics.
+__device__ __forceinline__ void stc_u32(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stc_i32(int32_t *p, int32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stc_u64(ull *p, ull v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+

@@ -66,6 +66,8 @@ struct GraphPeel {
   uint32_t k_prime;    // ceil(maxDensity); 0x7FFFFFFF for a graph without cliques
   uint32_t core_size;
   uint32_t max_core;
+  uint32_t levels;     // non-empty peeling levels
+  uint32_t pad;
 };
 
 // Core vertices and core cliques grouped by component (dense index d in

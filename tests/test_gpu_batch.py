@@ -32,8 +32,17 @@ def mixed_batch():
     return gs
 
 
+@pytest.mark.parametrize("grid_peel", [False, True], ids=["block_peel", "grid_peel"])
 @pytest.mark.parametrize("k", [3, 4])
-def test_batch_matches_oracle_and_solo(solver, k):
+def test_batch_matches_oracle_and_solo(solver, k, grid_peel):
+    solver.set_tuning(0, 1 if grid_peel else 0)
+    try:
+        _batch_matches(solver, k)
+    finally:
+        solver.set_tuning(0, 0)
+
+
+def _batch_matches(solver, k):
     gs = mixed_batch()
     res, sets = solver.solve_batch(gs, k)
     for i, (off, nbr) in enumerate(gs):

@@ -54,8 +54,18 @@ def graphs():
 GRAPHS = list(graphs())
 
 
+@pytest.mark.parametrize("grid_peel", [False, True], ids=["block_peel", "grid_peel"])
 @pytest.mark.parametrize("name,off,nbr,ks", GRAPHS, ids=[g[0] for g in GRAPHS])
-def test_peel_and_components(solver, name, off, nbr, ks):
+def test_peel_and_components(solver, name, off, nbr, ks, grid_peel):
+    """Both peels (workgroup per graph, and grid-wide step kernels) against the oracle."""
+    solver.set_tuning(0, 1 if grid_peel else 0)
+    try:
+        _peel_and_components(solver, name, off, nbr, ks)
+    finally:
+        solver.set_tuning(0, 0)
+
+
+def _peel_and_components(solver, name, off, nbr, ks):
     n = len(off) - 1
     for k in ks:
         solver.upload(off, nbr)
