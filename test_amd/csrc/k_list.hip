@@ -350,6 +350,20 @@ __global__ __launch_bounds__(64 * MAX_WAVES) void k_list_count(const uint32_t *_
   if (tid == 0 && tot) atomicAdd(mine, tot);
 }
 
+// gT[gid[r]] += rootT[r], reduced per wave when neighbours share a graph
+__global__ void k_graph_T(const ull *rootT, const uint32_t *gid, uint32_t n, ull *gT) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
+    if (rootT[r]) atomicAdd(&gT[gid[r]], rootT[r]);
+}
+
+__global__ void k_drop_graphs(ull *rootT, ull *cnt, const uint32_t *gid, const uint8_t *drop, uint32_t n) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
+    if (drop[gid[r]]) {
+      rootT[r] = 0;
+      cnt[r] = 0;
+    }
+}
+
 // cnt[v] = sum of the ncopy copies (copy 0 is cnt itself)
 __global__ void k_sum_copies(ull *cnt, uint64_t n, uint64_t cstride, uint32_t ncopy) {
   for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
@@ -685,6 +699,43 @@ void list_count(Ctx &c, uint32_t k) {
   c.listed = false;
   c.peeled = false;
   c.have_comp = false;
+}
+
+void graph_counts(Ctx &c, std::vector<ull> &gT) {
+  const uint32_t G = c.ngraphs, n = c.n;
+  gT.assign(G, 0);
+  if (G == 1) {
+    gT[0] = c.T;
+    return;
+  }
+  buf_reserve(c, c.tmp3, sizeof(ull) * ((size_t)G + 1));
+  ull *d = c.tmp3.as<ull>();
+  HIP_CHECK(hipMemsetAsync(d, 0, sizeof(ull) * G, c.stream));
+  if (n) {
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)c.num_cus * 8);
+    k_graph_T<<<grid, 256, 0, c.stream>>>(c.rootT.as<ull>(), c.gid.as<uint32_t>(), n, d);
+    KCDS_LAUNCH_CHECK();
+  }
+  HIP_CHECK(hipMemcpyAsync(gT.data(), d, sizeof(ull) * G, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+}
+
+ull drop_graphs(Ctx &c, const std::vector<uint8_t> &drop, const std::vector<ull> &gT) {
+  const uint32_t G = c.ngraphs, n = c.n;
+  ull keep = 0;
+  for (uint32_t g = 0; g < G; ++g)
+    if (!drop[g]) keep += gT[g];
+  buf_reserve(c, c.tmp3, (size_t)G + 8);
+  HIP_CHECK(hipMemcpyAsync(c.tmp3.p, drop.data(), G, hipMemcpyHostToDevice, c.stream));
+  if (n) {
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)c.num_cus * 8);
+    k_drop_graphs<<<grid, 256, 0, c.stream>>>(c.rootT.as<ull>(), c.cnt.as<ull>(), c.gid.as<uint32_t>(),
+                                              c.tmp3.as<uint8_t>(), n);
+    KCDS_LAUNCH_CHECK();
+  }
+  c.sync();  // drop lives on the host
+  c.T = keep;
+  return keep;
 }
 
 void list_materialize(Ctx &c, uint32_t k) {

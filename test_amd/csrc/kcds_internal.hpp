@@ -98,6 +98,7 @@ struct Ctx {
   uint32_t list_waves_per_root[4] = {0, 0, 0, 0};  // 0 = default of the class
   float tune_ms[4] = {0, 0, 0, 0};
   uint64_t order_steps = 0;  // work/control steps of the last GPU ordering
+  uint64_t graph_budget = 0; // per-graph cap on T * (8k + 4) bytes (0 = none)
 
   // pinned host staging for small readbacks
   void *pinned = nullptr;
@@ -183,6 +184,11 @@ void degeneracy_rank_gpu(Ctx &c);    // fills rank/inv from a GPU k-core peel
 // ---- listing (k_list.hip) ----
 void list_count(Ctx &c, uint32_t k);     // cnt, T
 void list_autotune(Ctx &c, uint32_t k);  // sets list_waves_per_root, tune_ms
+// Per-graph clique counts of the last count pass (needs graph ids for G > 1).
+void graph_counts(Ctx &c, std::vector<ull> &gT);
+// Drops the cliques of the flagged graphs from the count pass (roots and
+// per-vertex counts zeroed); returns the remaining T.
+ull drop_graphs(Ctx &c, const std::vector<uint8_t> &drop, const std::vector<ull> &gT);
 uint32_t list_waves_per_root(const Ctx &c, int cls, uint32_t k);  // effective waves per root of a class
 void list_materialize(Ctx &c, uint32_t k);  // cliques, incoff, inc (requires list_count)
 

@@ -113,16 +113,44 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
     c.oriented = true;
     R.ms_orient = t.stop();
   }
-  if (G > 1) check_partition(c);
+  if (G > 1) {
+    check_partition(c);
+    graph_ids(c);
+  }
+  std::vector<ull> gT;
+  std::vector<uint8_t> drop(G, 0);
   {
     Timer t(c.stream);
     list_count(c, k);
     R.T = c.T;
+    // per-graph HBM admission on the count pass: cliques, incidence, status
+    graph_counts(c, gT);
+    uint32_t ndrop = 0;
+    if (c.graph_budget)
+      for (uint32_t g = 0; g < G; ++g)
+        if ((double)gT[g] * (8.0 * k + 4.0) > (double)c.graph_budget) {
+          drop[g] = 1;
+          ++ndrop;
+        }
+    if (ndrop && G == 1)
+      fail(KCDS_E_NOMEM, "graph has %llu %u-cliques: %.0f bytes to materialise, over the graph budget %llu",
+           (unsigned long long)gT[0], k, (double)gT[0] * (8.0 * k + 4.0), (unsigned long long)c.graph_budget);
+    if (ndrop) drop_graphs(c, drop, gT);
     if (c.T) list_materialize(c, k);
     R.ms_list = t.stop();
   }
   if (in_set && c.n) memset(in_set, 0, c.n);
+  auto finish_status = [&]() {
+    for (uint32_t gi = 0; gi < G; ++gi)
+      if (drop[gi]) {
+        memset(&out[gi], 0, sizeof(kcds_graph_result));
+        out[gi].T = gT[gi];
+        out[gi].status = KCDS_GRAPH_OVER_BUDGET;
+      }
+  };
   if (c.T == 0) {  // main.cu:1431-1438: "Max Density 0 Size 0" for every graph
+    for (uint32_t gi = 0; gi < G; ++gi) out[gi].T = gT[gi];
+    finish_status();
     R.ms_total = total.stop();
     if (agg) *agg = R;
     if (per_graph) memcpy(per_graph, out.data(), sizeof(kcds_graph_result) * G);
@@ -224,6 +252,7 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
   R.rho_den = out[0].rho_den;
   for (uint32_t gi = 0; gi < G; ++gi) R.flow_calls += out[gi].flow_calls;
   R.ms_flow = tf.stop();
+  finish_status();
   if (in_set && c.n) {
     std::vector<uint8_t> tmp(c.n);
     std::vector<uint32_t> rank(c.n);

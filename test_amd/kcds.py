@@ -26,7 +26,10 @@ U64P = ctypes.POINTER(ctypes.c_uint64)
 EXPORTS = ("kcds_create", "kcds_destroy", "kcds_last_error", "kcds_upload_csr", "kcds_orient", "kcds_count",
            "kcds_list", "kcds_download_cliques", "kcds_peel", "kcds_components", "kcds_maxflow", "kcds_solve",
            "kcds_sync", "kcds_set_tuning", "kcds_get_list_info", "kcds_solve_batch", "kcds_set_list_shape",
-           "kcds_autotune_list")
+           "kcds_autotune_list", "kcds_set_graph_budget")
+
+GRAPH_SOLVED, GRAPH_OVER_BUDGET = 0, 1
+E_INVALID, E_HIP, E_NOMEM, E_STATE, E_LIMIT = -1, -2, -3, -4, -5
 
 
 class PeelStats(ctypes.Structure):
@@ -46,7 +49,7 @@ class Result(ctypes.Structure):
 class GraphResult(ctypes.Structure):
     _fields_ = [("T", ctypes.c_uint64), ("rho_num", ctypes.c_uint64), ("rho_den", ctypes.c_uint64),
                 ("k_prime", ctypes.c_uint32), ("core_vertices", ctypes.c_uint32), ("ncomp", ctypes.c_uint32),
-                ("flow_calls", ctypes.c_uint32)]
+                ("flow_calls", ctypes.c_uint32), ("status", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
 
 
 class ListInfo(ctypes.Structure):
@@ -89,6 +92,7 @@ def lib():
         L.kcds_get_list_info.argtypes = [V, ctypes.POINTER(ListInfo)]
         L.kcds_set_list_shape.argtypes = [V, U32P]
         L.kcds_autotune_list.argtypes = [V, ctypes.c_uint32, U32P, ctypes.POINTER(ctypes.c_float)]
+        L.kcds_set_graph_budget.argtypes = [V, ctypes.c_uint64]
         _LIB = L
     return _LIB
 
@@ -193,11 +197,25 @@ class Solver:
         sets = [ins[voff[i]:voff[i + 1]].astype(bool) for i in range(G)] if want_set else None
         return list(res), sets
 
+    def solve_resident_batch(self, voff, k, want_set=False):
+        """Solves the disjoint graphs of the already uploaded union, split at voff (G+1 offsets)."""
+        voff = np.ascontiguousarray(voff, dtype=np.uint32)
+        G = len(voff) - 1
+        res = (GraphResult * G)()
+        ins = np.zeros(max(self.n, 1), np.uint8) if want_set else None
+        self._chk(lib().kcds_solve_batch(self._h, k, G, _p(voff, U32P), res, _p(ins, U8P)))
+        sets = [ins[voff[i]:voff[i + 1]].astype(bool) for i in range(G)] if want_set else None
+        return list(res), sets
+
     def sync(self):
         self._chk(lib().kcds_sync(self._h))
 
     def set_tuning(self, waves_per_cu=0, flags=0):
         self._chk(lib().kcds_set_tuning(self._h, waves_per_cu, flags))
+
+    def set_graph_budget(self, nbytes):
+        """Per-graph cap on T*(8k+4) bytes of materialised cliques (0 = none)."""
+        self._chk(lib().kcds_set_graph_budget(self._h, int(nbytes)))
 
     def set_list_shape(self, waves_per_root=None):
         """Waves per root of the four width classes (None = library defaults)."""
