@@ -154,6 +154,9 @@ int kcds_sync(kcds_ctx *ctx);
  *                        graph fits the workgroup-per-graph peel (both give
  *                        the same results; used to test one against the other). */
 #define KCDS_TUNE_GRID_PEEL 1u
+/*   KCDS_TUNE_GRID_FLOW  solve every component with the grid-wide max-flow
+ *                        (normally only components of >= 65,536 cliques). */
+#define KCDS_TUNE_GRID_FLOW 2u
 int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
 
 /* Listing launch shape: waves per root (workgroup size / 64, 1..8) of each

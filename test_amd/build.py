@@ -23,7 +23,7 @@ HIP_FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 LIB_SOURCES = ["kcds_api.cpp", "kcds_solve.cpp", "k_util.hip", "k_orient.hip", "k_list.hip", "k_peel.hip",
-               "k_comp.hip", "k_flow.hip"]
+               "k_comp.hip", "k_flow.hip", "k_gflow.hip"]
 
 
 def _run(cmd):

@@ -1,0 +1,490 @@
+// Exact max-flow of one large clique flow network with the whole GPU.
+//
+// Reference: the same network and push-relabel scheme as the workgroup
+// solver (k_flow.hip; src/helpers.cu:1148-1509), whose single workgroup is
+// the bottleneck when one component holds hundreds of thousands of cliques.
+// Here each step of push-relabel is a grid-wide kernel:
+//   build     degree count, scan, then arcs (same layout as the workgroup
+//             solver: vertex v: [v->t, v->s, v->c ...]; clique c: k arcs;
+//             s: nc arcs; t: nc arcs) with capacities scaled by q;
+//   relabel   level-synchronous BFS from t over residual arcs, one kernel per
+//             level (frontier in NSEG segments, the last workgroup to finish
+//             a level swaps the frontiers), giving exact distances to t;
+//   sweeps    discharge of every active node (wave per vertex node, thread
+//             per clique node): push the excess along all residual arcs to
+//             the lowest neighbour height, or relabel; a sweep is a no-op
+//             when the previous one changed nothing.
+// The host runs the phases (one readback per phase) and the Dinkelbach
+// iteration.  Only a node's owner lowers its excess and the residuals of its
+// arcs; other workgroups only raise them (atomics), so an owner that reads a
+// value older than the latest pushes at most what is there.  Workgroups share
+// only atomic counters inside a kernel; kernel boundaries order the steps.
+#include <algorithm>
+#include <cstring>
+
+#include "kcds_device.hpp"
+#include "kcds_internal.hpp"
+
+namespace kcds {
+
+namespace {
+
+typedef long long i64;
+
+constexpr uint32_t NSEG = 8;
+constexpr uint32_t SWEEPS = 24;
+constexpr uint32_t MAX_PHASES = 1u << 14;
+constexpr uint32_t NO_ARC = 0xFFFFFFFFu;
+
+// Counters 64 words apart (same-address device atomics are serialised).
+struct GFCtl {
+  uint32_t dist, cur, done, pad0[64 - 3];
+  uint32_t fsize[2][NSEG][64];
+  uint32_t gblocks[NSEG][64];
+  uint32_t blocks[64];
+  uint32_t changed[SWEEPS + 1][64];
+  ull active[NSEG][8];
+  ull sums[2][8];  // side vertices, side cliques
+};
+
+__device__ __forceinline__ void add64(i64 *p, i64 v) { atomicAdd((ull *)p, (ull)v); }
+
+__global__ void k_gf_fill_u32(uint32_t *p, uint32_t v, uint64_t len) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// cnt[v] = 2 + number of cliques containing v (v->t, v->s, v->c arcs)
+__global__ void k_gf_deg(const uint32_t *ccl, uint64_t Tk, uint32_t *cnt) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < Tk; i += (uint64_t)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[ccl[i]], 1u);
+}
+
+__global__ void k_gf_topo_fixed(uint32_t nc, uint64_t Tc, uint32_t k, uint32_t *off, uint32_t *head, uint32_t *rev,
+                                uint32_t *fill) {
+  const uint32_t CA = off[nc];  // vertex arcs end here (off[0..nc] is the vertex scan)
+  const uint32_t SA = CA + (uint32_t)(Tc * k), TA = SA + nc, s = nc + (uint32_t)Tc, t = s + 1;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nc; v += stride) {
+    const uint32_t a = off[v];
+    head[a] = t;
+    rev[a] = TA + (uint32_t)v;
+    head[TA + v] = (uint32_t)v;
+    rev[TA + v] = a;
+    head[a + 1] = s;
+    rev[a + 1] = SA + (uint32_t)v;
+    head[SA + v] = (uint32_t)v;
+    rev[SA + v] = a + 1;
+    fill[v] = 0;
+  }
+  // clique c's arcs start at CA + c*k (c = Tc gives off[s] = SA); t and the
+  // end sentinel follow
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c <= Tc; c += stride)
+    off[nc + c] = CA + (uint32_t)(c * k);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    off[s + 1] = TA;
+    off[t + 1] = TA + nc;
+  }
+}
+
+__global__ void k_gf_topo_cliques(const uint32_t *ccl, uint64_t Tc, uint32_t k, uint32_t nc, const uint32_t *off,
+                                  uint32_t *head, uint32_t *rev, uint32_t *fill) {
+  const uint32_t CA = off[nc];
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < Tc * k; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = ccl[i];
+    const uint32_t a = off[v] + 2 + atomicAdd(&fill[v], 1u), b = CA + (uint32_t)i;
+    head[a] = nc + (uint32_t)(i / k);
+    rev[a] = b;
+    head[b] = v;
+    rev[b] = a;
+  }
+}
+
+// residual capacities at alpha = p/q after the preflow saturates s->v
+__global__ void k_gf_init(uint32_t nc, uint64_t Tc, uint32_t k, i64 p, i64 q, const uint32_t *off,
+                          const uint32_t *rev, i64 *res, i64 *excess) {
+  const uint32_t CA = off[nc], SA = CA + (uint32_t)(Tc * k), TA = SA + nc, N = nc + (uint32_t)Tc + 2;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nc; v += stride) {
+    const uint32_t a = off[v];
+    const i64 deg = (i64)(off[v + 1] - a - 2);
+    res[a] = (i64)k * p;
+    res[TA + v] = 0;
+    res[a + 1] = q * deg;
+    res[SA + v] = 0;
+    excess[v] = q * deg;
+  }
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < Tc * k; i += stride) {
+    const uint32_t b = CA + (uint32_t)i;
+    res[b] = q * (i64)(k - 1);
+    res[rev[b]] = q;
+  }
+  for (uint64_t x = nc + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < N; x += stride) excess[x] = 0;
+}
+
+// heights = N except t = 0; frontier {t}; phase counters cleared
+__global__ void k_gf_bfs_init(uint32_t N, uint32_t *height, uint32_t *fr0, GFCtl *ctl) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < N; x += stride)
+    height[x] = x == N - 1 ? 0u : N;
+  if (blockIdx.x == 0) {
+    const uint32_t t = threadIdx.x;
+    if (t < NSEG) {
+      ctl->fsize[0][t][0] = t == 0 ? 1u : 0u;
+      ctl->fsize[1][t][0] = 0;
+      ctl->gblocks[t][0] = 0;
+      ctl->active[t][0] = 0;
+    }
+    for (uint32_t i = t; i <= SWEEPS; i += blockDim.x) ctl->changed[i][0] = i == 0 ? 1u : 0u;
+    if (t == 0) {
+      fr0[0] = N - 1;
+      ctl->blocks[0] = 0;
+      ctl->dist = 0;
+      ctl->cur = 0;
+      ctl->done = 0;
+    }
+  }
+}
+
+// The last workgroup of a launch (two-level ticket) returns true.
+__device__ bool last_block(GFCtl *ctl, uint32_t *s_last) {
+  block_sync();
+  if (threadIdx.x == 0) {
+    const uint32_t seg = blockIdx.x % NSEG;
+    const uint32_t in_seg = (gridDim.x - seg + NSEG - 1) / NSEG;
+    uint32_t last = 0;
+    if (atomicAdd(&ctl->gblocks[seg][0], 1u) == in_seg - 1) last = atomicAdd(&ctl->blocks[0], 1u) == min(gridDim.x, NSEG) - 1;
+    *s_last = last;
+  }
+  block_sync();
+  return *s_last != 0;
+}
+
+// One BFS level: wave per frontier node y; x = head of an arc of y with
+// residual capacity on x -> y gets height dist + 1.  seg_cap: capacity of a
+// frontier segment.
+__global__ __launch_bounds__(256) void k_gf_bfs_level(uint32_t N, const uint32_t *off, const uint32_t *head,
+                                                      const uint32_t *rev, const i64 *res, uint32_t *height,
+                                                      uint32_t *fr0, uint32_t *fr1, uint32_t seg_cap, GFCtl *ctl) {
+  __shared__ uint32_t s_hdr[3], s_pre[NSEG + 1], s_last;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), seg = blockIdx.x % NSEG;
+  if (tid < 3) s_hdr[tid] = (&ctl->dist)[tid];
+  block_sync();
+  const uint32_t dist = s_hdr[0], cur = s_hdr[1], done = s_hdr[2];
+  if (done) return;
+  if (tid < NSEG) s_pre[tid + 1] = ctl->fsize[cur][tid][0];
+  if (tid == 0) s_pre[0] = 0;
+  block_sync();
+  if (tid == 0)
+    for (uint32_t g = 0; g < NSEG; ++g) s_pre[g + 1] += s_pre[g];
+  block_sync();
+  const uint32_t fs = s_pre[NSEG];
+  const uint32_t *fr = cur ? fr1 : fr0;
+  uint32_t *nx = cur ? fr0 : fr1;
+  const uint32_t s = N - 2;
+  const uint32_t wid = (blockIdx.x * blockDim.x + tid) / KCDS_WAVE, nw = gridDim.x * blockDim.x / KCDS_WAVE;
+  for (uint32_t i = wid; i < fs; i += nw) {
+    uint32_t g = 0;
+    while (i >= s_pre[g + 1]) ++g;
+    const uint32_t y = fr[(size_t)g * seg_cap + (i - s_pre[g])];
+    const uint32_t ye = off[y + 1];
+    for (uint32_t a0 = off[y]; a0 < ye; a0 += KCDS_WAVE) {
+      const uint32_t a = a0 + lane;
+      bool add = false;
+      uint32_t x = 0;
+      if (a < ye) {
+        x = head[a];
+        if (x != s && res[rev[a]] > 0 && height[x] == N) add = atomicCAS(&height[x], N, dist + 1) == N;
+      }
+      const ull bal = __ballot(add);
+      if (bal) {
+        const uint32_t lead = (uint32_t)__builtin_ctzll(bal);
+        uint32_t base = 0;
+        if (lane == lead) base = atomicAdd(&ctl->fsize[cur ^ 1][seg][0], (uint32_t)__popcll(bal));
+        base = __shfl(base, lead, 64);
+        if (add) nx[(size_t)seg * seg_cap + base + lanes_below(bal)] = x;
+      }
+    }
+  }
+  if (last_block(ctl, &s_last) && tid == 0) {
+    uint32_t nnext = 0;
+    for (uint32_t g2 = 0; g2 < NSEG; ++g2) {
+      nnext += atomicAdd(&ctl->fsize[cur ^ 1][g2][0], 0u);
+      ctl->fsize[cur][g2][0] = 0;
+      ctl->gblocks[g2][0] = 0;
+    }
+    ctl->blocks[0] = 0;
+    ctl->cur = cur ^ 1;
+    ctl->dist = dist + 1;
+    if (nnext == 0) ctl->done = 1;
+  }
+}
+
+// Active nodes (excess > 0 and can reach t), summed per segment.
+__global__ __launch_bounds__(256) void k_gf_active(uint32_t N, const i64 *excess, const uint32_t *height,
+                                                   GFCtl *ctl) {
+  __shared__ ull s;
+  if (threadIdx.x == 0) s = 0;
+  __syncthreads();
+  ull a = 0;
+  for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < N - 2; x += (uint64_t)gridDim.x * blockDim.x)
+    a += excess[x] > 0 && height[x] < N;
+  a = wave_sum_u64(a);
+  if (lane_id() == 0 && a) atomicAdd(&s, a);
+  __syncthreads();
+  if (threadIdx.x == 0 && s) atomicAdd(&ctl->active[blockIdx.x % NSEG][0], s);
+}
+
+// Vertex nodes, wave per node; sweep i counts into changed[i+1] and is a
+// no-op when changed[i] is zero.
+__global__ __launch_bounds__(256) void k_gf_sweep_v(uint32_t nc, uint32_t N, uint32_t sweep, const uint32_t *off,
+                                                    const uint32_t *head, const uint32_t *rev, i64 *res, i64 *excess,
+                                                    uint32_t *height, GFCtl *ctl) {
+  if (ctl->changed[sweep][0] == 0) return;
+  const uint32_t lane = lane_id();
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KCDS_WAVE, nw = gridDim.x * blockDim.x / KCDS_WAVE;
+  uint32_t changed = 0;
+  for (uint32_t u = wid; u < nc; u += nw) {
+    const i64 e = excess[u];
+    if (e <= 0 || height[u] >= N) continue;
+    const uint32_t b = off[u], en = off[u + 1];
+    uint32_t hmin = NO_ARC;
+    for (uint32_t a = b + lane; a < en; a += KCDS_WAVE)
+      if (res[a] > 0) hmin = min(hmin, height[head[a]]);
+    hmin = wave_min_u32(hmin);
+    if (hmin == NO_ARC) continue;
+    if (height[u] <= hmin) {
+      if (lane == 0) height[u] = hmin + 1;
+      ++changed;
+      continue;
+    }
+    i64 left = e;
+    for (uint32_t a0 = b; a0 < en && left > 0; a0 += KCDS_WAVE) {
+      const uint32_t a = a0 + lane;
+      i64 r = 0;
+      uint32_t x = 0;
+      if (a < en) {
+        r = res[a];
+        if (r > 0) {
+          x = head[a];
+          if (height[x] != hmin) r = 0;
+        } else {
+          r = 0;
+        }
+      }
+      const ull incl = wave_incl_scan_u64((ull)r);
+      const i64 before = (i64)incl - r;
+      const i64 d = r > 0 && left > before ? min(r, left - before) : 0;
+      if (d > 0) {
+        add64(&res[a], -d);
+        add64(&res[rev[a]], d);
+        add64(&excess[x], d);
+      }
+      const i64 tot = (i64)__shfl(incl, 63, 64);
+      left -= min(tot, left);
+    }
+    if (e > left) {
+      if (lane == 0) add64(&excess[u], -(e - left));
+      ++changed;
+    }
+  }
+  if (lane == 0 && changed) atomicAdd(&ctl->changed[sweep + 1][0], changed);
+}
+
+// Clique nodes, thread per node (k arcs each).
+__global__ __launch_bounds__(256) void k_gf_sweep_c(uint32_t nc, uint32_t N, uint32_t sweep, const uint32_t *off,
+                                                    const uint32_t *head, const uint32_t *rev, i64 *res, i64 *excess,
+                                                    uint32_t *height, GFCtl *ctl) {
+  if (ctl->changed[sweep][0] == 0) return;
+  uint32_t changed = 0;
+  const uint32_t s = N - 2;
+  for (uint64_t u = nc + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < s; u += (uint64_t)gridDim.x * blockDim.x) {
+    const i64 e = excess[u];
+    if (e <= 0 || height[u] >= N) continue;
+    const uint32_t b = off[u], en = off[u + 1];
+    uint32_t hmin = NO_ARC;
+    for (uint32_t a = b; a < en; ++a)
+      if (res[a] > 0) hmin = min(hmin, height[head[a]]);
+    if (hmin == NO_ARC) continue;
+    if (height[u] <= hmin) {
+      height[u] = hmin + 1;
+      ++changed;
+      continue;
+    }
+    i64 left = e;
+    for (uint32_t a = b; a < en && left > 0; ++a) {
+      const i64 r = res[a];
+      if (r <= 0) continue;
+      const uint32_t x = head[a];
+      if (height[x] != hmin) continue;
+      const i64 d = min(r, left);
+      add64(&res[a], -d);
+      add64(&res[rev[a]], d);
+      add64(&excess[x], d);
+      left -= d;
+    }
+    if (e > left) {
+      add64(&excess[u], -(e - left));
+      ++changed;
+    }
+  }
+  changed = wave_sum_u32(changed);
+  if (lane_id() == 0 && changed) atomicAdd(&ctl->changed[sweep + 1][0], changed);
+}
+
+// Source side of the maximal min cut: vertices that cannot reach t, and the
+// cliques with every member on that side; side flags into cside.
+__global__ __launch_bounds__(256) void k_gf_side(uint32_t nc, uint64_t Tc, uint32_t k, uint32_t N, const uint32_t *ccl,
+                                                 const uint32_t *height, uint8_t *cside, GFCtl *ctl) {
+  __shared__ ull s0, s1;
+  if (threadIdx.x == 0) {
+    s0 = 0;
+    s1 = 0;
+  }
+  __syncthreads();
+  ull sn = 0, sT = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nc; v += stride) {
+    const bool in = height[v] >= N;
+    cside[v] = in ? 1 : 0;
+    sn += in;
+  }
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < Tc; c += stride) {
+    bool all = true;
+    for (uint32_t j = 0; j < k; ++j) all = all && height[ccl[c * k + j]] >= N;
+    sT += all;
+  }
+  sn = wave_sum_u64(sn);
+  sT = wave_sum_u64(sT);
+  if (lane_id() == 0) {
+    if (sn) atomicAdd(&s0, sn);
+    if (sT) atomicAdd(&s1, sT);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s0) atomicAdd(&ctl->sums[0][0], s0);
+    if (s1) atomicAdd(&ctl->sums[1][0], s1);
+  }
+}
+
+uint32_t ggrid(const Ctx &c, uint64_t len) {
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((len + 255) / 256, (uint64_t)c.num_cus * 8));
+}
+
+ull gcd_u64(ull a, ull b) {
+  while (b) {
+    const ull t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+}  // namespace
+
+// Dinkelbach search on component d with grid-wide max-flows (as the
+// workgroup solver: at most max_iters flows; the final alpha, the maximal
+// min-cut side at it and the last flow value).
+void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t max_iters) {
+  CompData &g = c.cd;
+  FlowBufs &f = c.fb;
+  const uint32_t k = c.k, d = task.d;
+  const uint32_t nc = g.h_voff[d + 1] - g.h_voff[d];
+  const uint64_t Tc = g.h_coff[d + 1] - g.h_coff[d];
+  const uint64_t Nn = (uint64_t)nc + Tc + 2, A = 4ull * nc + 2ull * k * Tc;
+  if (Nn >= 0x7FFFFFFFull || A >= 0xFFFFFFFFull)
+    fail(KCDS_E_LIMIT, "flow network of component %u too large (%llu cliques)", d, (unsigned long long)Tc);
+  if ((double)task.q0 * k * (double)(Tc + 1) > 9.0e18 || (double)nc * k * (double)(Tc + 1) > 9.0e18)
+    fail(KCDS_E_LIMIT, "capacities of component %u exceed int64", d);
+  const uint32_t N = (uint32_t)Nn;
+  const uint32_t seg_cap = ((N + 63) / 64) * 64;
+  buf_reserve(c, f.g_off, sizeof(uint32_t) * (Nn + 2));
+  buf_reserve(c, f.g_head, sizeof(uint32_t) * (A + 1));
+  buf_reserve(c, f.g_rev, sizeof(uint32_t) * (A + 1));
+  buf_reserve(c, f.g_res, sizeof(i64) * (A + 1));
+  buf_reserve(c, f.g_excess, sizeof(i64) * (Nn + 1));
+  buf_reserve(c, f.g_height, sizeof(uint32_t) * (Nn + 1));
+  buf_reserve(c, f.g_fr, sizeof(uint32_t) * 2ull * NSEG * seg_cap);
+  buf_reserve(c, f.g_ctl, sizeof(GFCtl) + 256);
+  uint32_t *off = f.g_off.as<uint32_t>(), *head = f.g_head.as<uint32_t>(), *rev = f.g_rev.as<uint32_t>();
+  i64 *res = f.g_res.as<i64>(), *excess = f.g_excess.as<i64>();
+  uint32_t *height = f.g_height.as<uint32_t>();
+  uint32_t *fr0 = f.g_fr.as<uint32_t>(), *fr1 = fr0 + (size_t)NSEG * seg_cap;
+  GFCtl *ctl = f.g_ctl.as<GFCtl>();
+  const uint32_t *ccl = g.ccl.as<uint32_t>() + g.h_coff[d] * k;
+  uint8_t *cside = f.cside.as<uint8_t>() + g.h_voff[d];
+  // ---- topology: vertex arc counts, scan, fixed arcs, clique arcs ----
+  uint32_t *cnt = height;  // scratch before the first relabel
+  k_gf_fill_u32<<<ggrid(c, nc), 256, 0, c.stream>>>(cnt, 2u, nc);
+  k_gf_deg<<<ggrid(c, Tc * k), 256, 0, c.stream>>>(ccl, Tc * k, cnt);
+  KCDS_LAUNCH_CHECK();
+  scan_u32_to_u32(c, cnt, off, nc);  // off[0..nc]
+  uint32_t *fill = fr0;
+  k_gf_topo_fixed<<<ggrid(c, std::max<uint64_t>(nc, Tc + 1)), 256, 0, c.stream>>>(nc, Tc, k, off, head, rev, fill);
+  k_gf_topo_cliques<<<ggrid(c, Tc * k), 256, 0, c.stream>>>(ccl, Tc, k, nc, off, head, rev, fill);
+  KCDS_LAUNCH_CHECK();
+  GFCtl *h = (GFCtl *)c.host_scratch(sizeof(GFCtl));
+  const uint32_t bgrid = (uint32_t)c.num_cus * 4;  // BFS levels and sweeps: 256-thread blocks
+  ull p = task.p0, q = task.q0;
+  uint32_t flows = 0;
+  i64 flow = 0;
+  ull sn = 0, sT = 0;
+  for (uint32_t it = 0;; ++it) {
+    k_gf_init<<<ggrid(c, std::max<uint64_t>(Nn, Tc * k)), 256, 0, c.stream>>>(nc, Tc, k, (i64)p, (i64)q, off, rev, res,
+                                                                             excess);
+    KCDS_LAUNCH_CHECK();
+    uint32_t phase = 0;
+    for (;; ++phase) {
+      if (phase >= MAX_PHASES) fail(KCDS_E_LIMIT, "grid max-flow of component %u exceeded %u phases", d, MAX_PHASES);
+      k_gf_bfs_init<<<ggrid(c, Nn), 256, 0, c.stream>>>(N, height, fr0, ctl);
+      KCDS_LAUNCH_CHECK();
+      uint32_t levels = 0;
+      while (true) {
+        const uint32_t batch = 16;
+        for (uint32_t i = 0; i < batch; ++i)
+          k_gf_bfs_level<<<bgrid, 256, 0, c.stream>>>(N, off, head, rev, res, height, fr0, fr1, seg_cap, ctl);
+        KCDS_LAUNCH_CHECK();
+        levels += batch;
+        k_gf_active<<<ggrid(c, Nn), 256, 0, c.stream>>>(N, excess, height, ctl);
+        KCDS_LAUNCH_CHECK();
+        HIP_CHECK(hipMemcpyAsync(h, ctl, sizeof(GFCtl), hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        if (h->done) break;
+        if (levels > N + 16) fail(KCDS_E_HIP, "relabel BFS of component %u did not finish", d);
+        // not finished: the active count was taken too early; clear it
+        HIP_CHECK(hipMemsetAsync(&ctl->active[0][0], 0, sizeof(ctl->active), c.stream));
+      }
+      ull active = 0;
+      for (uint32_t s = 0; s < NSEG; ++s) active += h->active[s][0];
+      if (active == 0) break;
+      for (uint32_t sw = 0; sw < SWEEPS; ++sw) {
+        k_gf_sweep_v<<<bgrid, 256, 0, c.stream>>>(nc, N, sw, off, head, rev, res, excess, height, ctl);
+        k_gf_sweep_c<<<ggrid(c, Tc), 256, 0, c.stream>>>(nc, N, sw, off, head, rev, res, excess, height, ctl);
+      }
+      KCDS_LAUNCH_CHECK();
+    }
+    ++flows;
+    HIP_CHECK(hipMemsetAsync(&ctl->sums[0][0], 0, sizeof(ctl->sums), c.stream));
+    k_gf_side<<<ggrid(c, std::max<uint64_t>(nc, Tc)), 256, 0, c.stream>>>(nc, Tc, k, N, ccl, height, cside, ctl);
+    KCDS_LAUNCH_CHECK();
+    HIP_CHECK(hipMemcpyAsync(&flow, excess + (N - 1), sizeof(i64), hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipMemcpyAsync(h, ctl, sizeof(GFCtl), hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    sn = h->sums[0][0];
+    sT = h->sums[1][0];
+    const ull full = q * k * Tc;
+    if ((ull)flow >= full || it + 1 >= max_iters) break;  // no set denser than alpha
+    if (sn == 0 || it + 1 >= 64) fail(KCDS_E_LIMIT, "grid search on component %u did not converge", d);
+    const ull gg = gcd_u64(sT, sn);
+    p = sT / gg;
+    q = sn / gg;
+  }
+  out.p = p;
+  out.q = q;
+  out.side_n = sn;
+  out.side_T = sT;
+  out.flow = flow;
+  out.flows = flows;
+}
+
+}  // namespace kcds

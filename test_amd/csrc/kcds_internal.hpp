@@ -85,6 +85,7 @@ struct FlowBufs {
   DevBuf scal;
   DevBuf cside;  // uint8 per core vertex (component-grouped): min-cut side of each solved component
   DevBuf b_off, b_excess, b_height, b_fr0, b_fr1, b_head, b_rev, b_res, b_tasks;  // block solver slices
+  DevBuf g_off, g_head, g_rev, g_res, g_excess, g_height, g_fr, g_ctl;           // grid solver (one component)
 };
 
 struct Ctx {
@@ -215,6 +216,13 @@ struct BlockResult {
   uint32_t flows = 0;
 };
 void dinkelbach_blocks(Ctx &c, const std::vector<BlockTask> &in, std::vector<BlockResult> &out, uint32_t max_iters);
+void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t max_iters);
+// Components with at least this many cliques are solved by the grid solver.
+constexpr uint64_t GRID_FLOW_MIN_T = 1u << 16;
+// All tasks: large components one after another on the whole GPU, the rest in
+// one workgroup-per-component launch.
+void dinkelbach_components(Ctx &c, const std::vector<BlockTask> &in, std::vector<BlockResult> &out,
+                           uint32_t max_iters);
 void graph_ids(Ctx &c);
 void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_graph, uint8_t *in_set);
 void graph_ids(Ctx &c);

@@ -77,13 +77,39 @@ bool may_reach(uint32_t k, uint32_t max_core, ull Tc, ull num, ull den) {
 
 }  // namespace
 
+void dinkelbach_components(Ctx &c, const std::vector<BlockTask> &in, std::vector<BlockResult> &out,
+                           uint32_t max_iters) {
+  CompData &g = c.cd;
+  FlowBufs &f = c.fb;
+  out.assign(in.size(), BlockResult{});
+  if (f.cside.bytes < (size_t)g.h_voff.back() + 8) {
+    buf_reserve(c, f.cside, (size_t)g.h_voff.back() + 8);
+    HIP_CHECK(hipMemsetAsync(f.cside.p, 0, (size_t)g.h_voff.back() + 8, c.stream));
+  }
+  const bool all_grid = (c.tune_flags & KCDS_TUNE_GRID_FLOW) != 0;
+  std::vector<BlockTask> small;
+  std::vector<size_t> small_at;
+  for (size_t i = 0; i < in.size(); ++i) {
+    const ull Tc = g.h_coff[in[i].d + 1] - g.h_coff[in[i].d];
+    if (all_grid || Tc >= GRID_FLOW_MIN_T) {
+      dinkelbach_grid(c, in[i], out[i], max_iters);
+    } else {
+      small.push_back(in[i]);
+      small_at.push_back(i);
+    }
+  }
+  std::vector<BlockResult> r;
+  dinkelbach_blocks(c, small, r, max_iters);
+  for (size_t j = 0; j < small.size(); ++j) out[small_at[j]] = r[j];
+}
+
 void maxflow_component(Ctx &c, uint32_t comp_id, uint64_t p, uint64_t q, int64_t *flow, uint8_t *side) {
   CompData &g = c.cd;
   auto it = std::lower_bound(g.h_label.begin(), g.h_label.end(), comp_id);
   if (it == g.h_label.end() || *it != comp_id) fail(KCDS_E_INVALID, "no component with id %u", comp_id);
   const uint32_t d = (uint32_t)(it - g.h_label.begin());
   std::vector<BlockResult> r;
-  dinkelbach_blocks(c, {BlockTask{d, p, q}}, r, 1);
+  dinkelbach_components(c, {BlockTask{d, p, q}}, r, 1);
   if (flow) *flow = r[0].flow;
   if (side && c.n) {
     buf_reserve(c, c.inset, (size_t)c.n + 8);
@@ -210,7 +236,7 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
   std::vector<uint8_t> solved(C, 0);
   {
     std::vector<BlockResult> r;
-    dinkelbach_blocks(c, tasks, r, ALL_ITERATIONS);
+    dinkelbach_components(c, tasks, r, ALL_ITERATIONS);
     for (size_t i = 0; i < tasks.size(); ++i) {
       res[tasks[i].d] = r[i];
       solved[tasks[i].d] = 1;

@@ -87,8 +87,18 @@ def _peel_and_components(solver, name, off, nbr, ks):
         assert ncomp == len(set(int(x) for x in comp_r if x != 0xFFFFFFFF))
 
 
+@pytest.mark.parametrize("grid_flow", [False, True], ids=["block_flow", "grid_flow"])
 @pytest.mark.parametrize("name,off,nbr,ks", GRAPHS, ids=[g[0] for g in GRAPHS])
-def test_maxflow_values(solver, name, off, nbr, ks):
+def test_maxflow_values(solver, name, off, nbr, ks, grid_flow):
+    """Max-flow values and maximal min-cut sides of the workgroup and the grid solver."""
+    solver.set_tuning(0, 2 if grid_flow else 0)
+    try:
+        _maxflow_values(solver, name, off, nbr, ks)
+    finally:
+        solver.set_tuning(0, 0)
+
+
+def _maxflow_values(solver, name, off, nbr, ks):
     n = len(off) - 1
     k = ks[0]
     solver.upload(off, nbr)
@@ -114,8 +124,17 @@ def test_maxflow_values(solver, name, off, nbr, ks):
             assert got == exp, (cid, p, q)
 
 
+@pytest.mark.parametrize("grid_flow", [False, True], ids=["block_flow", "grid_flow"])
 @pytest.mark.parametrize("name,off,nbr,ks", GRAPHS, ids=[g[0] for g in GRAPHS])
-def test_solve_matches_oracle(solver, name, off, nbr, ks):
+def test_solve_matches_oracle(solver, name, off, nbr, ks, grid_flow):
+    solver.set_tuning(0, 2 if grid_flow else 0)
+    try:
+        _solve_matches(solver, name, off, nbr, ks)
+    finally:
+        solver.set_tuning(0, 0)
+
+
+def _solve_matches(solver, name, off, nbr, ks):
     for k in ks:
         solver.upload(off, nbr)
         res, ins = solver.solve(k)
