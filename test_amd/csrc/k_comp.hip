@@ -108,20 +108,26 @@ __global__ void k_scatter_vertices(uint32_t n, const uint32_t *cid, const uint32
 
 __global__ void k_count_comp_cliques(uint64_t T, uint32_t k, const uint32_t *cl, const int32_t *status,
                                      const GraphPeel *gp, const uint32_t *gid, const uint32_t *cid, ull *ccount) {
-  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
-       c += (uint64_t)gridDim.x * blockDim.x)
-    if (core_clique(c, k, cl, status, gp, gid)) atomicAdd(&ccount[cid[cl[c * k]]], 1ull);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t c0 = (uint64_t)blockIdx.x * blockDim.x; c0 < T; c0 += stride) {
+    const uint64_t c = c0 + threadIdx.x;
+    const bool in = c < T && core_clique(c, k, cl, status, gp, gid);
+    wave_slot_by_key<ull>(in, in ? cid[cl[c * k]] : 0u, ccount);
+  }
 }
 
 // core cliques grouped by component, members as local vertex ids
 __global__ void k_scatter_cliques(uint64_t T, uint32_t k, const uint32_t *cl, const int32_t *status,
                                   const GraphPeel *gp, const uint32_t *gid, const uint32_t *cid, const uint32_t *local,
                                   const ull *coff, ull *cfill, uint32_t *ccl) {
-  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
-       c += (uint64_t)gridDim.x * blockDim.x) {
-    if (!core_clique(c, k, cl, status, gp, gid)) continue;
-    const uint32_t d = cid[cl[c * k]];
-    const ull at = coff[d] + atomicAdd(&cfill[d], 1ull);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t c0 = (uint64_t)blockIdx.x * blockDim.x; c0 < T; c0 += stride) {
+    const uint64_t c = c0 + threadIdx.x;
+    const bool in = c < T && core_clique(c, k, cl, status, gp, gid);
+    const uint32_t d = in ? cid[cl[c * k]] : 0u;
+    const ull slot = wave_slot_by_key<ull>(in, d, cfill);
+    if (!in) continue;
+    const ull at = coff[d] + slot;
     if (at >= coff[d + 1]) continue;  // never taken: counts and scatter agree
     for (uint32_t t = 0; t < k; ++t) ccl[at * k + t] = local[cl[c * k + t]];
   }

@@ -56,8 +56,10 @@ __global__ void k_gf_fill_u32(uint32_t *p, uint32_t v, uint64_t len) {
 
 // cnt[v] = 2 + number of cliques containing v (v->t, v->s, v->c arcs)
 __global__ void k_gf_deg(const uint32_t *ccl, uint64_t Tk, uint32_t *cnt) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < Tk; i += (uint64_t)gridDim.x * blockDim.x)
-    atomicAdd(&cnt[ccl[i]], 1u);
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < Tk; i0 += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = i0 + threadIdx.x;
+    wave_slot_by_key<uint32_t>(i < Tk, i < Tk ? ccl[i] : 0u, cnt);  // hub vertices repeat across lanes
+  }
 }
 
 __global__ void k_gf_topo_fixed(uint32_t nc, uint64_t Tc, uint32_t k, uint32_t *off, uint32_t *head, uint32_t *rev,
@@ -90,9 +92,14 @@ __global__ void k_gf_topo_fixed(uint32_t nc, uint64_t Tc, uint32_t k, uint32_t *
 __global__ void k_gf_topo_cliques(const uint32_t *ccl, uint64_t Tc, uint32_t k, uint32_t nc, const uint32_t *off,
                                   uint32_t *head, uint32_t *rev, uint32_t *fill) {
   const uint32_t CA = off[nc];
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < Tc * k; i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t v = ccl[i];
-    const uint32_t a = off[v] + 2 + atomicAdd(&fill[v], 1u), b = CA + (uint32_t)i;
+  const uint64_t Tk = Tc * k;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < Tk; i0 += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = i0 + threadIdx.x;
+    const bool valid = i < Tk;
+    const uint32_t v = valid ? ccl[i] : 0u;
+    const uint32_t slot = wave_slot_by_key<uint32_t>(valid, v, fill);
+    if (!valid) continue;
+    const uint32_t a = off[v] + 2 + slot, b = CA + (uint32_t)i;
     head[a] = nc + (uint32_t)(i / k);
     rev[a] = b;
     head[b] = v;

@@ -465,12 +465,16 @@ __global__ __launch_bounds__(256) void k_class_fill(const uint32_t *doff, uint32
 // incidence index: inc[incoff[v] + i] = id of the i-th clique containing v
 __global__ void k_incidence(const uint32_t *cl, uint64_t T, uint32_t k, const ull *incoff, uint32_t *fillc,
                             uint32_t *inc) {
-  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
-       c += (uint64_t)gridDim.x * blockDim.x) {
+  // lanes take consecutive cliques: their roots and hub members repeat, so
+  // the slot counters are bumped once per group of equal vertices
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t c0 = (uint64_t)blockIdx.x * blockDim.x; c0 < T; c0 += stride) {
+    const uint64_t c = c0 + threadIdx.x;
+    const bool valid = c < T;
     for (uint32_t t = 0; t < k; ++t) {
-      uint32_t v = cl[c * k + t];
-      uint32_t slot = atomicAdd(&fillc[v], 1u);
-      if (incoff[v] + slot < incoff[v + 1]) inc[incoff[v] + slot] = (uint32_t)c;
+      const uint32_t v = valid ? cl[c * k + t] : 0u;
+      const uint32_t slot = wave_slot_by_key<uint32_t>(valid, v, fillc);
+      if (valid && incoff[v] + slot < incoff[v + 1]) inc[incoff[v] + slot] = (uint32_t)c;
     }
   }
 }

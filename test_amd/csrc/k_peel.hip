@@ -8,27 +8,22 @@
 // remaining vertices; the largest density (ties keep the later core,
 // main.cu:407) is maxDensity.
 //
-// Here one peeling sub-round is a fixed sequence of step kernels whose work
-// is decided by device-side control state, so the host enqueues many
-// sub-rounds back to back and only reads the "done" flag now and then:
-//   touch / min / open / collect   (no-ops unless the frontier is empty)
-//     a new level starts only when the frontier is empty; its value is the
-//     minimum live count (empty levels change neither density nor cores, so
-//     they are skipped by a min-reduction); the density of the level just
-//     completed is recorded first, as an exact rational;
-//   mark      frontier vertices take the level as their core number;
-//   process   each live clique of a removed vertex is claimed exactly once
-//             (atomicCAS(status, -1, L)) and its surviving members are
-//             decremented; a member whose count falls from L+1 to L joins the
-//             next sub-round (each vertex crosses that value once);
-//   finish    advance to the next frontier.
-// Kernel boundaries order every step, so no step needs a grid-wide barrier.
-// Frontier vertices with long incidence lists are processed grid-wide, the
-// rest wave-per-vertex.  With several graphs in one batch, every graph keeps
-// its own live cliques / remaining vertices and records its own densities at
-// the end of each level that touched it -- exactly what its own peel records,
-// because a graph's vertices leave at their core numbers whatever else is
-// peeled alongside.
+// Here the peel of large graphs is one kernel per step (k_pl_step: OPEN a
+// level or PROCESS a sub-round) driven by device-side control state; the
+// last workgroup to finish a step runs the control step, and the host
+// enqueues steps in batches, reading the "done" flag now and then.  A
+// frontier carries each removed vertex's incidence list cut into pieces, one
+// wave per piece, so a vertex in millions of cliques is spread over the GPU;
+// decrements of one vertex by several lanes of a wave are one atomic.
+// Workgroups share only atomic counters inside a step; kernel boundaries
+// order the steps.  Graphs with few vertices and cliques are instead peeled
+// one workgroup each (k_peel_block), all in one launch.  With several graphs
+// in one batch, every graph keeps its own live cliques / remaining vertices
+// and records its own densities at the end of each level that touched it --
+// exactly what its own peel records, because a graph's vertices leave at
+// their core numbers whatever else is peeled alongside.
+#include <cstring>
+
 #include "kcds_device.hpp"
 #include "kcds_internal.hpp"
 
@@ -36,22 +31,24 @@ namespace kcds {
 
 namespace {
 
-constexpr uint32_t BIG_LIST = 1u << 14;
+constexpr uint32_t PSEG = 4;        // frontier segments / counter shards
+constexpr uint32_t PPIECE = 1024;   // incidence entries per work item
 
+// Counters 64 words apart (same-address device atomics are serialised).
 struct PeelCtl {
-  ull level;      // current level
-  ull minval;     // min live count over remaining vertices (reduction target)
-  ull live;       // live cliques (one-graph bookkeeping)
-  ull max_num, max_den;
-  ull destroyed;  // cliques destroyed in the current sub-round
-  uint32_t removed;
-  uint32_t fsize[2];
-  uint32_t cur;
-  uint32_t nbig;
-  uint32_t levels;
-  uint32_t ntouch;
-  uint32_t done;
-  uint32_t opening;  // a level was opened in this sub-round: collect its frontier
+  ull level, target;
+  uint32_t cur, done, phase, steps, levels, pad0[64 - 9];
+  uint32_t fsize[2][PSEG][64];  // vertex frontier segments
+  uint32_t psize[2][PSEG][64];  // incidence-piece frontier segments
+  ull gmin[PSEG][8];            // minimum live count above target (OPEN)
+  uint32_t gblocks[PSEG][64];
+  uint32_t blocks[64];
+  uint32_t ntouch[64];          // graphs touched by the current level
+};
+
+struct PItem {
+  ull st, en;  // incidence entries [st, en) of vertex v
+  uint32_t v, pad;
 };
 
 // a/b >= c/d for 64-bit non-negative integers, b, d > 0 (exact, 128-bit products)
@@ -61,159 +58,258 @@ __device__ __forceinline__ bool ratio_ge(ull a, ull b, ull c, ull d) {
   return hi1 > hi2 || (hi1 == hi2 && lo1 >= lo2);
 }
 
-__device__ __forceinline__ bool level_end(const PeelCtl *ctl) {
-  return !ctl->done && ctl->fsize[ctl->cur] == 0;
-}
-
-// multi-graph: record the density of every graph touched by the level that
-// just completed
-__global__ void k_pl_touch(PeelCtl *ctl, GraphPeel *gp, uint32_t *tlist) {
-  if (!level_end(ctl)) return;
-  const uint32_t nt = ctl->ntouch;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x) {
-    GraphPeel &G = gp[tlist[i]];
-    const ull rem = G.rem;
-    if (rem > 0 && ratio_ge(G.live, rem, G.max_num, G.max_den)) {
-      G.max_num = G.live;
-      G.max_den = rem;
-    }
-    G.touch = 0;
+// Appends v (where want) to a vertex frontier segment and its incidence list,
+// cut in PPIECE-entry pieces, to a piece segment; one counter atomic of each
+// kind per wave.  Every lane calls it.
+__device__ __forceinline__ void pl_append(bool want, uint32_t v, const ull *__restrict__ incoff, uint32_t *fv,
+                                          uint32_t *fvsize, PItem *fp, uint32_t *fpsize) {
+  const ull bal = __ballot(want);
+  if (!bal) return;
+  const uint32_t lane = lane_id(), lead = (uint32_t)__builtin_ctzll(bal);
+  ull b = 0, e = 0;
+  uint32_t np = 0;
+  if (want) {
+    b = incoff[v];
+    e = incoff[v + 1];
+    np = (uint32_t)((e - b + PPIECE - 1) / PPIECE);
   }
-}
-
-__global__ void k_pl_min(PeelCtl *ctl, uint32_t n, const ull *cnt, const int32_t *core) {
-  if (!level_end(ctl)) return;
-  ull m = ~0ull;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
-    if (core[v] == -1) m = min(m, cnt[v]);
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) m = min(m, (ull)__shfl_xor(m, o, 64));
-  if (lane_id() == 0 && m != ~0ull) atomicMin(&ctl->minval, m);
-}
-
-// one thread: close the completed level, open the next one
-__global__ void k_pl_open(PeelCtl *ctl, uint32_t n, uint32_t multi) {
-  if (!level_end(ctl)) return;
-  if (!multi) {
-    const ull rem = (ull)(n - ctl->removed);
-    if (rem > 0 && ratio_ge(ctl->live, rem, ctl->max_num, ctl->max_den)) {
-      ctl->max_num = ctl->live;
-      ctl->max_den = rem;
-    }
+  const uint32_t incl = wave_incl_scan_u32(np);
+  const uint32_t tot = __shfl(incl, 63, 64);
+  uint32_t vb = 0, pb = 0;
+  if (lane == lead) {
+    vb = atomicAdd(fvsize, (uint32_t)__popcll(bal));
+    if (tot) pb = atomicAdd(fpsize, tot);
   }
-  ctl->ntouch = 0;
-  if (ctl->minval == ~0ull) {  // nothing left (cannot happen while removed < n)
-    ctl->done = 1;
-    return;
-  }
-  ctl->level = max(ctl->level, ctl->minval);
-  ctl->minval = ~0ull;
-  ctl->levels += 1;
-  ctl->opening = 1;
-}
-
-__global__ void k_pl_collect(PeelCtl *ctl, uint32_t n, const ull *cnt, const int32_t *core, uint32_t *fr0,
-                             uint32_t *fr1) {
-  if (ctl->done || !ctl->opening) return;
-  const ull L = ctl->level;
-  const uint32_t cur = ctl->cur;
-  uint32_t *fr = cur ? fr1 : fr0;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
-    if (core[v] == -1 && cnt[v] <= L) fr[atomicAdd(&ctl->fsize[cur], 1u)] = v;
-}
-
-__global__ void k_pl_mark(PeelCtl *ctl, const uint32_t *fr0, const uint32_t *fr1, const ull *incoff, int32_t *core,
-                          uint32_t *big, GraphPeel *gp, const uint32_t *gid, uint32_t *tlist) {
-  if (ctl->done) return;
-  const uint32_t cur = ctl->cur, fs = ctl->fsize[cur];
-  const uint32_t *fr = cur ? fr1 : fr0;
-  const int32_t L = (int32_t)ctl->level;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < fs; i += gridDim.x * blockDim.x) {
-    const uint32_t v = fr[i];
-    core[v] = L;
-    if (incoff[v + 1] - incoff[v] >= BIG_LIST) big[atomicAdd(&ctl->nbig, 1u)] = v;
-    if (gp) {
-      const uint32_t g = gid[v];
-      atomicSub(&gp[g].rem, 1u);
-      if (atomicExch(&gp[g].touch, 1u) == 0u) tlist[atomicAdd(&ctl->ntouch, 1u)] = g;
+  vb = __shfl(vb, lead, 64);
+  pb = __shfl(pb, lead, 64);
+  if (want) {
+    fv[vb + lanes_below(bal)] = v;
+    uint32_t p = pb + incl - np;
+    for (ull st = b; st < e; st += PPIECE) {
+      PItem it;
+      it.st = st;
+      it.en = min(st + PPIECE, e);
+      it.v = v;
+      it.pad = 0;
+      fp[p++] = it;
     }
   }
 }
 
-__device__ __forceinline__ void claim_clique(uint32_t c, uint32_t v, uint32_t k, ull L, const uint32_t *cl,
-                                             int32_t *status, const int32_t *core, ull *cnt, uint32_t *nx,
-                                             uint32_t *nx_size, ull &destroyed, GraphPeel *gp,
-                                             const uint32_t *gid) {
-  if (__hip_atomic_load(&status[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != -1) return;
-  if (atomicCAS(&status[c], -1, (int32_t)L) != -1) return;
-  destroyed += 1;
-  if (gp) atomicSub(&gp[gid[v]].live, 1ull);
-  for (uint32_t t = 0; t < k; ++t) {
-    const uint32_t u = cl[(ull)c * k + t];
-    if (u == v) continue;
-    if (core[u] != -1) continue;  // removed (core numbers were set by the mark step)
-    const ull old = atomicSub(&cnt[u], 1ull);
-    if (old == L + 1) nx[atomicAdd(nx_size, 1u)] = u;
-  }
-}
-
-// wave per frontier vertex with a short incidence list
-__global__ void k_pl_proc(PeelCtl *ctl, uint32_t k, const uint32_t *cl, const ull *incoff, const uint32_t *inc,
-                          ull *cnt, const int32_t *core, int32_t *status, uint32_t *fr0, uint32_t *fr1,
-                          GraphPeel *gp, const uint32_t *gid) {
-  if (ctl->done) return;
-  const uint32_t cur = ctl->cur, fs = ctl->fsize[cur];
-  const uint32_t *fr = cur ? fr1 : fr0;
-  uint32_t *nx = cur ? fr0 : fr1;
-  const ull L = ctl->level;
+// Lanes with want grouped by key (the first eight groups; later lanes each
+// alone): returns the group size on the group's leader lane and 0 elsewhere;
+// leader = the leader's lane, rank = the lane's position in its group.
+__device__ __forceinline__ uint32_t wave_group(bool want, uint32_t key, uint32_t &rank, uint32_t &leader) {
   const uint32_t lane = lane_id();
-  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KCDS_WAVE;
-  const uint32_t nw = gridDim.x * blockDim.x / KCDS_WAVE;
-  ull destroyed = 0;
-  for (uint32_t i = wid; i < fs; i += nw) {
-    const uint32_t v = fr[i];
-    const ull b = incoff[v], e = incoff[v + 1];
-    if (e - b >= BIG_LIST) continue;
-    for (ull j = b + lane; j < e; j += KCDS_WAVE)
-      claim_clique(inc[j], v, k, L, cl, status, core, cnt, nx, &ctl->fsize[cur ^ 1], destroyed, gp, gid);
+  uint32_t size = 0;
+  rank = 0;
+  leader = lane;
+  ull pending = __ballot(want);
+#pragma unroll 1
+  for (int round = 0; round < 8 && pending; ++round) {
+    const uint32_t lead = (uint32_t)__builtin_ctzll(pending);
+    const uint32_t kl = __shfl(key, lead, 64);
+    const ull same = __ballot(want && key == kl) & pending;
+    if ((same >> lane) & 1ull) {
+      rank = lanes_below(same);
+      leader = lead;
+    }
+    if (lane == lead) size = (uint32_t)__popcll(same);
+    pending &= ~same;
   }
-  destroyed = wave_sum_u64(destroyed);
-  if (lane == 0 && destroyed) atomicAdd(&ctl->destroyed, destroyed);
+  if ((pending >> lane) & 1ull) size = 1;
+  return size;
 }
 
-// the whole grid over each frontier vertex with a long incidence list
-__global__ void k_pl_proc_big(PeelCtl *ctl, uint32_t k, const uint32_t *cl, const ull *incoff, const uint32_t *inc,
-                              ull *cnt, const int32_t *core, int32_t *status, uint32_t *fr0, uint32_t *fr1,
-                              const uint32_t *big, GraphPeel *gp, const uint32_t *gid) {
-  if (ctl->done) return;
-  const uint32_t nbig = ctl->nbig;
-  if (nbig == 0) return;
-  const uint32_t cur = ctl->cur;
-  uint32_t *nx = cur ? fr0 : fr1;
-  const ull L = ctl->level;
-  const ull tid = (ull)blockIdx.x * blockDim.x + threadIdx.x, nt = (ull)gridDim.x * blockDim.x;
-  ull destroyed = 0;
-  for (uint32_t bi = 0; bi < nbig; ++bi) {
-    const uint32_t v = big[bi];
-    for (ull j = incoff[v] + tid; j < incoff[v + 1]; j += nt)
-      claim_clique(inc[j], v, k, L, cl, status, core, cnt, nx, &ctl->fsize[cur ^ 1], destroyed, gp, gid);
+// The last workgroup of a launch (two-level ticket) returns true.
+__device__ bool pl_last_block(PeelCtl *ctl, uint32_t *s_last) {
+  block_sync();
+  if (threadIdx.x == 0) {
+    const uint32_t seg = blockIdx.x % PSEG;
+    const uint32_t in_seg = (gridDim.x - seg + PSEG - 1) / PSEG;
+    uint32_t last = 0;
+    if (atomicAdd(&ctl->gblocks[seg][0], 1u) == in_seg - 1)
+      last = atomicAdd(&ctl->blocks[0], 1u) == min(gridDim.x, PSEG) - 1;
+    *s_last = last;
   }
-  destroyed = wave_sum_u64(destroyed);
-  if (lane_id() == 0 && destroyed) atomicAdd(&ctl->destroyed, destroyed);
+  block_sync();
+  return *s_last != 0;
 }
 
-// one thread: advance to the next frontier
-__global__ void k_pl_finish(PeelCtl *ctl, uint32_t n) {
-  if (ctl->done) return;
-  const uint32_t cur = ctl->cur;
-  ctl->removed += ctl->fsize[cur];
-  ctl->live -= ctl->destroyed;
-  ctl->destroyed = 0;
-  ctl->fsize[cur] = 0;
-  ctl->nbig = 0;
-  ctl->opening = 0;
-  ctl->cur = cur ^ 1;
-  if (ctl->removed >= n) ctl->done = 1;
+// One peeling step.
+//   OPEN    (frontier empty) records the density of every graph the level
+//           just completed touched, then collects every live vertex whose
+//           count is <= target and the minimum count of the others; the
+//           control step starts level = target, or retries at that minimum
+//           (empty levels cost one pass), or ends the peel.
+//   PROCESS frontier vertices take the level as their core number; each live
+//           clique of a removed vertex is claimed exactly once
+//           (atomicCAS(status, -1, L)) and its surviving members are
+//           decremented; a member whose count falls from L+1 to L joins the
+//           next sub-round (each vertex crosses that value once).  Members
+//           removed in this same step may still be decremented: their counts
+//           no longer matter.
+// Per graph: remaining vertices and live cliques (batches peel many graphs
+// at once; each records its own densities, exactly what its own peel
+// records, because a graph's vertices leave at their core numbers whatever
+// else is peeled alongside).
+__global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint32_t k, const uint32_t *__restrict__ cl,
+                                                 const ull *__restrict__ incoff, const uint32_t *__restrict__ inc,
+                                                 ull *cnt, int32_t *core, int32_t *status, uint32_t *fv0, uint32_t *fv1,
+                                                 PItem *fp0, PItem *fp1, uint32_t seg_v, uint32_t seg_p,
+                                                 GraphPeel *gp, const uint32_t *__restrict__ gid, uint32_t *tlist) {
+  __shared__ ull s_hdr[2];
+  __shared__ uint32_t s_hdr2[3], s_f[PSEG + 1], s_p[PSEG + 1], s_last;
+  __shared__ ull s_min;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), seg = blockIdx.x % PSEG;
+  if (tid < 2) s_hdr[tid] = (&ctl->level)[tid];
+  if (tid < 3) s_hdr2[tid] = (&ctl->cur)[tid];
+  block_sync();
+  const ull L = s_hdr[0], target = s_hdr[1];
+  const uint32_t cur = s_hdr2[0], done = s_hdr2[1], phase = s_hdr2[2];
+  if (done) return;
+  uint32_t *fv = cur ? fv1 : fv0, *nv = cur ? fv0 : fv1;
+  PItem *fp = cur ? fp1 : fp0, *np = cur ? fp0 : fp1;
+  const uint32_t gtid = blockIdx.x * blockDim.x + tid, gstride = gridDim.x * blockDim.x;
+  if (phase == 0) {
+    // densities of the graphs the completed level touched
+    const uint32_t nt = ctl->ntouch[0];
+    for (uint32_t i = gtid; i < nt; i += gstride) {
+      GraphPeel &P = gp[tlist[i]];
+      const ull rem = P.rem;
+      if (rem > 0 && ratio_ge(P.live, rem, P.max_num, P.max_den)) {
+        P.max_num = P.live;
+        P.max_den = rem;
+      }
+      P.touch = 0;
+    }
+    if (tid == 0) s_min = ~0ull;
+    block_sync();
+    ull m = ~0ull;
+    for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < n; v0 += gstride) {
+      const uint32_t v = v0 + tid;
+      const bool live = v < n && core[v] == -1;
+      const ull cv = live ? cnt[v] : ~0ull;
+      pl_append(live && cv <= target, v, incoff, fv + (size_t)seg * seg_v, &ctl->fsize[cur][seg][0],
+                fp + (size_t)seg * seg_p, &ctl->psize[cur][seg][0]);
+      if (live && cv > target) m = min(m, cv);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = min(m, (ull)__shfl_xor(m, o, 64));
+    if (lane == 0 && m != ~0ull) atomicMin(&s_min, m);
+    block_sync();
+    if (tid == 0 && s_min != ~0ull) atomicMin(&ctl->gmin[seg][0], s_min);
+  } else {
+    if (tid < PSEG) {
+      s_f[tid + 1] = ctl->fsize[cur][tid][0];
+      s_p[tid + 1] = ctl->psize[cur][tid][0];
+    }
+    if (tid == 0) {
+      s_f[0] = 0;
+      s_p[0] = 0;
+    }
+    block_sync();
+    if (tid == 0)
+      for (uint32_t g = 0; g < PSEG; ++g) {
+        s_f[g + 1] += s_f[g];
+        s_p[g + 1] += s_p[g];
+      }
+    block_sync();
+    const uint32_t fs = s_f[PSEG], ps = s_p[PSEG];
+    // mark: core numbers, remaining vertices and the touched list per graph
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < fs; i0 += gstride) {
+      const uint32_t i = i0 + tid;
+      uint32_t v = 0, g = 0;
+      const bool valid = i < fs;
+      if (valid) {
+        uint32_t sg = 0;
+        while (i >= s_f[sg + 1]) ++sg;
+        v = fv[(size_t)sg * seg_v + (i - s_f[sg])];
+        __hip_atomic_store(&core[v], (int32_t)L, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        g = gid[v];
+      }
+      uint32_t rank, leader;
+      const uint32_t gsize = wave_group(valid, g, rank, leader);
+      if (gsize) {
+        atomicSub(&gp[g].rem, gsize);
+        if (atomicExch(&gp[g].touch, 1u) == 0u) tlist[atomicAdd(&ctl->ntouch[0], 1u)] = g;
+      }
+    }
+    // incidence pieces, wave per piece
+    const uint32_t wid = gtid / KCDS_WAVE, nw = gstride / KCDS_WAVE;
+    for (uint32_t it = wid; it < ps; it += nw) {
+      uint32_t sg = 0;
+      while (it >= s_p[sg + 1]) ++sg;
+      const PItem item = fp[(size_t)sg * seg_p + (it - s_p[sg])];
+      const uint32_t v = item.v;
+      uint32_t destroyed = 0;
+      for (ull j0 = item.st; j0 < item.en; j0 += KCDS_WAVE) {
+        const ull j = j0 + lane;
+        bool mine = false;
+        uint32_t c = 0;
+        if (j < item.en) {
+          c = inc[j];
+          mine = status[c] == -1 && atomicCAS(&status[c], -1, (int32_t)L) == -1;
+        }
+        destroyed += mine;
+        for (uint32_t t = 0; t < k; ++t) {
+          uint32_t u = 0;
+          bool dec = false;
+          if (mine) {
+            u = cl[(ull)c * k + t];
+            dec = u != v && core[u] == -1;
+          }
+          // one atomic per group of lanes decrementing the same (hub) vertex;
+          // each lane's own old value is the leader's minus its rank
+          uint32_t rank, leader;
+          const uint32_t gsize = wave_group(dec, u, rank, leader);
+          ull old = 0;
+          if (gsize) old = atomicSub(&cnt[u], (ull)gsize);
+          const ull lold = __shfl(old, leader, 64);
+          const bool join = dec && lold - rank == L + 1;
+          pl_append(join, u, incoff, nv + (size_t)seg * seg_v, &ctl->fsize[cur ^ 1][seg][0],
+                    np + (size_t)seg * seg_p, &ctl->psize[cur ^ 1][seg][0]);
+        }
+      }
+      destroyed = wave_sum_u32(destroyed);
+      if (lane == 0 && destroyed) atomicSub(&gp[gid[v]].live, (ull)destroyed);
+    }
+  }
+  // control step: the last workgroup to finish
+  if (pl_last_block(ctl, &s_last) && tid == 0) {
+    uint32_t fcur = 0, fnext = 0;
+    ull mn = ~0ull;
+    for (uint32_t g = 0; g < PSEG; ++g) {
+      fcur += atomicAdd(&ctl->fsize[cur][g][0], 0u);
+      fnext += atomicAdd(&ctl->fsize[cur ^ 1][g][0], 0u);
+      mn = min(mn, (ull)atomicExch(&ctl->gmin[g][0], ~0ull));
+      ctl->gblocks[g][0] = 0;
+    }
+    ctl->blocks[0] = 0;
+    ++ctl->steps;
+    if (phase == 1) {
+      for (uint32_t g = 0; g < PSEG; ++g) {
+        ctl->fsize[cur][g][0] = 0;
+        ctl->psize[cur][g][0] = 0;
+      }
+      ctl->cur = cur ^ 1;
+      if (fnext == 0) {
+        ctl->phase = 0;
+        ctl->target = L + 1;
+      }
+    } else {
+      ctl->ntouch[0] = 0;  // the densities were recorded by this step
+      if (fcur > 0) {
+        ctl->level = target;
+        ctl->phase = 1;
+        ++ctl->levels;
+      } else if (mn == ~0ull) {
+        ctl->done = 1;
+      } else {
+        ctl->target = mn;
+      }
+    }
+  }
 }
 
 __global__ void k_peel_init(uint32_t n, uint64_t T, const ull *cnt0, ull *cnt, int32_t *core, int32_t *status) {
@@ -231,10 +327,23 @@ __global__ void k_gid(uint32_t n, const uint32_t *inv, const uint32_t *gvoff, ui
 }
 
 __global__ void k_graph_accum(uint32_t n, const uint32_t *gid, const ull *cnt, GraphPeel *gp) {
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-    GraphPeel &G = gp[gid[r]];
-    if (cnt[r]) atomicAdd(&G.live, cnt[r]);
-    atomicAdd(&G.n, 1u);
+  for (uint32_t r0 = blockIdx.x * blockDim.x; r0 < n; r0 += gridDim.x * blockDim.x) {
+    const uint32_t r = r0 + threadIdx.x;
+    const bool valid = r < n;
+    const uint32_t g = valid ? gid[r] : 0u;
+    const ull cv = valid ? cnt[r] : 0ull;
+    const uint32_t g0 = __shfl(g, 0, 64);
+    if (__ballot(valid && g != g0) == 0ull) {  // one graph for the whole wave
+      const ull sum = wave_sum_u64(cv);
+      const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
+      if (lane_id() == 0) {
+        if (sum) atomicAdd(&gp[g0].live, sum);
+        if (nv) atomicAdd(&gp[g0].n, nv);
+      }
+    } else if (valid) {
+      if (cv) atomicAdd(&gp[g].live, cv);
+      atomicAdd(&gp[g].n, 1u);
+    }
   }
 }
 
@@ -250,11 +359,6 @@ __global__ void k_graph_init(uint32_t G, uint32_t k, GraphPeel *gp) {
   }
 }
 
-__global__ void k_graph_single(const PeelCtl *ctl, GraphPeel *gp) {
-  gp[0].max_num = ctl->max_num;
-  gp[0].max_den = ctl->max_den;
-}
-
 // k' = ceil(maxDensity) per graph (main.cu:1476)
 __global__ void k_graph_kprime(uint32_t G, GraphPeel *gp) {
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
@@ -266,19 +370,43 @@ __global__ void k_graph_kprime(uint32_t G, GraphPeel *gp) {
   }
 }
 
-// k'-core size, cliques inside it and the largest core, per graph
+// k'-core size, cliques inside it and the largest core, per graph.  Lanes of
+// one graph are combined first (one atomic per wave when they agree).
 __global__ void k_graph_summary(uint32_t n, uint64_t T, uint32_t k, const uint32_t *gid, const int32_t *core,
                                 const int32_t *status, const uint32_t *cl, GraphPeel *gp) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    GraphPeel &P = gp[gid[i]];
-    const int32_t cv = core[i];
-    if (cv >= (int32_t)P.k_prime) atomicAdd(&P.core_size, 1u);
-    if (cv > 0) atomicMax(&P.max_core, (uint32_t)cv);
+  const uint32_t lane = lane_id();
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+    const uint64_t i = i0 + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t g = valid ? gid[i] : 0u;
+    const int32_t cv = valid ? core[i] : -1;
+    const bool in = valid && cv >= (int32_t)gp[g].k_prime;
+    const uint32_t g0 = __shfl(g, 0, 64);
+    if (__ballot(valid && g != g0) == 0ull) {  // the whole wave is in graph g0
+      const uint32_t cs = (uint32_t)__popcll(__ballot(in));
+      const uint32_t mc = wave_max_u32(cv > 0 ? (uint32_t)cv : 0u);
+      if (lane == 0) {
+        if (cs) atomicAdd(&gp[g0].core_size, cs);
+        if (mc) atomicMax(&gp[g0].max_core, mc);
+      }
+    } else if (valid) {
+      if (in) atomicAdd(&gp[g].core_size, 1u);
+      if (cv > 0) atomicMax(&gp[g].max_core, (uint32_t)cv);
+    }
   }
-  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T; c += stride) {
-    GraphPeel &P = gp[gid[cl[c * k]]];
-    if (status[c] >= (int32_t)P.k_prime) atomicAdd(&P.core_cliques, 1ull);
+  for (uint64_t c0 = (uint64_t)blockIdx.x * blockDim.x; c0 < T; c0 += stride) {
+    const uint64_t c = c0 + threadIdx.x;
+    const bool valid = c < T;
+    const uint32_t g = valid ? gid[cl[c * k]] : 0u;
+    const bool in = valid && status[c] >= (int32_t)gp[g].k_prime;
+    const uint32_t g0 = __shfl(g, 0, 64);
+    if (__ballot(valid && g != g0) == 0ull) {
+      const uint32_t cc = (uint32_t)__popcll(__ballot(in));
+      if (lane == 0 && cc) atomicAdd(&gp[g0].core_cliques, (ull)cc);
+    } else if (in) {
+      atomicAdd(&gp[g].core_cliques, 1ull);
+    }
   }
 }
 
@@ -318,13 +446,15 @@ __device__ __forceinline__ ull block_sum_u64(ull v, ull *slot) {
   return r;
 }
 
-__global__ __launch_bounds__(BP_THREADS) void k_peel_block(uint32_t G, const uint32_t *__restrict__ gvoff,
+__global__ __launch_bounds__(BP_THREADS) void k_peel_block(uint32_t G, const uint32_t *__restrict__ glist,
+                                                          const uint32_t *__restrict__ gvoff,
                                                           const uint32_t *__restrict__ rank, uint32_t k,
                                                           const ull *__restrict__ cnt0, ull *cnt, int32_t *core,
                                                           int32_t *status, const uint32_t *__restrict__ cl,
                                                           const ull *__restrict__ incoff,
                                                           const uint32_t *__restrict__ inc, uint32_t *fr0,
-                                                          uint32_t *fr1, uint32_t *queue, GraphPeel *gp) {
+                                                          uint32_t *fr1, uint32_t *queue, GraphPeel *gp,
+                                                          uint32_t init_status) {
   __shared__ ull wsum[16];
   __shared__ ull s_red;
   __shared__ ull s_pre[BP_THREADS + 1];
@@ -336,9 +466,10 @@ __global__ __launch_bounds__(BP_THREADS) void k_peel_block(uint32_t G, const uin
   while (true) {
     if (tid == 0) s_g = queue_next(queue, G, qc);
     block_sync();
-    const uint32_t g = s_g;
+    const uint32_t gi = s_g;
     block_sync();
-    if (g >= G) break;
+    if (gi >= G) break;
+    const uint32_t g = glist[gi];
     const uint32_t vb = gvoff[g], ng = gvoff[g + 1] - vb;
     uint32_t *F0 = fr0 + vb, *F1 = fr1 + vb;
     ull mine = 0;
@@ -348,6 +479,8 @@ __global__ __launch_bounds__(BP_THREADS) void k_peel_block(uint32_t G, const uin
       stc_u64(&cnt[r], c0);
       stc_i32(&core[r], -1);
       mine += c0;
+      if (init_status)  // the graph's cliques (when not reset grid-wide before the launch)
+        for (ull j = incoff[r]; j < incoff[r + 1]; ++j) stc_i32(&status[inc[j]], -1);
     }
     const ull T = block_sum_u64(mine, &s_red) / k;  // sum of per-vertex counts = k T
     ull live = T, max_num = T, max_den = ng ? ng : 1, L = 0;
@@ -486,34 +619,75 @@ void graph_ids(Ctx &c) {
 
 // Graphs up to this size are peeled by one workgroup each.
 constexpr uint32_t BLOCK_PEEL_MAX_N = 1u << 15;
+constexpr ull BLOCK_PEEL_MAX_T = 1u << 18;
 
-void peel_blocks(Ctx &c) {
+// Graph 0's peel summary into c.peel (the single-graph API reports it).
+void set_peel_stats(Ctx &c, uint32_t levels) {
+  const GraphPeel &P = c.h_gpeel[0];
+  kcds_peel_stats ps{};
+  ull g0 = P.max_num, b0 = P.max_den;
+  while (b0) {
+    const ull t = g0 % b0;
+    g0 = b0;
+    b0 = t;
+  }
+  if (!g0) g0 = 1;
+  ps.max_num = P.max_num / g0;
+  ps.max_den = P.max_den / g0;
+  ps.k_prime = P.k_prime;
+  ps.levels = levels;
+  ps.core_size = P.core_size;
+  ps.core_cliques = P.core_cliques;
+  ps.max_core = (int32_t)P.max_core;
+  c.peel = ps;
+  c.peeled = true;
+  c.have_comp = false;
+}
+
+// Workgroup-per-graph peel of the graphs in ids (all graphs when ids is
+// empty); writes their GraphPeel entries on the device and in c.h_gpeel.
+void peel_blocks(Ctx &c, const std::vector<uint32_t> &ids) {
   const uint32_t n = c.n, k = c.k, G = c.ngraphs;
   const uint64_t T = c.T;
+  std::vector<uint32_t> all;
+  if (ids.empty()) {
+    all.resize(G);
+    for (uint32_t g = 0; g < G; ++g) all[g] = g;
+  }
+  const std::vector<uint32_t> &list = ids.empty() ? all : ids;
+  const uint32_t L = (uint32_t)list.size();
   buf_reserve(c, c.core, sizeof(int32_t) * ((size_t)n + 1));
   buf_reserve(c, c.status, sizeof(int32_t) * (T + 1));
   buf_reserve(c, c.gpeel, sizeof(GraphPeel) * ((size_t)G + 1));
   buf_reserve(c, c.gvoff, sizeof(uint32_t) * ((size_t)G + 1));
   HIP_CHECK(hipMemcpyAsync(c.gvoff.p, c.h_gvoff.data(), sizeof(uint32_t) * ((size_t)G + 1), hipMemcpyHostToDevice,
                            c.stream));
-  const size_t words = (size_t)n * 2 /*cnt*/ + (size_t)n * 2 /*frontiers*/ + QUEUE_WORDS + 64;
+  const size_t words = (size_t)n * 2 /*cnt*/ + (size_t)n * 2 /*frontiers*/ + QUEUE_WORDS + (size_t)L + 64;
   buf_reserve(c, c.tmp0, words * sizeof(uint32_t) + 256);
   ull *cnt = c.tmp0.as<ull>();
-  uint32_t *fr0 = (uint32_t *)(cnt + n), *fr1 = fr0 + n, *queue = fr1 + n;
+  uint32_t *fr0 = (uint32_t *)(cnt + n), *fr1 = fr0 + n, *queue = fr1 + n, *glist = queue + QUEUE_WORDS;
   HIP_CHECK(hipMemsetAsync(queue, 0, QUEUE_WORDS * sizeof(uint32_t), c.stream));
-  if (T) {
+  HIP_CHECK(hipMemcpyAsync(glist, list.data(), sizeof(uint32_t) * L, hipMemcpyHostToDevice, c.stream));
+  if (ids.empty() && T) {  // every graph: reset all clique statuses at once
     k_peel_init<<<grid1d(T), 256, 0, c.stream>>>(0, T, nullptr, nullptr, nullptr, c.status.as<int32_t>());
     KCDS_LAUNCH_CHECK();
   }
-  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(G, (uint32_t)c.num_cus * 2));
-  k_peel_block<<<grid, BP_THREADS, 0, c.stream>>>(G, c.gvoff.as<uint32_t>(), c.rank.as<uint32_t>(), k,
+  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(L, (uint32_t)c.num_cus * 2));
+  k_peel_block<<<grid, BP_THREADS, 0, c.stream>>>(L, glist, c.gvoff.as<uint32_t>(), c.rank.as<uint32_t>(), k,
                                                   c.cnt.as<ull>(), cnt, c.core.as<int32_t>(), c.status.as<int32_t>(),
                                                   c.cliques.as<uint32_t>(), c.incoff.as<ull>(), c.inc.as<uint32_t>(),
-                                                  fr0, fr1, queue, c.gpeel.as<GraphPeel>());
+                                                  fr0, fr1, queue, c.gpeel.as<GraphPeel>(), ids.empty() ? 0u : 1u);
   KCDS_LAUNCH_CHECK();
   c.h_gpeel.resize(G);
   HIP_CHECK(hipMemcpyAsync(c.h_gpeel.data(), c.gpeel.p, sizeof(GraphPeel) * G, hipMemcpyDeviceToHost, c.stream));
-  c.sync();
+  c.sync();  // list lives on the host
+}
+
+// vertices of graphs peeled elsewhere take core -2: the grid peel treats
+// them as neither live nor removed
+__global__ void k_exclude(uint32_t n, const uint32_t *gid, const uint8_t *excl, int32_t *core) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
+    if (excl[gid[r]]) core[r] = -2;
 }
 
 void peel_run(Ctx &c) {
@@ -525,29 +699,20 @@ void peel_run(Ctx &c) {
   if (T > 0x7FFFFFFFull && max_u64_host(c, c.cnt.as<ull>(), n) > 0x7FFFFFFFull)
     fail(KCDS_E_LIMIT, "a per-vertex clique count exceeds the int32 clique-core range");
   graph_ids(c);
-  uint32_t largest = 0;
-  for (uint32_t g = 0; g < G; ++g) largest = std::max(largest, c.h_gvoff[g + 1] - c.h_gvoff[g]);
-  if (largest <= BLOCK_PEEL_MAX_N && !(c.tune_flags & KCDS_TUNE_GRID_PEEL)) {
-    peel_blocks(c);
-    const GraphPeel &P = c.h_gpeel[0];
-    kcds_peel_stats ps{};
-    ull g0 = P.max_num, b0 = P.max_den;
-    while (b0) {
-      ull t = g0 % b0;
-      g0 = b0;
-      b0 = t;
-    }
-    if (!g0) g0 = 1;
-    ps.max_num = P.max_num / g0;
-    ps.max_den = P.max_den / g0;
-    ps.k_prime = P.k_prime;
-    ps.levels = P.levels;
-    ps.core_size = P.core_size;
-    ps.core_cliques = P.core_cliques;
-    ps.max_core = (int32_t)P.max_core;
-    c.peel = ps;
-    c.peeled = true;
-    c.have_comp = false;
+  // graphs small enough (vertices and cliques) for one workgroup each
+  std::vector<ull> gT;
+  graph_counts(c, gT);
+  std::vector<uint32_t> small_ids;
+  std::vector<uint8_t> small(G, 0);
+  if (!(c.tune_flags & KCDS_TUNE_GRID_PEEL))
+    for (uint32_t g = 0; g < G; ++g)
+      if (c.h_gvoff[g + 1] - c.h_gvoff[g] <= BLOCK_PEEL_MAX_N && gT[g] <= BLOCK_PEEL_MAX_T) {
+        small[g] = 1;
+        small_ids.push_back(g);
+      }
+  if (small_ids.size() == G) {
+    peel_blocks(c, {});
+    set_peel_stats(c, c.h_gpeel[0].levels);
     return;
   }
   buf_reserve(c, c.core, sizeof(int32_t) * ((size_t)n + 1));
@@ -556,22 +721,35 @@ void peel_run(Ctx &c) {
   buf_reserve(c, c.tlist, sizeof(uint32_t) * ((size_t)G + 1));
   GraphPeel *gp = c.gpeel.as<GraphPeel>();
   HIP_CHECK(hipMemsetAsync(gp, 0, sizeof(GraphPeel) * G, c.stream));
-  // working counts, two frontiers, the big-vertex list and the control block
-  const size_t words = (size_t)n * 2 /*cnt ull*/ + (size_t)n * 3 + 64;
-  buf_reserve(c, c.tmp0, words * sizeof(uint32_t) + sizeof(PeelCtl) + 256);
+  // working counts, two frontiers (vertices, incidence pieces) in PSEG
+  // segments that can each hold any frontier, and the control block
+  const uint32_t seg_v = ((n + 63) / 64) * 64;
+  const uint64_t seg_p64 = (((uint64_t)n + (T * k) / PPIECE + 64 + 63) / 64) * 64;
+  if (seg_p64 * PSEG >= (1ull << 32)) fail(KCDS_E_LIMIT, "too many cliques for the peeling frontiers");
+  const uint32_t seg_p = (uint32_t)seg_p64;
+  const size_t bytes = sizeof(ull) * (size_t)seg_v + sizeof(uint32_t) * 2 * PSEG * (size_t)seg_v +
+                       sizeof(PItem) * 2 * PSEG * (size_t)seg_p + sizeof(PeelCtl) + 1024;
+  buf_reserve(c, c.tmp0, bytes);
   ull *cnt = c.tmp0.as<ull>();
-  uint32_t *fr0 = (uint32_t *)(cnt + n), *fr1 = fr0 + n, *big = fr1 + n;
-  PeelCtl *ctl = (PeelCtl *)(((uintptr_t)(big + n) + 255) & ~(uintptr_t)255);
-  PeelCtl h0{};
-  h0.minval = ~0ull;
-  h0.live = T;
-  h0.max_num = T;
-  h0.max_den = n ? n : 1;  // main.cu:361-366: the whole graph's density T/n
-  h0.done = n == 0;
-  HIP_CHECK(hipMemcpyAsync(ctl, &h0, sizeof h0, hipMemcpyHostToDevice, c.stream));
+  uint32_t *fv0 = (uint32_t *)(cnt + seg_v), *fv1 = fv0 + (size_t)PSEG * seg_v;
+  PItem *fp0 = (PItem *)(((uintptr_t)(fv1 + (size_t)PSEG * seg_v) + 255) & ~(uintptr_t)255);
+  PItem *fp1 = fp0 + (size_t)PSEG * seg_p;
+  PeelCtl *ctl = (PeelCtl *)(((uintptr_t)(fp1 + (size_t)PSEG * seg_p) + 255) & ~(uintptr_t)255);
+  PeelCtl *h0 = (PeelCtl *)c.host_scratch(sizeof(PeelCtl));
+  memset(h0, 0, sizeof(PeelCtl));
+  for (uint32_t g = 0; g < PSEG; ++g) h0->gmin[g][0] = ~0ull;
+  h0->done = n == 0;
+  HIP_CHECK(hipMemcpyAsync(ctl, h0, sizeof(PeelCtl), hipMemcpyHostToDevice, c.stream));
+  c.sync();  // h0 is reused below
   const uint32_t grid = grid1d(std::max<uint64_t>(n, T));
   k_peel_init<<<grid, 256, 0, c.stream>>>(n, T, c.cnt.as<ull>(), cnt, c.core.as<int32_t>(), c.status.as<int32_t>());
   KCDS_LAUNCH_CHECK();
+  if (!small_ids.empty() && n) {
+    buf_reserve(c, c.tmp3, (size_t)G + 8);
+    HIP_CHECK(hipMemcpyAsync(c.tmp3.p, small.data(), G, hipMemcpyHostToDevice, c.stream));
+    k_exclude<<<grid1d(n), 256, 0, c.stream>>>(n, c.gid.as<uint32_t>(), c.tmp3.as<uint8_t>(), c.core.as<int32_t>());
+    KCDS_LAUNCH_CHECK();
+  }
   const uint32_t *gidp = c.gid.as<uint32_t>();
   if (n) {
     k_graph_accum<<<grid1d(n), 256, 0, c.stream>>>(n, gidp, c.cnt.as<ull>(), gp);
@@ -582,68 +760,44 @@ void peel_run(Ctx &c) {
   const uint32_t *cl = c.cliques.as<uint32_t>(), *inc = c.inc.as<uint32_t>();
   const ull *incoff = c.incoff.as<ull>();
   int32_t *core = c.core.as<int32_t>(), *status = c.status.as<int32_t>();
-  GraphPeel *gpa = multi ? gp : nullptr;
   uint32_t *tl = c.tlist.as<uint32_t>();
-  const uint32_t vgrid = grid1d(n, (uint32_t)c.num_cus * 8);
-  const uint32_t pgrid = (uint32_t)c.num_cus * 8;  // 256-thread blocks, 4 waves each
-  uint32_t *hdone = (uint32_t *)c.host_scratch(64);
-  // enqueue sub-rounds in growing batches; every step is a no-op once done
-  uint32_t batch = 4;
+  const uint32_t sgrid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, c.num_cus * 2ull));
+  // the number of steps is a property of the graph: start from the last count
+  uint32_t batch = c.peel_steps ? (uint32_t)std::min<uint64_t>(c.peel_steps, 1u << 20) : 32;
   uint64_t rounds = 0;
   while (n) {
     for (uint32_t r = 0; r < batch; ++r) {
-      if (multi) k_pl_touch<<<grid1d(G, 1024), 256, 0, c.stream>>>(ctl, gp, tl);
-      k_pl_min<<<vgrid, 256, 0, c.stream>>>(ctl, n, cnt, core);
-      k_pl_open<<<1, 1, 0, c.stream>>>(ctl, n, multi ? 1u : 0u);
-      k_pl_collect<<<vgrid, 256, 0, c.stream>>>(ctl, n, cnt, core, fr0, fr1);
-      k_pl_mark<<<vgrid, 256, 0, c.stream>>>(ctl, fr0, fr1, incoff, core, big, gpa, gidp, tl);
-      k_pl_proc<<<pgrid, 256, 0, c.stream>>>(ctl, k, cl, incoff, inc, cnt, core, status, fr0, fr1, gpa, gidp);
-      k_pl_proc_big<<<pgrid, 256, 0, c.stream>>>(ctl, k, cl, incoff, inc, cnt, core, status, fr0, fr1, big, gpa,
-                                                 gidp);
-      k_pl_finish<<<1, 1, 0, c.stream>>>(ctl, n);
+      k_pl_step<<<sgrid, 256, 0, c.stream>>>(ctl, n, k, cl, incoff, inc, cnt, core, status, fv0, fv1, fp0, fp1, seg_v,
+                                             seg_p, gp, gidp, tl);
       KCDS_LAUNCH_CHECK();
     }
     rounds += batch;
-    HIP_CHECK(hipMemcpyAsync(hdone, &ctl->done, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipMemcpyAsync(h0, ctl, 64 * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
     c.sync();
-    if (*hdone) break;
-    // every sub-round removes at least one vertex, so n + 1 sub-rounds suffice
-    if (rounds > (uint64_t)n + 1)
-      fail(KCDS_E_HIP, "peeling did not finish within %llu sub-rounds", (unsigned long long)rounds);
-    batch = std::min<uint32_t>(batch * 2, 256);
+    if (h0->done) break;
+    // every PROCESS step removes at least one vertex; an OPEN step is
+    // followed by a PROCESS step or by one OPEN step that succeeds
+    if (rounds > 3ull * n + 8)
+      fail(KCDS_E_HIP, "peeling did not finish within %llu steps", (unsigned long long)rounds);
+    batch = std::min<uint32_t>(std::max<uint32_t>(batch, 32) * 2, 128);
   }
-  if (!multi) {
-    k_graph_single<<<1, 1, 0, c.stream>>>(ctl, gp);
-    KCDS_LAUNCH_CHECK();
-  }
+  c.peel_steps = h0->steps;
   k_graph_kprime<<<grid1d(G), 256, 0, c.stream>>>(G, gp);
   KCDS_LAUNCH_CHECK();
   k_graph_summary<<<grid, 256, 0, c.stream>>>(n, T, k, gidp, core, status, cl, gp);
   KCDS_LAUNCH_CHECK();
   PeelCtl *hs = (PeelCtl *)c.host_scratch(sizeof(PeelCtl) + 64);
   HIP_CHECK(hipMemcpyAsync(hs, ctl, sizeof(PeelCtl), hipMemcpyDeviceToHost, c.stream));
-  c.h_gpeel.resize(G);
-  HIP_CHECK(hipMemcpyAsync(c.h_gpeel.data(), gp, sizeof(GraphPeel) * G, hipMemcpyDeviceToHost, c.stream));
   c.sync();
-  const GraphPeel &P = c.h_gpeel[0];
-  kcds_peel_stats ps{};
-  ull g = P.max_num, b = P.max_den;
-  while (b) {
-    ull t = g % b;
-    g = b;
-    b = t;
+  const uint32_t levels = hs->levels;
+  if (!small_ids.empty()) {
+    peel_blocks(c, small_ids);  // also brings every graph's entry to the host
+  } else {
+    c.h_gpeel.resize(G);
+    HIP_CHECK(hipMemcpyAsync(c.h_gpeel.data(), gp, sizeof(GraphPeel) * G, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
   }
-  if (!g) g = 1;
-  ps.max_num = P.max_num / g;
-  ps.max_den = P.max_den / g;
-  ps.k_prime = P.k_prime;
-  ps.levels = hs->levels;
-  ps.core_size = P.core_size;
-  ps.core_cliques = P.core_cliques;
-  ps.max_core = (int32_t)P.max_core;
-  c.peel = ps;
-  c.peeled = true;
-  c.have_comp = false;
+  set_peel_stats(c, levels);
 }
 
 }  // namespace kcds

@@ -189,3 +189,28 @@ __device__ __forceinline__ void stc_u64(ull *p, ull v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Slot of this lane in base[key] for every lane with want: base[key] grows by
+// the number of such lanes.  Lanes that share a key (a hub vertex, the same
+// component) are served by one device atomic per group for the first eight
+// groups; the remaining lanes use one atomic each.  Every lane of the wave
+// must call it.
+template <typename T>
+__device__ __forceinline__ T wave_slot_by_key(bool want, uint32_t key, T *base) {
+  const uint32_t lane = lane_id();
+  T slot = 0;
+  ull pending = __ballot(want);
+#pragma unroll 1
+  for (int round = 0; round < 8 && pending; ++round) {
+    const uint32_t lead = (uint32_t)__builtin_ctzll(pending);
+    const uint32_t kl = __shfl(key, lead, 64);
+    const ull same = __ballot(want && key == kl) & pending;
+    T b = 0;
+    if (lane == lead) b = atomicAdd(&base[kl], (T)__popcll(same));
+    b = __shfl(b, lead, 64);
+    if ((same >> lane) & 1ull) slot = b + (T)lanes_below(same);
+    pending &= ~same;
+  }
+  if ((pending >> lane) & 1ull) slot = atomicAdd(&base[key], (T)1);
+  return slot;
+}
+

@@ -99,6 +99,7 @@ struct Ctx {
   uint32_t list_waves_per_root[4] = {0, 0, 0, 0};  // 0 = default of the class
   float tune_ms[4] = {0, 0, 0, 0};
   uint64_t order_steps = 0;  // work/control steps of the last GPU ordering
+  uint64_t peel_steps = 0;   // steps of the last grid peel
   uint64_t graph_budget = 0; // per-graph cap on T * (8k + 4) bytes (0 = none)
 
   // pinned host staging for small readbacks

@@ -86,3 +86,22 @@ def test_batch_rejects_cross_edges(solver):
     res = (kcds.GraphResult * 2)()
     rc = kcds.lib().kcds_solve_batch(solver._h, 3, 2, voff.ctypes.data_as(kcds.U32P), res, None)
     assert rc == -1  # KCDS_E_INVALID: edges between the two declared graphs
+
+
+def test_batch_mixed_peels(solver):
+    """A batch whose graphs go to different peels: one with more cliques than a
+    workgroup peel takes (K40 plus noise at k = 5: C(40,5) = 658,008 cliques)
+    is peeled by the grid kernels while the small ones are peeled a workgroup
+    each; every graph's answer equals the oracle's."""
+    gs = mixed_batch()[:6]
+    g = gen.er(120, 0.05, 40, seed=77)
+    gs.append((g.offset, g.nbr))
+    res, sets = solver.solve_batch(gs, 5)
+    for i, (off, nbr) in enumerate(gs):
+        r, pv, core, comp, ins_r = O.solve(off, nbr, 5)
+        assert res[i].T == r.T
+        if r.T == 0:
+            continue
+        assert (res[i].rho_num, res[i].rho_den) == (r.rho_num, r.rho_den)
+        np.testing.assert_array_equal(sets[i], ins_r)
+    assert res[-1].T > (1 << 18)
