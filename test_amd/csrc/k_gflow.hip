@@ -7,13 +7,18 @@
 //   build     degree count, scan, then arcs (same layout as the workgroup
 //             solver: vertex v: [v->t, v->s, v->c ...]; clique c: k arcs;
 //             s: nc arcs; t: nc arcs) with capacities scaled by q;
-//   relabel   level-synchronous BFS from t over residual arcs, one kernel per
-//             level (frontier in NSEG segments, the last workgroup to finish
-//             a level swaps the frontiers), giving exact distances to t;
-//   sweeps    discharge of every active node (wave per vertex node, thread
-//             per clique node): push the excess along all residual arcs to
-//             the lowest neighbour height, or relabel; a sweep is a no-op
-//             when the previous one changed nothing.
+//   relabel   level-synchronous BFS from t over residual arcs, two kernels
+//             per level (vertex nodes of the frontier by arc chunks, then the
+//             rest a wave per node; frontier in NSEG segments, the last
+//             workgroup to finish a level swaps the frontiers), giving exact
+//             distances to t;
+//   sweeps    discharge of every active node: push the excess along all
+//             residual arcs to the lowest neighbour height, or relabel; a
+//             sweep is a no-op when the previous one changed nothing.  Clique
+//             nodes (k arcs) take a thread each; vertex nodes (up to millions
+//             of arcs) are cut into chunks of VCH arcs, a wave per chunk, in
+//             three kernels (lowest height, residual to it per chunk, push in
+//             arc order from each chunk's share of the excess).
 // The host runs the phases (one readback per phase) and the Dinkelbach
 // iteration.  Only a node's owner lowers its excess and the residuals of its
 // arcs; other workgroups only raise them (atomics), so an owner that reads a
@@ -35,6 +40,7 @@ constexpr uint32_t NSEG = 8;
 constexpr uint32_t SWEEPS = 24;
 constexpr uint32_t MAX_PHASES = 1u << 14;
 constexpr uint32_t NO_ARC = 0xFFFFFFFFu;
+constexpr uint32_t VCH = 4096;  // arcs of a vertex node per work item
 
 // Counters 64 words apart (same-address device atomics are serialised).
 struct GFCtl {
@@ -170,7 +176,8 @@ __device__ bool last_block(GFCtl *ctl, uint32_t *s_last) {
 // One BFS level: wave per frontier node y; x = head of an arc of y with
 // residual capacity on x -> y gets height dist + 1.  seg_cap: capacity of a
 // frontier segment.
-__global__ __launch_bounds__(256) void k_gf_bfs_level(uint32_t N, const uint32_t *off, const uint32_t *head,
+__global__ __launch_bounds__(256) void k_gf_bfs_level(uint32_t N, uint32_t nv_nodes, const uint32_t *off,
+                                                      const uint32_t *head,
                                                       const uint32_t *rev, const i64 *res, uint32_t *height,
                                                       uint32_t *fr0, uint32_t *fr1, uint32_t seg_cap, GFCtl *ctl) {
   __shared__ uint32_t s_hdr[3], s_pre[NSEG + 1], s_last;
@@ -194,6 +201,7 @@ __global__ __launch_bounds__(256) void k_gf_bfs_level(uint32_t N, const uint32_t
     uint32_t g = 0;
     while (i >= s_pre[g + 1]) ++g;
     const uint32_t y = fr[(size_t)g * seg_cap + (i - s_pre[g])];
+    if (y < nv_nodes) continue;  // vertex nodes: k_gf_bfs_vchunks
     const uint32_t ye = off[y + 1];
     for (uint32_t a0 = off[y]; a0 < ye; a0 += KCDS_WAVE) {
       const uint32_t a = a0 + lane;
@@ -242,56 +250,167 @@ __global__ __launch_bounds__(256) void k_gf_active(uint32_t N, const i64 *excess
   if (threadIdx.x == 0 && s) atomicAdd(&ctl->active[blockIdx.x % NSEG][0], s);
 }
 
-// Vertex nodes, wave per node; sweep i counts into changed[i+1] and is a
-// no-op when changed[i] is zero.
-__global__ __launch_bounds__(256) void k_gf_sweep_v(uint32_t nc, uint32_t N, uint32_t sweep, const uint32_t *off,
-                                                    const uint32_t *head, const uint32_t *rev, i64 *res, i64 *excess,
-                                                    uint32_t *height, GFCtl *ctl) {
+// Vertex nodes hold most of the arcs (a vertex is in up to millions of
+// cliques), so their work is cut into chunks of VCH arcs: nch[u] chunks of
+// vertex u, chunk ids choff[u] .. choff[u+1], chv[chunk] = u.
+__global__ void k_gf_nch(uint32_t nc, const uint32_t *off, uint32_t *nch) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < nc; u += gridDim.x * blockDim.x)
+    nch[u] = (off[u + 1] - off[u] + VCH - 1) / VCH;
+}
+
+__global__ void k_gf_chv(uint32_t nc, const uint32_t *choff, uint32_t *chv) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < nc; u += gridDim.x * blockDim.x)
+    for (uint32_t ch = choff[u]; ch < choff[u + 1]; ++ch) chv[ch] = u;
+}
+
+// BFS level, vertex nodes of the frontier (those at height dist): wave per
+// chunk of their arcs.  Runs before k_gf_bfs_level of the same level, which
+// handles the other frontier nodes and advances the level.
+__global__ __launch_bounds__(256) void k_gf_bfs_vchunks(uint32_t N, uint32_t nch, const uint32_t *choff,
+                                                        const uint32_t *chv, const uint32_t *off, const uint32_t *head,
+                                                        const uint32_t *rev, const i64 *res, uint32_t *height,
+                                                        uint32_t *fr0, uint32_t *fr1, uint32_t seg_cap, GFCtl *ctl) {
+  const uint32_t dist = ctl->dist, cur = ctl->cur;
+  if (ctl->done) return;
+  uint32_t *nx = cur ? fr0 : fr1;
+  const uint32_t lane = lane_id(), seg = blockIdx.x % NSEG, s = N - 2;
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KCDS_WAVE, nw = gridDim.x * blockDim.x / KCDS_WAVE;
+  for (uint32_t ch = wid; ch < nch; ch += nw) {
+    const uint32_t u = chv[ch];
+    if (height[u] != dist) continue;
+    const uint32_t a0 = off[u] + (ch - choff[u]) * VCH, a1 = min(a0 + VCH, off[u + 1]);
+    for (uint32_t b = a0; b < a1; b += KCDS_WAVE) {
+      const uint32_t a = b + lane;
+      bool add = false;
+      uint32_t x = 0;
+      if (a < a1) {
+        x = head[a];
+        if (x != s && res[rev[a]] > 0 && height[x] == N) add = atomicCAS(&height[x], N, dist + 1) == N;
+      }
+      const ull bal = __ballot(add);
+      if (bal) {
+        const uint32_t lead = (uint32_t)__builtin_ctzll(bal);
+        uint32_t base = 0;
+        if (lane == lead) base = atomicAdd(&ctl->fsize[cur ^ 1][seg][0], (uint32_t)__popcll(bal));
+        base = __shfl(base, lead, 64);
+        if (add) nx[(size_t)seg * seg_cap + base + lanes_below(bal)] = x;
+      }
+    }
+  }
+}
+
+// Vertex discharge in three kernels (wave per chunk).  Vertex nodes only
+// push to clique nodes and t, whose heights and the vertices' own excess do
+// not change while these run.
+//   A: vmin[u] = lowest head height over u's residual arcs
+__global__ __launch_bounds__(256) void k_gf_vmin(uint32_t N, uint32_t nch, uint32_t sweep, const uint32_t *choff,
+                                                 const uint32_t *chv, const uint32_t *off, const uint32_t *head,
+                                                 const i64 *res, const i64 *excess, const uint32_t *height,
+                                                 uint32_t *vmin, const GFCtl *ctl) {
+  if (ctl->changed[sweep][0] == 0) return;
+  const uint32_t lane = lane_id();
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KCDS_WAVE, nw = gridDim.x * blockDim.x / KCDS_WAVE;
+  for (uint32_t ch = wid; ch < nch; ch += nw) {
+    const uint32_t u = chv[ch];
+    if (excess[u] <= 0 || height[u] >= N) continue;
+    const uint32_t a0 = off[u] + (ch - choff[u]) * VCH, a1 = min(a0 + VCH, off[u + 1]);
+    uint32_t m = NO_ARC;
+    for (uint32_t a = a0 + lane; a < a1; a += KCDS_WAVE)
+      if (res[a] > 0) m = min(m, height[head[a]]);
+    m = wave_min_u32(m);
+    if (lane == 0 && m != NO_ARC) atomicMin(&vmin[u], m);
+  }
+}
+
+//   B: csum[chunk] = residual on the chunk's arcs to height vmin[u] when u
+//      pushes (height[u] > vmin[u]); esnap[u] = excess[u]
+__global__ __launch_bounds__(256) void k_gf_vsum(uint32_t N, uint32_t nch, uint32_t sweep, const uint32_t *choff,
+                                                 const uint32_t *chv, const uint32_t *off, const uint32_t *head,
+                                                 const i64 *res, const i64 *excess, const uint32_t *height,
+                                                 const uint32_t *vmin, i64 *csum, i64 *esnap, const GFCtl *ctl) {
+  if (ctl->changed[sweep][0] == 0) return;
+  const uint32_t lane = lane_id();
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KCDS_WAVE, nw = gridDim.x * blockDim.x / KCDS_WAVE;
+  for (uint32_t ch = wid; ch < nch; ch += nw) {
+    const uint32_t u = chv[ch];
+    const i64 e = excess[u];
+    const uint32_t hm = vmin[u];
+    i64 sum = 0;
+    if (e > 0 && height[u] < N && hm != NO_ARC && height[u] > hm) {
+      const uint32_t a0 = off[u] + (ch - choff[u]) * VCH, a1 = min(a0 + VCH, off[u + 1]);
+      for (uint32_t a = a0 + lane; a < a1; a += KCDS_WAVE) {
+        const i64 r = res[a];
+        if (r > 0 && height[head[a]] == hm) sum += r;
+      }
+      sum = (i64)wave_sum_u64((ull)sum);
+    }
+    if (lane == 0) {
+      csum[ch] = sum;
+      if (ch == choff[u]) esnap[u] = e;
+    }
+  }
+}
+
+//   C: push esnap[u] along the arcs to height vmin[u] in arc order (chunk by
+//      chunk, each from its share: esnap minus the earlier chunks' sums), or
+//      relabel u when nothing is lower (first chunk); vnext[u] is cleared for
+//      the next sweep.
+__global__ __launch_bounds__(256) void k_gf_vpush(uint32_t N, uint32_t nch, uint32_t sweep, const uint32_t *choff,
+                                                  const uint32_t *chv, const uint32_t *off, const uint32_t *head,
+                                                  const uint32_t *rev, i64 *res, i64 *excess, uint32_t *height,
+                                                  const uint32_t *vmin, uint32_t *vnext, const i64 *csum,
+                                                  const i64 *esnap, GFCtl *ctl) {
   if (ctl->changed[sweep][0] == 0) return;
   const uint32_t lane = lane_id();
   const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KCDS_WAVE, nw = gridDim.x * blockDim.x / KCDS_WAVE;
   uint32_t changed = 0;
-  for (uint32_t u = wid; u < nc; u += nw) {
-    const i64 e = excess[u];
-    if (e <= 0 || height[u] >= N) continue;
-    const uint32_t b = off[u], en = off[u + 1];
-    uint32_t hmin = NO_ARC;
-    for (uint32_t a = b + lane; a < en; a += KCDS_WAVE)
-      if (res[a] > 0) hmin = min(hmin, height[head[a]]);
-    hmin = wave_min_u32(hmin);
-    if (hmin == NO_ARC) continue;
-    if (height[u] <= hmin) {
-      if (lane == 0) height[u] = hmin + 1;
-      ++changed;
+  for (uint32_t ch = wid; ch < nch; ch += nw) {
+    const uint32_t u = chv[ch], c0 = choff[u];
+    const uint32_t hm = vmin[u];
+    const uint32_t hu = height[u];
+    if (ch == c0 && lane == 0) vnext[u] = NO_ARC;
+    if (esnap[u] <= 0 || hu >= N || hm == NO_ARC) continue;
+    if (hu <= hm) {
+      if (ch == c0) {
+        if (lane == 0) height[u] = hm + 1;
+        ++changed;
+      }
       continue;
     }
-    i64 left = e;
-    for (uint32_t a0 = b; a0 < en && left > 0; a0 += KCDS_WAVE) {
-      const uint32_t a = a0 + lane;
+    i64 before = 0;
+    for (uint32_t q = c0; q < ch; ++q) before += csum[q];
+    i64 left = esnap[u] - before;
+    if (left <= 0 || csum[ch] == 0) continue;
+    const uint32_t a0 = off[u] + (ch - c0) * VCH, a1 = min(a0 + VCH, off[u + 1]);
+    i64 pushed = 0;
+    for (uint32_t b = a0; b < a1 && left > 0; b += KCDS_WAVE) {
+      const uint32_t a = b + lane;
       i64 r = 0;
       uint32_t x = 0;
-      if (a < en) {
+      if (a < a1) {
         r = res[a];
         if (r > 0) {
           x = head[a];
-          if (height[x] != hmin) r = 0;
+          if (height[x] != hm) r = 0;
         } else {
           r = 0;
         }
       }
       const ull incl = wave_incl_scan_u64((ull)r);
-      const i64 before = (i64)incl - r;
-      const i64 d = r > 0 && left > before ? min(r, left - before) : 0;
+      const i64 pre = (i64)incl - r;
+      const i64 d = r > 0 && left > pre ? min(r, left - pre) : 0;
       if (d > 0) {
         add64(&res[a], -d);
         add64(&res[rev[a]], d);
         add64(&excess[x], d);
       }
       const i64 tot = (i64)__shfl(incl, 63, 64);
-      left -= min(tot, left);
+      const i64 used = min(tot, left);
+      left -= used;
+      pushed += used;
     }
-    if (e > left) {
-      if (lane == 0) add64(&excess[u], -(e - left));
+    if (pushed > 0) {
+      if (lane == 0) add64(&excess[u], -pushed);
       ++changed;
     }
   }
@@ -302,7 +421,7 @@ __global__ __launch_bounds__(256) void k_gf_sweep_v(uint32_t nc, uint32_t N, uin
 __global__ __launch_bounds__(256) void k_gf_sweep_c(uint32_t nc, uint32_t N, uint32_t sweep, const uint32_t *off,
                                                     const uint32_t *head, const uint32_t *rev, i64 *res, i64 *excess,
                                                     uint32_t *height, GFCtl *ctl) {
-  if (ctl->changed[sweep][0] == 0) return;
+  if (ctl->changed[sweep][0] == 0) return;  // (also skips when the vertex kernels of this sweep did)
   uint32_t changed = 0;
   const uint32_t s = N - 2;
   for (uint64_t u = nc + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < s; u += (uint64_t)gridDim.x * blockDim.x) {
@@ -430,6 +549,24 @@ void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t m
   k_gf_topo_fixed<<<ggrid(c, std::max<uint64_t>(nc, Tc + 1)), 256, 0, c.stream>>>(nc, Tc, k, off, head, rev, fill);
   k_gf_topo_cliques<<<ggrid(c, Tc * k), 256, 0, c.stream>>>(ccl, Tc, k, nc, off, head, rev, fill);
   KCDS_LAUNCH_CHECK();
+  // ---- vertex arc chunks ----
+  buf_reserve(c, f.g_nch, sizeof(uint32_t) * (nc + 1));
+  buf_reserve(c, f.g_choff, sizeof(uint32_t) * (nc + 1));
+  buf_reserve(c, f.g_vmin, sizeof(uint32_t) * 2ull * nc);
+  buf_reserve(c, f.g_esnap, sizeof(i64) * nc);
+  uint32_t *nchv = f.g_nch.as<uint32_t>(), *choff = f.g_choff.as<uint32_t>();
+  k_gf_nch<<<ggrid(c, nc), 256, 0, c.stream>>>(nc, off, nchv);
+  KCDS_LAUNCH_CHECK();
+  scan_u32_to_u32(c, nchv, choff, nc);  // choff[0..nc]
+  uint32_t nch = 0;
+  HIP_CHECK(hipMemcpyAsync(&nch, choff + nc, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  buf_reserve(c, f.g_chv, sizeof(uint32_t) * (nch + 1));
+  buf_reserve(c, f.g_csum, sizeof(i64) * (nch + 1));
+  uint32_t *chv = f.g_chv.as<uint32_t>(), *vmin = f.g_vmin.as<uint32_t>();
+  i64 *csum = f.g_csum.as<i64>(), *esnap = f.g_esnap.as<i64>();
+  k_gf_chv<<<ggrid(c, nc), 256, 0, c.stream>>>(nc, choff, chv);
+  KCDS_LAUNCH_CHECK();
   GFCtl *h = (GFCtl *)c.host_scratch(sizeof(GFCtl));
   const uint32_t bgrid = (uint32_t)c.num_cus * 4;  // BFS levels and sweeps: 256-thread blocks
   ull p = task.p0, q = task.q0;
@@ -448,8 +585,11 @@ void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t m
       uint32_t levels = 0;
       while (true) {
         const uint32_t batch = 16;
-        for (uint32_t i = 0; i < batch; ++i)
-          k_gf_bfs_level<<<bgrid, 256, 0, c.stream>>>(N, off, head, rev, res, height, fr0, fr1, seg_cap, ctl);
+        for (uint32_t i = 0; i < batch; ++i) {
+          k_gf_bfs_vchunks<<<bgrid, 256, 0, c.stream>>>(N, nch, choff, chv, off, head, rev, res, height, fr0, fr1,
+                                                        seg_cap, ctl);
+          k_gf_bfs_level<<<bgrid, 256, 0, c.stream>>>(N, nc, off, head, rev, res, height, fr0, fr1, seg_cap, ctl);
+        }
         KCDS_LAUNCH_CHECK();
         levels += batch;
         k_gf_active<<<ggrid(c, Nn), 256, 0, c.stream>>>(N, excess, height, ctl);
@@ -464,8 +604,14 @@ void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t m
       ull active = 0;
       for (uint32_t s = 0; s < NSEG; ++s) active += h->active[s][0];
       if (active == 0) break;
+      k_gf_fill_u32<<<ggrid(c, 2ull * nc), 256, 0, c.stream>>>(vmin, NO_ARC, 2ull * nc);
       for (uint32_t sw = 0; sw < SWEEPS; ++sw) {
-        k_gf_sweep_v<<<bgrid, 256, 0, c.stream>>>(nc, N, sw, off, head, rev, res, excess, height, ctl);
+        uint32_t *vm = vmin + (size_t)(sw & 1) * nc, *vn = vmin + (size_t)((sw + 1) & 1) * nc;
+        k_gf_vmin<<<bgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, res, excess, height, vm, ctl);
+        k_gf_vsum<<<bgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, res, excess, height, vm, csum, esnap,
+                                               ctl);
+        k_gf_vpush<<<bgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, rev, res, excess, height, vm, vn,
+                                                csum, esnap, ctl);
         k_gf_sweep_c<<<ggrid(c, Tc), 256, 0, c.stream>>>(nc, N, sw, off, head, rev, res, excess, height, ctl);
       }
       KCDS_LAUNCH_CHECK();
