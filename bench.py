@@ -14,7 +14,10 @@ Also reported on the same JSON line:
                 root's out-neighbour rows staged, int64 counts written) over
                 their hipEvent-timed duration, against 8 TB/s;
   cpu_baseline  the CPU oracle's count of the same graph (all host threads);
-  extras        exact k-CDS solves/s on config C1 (ER 2000 + K40, k = 3).
+  extras        exact k-CDS solves/s on config C1 (ER 2000 + K40, k = 3);
+                C4 batched vs plain solves/s (4,096 graphs, k = 4); C5 served
+                vs plain solves/s and p50/p99 latency on the first requests of
+                the Poisson 200/s stream.
 """
 import argparse
 import json
@@ -71,6 +74,42 @@ def bench_c4(np, gen, kcds, plain_sample):
             "C4_T_total": int(sum(r.T for r in res)), "C4_waves_per_root": tuned[0]}
 
 
+def bench_c5(count, graph_budget, slots):
+    """C5: the first `count` requests of the Poisson 200/s R-MAT stream (k in 3..6).
+
+    served: the continuous-batching loop over `slots` device contexts; plain:
+    the same loop with one slot and one graph per call.  Requests whose
+    cliques exceed the per-graph budget are answered over-budget (rejected
+    on the count pass) in both forms.  Latency runs from the scheduled
+    arrival to the result on the host."""
+    import copy
+    from test_amd import kcds, serve
+    t0 = time.perf_counter()
+    reqs = serve.c5_requests(count=count)
+    t_gen = time.perf_counter() - t0
+    out = {"C5_requests": len(reqs), "C5_stream_seconds": reqs[-1].t_arrival if reqs else 0.0,
+           "C5_graph_budget_bytes": graph_budget, "C5_slots": slots, "C5_generate_s": t_gen}
+    with serve.Server(slots=slots, batch_edges=4_000_000, graph_budget=graph_budget) as srv:
+        srv.run([copy.copy(r) for r in reqs[: min(8, len(reqs))]])  # warm every slot's kernels
+        for form, plain in (("served", False), ("plain", True)):
+            done, st = srv.run([copy.copy(r) for r in reqs], plain=plain)
+            out[f"C5_{form}_solves_per_s"] = st.solves_per_s
+            out[f"C5_{form}_p50_ms"] = st.p50_ms
+            out[f"C5_{form}_p99_ms"] = st.p99_ms
+            out[f"C5_{form}_wall_s"] = st.wall_s
+            out[f"C5_{form}_solved"] = st.solved
+            out[f"C5_{form}_over_budget"] = st.over_budget
+            out[f"C5_{form}_failed"] = st.failed
+            if not plain:
+                out["C5_served_mean_batch"] = st.mean_batch
+                served = {r.rid: (r.status, r.T, r.rho_num, r.rho_den) for r in done}
+            else:
+                same = all(served[r.rid] == (r.status, r.T, r.rho_num, r.rho_den) for r in done
+                           if r.status == kcds.GRAPH_SOLVED)
+                out["C5_served_equals_plain"] = bool(same)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +121,11 @@ def main():
     ap.add_argument("--shape", default=None, help="waves per root of the 4 width classes, e.g. 2,2,4,8 (skips the autotuner)")
     ap.add_argument("--c4-plain-sample", type=int, default=256,
                     help="graphs of C4 solved one per call for the plain-form rate")
+    ap.add_argument("--c5-requests", type=int, default=100,
+                    help="first requests of the C5 stream served (0 skips the serving leg)")
+    ap.add_argument("--c5-graph-budget", type=int, default=256 << 20,
+                    help="per-graph HBM admission budget of the serving leg, bytes")
+    ap.add_argument("--c5-slots", type=int, default=4)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -181,6 +225,8 @@ def main():
         extras["C1_rho"] = f"{res.rho_num}/{res.rho_den}"
         s1.close()
         extras.update(bench_c4(np, gen, kcds, args.c4_plain_sample))
+        if args.c5_requests > 0:
+            extras.update(bench_c5(args.c5_requests, args.c5_graph_budget, args.c5_slots))
 
     line = {
         "metric": "k-cliques listed/s", "value": value, "unit": "cliques/s", "n_gpus": 1,

@@ -120,7 +120,8 @@ typedef struct kcds_graph_result {
   uint32_t core_vertices;
   uint32_t ncomp;
   uint32_t flow_calls;
-  uint32_t status;     /* KCDS_GRAPH_SOLVED, or KCDS_GRAPH_OVER_BUDGET (T is the count; nothing else solved) */
+  uint32_t status;     /* KCDS_GRAPH_SOLVED, or KCDS_GRAPH_OVER_BUDGET (nothing solved; T is then a lower
+                          bound above the cap: counting stops once the graph is over budget) */
   uint32_t pad;
 } kcds_graph_result;
 
@@ -128,10 +129,12 @@ typedef struct kcds_graph_result {
 #define KCDS_GRAPH_OVER_BUDGET 1u
 
 /* Per-graph HBM admission: a graph whose materialised cliques, incidence
- * and status (T * (8k + 4) bytes, known after the count pass) exceed
- * bytes is not solved -- kcds_solve fails with KCDS_E_NOMEM, a batch marks
- * the graph KCDS_GRAPH_OVER_BUDGET and solves the others.  0 = no per-graph
- * limit (the context budget still applies).  No counterpart in the reference. */
+ * and status (T * (8k + 4) bytes) exceed bytes is not solved -- kcds_solve
+ * fails with KCDS_E_NOMEM, a batch marks the graph KCDS_GRAPH_OVER_BUDGET
+ * and solves the others.  It is decided on the count pass, which stops
+ * counting a graph as soon as its count passes bytes / (8k + 4).  0 = no
+ * per-graph limit (the context budget still applies).  No counterpart in
+ * the reference. */
 int kcds_set_graph_budget(kcds_ctx *ctx, uint64_t bytes);
 
 /* Batched solve: the uploaded CSR is the disjoint union of ngraphs graphs,

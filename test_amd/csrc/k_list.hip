@@ -54,6 +54,7 @@ template <int W> struct Frame {
   ull A[W * D];      // A[w*D + j]: bits 64w..64w+63 of row j
   ull rt;            // cliques of the current root (summed over waves)
   uint32_t root;
+  uint32_t skip;     // the root's graph is already over its admission cap
 };
 
 constexpr uint32_t UNROLL = 4;  // 64-element chunks of one out-list in flight per wave
@@ -283,12 +284,26 @@ template <int W> __device__ __forceinline__ uint32_t share_root(Frame<W> &f, uin
   return idx;
 }
 
+// Adds E (wave-uniform) to the wave's pending count of graph g; at theta the
+// pending count goes to gsum[g].  True when gsum[g] has passed cap.
+__device__ __forceinline__ bool admit(ull *gsum, uint32_t g, ull E, ull &pend, ull cap, ull theta) {
+  pend += E;
+  if (pend < theta) return false;
+  ull old = 0;
+  if (lane_id() == 0) old = atomicAdd(&gsum[g], pend);
+  old = __shfl(old, 0, 64);
+  const bool over = old + pend > cap;
+  pend = 0;
+  return over;
+}
+
 template <int W, int K>
 __global__ __launch_bounds__(64 * MAX_WAVES) void k_list_count(const uint32_t *__restrict__ doff,
                                                                const uint32_t *__restrict__ dnbr,
                                                                const uint32_t *__restrict__ roots, uint32_t nroots,
                                                                uint32_t *queue, ull *gcnt, uint64_t cstride,
-                                                               uint32_t ncopy, ull *rootT, ull *gtotal) {
+                                                               uint32_t ncopy, ull *rootT, ull *gtotal,
+                                                               const uint32_t *gid, ull *gsum, ull cap, ull theta) {
   __shared__ Frame<W> f;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
   ull tot = 0, w2 = 0;
@@ -296,13 +311,36 @@ __global__ __launch_bounds__(64 * MAX_WAVES) void k_list_count(const uint32_t *_
   // per-vertex counts go to one of ncopy copies (summed afterwards), so a
   // hub's counter is not one serialised address for the whole launch
   gcnt += (uint64_t)(blockIdx.x % ncopy) * cstride;
+  // Admission cap (cap > 0): gsum[g] gathers a lower bound of graph g's
+  // cliques (each wave adds its pending count once it reaches theta); once
+  // it passes cap the graph is over budget and its remaining work is
+  // skipped (its counts are then partial and the caller drops it).
+  ull pend = 0;
+  uint32_t pg = 0xFFFFFFFFu;
   QueueCursor qc = queue_start(blockIdx.x);
   uint32_t pre = tid == 0 ? queue_next(queue, nroots, qc) : 0u;  // next root (thread 0)
   while (true) {
-    const uint32_t idx = share_root(f, pre);
+    if (tid == 0) {
+      f.root = pre;
+      uint32_t sk = 0;
+      if (cap && pre < nroots) sk = ldc_u64(&gsum[gid ? gid[roots[pre]] : 0u]) > cap;
+      f.skip = sk;
+    }
+    block_sync();
+    const uint32_t idx = f.root, skip = f.skip;
+    block_sync();  // every thread has read the frame header before it is reused
     if (idx >= nroots) break;
     if (tid == 0) pre = queue_next(queue, nroots, qc);  // in flight while this root is processed
+    if (skip) continue;
     const uint32_t r = roots[idx];
+    uint32_t g = 0;
+    if (cap) {
+      g = gid ? gid[r] : 0u;
+      if (g != pg) {
+        pend = 0;
+        pg = g;
+      }
+    }
     const uint32_t base = doff[r], d = doff[r + 1] - base;
     w2 += build_rows<W>(f, doff, dnbr, base, d);
     ull cnt[W];
@@ -313,6 +351,7 @@ __global__ __launch_bounds__(64 * MAX_WAVES) void k_list_count(const uint32_t *_
     ull rt = 0;
     if constexpr (K == 3) {
       rt = edges_in<W>(f, S, cnt, wv, nw);  // twice the wave's share of edges
+      if (cap) admit(gsum, g, rt >> 1, pend, cap, theta);
     } else {
       for (uint32_t c = wv; c < d; c += nw) {
         ull S2[W];
@@ -326,6 +365,7 @@ __global__ __launch_bounds__(64 * MAX_WAVES) void k_list_count(const uint32_t *_
         const ull E = Count<W, K - 2>::run(f, S2, cnt);
         credit<W>(cnt, c, E);
         rt += E;
+        if (cap && admit(gsum, g, E, pend, cap, theta)) break;
       }
     }
 #pragma unroll
@@ -480,7 +520,7 @@ __global__ void k_incidence(const uint32_t *cl, uint64_t T, uint32_t k, const ul
 }
 
 typedef void (*CountFn)(const uint32_t *, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, ull *, uint64_t,
-                        uint32_t, ull *, ull *);
+                        uint32_t, ull *, ull *, const uint32_t *, ull *, ull, ull);
 typedef void (*EmitFn)(const uint32_t *, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, const ull *,
                        uint32_t *);
 
@@ -580,12 +620,15 @@ void classify_roots(Ctx &c, uint32_t k) {
 
 // One launch of class cls over its roots; counters accumulate into the
 // ncopy copies of gcnt, rootT and the sharded totals.
-void launch_class(Ctx &c, int cls, uint32_t k, uint32_t nw, uint32_t *queue, ull *gcnt, ull *rootT, ull *totals) {
+void launch_class(Ctx &c, int cls, uint32_t k, uint32_t nw, uint32_t *queue, ull *gcnt, ull *rootT, ull *totals,
+                  ull cap = 0) {
   const uint32_t nr = c.class_count[cls];
   const uint32_t *roots = c.roots.as<uint32_t>() + c.class_start[cls];
+  const uint32_t *gid = c.ngraphs > 1 ? c.gid.as<uint32_t>() : nullptr;
+  const ull theta = cap / 64 + 1;
   hipLaunchKernelGGL(class_count_fn(cls, k), dim3(wg_grid(c, cls, nr, nw)), dim3(64 * nw), 0, c.stream,
                      c.doff.as<uint32_t>(), c.dnbr.as<uint32_t>(), roots, nr, queue, gcnt, copy_stride(c.n),
-                     count_copies(c.n), rootT, totals);
+                     count_copies(c.n), rootT, totals, gid, cap ? c.gsum.as<ull>() : nullptr, cap, theta);
   KCDS_LAUNCH_CHECK();
 }
 
@@ -656,7 +699,7 @@ void list_autotune(Ctx &c, uint32_t k) {
   HIP_CHECK(hipEventDestroy(e1));
 }
 
-void list_count(Ctx &c, uint32_t k) {
+void list_count(Ctx &c, uint32_t k, ull cap) {
   if (!c.oriented) fail(KCDS_E_STATE, "kcds_count/kcds_list before kcds_orient");
   if (k < 3 || k > 8) fail(KCDS_E_INVALID, "k = %u outside 3..8", k);
   const uint32_t n = c.n;
@@ -670,6 +713,11 @@ void list_count(Ctx &c, uint32_t k) {
   HIP_CHECK(hipMemsetAsync(c.rootT.p, 0, sizeof(ull) * ((size_t)n + 2), c.stream));
   ull *totals = c.lsum.as<ull>();
   HIP_CHECK(hipMemsetAsync(totals, 0, sizeof(ull) * QSHARDS * 8, c.stream));
+  if (cap) {
+    buf_reserve(c, c.gsum, sizeof(ull) * ((size_t)c.ngraphs + 1));
+    HIP_CHECK(hipMemsetAsync(c.gsum.p, 0, sizeof(ull) * c.ngraphs, c.stream));
+  }
+  c.count_cap = cap;
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
@@ -678,7 +726,7 @@ void list_count(Ctx &c, uint32_t k) {
   for (int cls = NCLASS - 1; cls >= 0; --cls) {
     if (!c.class_count[cls]) continue;
     launch_class(c, cls, k, list_waves_per_root(c, cls, k), count_queue(c, cls), c.cnt.as<ull>(), c.rootT.as<ull>(),
-                 totals);
+                 totals, cap);
     ++launches;
   }
   if (ncopy > 1 && n) {
@@ -710,17 +758,23 @@ void graph_counts(Ctx &c, std::vector<ull> &gT) {
   gT.assign(G, 0);
   if (G == 1) {
     gT[0] = c.T;
-    return;
+  } else {
+    buf_reserve(c, c.tmp3, sizeof(ull) * ((size_t)G + 1));
+    ull *d = c.tmp3.as<ull>();
+    HIP_CHECK(hipMemsetAsync(d, 0, sizeof(ull) * G, c.stream));
+    if (n) {
+      const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)c.num_cus * 8);
+      k_graph_T<<<grid, 256, 0, c.stream>>>(c.rootT.as<ull>(), c.gid.as<uint32_t>(), n, d);
+      KCDS_LAUNCH_CHECK();
+    }
+    HIP_CHECK(hipMemcpyAsync(gT.data(), d, sizeof(ull) * G, hipMemcpyDeviceToHost, c.stream));
   }
-  buf_reserve(c, c.tmp3, sizeof(ull) * ((size_t)G + 1));
-  ull *d = c.tmp3.as<ull>();
-  HIP_CHECK(hipMemsetAsync(d, 0, sizeof(ull) * G, c.stream));
-  if (n) {
-    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)c.num_cus * 8);
-    k_graph_T<<<grid, 256, 0, c.stream>>>(c.rootT.as<ull>(), c.gid.as<uint32_t>(), n, d);
-    KCDS_LAUNCH_CHECK();
+  if (c.count_cap) {  // graphs stopped at the cap: their lower bound (counts are partial)
+    std::vector<ull> gs(G);
+    HIP_CHECK(hipMemcpyAsync(gs.data(), c.gsum.p, sizeof(ull) * G, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    for (uint32_t g = 0; g < G; ++g) gT[g] = std::max(gT[g], gs[g]);
   }
-  HIP_CHECK(hipMemcpyAsync(gT.data(), d, sizeof(ull) * G, hipMemcpyDeviceToHost, c.stream));
   c.sync();
 }
 

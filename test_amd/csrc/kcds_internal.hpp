@@ -130,6 +130,8 @@ struct Ctx {
   uint32_t k = 0;
   DevBuf cnt;     // ull[n] per-vertex counts in new-id space (copy 0 of the listing's copies)
   DevBuf lsum;    // ull sharded listing totals
+  DevBuf gsum;    // ull per graph: admission lower bound of the count pass
+  ull count_cap = 0;  // cliques per graph above which the last count pass stopped a graph (0 = none)
   DevBuf scal;    // ull scratch scalars (T, counters)
   DevBuf rootT;   // ull[n+1] per-root clique counts -> offsets
   DevBuf cliques; // uint32[T*k] new ids
@@ -185,7 +187,7 @@ void orient_from_rank(Ctx &c);       // rank/inv set -> doff, dnbr, dmax, darcs,
 void degeneracy_rank_gpu(Ctx &c);    // fills rank/inv from a GPU k-core peel
 
 // ---- listing (k_list.hip) ----
-void list_count(Ctx &c, uint32_t k);     // cnt, T
+void list_count(Ctx &c, uint32_t k, ull cap = 0);  // cnt, T (cap: per-graph admission cap, 0 = none)
 void list_autotune(Ctx &c, uint32_t k);  // sets list_waves_per_root, tune_ms
 // Per-graph clique counts of the last count pass (needs graph ids for G > 1).
 void graph_counts(Ctx &c, std::vector<ull> &gT);

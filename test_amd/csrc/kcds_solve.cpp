@@ -147,20 +147,26 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
   std::vector<uint8_t> drop(G, 0);
   {
     Timer t(c.stream);
-    list_count(c, k);
+    // per-graph HBM admission on the count pass: cliques, incidence and
+    // status take T * (8k + 4) bytes, so a graph is admitted iff T <= cap;
+    // the count pass stops counting a graph once it has passed cap
+    const ull cap = c.graph_budget ? c.graph_budget / (8ull * k + 4ull) : 0;
+    list_count(c, k, cap);
     R.T = c.T;
-    // per-graph HBM admission on the count pass: cliques, incidence, status
     graph_counts(c, gT);
     uint32_t ndrop = 0;
-    if (c.graph_budget)
+    if (cap)
       for (uint32_t g = 0; g < G; ++g)
-        if ((double)gT[g] * (8.0 * k + 4.0) > (double)c.graph_budget) {
+        if (gT[g] > cap) {
           drop[g] = 1;
           ++ndrop;
         }
-    if (ndrop && G == 1)
-      fail(KCDS_E_NOMEM, "graph has %llu %u-cliques: %.0f bytes to materialise, over the graph budget %llu",
-           (unsigned long long)gT[0], k, (double)gT[0] * (8.0 * k + 4.0), (unsigned long long)c.graph_budget);
+    if (ndrop && G == 1) {
+      c.counted = false;  // the counts stopped at the cap
+      fail(KCDS_E_NOMEM, "graph has more than %llu %u-cliques (counted to %llu): over %.0f bytes to materialise, "
+           "over the graph budget %llu", (unsigned long long)cap, k, (unsigned long long)gT[0],
+           (double)gT[0] * (8.0 * k + 4.0), (unsigned long long)c.graph_budget);
+    }
     if (ndrop) drop_graphs(c, drop, gT);
     if (c.T) list_materialize(c, k);
     R.ms_list = t.stop();

@@ -216,21 +216,30 @@ class Server:
         return reqs, stats
 
 
-def c5_requests(count=None, duration=None, time_scale=1.0, max_n=None):
+def c5_requests(count=None, duration=None, time_scale=1.0, max_n=None, threads=8):
     """The C5 request stream (Poisson 200/s, R-MAT, k in 3..6), generated in order.
 
     count / duration bound the sample (first requests of the stream);
-    time_scale stretches arrival times; max_n skips requests above that size."""
+    time_scale stretches arrival times; max_n skips requests above that size.
+    Graphs are generated on `threads` host threads (the generator is native
+    and releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
     from . import gen
-    out = []
+    picked = []
     for i, spec in enumerate(gen.request_specs()):
-        if count is not None and len(out) >= count:
+        if count is not None and len(picked) >= count:
             break
         if duration is not None and spec[0] >= duration:
             break
         if max_n is not None and spec[1] > max_n:
             continue
+        picked.append((i, spec))
+
+    def make(item):
+        i, spec = item
         g, k = gen.request_graph(spec)
-        out.append(Request(rid=i, t_arrival=spec[0] * time_scale, offset=np.asarray(g.offset, np.uint32),
-                           nbr=np.asarray(g.nbr, np.uint32), k=k))
-    return out
+        return Request(rid=i, t_arrival=spec[0] * time_scale, offset=np.asarray(g.offset, np.uint32),
+                       nbr=np.asarray(g.nbr, np.uint32), k=k)
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        return list(ex.map(make, picked))
