@@ -15,6 +15,7 @@ Also reported on the same JSON line:
                 their hipEvent-timed duration, against 8 TB/s;
   cpu_baseline  the CPU oracle's count of the same graph (all host threads);
   extras        exact k-CDS solves/s on config C1 (ER 2000 + K40, k = 3);
+                the same counting step on C3 (Chung-Lu 1M, k = 5);
                 C4 batched vs plain solves/s (4,096 graphs, k = 4); C5 served
                 vs plain solves/s and p50/p99 latency on the first requests of
                 the Poisson 200/s stream.
@@ -108,6 +109,29 @@ def bench_c5(count, graph_budget, slots):
                            if r.status == kcds.GRAPH_SOLVED)
                 out["C5_served_equals_plain"] = bool(same)
     return out
+
+
+def bench_c3(gen, kcds, reps=3):
+    """C3 (Chung-Lu n = 1M, ~10M edges, near-K80, k = 5): the same step as the
+    headline (GPU order + DAG + per-vertex counting pass).  T = 7.3e10 is
+    beyond 2^32 clique ids, so C3 is counted, never materialised."""
+    g, k = gen.config(3)
+    s = kcds.Solver(0)
+    s.upload(g.offset, g.nbr)
+    s.orient(None)
+    s.autotune_list(k)
+    s.count(k, per_vertex=False)
+    s.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.orient(None)
+        T, _ = s.count(k, per_vertex=False)
+    s.sync()
+    dt = (time.perf_counter() - t0) / reps
+    info = s.list_info()
+    s.close()
+    return {"C3_T": int(T), "C3_step_ms": dt * 1e3, "C3_cliques_per_s": T / dt, "C3_list_pass_ms": info.kernel_ms,
+            "C3_max_outdeg": info.max_outdeg, "C3_waves_per_root": list(info.waves_per_root)}
 
 
 def main():
@@ -224,6 +248,7 @@ def main():
         extras["C1_solve_ms"] = t_solve * 1e3
         extras["C1_rho"] = f"{res.rho_num}/{res.rho_den}"
         s1.close()
+        extras.update(bench_c3(gen, kcds))
         extras.update(bench_c4(np, gen, kcds, args.c4_plain_sample))
         if args.c5_requests > 0:
             extras.update(bench_c5(args.c5_requests, args.c5_graph_budget, args.c5_slots))

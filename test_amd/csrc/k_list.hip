@@ -25,6 +25,9 @@
 //      its own counter; the wave sum / 2 is credited to every prefix vertex.
 //      Per-vertex counts are exact without touching individual cliques.
 //   4. Counters flush to HBM with one 64-bit atomic per (wave, root, member).
+// The workgroup exchanges data only through LDS, so its barriers wait for LDS
+// operations alone: the flush atomics and the next root's queue ticket stay
+// in flight across them.
 // The materialising variant (one wave per root) enumerates the same cliques
 // in a fixed order and writes them row-major (k new ids per clique) at the
 // root's offset.
@@ -47,19 +50,21 @@ template <int W> struct Frame {
   static constexpr uint32_t HB = W == 1 ? 8 : W == 2 ? 9 : W == 4 ? 10 : 11;
   static constexpr uint32_t H = 1u << HB;  // table slots (load <= 1/4)
   uint32_t L[D];     // N+(r), ascending
-  uint32_t soff[D];  // doff[L[i]]
-  uint32_t slen[D];  // outdeg(L[i]); 0 for the last (it has no later candidate)
-  uint32_t hkey[H];  // L[i], EMPTY_KEY when free
-  uint16_t hval[H];  // i
+  uint2 sinfo[D];    // (doff[L[i]], outdeg(L[i])); length 0 for the last (it has no later candidate)
+  ull hent[H];       // table entry: low word L[i] (EMPTY_KEY when free), high word i
   ull A[W * D];      // A[w*D + j]: bits 64w..64w+63 of row j
   ull rt;            // cliques of the current root (summed over waves)
   uint32_t root;
   uint32_t skip;     // the root's graph is already over its admission cap
 };
 
-constexpr uint32_t UNROLL = 4;  // 64-element chunks of one out-list in flight per wave
+constexpr uint32_t UNROLL = 8;  // 64-element chunks in flight per wave (from one or more out-lists)
 
-__device__ __forceinline__ uint32_t hslot(uint32_t key, uint32_t hb) { return (key * 0x9E3779B1u) >> (32 - hb); }
+// Fibonacci hashing on a 24-bit multiply (full rate; a 32-bit multiply is
+// quarter rate): the top bits of the product's low word mix every key bit.
+__device__ __forceinline__ uint32_t hslot(uint32_t key, uint32_t hb) {
+  return (uint32_t)__umul24(key ^ (key >> 24), 0x9E3779u) >> (32 - hb);
+}
 
 __device__ __forceinline__ ull above_mask(int q, uint32_t c) {
   const int cq = (int)(c >> 6), cb = (int)(c & 63);
@@ -76,61 +81,98 @@ template <int W> __device__ __forceinline__ ull row(const Frame<W> &f, uint32_t 
 // Every thread of the workgroup calls it.  Returns this lane's share of
 // sum over u in N+(r) of outdeg(u).
 template <int W>
-__device__ uint32_t build_rows(Frame<W> &f, const uint32_t *__restrict__ doff, const uint32_t *__restrict__ dnbr,
-                               uint32_t base, uint32_t d) {
+__device__ __forceinline__ uint32_t build_rows(Frame<W> &f, const uint32_t *__restrict__ doff,
+                                               const uint32_t *__restrict__ dnbr, uint32_t base, uint32_t d) {
   constexpr uint32_t D = Frame<W>::D, H = Frame<W>::H, HB = Frame<W>::HB;
-  const uint32_t tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
-  for (uint32_t t = tid; t < H; t += nt) f.hkey[t] = EMPTY_KEY;
+  const uint32_t tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, nw = nt >> 6;
+  const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar loops
+  for (uint32_t t = tid; t < H; t += nt) f.hent[t] = ~0ull;
   for (uint32_t j = tid; j < d; j += nt) {
 #pragma unroll
     for (int w = 0; w < W; ++w) f.A[(uint32_t)w * D + j] = 0ull;
   }
-  block_sync();
+  lds_sync();
   // stage N+(r), the out-list extents and the id -> index table
+  uint32_t *hk = reinterpret_cast<uint32_t *>(f.hent);
   uint32_t raw = 0;
   for (uint32_t t = tid; t < d; t += nt) {
     const uint32_t u = dnbr[base + t];
     const uint32_t ou = doff[u], du = doff[u + 1] - ou;
     f.L[t] = u;
-    f.soff[t] = ou;
-    f.slen[t] = t + 1 < d ? du : 0u;
+    f.sinfo[t] = make_uint2(ou, t + 1 < d ? du : 0u);
     raw += du;
     uint32_t h = hslot(u, HB);
-    while (atomicCAS(&f.hkey[h], EMPTY_KEY, u) != EMPTY_KEY) h = (h + 1) & (H - 1);
-    f.hval[h] = (uint16_t)t;
+    while (atomicCAS(&hk[2 * h], EMPTY_KEY, u) != EMPTY_KEY) h = (h + 1) & (H - 1);
+    hk[2 * h + 1] = t;
   }
-  block_sync();
-  // row i collects N+(L[i]) inside N+(r): the waves take out-lists in turn,
-  // 64 * UNROLL elements per step with every load issued before the lookups
+  lds_sync();
+  // row i collects N+(L[i]) inside N+(r).  The waves take out-lists in turn
+  // (i = wv, wv + nw, ...) and walk them in 64-element chunks; a step takes
+  // the next UNROLL chunks, across list boundaries, and issues every load,
+  // then every first probe, before any is compared (collisions, rare at load
+  // <= 1/4, probe on in a loop afterwards).  Out-lists average a few chunks,
+  // so one step overlaps the loads of several lists.
   const uint32_t last = f.L[d - 1];
-  for (uint32_t i = wv; i + 1 < d; i += nw) {
-    const uint32_t o = f.soff[i], len = f.slen[i];
-    for (uint32_t t0 = 0; t0 < len; t0 += 64 * UNROLL) {
-      uint32_t w[UNROLL];
+  // cursor: list ci (its (start, length) in cur, the next list's already
+  // requested in nxt) and the offset ct in it
+  // (cur is wave-uniform: scalar registers and branches; nxt is made uniform
+  // only when it is used, so its LDS read is not waited for at once)
+  auto info = [&](uint32_t i) -> uint2 { return i + 1 < d ? f.sinfo[i] : make_uint2(0u, 0u); };
+  auto uniform = [](uint2 v) {
+    return make_uint2(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y));
+  };
+  uint32_t ci = wv, ct = 0;
+  uint2 cur = uniform(info(ci)), nxt = info(ci + nw);
+  auto advance = [&]() {  // to the next non-empty list of this wave
+    do {
+      ci += nw;
+      cur = uniform(nxt);
+      nxt = info(ci + nw);
+    } while (ci + 1 < d && cur.y == 0);
+  };
+  if (ci + 1 < d && cur.y == 0) advance();
+  while (ci + 1 < d) {
+    uint32_t w[UNROLL], h[UNROLL], li[UNROLL];
+    ull e[UNROLL];
 #pragma unroll
-      for (uint32_t x = 0; x < UNROLL; ++x) {
-        const uint32_t t = t0 + x * 64 + lane;
-        w[x] = t < len ? dnbr[o + t] : 0xFFFFFFFFu;
-      }
-#pragma unroll
-      for (uint32_t x = 0; x < UNROLL; ++x) {
-        if (w[x] > last) continue;  // also the padding
-        uint32_t h = hslot(w[x], HB);
-        while (true) {
-          const uint32_t key = f.hkey[h];
-          if (key == w[x]) {
-            const uint32_t j = f.hval[h];  // j > i: ids ascend along arcs and L is ascending
-            atomicOr(&f.A[(j >> 6) * D + i], 1ull << (j & 63));
-            atomicOr(&f.A[(i >> 6) * D + j], 1ull << (i & 63));
-            break;
-          }
-          if (key == EMPTY_KEY) break;
-          h = (h + 1) & (H - 1);
+    for (uint32_t x = 0; x < UNROLL; ++x) {
+      li[x] = ci;
+      w[x] = 0xFFFFFFFFu;
+      if (ci + 1 < d) {
+        const uint32_t t = ct + lane;
+        if (t < cur.y) w[x] = dnbr[cur.x + t];
+        ct += 64;
+        if (ct >= cur.y) {
+          ct = 0;
+          advance();
         }
       }
     }
+#pragma unroll
+    for (uint32_t x = 0; x < UNROLL; ++x) {
+      h[x] = hslot(w[x], HB);
+      e[x] = f.hent[h[x]];
+    }
+#pragma unroll
+    for (uint32_t x = 0; x < UNROLL; ++x) {
+      const bool valid = w[x] <= last;  // also excludes the padding
+      uint32_t key = (uint32_t)e[x];
+      bool more = valid && key != w[x] && key != EMPTY_KEY;
+      while (more) {
+        h[x] = (h[x] + 1) & (H - 1);
+        e[x] = f.hent[h[x]];
+        key = (uint32_t)e[x];
+        more = key != w[x] && key != EMPTY_KEY;
+      }
+      if (valid && key == w[x]) {
+        const uint32_t i = li[x];
+        const uint32_t j = (uint32_t)(e[x] >> 32);  // j > i: ids ascend along arcs and L is ascending
+        atomicOr(&f.A[(j >> 6) * D + i], 1ull << (j & 63));
+        atomicOr(&f.A[(i >> 6) * D + j], 1ull << (i & 63));
+      }
+    }
   }
-  block_sync();
+  lds_sync();
   return raw;
 }
 
@@ -278,9 +320,9 @@ template <int W> __device__ __forceinline__ void all_bits(ull *S, uint32_t d) {
 // Hands the root index held by thread 0 to the workgroup.
 template <int W> __device__ __forceinline__ uint32_t share_root(Frame<W> &f, uint32_t idx0) {
   if (threadIdx.x == 0) f.root = idx0;
-  block_sync();
+  lds_sync();
   const uint32_t idx = f.root;
-  block_sync();  // every thread has read f.root before it is reused
+  lds_sync();  // every thread has read f.root before it is reused
   return idx;
 }
 
@@ -305,7 +347,8 @@ __global__ __launch_bounds__(64 * MAX_WAVES) void k_list_count(const uint32_t *_
                                                                uint32_t ncopy, ull *rootT, ull *gtotal,
                                                                const uint32_t *gid, ull *gsum, ull cap, ull theta) {
   __shared__ Frame<W> f;
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, nw = blockDim.x >> 6;
+  const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar loops
   ull tot = 0, w2 = 0;
   if (tid == 0) f.rt = 0;
   // per-vertex counts go to one of ncopy copies (summed afterwards), so a
@@ -326,9 +369,9 @@ __global__ __launch_bounds__(64 * MAX_WAVES) void k_list_count(const uint32_t *_
       if (cap && pre < nroots) sk = ldc_u64(&gsum[gid ? gid[roots[pre]] : 0u]) > cap;
       f.skip = sk;
     }
-    block_sync();
+    lds_sync();
     const uint32_t idx = f.root, skip = f.skip;
-    block_sync();  // every thread has read the frame header before it is reused
+    lds_sync();  // every thread has read the frame header before it is reused
     if (idx >= nroots) break;
     if (tid == 0) pre = queue_next(queue, nroots, qc);  // in flight while this root is processed
     if (skip) continue;
@@ -374,7 +417,7 @@ __global__ __launch_bounds__(64 * MAX_WAVES) void k_list_count(const uint32_t *_
       if (j < d && cnt[q]) atomicAdd(&gcnt[f.L[j]], cnt[q]);
     }
     if (lane == 0 && rt) atomicAdd(&f.rt, rt);
-    block_sync();
+    lds_sync();
     if (tid == 0) {
       const ull all = K == 3 ? (f.rt >> 1) : f.rt;
       f.rt = 0;

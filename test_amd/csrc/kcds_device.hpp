@@ -28,6 +28,15 @@ __device__ __forceinline__ void block_sync() {
   __syncthreads();
 }
 
+// Workgroup barrier for data exchanged through LDS only: it waits for the
+// wave's LDS operations but leaves its global loads, stores and atomics in
+// flight (a plain __syncthreads also drains those).
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ uint32_t wave_bcast0(uint32_t v) {
   return __builtin_amdgcn_readfirstlane(v);
 }

@@ -99,6 +99,33 @@ def test_config2_full(solver):
     assert np.all(pv[g.planted[:60]] >= math.comb(59, 3))
 
 
+@pytest.mark.slow
+def test_config3_full_counts_match_oracle_golden(solver):
+    """C3 at full size (n = 1,000,000, ~10M edges, k = 5): T = 7.3e10 5-cliques
+    (beyond 2^32, so C3 is counted, never materialised).  T and checksums of
+    the per-vertex counts equal the oracle's, kept in golden/c3_counts.json."""
+    import json
+    with open(os.path.join(GOLD, "c3_counts.json")) as fh:
+        ref = json.load(fh)
+    g, k = gen.config(3)
+    assert (g.n, g.m, k) == (ref["n"], ref["m"], ref["k"])
+    solver.upload(g.offset, g.nbr)
+    solver.orient(None)
+    T, pv = solver.count(k)
+    print(f"  C3: T = {T}, list pass {solver.list_info().kernel_ms:.1f} ms", flush=True)
+    assert T == ref["T"]
+    pv = np.asarray(pv, dtype=np.int64)
+    assert int(pv.sum()) == ref["sum"] == k * ref["T"]
+    u = pv.astype(np.uint64)
+    idx = np.arange(1, len(pv) + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        assert int(np.sum(u * idx, dtype=np.uint64)) == ref["check1"]
+        assert int(np.sum(u * (idx * idx + np.uint64(7)), dtype=np.uint64)) == ref["check2"]
+    assert int(pv.max()) == ref["max"]
+    for v, c in ref["sample"].items():
+        assert int(pv[int(v)]) == c
+
+
 def test_materialize_matches_oracle(solver):
     g = gen.er(300, 0.05, 15, seed=3)
     for k in (3, 4, 5):
