@@ -67,6 +67,13 @@ __global__ void k_uf_hook(uint64_t T, uint32_t k, const uint32_t *cl, const int3
        c += (uint64_t)gridDim.x * blockDim.x) {
     if (!core_clique(c, k, cl, status, gp, gid)) continue;
     const uint32_t a = cl[c * k];
+    // members with one common parent are in one set already: most cliques,
+    // once the big components have formed, and this test reads no root (a
+    // root's entry is read by every find and would be one hot L2 line)
+    const uint32_t pa = ld(&par[a]);
+    bool same = true;
+    for (uint32_t t = 1; t < k; ++t) same = same && ld(&par[cl[c * k + t]]) == pa;
+    if (same) continue;
     for (uint32_t t = 1; t < k; ++t) uf_unite(par, key, a, cl[c * k + t]);
   }
 }
@@ -86,15 +93,16 @@ __global__ void k_uf_label(uint32_t n, const int32_t *core, const GraphPeel *gp,
 // dense component index, vertex counts and local vertex ids
 __global__ void k_group_vertices(uint32_t n, const uint32_t *comp, const uint32_t *dense_of_orig,
                                  const uint32_t *rank, uint32_t *cid, uint32_t *vcount, uint32_t *local) {
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-    const uint32_t lab = comp[r];
-    if (lab == 0xFFFFFFFFu) {
-      cid[r] = 0xFFFFFFFFu;
-      continue;
-    }
-    const uint32_t d = dense_of_orig[lab];
-    cid[r] = d;
-    local[r] = atomicAdd(&vcount[d], 1u);
+  // one atomic per group of lanes in the same component (one giant
+  // component is the common case)
+  for (uint32_t r0 = blockIdx.x * blockDim.x; r0 < n; r0 += gridDim.x * blockDim.x) {
+    const uint32_t r = r0 + threadIdx.x;
+    const uint32_t lab = r < n ? comp[r] : 0xFFFFFFFFu;
+    const bool in = lab != 0xFFFFFFFFu;
+    const uint32_t d = in ? dense_of_orig[lab] : 0u;
+    if (r < n) cid[r] = in ? d : 0xFFFFFFFFu;
+    const uint32_t slot = wave_slot_by_key<uint32_t>(in, d, vcount);
+    if (in) local[r] = slot;
   }
 }
 

@@ -16,7 +16,8 @@
 //             residual arcs to the lowest neighbour height, or relabel; a
 //             sweep is a no-op when the previous one changed nothing.  Clique
 //             nodes (k arcs) take a thread each; vertex nodes (up to millions
-//             of arcs) are cut into chunks of VCH arcs, a wave per chunk, in
+//             of arcs) are cut into chunks of VCH arcs, a wave per chunk
+//             (four 64-arc sub-chunks' loads in flight together), in
 //             three kernels (lowest height, residual to it per chunk, push in
 //             arc order from each chunk's share of the excess).
 // The host runs the phases (one readback per phase) and the Dinkelbach
@@ -40,7 +41,8 @@ constexpr uint32_t NSEG = 8;
 constexpr uint32_t SWEEPS = 24;
 constexpr uint32_t MAX_PHASES = 1u << 14;
 constexpr uint32_t NO_ARC = 0xFFFFFFFFu;
-constexpr uint32_t VCH = 4096;  // arcs of a vertex node per work item
+constexpr uint32_t VCH = 1024;  // arcs of a vertex node per work item (a wave, 4 x 64 arcs per step)
+constexpr uint32_t VU = 4;
 
 // Counters 64 words apart (same-address device atomics are serialised).
 struct GFCtl {
@@ -60,11 +62,24 @@ __global__ void k_gf_fill_u32(uint32_t *p, uint32_t v, uint64_t len) {
     p[i] = v;
 }
 
-// cnt[v] = 2 + number of cliques containing v (v->t, v->s, v->c arcs)
-__global__ void k_gf_deg(const uint32_t *ccl, uint64_t Tk, uint32_t *cnt) {
-  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < Tk; i0 += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t i = i0 + threadIdx.x;
-    wave_slot_by_key<uint32_t>(i < Tk, i < Tk ? ccl[i] : 0u, cnt);  // hub vertices repeat across lanes
+// cnt[v] = 2 + number of cliques containing v (v->t, v->s, v->c arcs).
+// Rounds of GF_E * 256 consecutive clique entries per workgroup, one atomic
+// per distinct vertex per round (hub vertices repeat across the round).
+constexpr int GF_E = 8, GF_TB = 4096;
+
+__global__ __launch_bounds__(256) void k_gf_deg(const uint32_t *ccl, uint64_t Tk, uint32_t *cnt) {
+  __shared__ KeyTable<GF_TB> tb;
+  const uint64_t per = (uint64_t)GF_E * 256;
+  for (uint64_t r0 = (uint64_t)blockIdx.x * per; r0 < Tk; r0 += (uint64_t)gridDim.x * per) {
+    bool want[GF_E];
+    uint32_t v[GF_E], slot[GF_E];
+#pragma unroll
+    for (int e = 0; e < GF_E; ++e) {
+      const uint64_t i = r0 + (uint64_t)e * 256 + threadIdx.x;
+      want[e] = i < Tk;
+      v[e] = want[e] ? ccl[i] : 0u;
+    }
+    block_slot_by_key<GF_E, GF_TB, uint32_t>(tb, want, v, slot, cnt);
   }
 }
 
@@ -95,21 +110,35 @@ __global__ void k_gf_topo_fixed(uint32_t nc, uint64_t Tc, uint32_t k, uint32_t *
   }
 }
 
-__global__ void k_gf_topo_cliques(const uint32_t *ccl, uint64_t Tc, uint32_t k, uint32_t nc, const uint32_t *off,
-                                  uint32_t *head, uint32_t *rev, uint32_t *fill) {
+// Clique arcs: vertex v's arc to clique c takes the next slot of v's list
+// (slots reserved per workgroup round as in k_gf_deg); the clique's k arcs
+// sit at CA + c*k.
+__global__ __launch_bounds__(256) void k_gf_topo_cliques(const uint32_t *ccl, uint64_t Tc, uint32_t k, uint32_t nc,
+                                                         const uint32_t *off, uint32_t *head, uint32_t *rev,
+                                                         uint32_t *fill) {
+  __shared__ KeyTable<GF_TB> tb;
   const uint32_t CA = off[nc];
-  const uint64_t Tk = Tc * k;
-  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < Tk; i0 += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t i = i0 + threadIdx.x;
-    const bool valid = i < Tk;
-    const uint32_t v = valid ? ccl[i] : 0u;
-    const uint32_t slot = wave_slot_by_key<uint32_t>(valid, v, fill);
-    if (!valid) continue;
-    const uint32_t a = off[v] + 2 + slot, b = CA + (uint32_t)i;
-    head[a] = nc + (uint32_t)(i / k);
-    rev[a] = b;
-    head[b] = v;
-    rev[b] = a;
+  const uint64_t Tk = Tc * k, per = (uint64_t)GF_E * 256;
+  for (uint64_t r0 = (uint64_t)blockIdx.x * per; r0 < Tk; r0 += (uint64_t)gridDim.x * per) {
+    bool want[GF_E];
+    uint32_t v[GF_E], slot[GF_E];
+#pragma unroll
+    for (int e = 0; e < GF_E; ++e) {
+      const uint64_t i = r0 + (uint64_t)e * 256 + threadIdx.x;
+      want[e] = i < Tk;
+      v[e] = want[e] ? ccl[i] : 0u;
+    }
+    block_slot_by_key<GF_E, GF_TB, uint32_t>(tb, want, v, slot, fill);
+#pragma unroll
+    for (int e = 0; e < GF_E; ++e) {
+      if (!want[e]) continue;
+      const uint64_t i = r0 + (uint64_t)e * 256 + threadIdx.x;
+      const uint32_t a = off[v[e]] + 2 + slot[e], b = CA + (uint32_t)i;
+      head[a] = nc + (uint32_t)(i / k);
+      rev[a] = b;
+      head[b] = v[e];
+      rev[b] = a;
+    }
   }
 }
 
@@ -135,26 +164,47 @@ __global__ void k_gf_init(uint32_t nc, uint64_t Tc, uint32_t k, i64 p, i64 q, co
   for (uint64_t x = nc + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < N; x += stride) excess[x] = 0;
 }
 
-// heights = N except t = 0; frontier {t}; phase counters cleared
-__global__ void k_gf_bfs_init(uint32_t N, uint32_t *height, uint32_t *fr0, GFCtl *ctl) {
+// heights = N except t = 0; phase counters cleared; the frontier starts at
+// level 1 (k_gf_bfs_seed), so it never holds t, whose nc arcs would be one
+// node's work.
+__global__ void k_gf_bfs_init(uint32_t N, uint32_t *height, GFCtl *ctl) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < N; x += stride)
     height[x] = x == N - 1 ? 0u : N;
   if (blockIdx.x == 0) {
     const uint32_t t = threadIdx.x;
     if (t < NSEG) {
-      ctl->fsize[0][t][0] = t == 0 ? 1u : 0u;
+      ctl->fsize[0][t][0] = 0;
       ctl->fsize[1][t][0] = 0;
       ctl->gblocks[t][0] = 0;
       ctl->active[t][0] = 0;
     }
     for (uint32_t i = t; i <= SWEEPS; i += blockDim.x) ctl->changed[i][0] = i == 0 ? 1u : 0u;
     if (t == 0) {
-      fr0[0] = N - 1;
       ctl->blocks[0] = 0;
-      ctl->dist = 0;
+      ctl->dist = 1;
       ctl->cur = 0;
       ctl->done = 0;
+    }
+  }
+}
+
+// Level 1 of the BFS from t: the vertices with residual capacity to t (arc
+// off[v] is v -> t) get height 1 and form the frontier in fr0.
+__global__ __launch_bounds__(256) void k_gf_bfs_seed(uint32_t nc, const uint32_t *off, const i64 *res,
+                                                     uint32_t *height, uint32_t *fr0, uint32_t seg_cap, GFCtl *ctl) {
+  const uint32_t lane = lane_id(), seg = blockIdx.x % NSEG;
+  for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < nc; v0 += gridDim.x * blockDim.x) {
+    const uint32_t v = v0 + threadIdx.x;
+    const bool add = v < nc && res[off[v]] > 0;
+    if (add) height[v] = 1;
+    const ull bal = __ballot(add);
+    if (bal) {
+      const uint32_t lead = (uint32_t)__builtin_ctzll(bal);
+      uint32_t base = 0;
+      if (lane == lead) base = atomicAdd(&ctl->fsize[0][seg][0], (uint32_t)__popcll(bal));
+      base = __shfl(base, lead, 64);
+      if (add) fr0[(size_t)seg * seg_cap + base + lanes_below(bal)] = v;
     }
   }
 }
@@ -173,13 +223,16 @@ __device__ bool last_block(GFCtl *ctl, uint32_t *s_last) {
   return *s_last != 0;
 }
 
-// One BFS level: wave per frontier node y; x = head of an arc of y with
-// residual capacity on x -> y gets height dist + 1.  seg_cap: capacity of a
-// frontier segment.
-__global__ __launch_bounds__(256) void k_gf_bfs_level(uint32_t N, uint32_t nv_nodes, const uint32_t *off,
-                                                      const uint32_t *head,
-                                                      const uint32_t *rev, const i64 *res, uint32_t *height,
-                                                      uint32_t *fr0, uint32_t *fr1, uint32_t seg_cap, GFCtl *ctl) {
+// One BFS level, clique nodes of the frontier: a thread per node y (k arcs);
+// x = head of an arc of y with residual capacity on x -> y gets height
+// dist + 1.  Vertex nodes of the frontier are handled by k_gf_bfs_vchunks
+// (they hold up to millions of arcs); t is never in the frontier and s is
+// never added.  seg_cap: capacity of a frontier segment.  The last
+// workgroup swaps the frontiers and advances the level.
+__global__ __launch_bounds__(256) void k_gf_bfs_level(uint32_t N, uint32_t nv_nodes, uint32_t k, const uint32_t *off,
+                                                      const uint32_t *head, const uint32_t *rev, const i64 *res,
+                                                      uint32_t *height, uint32_t *fr0, uint32_t *fr1,
+                                                      uint32_t seg_cap, GFCtl *ctl) {
   __shared__ uint32_t s_hdr[3], s_pre[NSEG + 1], s_last;
   const uint32_t tid = threadIdx.x, lane = lane_id(), seg = blockIdx.x % NSEG;
   if (tid < 3) s_hdr[tid] = (&ctl->dist)[tid];
@@ -196,20 +249,24 @@ __global__ __launch_bounds__(256) void k_gf_bfs_level(uint32_t N, uint32_t nv_no
   const uint32_t *fr = cur ? fr1 : fr0;
   uint32_t *nx = cur ? fr0 : fr1;
   const uint32_t s = N - 2;
-  const uint32_t wid = (blockIdx.x * blockDim.x + tid) / KCDS_WAVE, nw = gridDim.x * blockDim.x / KCDS_WAVE;
-  for (uint32_t i = wid; i < fs; i += nw) {
-    uint32_t g = 0;
-    while (i >= s_pre[g + 1]) ++g;
-    const uint32_t y = fr[(size_t)g * seg_cap + (i - s_pre[g])];
-    if (y < nv_nodes) continue;  // vertex nodes: k_gf_bfs_vchunks
-    const uint32_t ye = off[y + 1];
-    for (uint32_t a0 = off[y]; a0 < ye; a0 += KCDS_WAVE) {
-      const uint32_t a = a0 + lane;
+  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < fs; i0 += gridDim.x * blockDim.x) {
+    const uint32_t i = i0 + tid;
+    uint32_t y = 0;
+    bool clique = false;
+    if (i < fs) {
+      uint32_t g = 0;
+      while (i >= s_pre[g + 1]) ++g;
+      y = fr[(size_t)g * seg_cap + (i - s_pre[g])];
+      clique = y >= nv_nodes && y < s;
+    }
+    const uint32_t a0 = clique ? off[y] : 0u;
+    for (uint32_t t = 0; t < k; ++t) {
       bool add = false;
       uint32_t x = 0;
-      if (a < ye) {
+      if (clique) {
+        const uint32_t a = a0 + t;
         x = head[a];
-        if (x != s && res[rev[a]] > 0 && height[x] == N) add = atomicCAS(&height[x], N, dist + 1) == N;
+        if (res[rev[a]] > 0 && height[x] == N) add = atomicCAS(&height[x], N, dist + 1) == N;
       }
       const ull bal = __ballot(add);
       if (bal) {
@@ -279,21 +336,28 @@ __global__ __launch_bounds__(256) void k_gf_bfs_vchunks(uint32_t N, uint32_t nch
     const uint32_t u = chv[ch];
     if (height[u] != dist) continue;
     const uint32_t a0 = off[u] + (ch - choff[u]) * VCH, a1 = min(a0 + VCH, off[u + 1]);
-    for (uint32_t b = a0; b < a1; b += KCDS_WAVE) {
-      const uint32_t a = b + lane;
-      bool add = false;
-      uint32_t x = 0;
-      if (a < a1) {
-        x = head[a];
-        if (x != s && res[rev[a]] > 0 && height[x] == N) add = atomicCAS(&height[x], N, dist + 1) == N;
+    for (uint32_t b = a0; b < a1; b += VU * KCDS_WAVE) {
+      uint32_t x[VU], ra[VU];
+      i64 rr[VU];
+#pragma unroll
+      for (uint32_t q = 0; q < VU; ++q) {  // loads of the 4 sub-chunks issued together
+        const uint32_t a = b + q * KCDS_WAVE + lane;
+        x[q] = a < a1 ? head[a] : s;
+        ra[q] = a < a1 ? rev[a] : 0u;
       }
-      const ull bal = __ballot(add);
-      if (bal) {
-        const uint32_t lead = (uint32_t)__builtin_ctzll(bal);
-        uint32_t base = 0;
-        if (lane == lead) base = atomicAdd(&ctl->fsize[cur ^ 1][seg][0], (uint32_t)__popcll(bal));
-        base = __shfl(base, lead, 64);
-        if (add) nx[(size_t)seg * seg_cap + base + lanes_below(bal)] = x;
+#pragma unroll
+      for (uint32_t q = 0; q < VU; ++q) rr[q] = x[q] != s ? res[ra[q]] : 0;
+#pragma unroll
+      for (uint32_t q = 0; q < VU; ++q) {
+        const bool add = rr[q] > 0 && height[x[q]] == N && atomicCAS(&height[x[q]], N, dist + 1) == N;
+        const ull bal = __ballot(add);
+        if (bal) {
+          const uint32_t lead = (uint32_t)__builtin_ctzll(bal);
+          uint32_t base = 0;
+          if (lane == lead) base = atomicAdd(&ctl->fsize[cur ^ 1][seg][0], (uint32_t)__popcll(bal));
+          base = __shfl(base, lead, 64);
+          if (add) nx[(size_t)seg * seg_cap + base + lanes_below(bal)] = x[q];
+        }
       }
     }
   }
@@ -315,8 +379,20 @@ __global__ __launch_bounds__(256) void k_gf_vmin(uint32_t N, uint32_t nch, uint3
     if (excess[u] <= 0 || height[u] >= N) continue;
     const uint32_t a0 = off[u] + (ch - choff[u]) * VCH, a1 = min(a0 + VCH, off[u + 1]);
     uint32_t m = NO_ARC;
-    for (uint32_t a = a0 + lane; a < a1; a += KCDS_WAVE)
-      if (res[a] > 0) m = min(m, height[head[a]]);
+    for (uint32_t b = a0; b < a1; b += VU * KCDS_WAVE) {
+      i64 r[VU];
+      uint32_t x[VU];
+#pragma unroll
+      for (uint32_t q = 0; q < VU; ++q) {
+        const uint32_t a = b + q * KCDS_WAVE + lane;
+        r[q] = a < a1 ? res[a] : 0;
+      }
+#pragma unroll
+      for (uint32_t q = 0; q < VU; ++q) x[q] = r[q] > 0 ? head[b + q * KCDS_WAVE + lane] : 0u;
+#pragma unroll
+      for (uint32_t q = 0; q < VU; ++q)
+        if (r[q] > 0) m = min(m, height[x[q]]);
+    }
     m = wave_min_u32(m);
     if (lane == 0 && m != NO_ARC) atomicMin(&vmin[u], m);
   }
@@ -338,9 +414,19 @@ __global__ __launch_bounds__(256) void k_gf_vsum(uint32_t N, uint32_t nch, uint3
     i64 sum = 0;
     if (e > 0 && height[u] < N && hm != NO_ARC && height[u] > hm) {
       const uint32_t a0 = off[u] + (ch - choff[u]) * VCH, a1 = min(a0 + VCH, off[u + 1]);
-      for (uint32_t a = a0 + lane; a < a1; a += KCDS_WAVE) {
-        const i64 r = res[a];
-        if (r > 0 && height[head[a]] == hm) sum += r;
+      for (uint32_t b = a0; b < a1; b += VU * KCDS_WAVE) {
+        i64 r[VU];
+        uint32_t x[VU];
+#pragma unroll
+        for (uint32_t q = 0; q < VU; ++q) {
+          const uint32_t a = b + q * KCDS_WAVE + lane;
+          r[q] = a < a1 ? res[a] : 0;
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < VU; ++q) x[q] = r[q] > 0 ? head[b + q * KCDS_WAVE + lane] : 0u;
+#pragma unroll
+        for (uint32_t q = 0; q < VU; ++q)
+          if (r[q] > 0 && height[x[q]] == hm) sum += r[q];
       }
       sum = (i64)wave_sum_u64((ull)sum);
     }
@@ -377,37 +463,42 @@ __global__ __launch_bounds__(256) void k_gf_vpush(uint32_t N, uint32_t nch, uint
       }
       continue;
     }
-    i64 before = 0;
-    for (uint32_t q = c0; q < ch; ++q) before += csum[q];
+    i64 before = 0;  // the earlier chunks' residual to hm, summed across the lanes
+    for (uint32_t q = c0 + lane; q < ch; q += KCDS_WAVE) before += csum[q];
+    before = (i64)wave_sum_u64((ull)before);
     i64 left = esnap[u] - before;
     if (left <= 0 || csum[ch] == 0) continue;
     const uint32_t a0 = off[u] + (ch - c0) * VCH, a1 = min(a0 + VCH, off[u + 1]);
     i64 pushed = 0;
-    for (uint32_t b = a0; b < a1 && left > 0; b += KCDS_WAVE) {
-      const uint32_t a = b + lane;
-      i64 r = 0;
-      uint32_t x = 0;
-      if (a < a1) {
-        r = res[a];
-        if (r > 0) {
-          x = head[a];
-          if (height[x] != hm) r = 0;
-        } else {
-          r = 0;
+    for (uint32_t b = a0; b < a1 && left > 0; b += VU * KCDS_WAVE) {
+      i64 r[VU];
+      uint32_t x[VU];
+#pragma unroll
+      for (uint32_t q = 0; q < VU; ++q) {
+        const uint32_t a = b + q * KCDS_WAVE + lane;
+        r[q] = a < a1 ? res[a] : 0;
+      }
+#pragma unroll
+      for (uint32_t q = 0; q < VU; ++q) x[q] = r[q] > 0 ? head[b + q * KCDS_WAVE + lane] : 0u;
+#pragma unroll
+      for (uint32_t q = 0; q < VU; ++q)
+        if (r[q] > 0 && height[x[q]] != hm) r[q] = 0;
+#pragma unroll
+      for (uint32_t q = 0; q < VU; ++q) {  // in arc order: sub-chunk by sub-chunk, lanes by prefix
+        const uint32_t a = b + q * KCDS_WAVE + lane;
+        const ull incl = wave_incl_scan_u64((ull)r[q]);
+        const i64 pre = (i64)incl - r[q];
+        const i64 d = r[q] > 0 && left > pre ? min(r[q], left - pre) : 0;
+        if (d > 0) {
+          add64(&res[a], -d);
+          add64(&res[rev[a]], d);
+          add64(&excess[x[q]], d);
         }
+        const i64 tot = (i64)__shfl(incl, 63, 64);
+        const i64 used = min(tot, left);
+        left -= used;
+        pushed += used;
       }
-      const ull incl = wave_incl_scan_u64((ull)r);
-      const i64 pre = (i64)incl - r;
-      const i64 d = r > 0 && left > pre ? min(r, left - pre) : 0;
-      if (d > 0) {
-        add64(&res[a], -d);
-        add64(&res[rev[a]], d);
-        add64(&excess[x], d);
-      }
-      const i64 tot = (i64)__shfl(incl, 63, 64);
-      const i64 used = min(tot, left);
-      left -= used;
-      pushed += used;
     }
     if (pushed > 0) {
       if (lane == 0) add64(&excess[u], -pushed);
@@ -493,6 +584,12 @@ __global__ __launch_bounds__(256) void k_gf_side(uint32_t nc, uint64_t Tc, uint3
   }
 }
 
+// workgroups for the keyed-round kernels (64 KB of LDS each: two per CU)
+uint32_t kgrid(const Ctx &c, uint64_t entries) {
+  return (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>((entries + GF_E * 256 - 1) / (GF_E * 256), (uint64_t)c.num_cus * 2));
+}
+
 uint32_t ggrid(const Ctx &c, uint64_t len) {
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((len + 255) / 256, (uint64_t)c.num_cus * 8));
 }
@@ -542,12 +639,12 @@ void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t m
   // ---- topology: vertex arc counts, scan, fixed arcs, clique arcs ----
   uint32_t *cnt = height;  // scratch before the first relabel
   k_gf_fill_u32<<<ggrid(c, nc), 256, 0, c.stream>>>(cnt, 2u, nc);
-  k_gf_deg<<<ggrid(c, Tc * k), 256, 0, c.stream>>>(ccl, Tc * k, cnt);
+  k_gf_deg<<<kgrid(c, Tc * k), 256, 0, c.stream>>>(ccl, Tc * k, cnt);
   KCDS_LAUNCH_CHECK();
   scan_u32_to_u32(c, cnt, off, nc);  // off[0..nc]
   uint32_t *fill = fr0;
   k_gf_topo_fixed<<<ggrid(c, std::max<uint64_t>(nc, Tc + 1)), 256, 0, c.stream>>>(nc, Tc, k, off, head, rev, fill);
-  k_gf_topo_cliques<<<ggrid(c, Tc * k), 256, 0, c.stream>>>(ccl, Tc, k, nc, off, head, rev, fill);
+  k_gf_topo_cliques<<<kgrid(c, Tc * k), 256, 0, c.stream>>>(ccl, Tc, k, nc, off, head, rev, fill);
   KCDS_LAUNCH_CHECK();
   // ---- vertex arc chunks ----
   buf_reserve(c, f.g_nch, sizeof(uint32_t) * (nc + 1));
@@ -568,7 +665,8 @@ void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t m
   k_gf_chv<<<ggrid(c, nc), 256, 0, c.stream>>>(nc, choff, chv);
   KCDS_LAUNCH_CHECK();
   GFCtl *h = (GFCtl *)c.host_scratch(sizeof(GFCtl));
-  const uint32_t bgrid = (uint32_t)c.num_cus * 4;  // BFS levels and sweeps: 256-thread blocks
+  const uint32_t bgrid = (uint32_t)c.num_cus * 4;  // BFS levels (last-workgroup control): 256-thread blocks
+  const uint32_t vgrid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((nch + 3) / 4, (uint64_t)c.num_cus * 8));
   ull p = task.p0, q = task.q0;
   uint32_t flows = 0;
   i64 flow = 0;
@@ -580,15 +678,16 @@ void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t m
     uint32_t phase = 0;
     for (;; ++phase) {
       if (phase >= MAX_PHASES) fail(KCDS_E_LIMIT, "grid max-flow of component %u exceeded %u phases", d, MAX_PHASES);
-      k_gf_bfs_init<<<ggrid(c, Nn), 256, 0, c.stream>>>(N, height, fr0, ctl);
+      k_gf_bfs_init<<<ggrid(c, Nn), 256, 0, c.stream>>>(N, height, ctl);
+      k_gf_bfs_seed<<<ggrid(c, nc), 256, 0, c.stream>>>(nc, off, res, height, fr0, seg_cap, ctl);
       KCDS_LAUNCH_CHECK();
       uint32_t levels = 0;
       while (true) {
         const uint32_t batch = 16;
         for (uint32_t i = 0; i < batch; ++i) {
-          k_gf_bfs_vchunks<<<bgrid, 256, 0, c.stream>>>(N, nch, choff, chv, off, head, rev, res, height, fr0, fr1,
+          k_gf_bfs_vchunks<<<vgrid, 256, 0, c.stream>>>(N, nch, choff, chv, off, head, rev, res, height, fr0, fr1,
                                                         seg_cap, ctl);
-          k_gf_bfs_level<<<bgrid, 256, 0, c.stream>>>(N, nc, off, head, rev, res, height, fr0, fr1, seg_cap, ctl);
+          k_gf_bfs_level<<<bgrid, 256, 0, c.stream>>>(N, nc, k, off, head, rev, res, height, fr0, fr1, seg_cap, ctl);
         }
         KCDS_LAUNCH_CHECK();
         levels += batch;
@@ -607,10 +706,10 @@ void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t m
       k_gf_fill_u32<<<ggrid(c, 2ull * nc), 256, 0, c.stream>>>(vmin, NO_ARC, 2ull * nc);
       for (uint32_t sw = 0; sw < SWEEPS; ++sw) {
         uint32_t *vm = vmin + (size_t)(sw & 1) * nc, *vn = vmin + (size_t)((sw + 1) & 1) * nc;
-        k_gf_vmin<<<bgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, res, excess, height, vm, ctl);
-        k_gf_vsum<<<bgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, res, excess, height, vm, csum, esnap,
+        k_gf_vmin<<<vgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, res, excess, height, vm, ctl);
+        k_gf_vsum<<<vgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, res, excess, height, vm, csum, esnap,
                                                ctl);
-        k_gf_vpush<<<bgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, rev, res, excess, height, vm, vn,
+        k_gf_vpush<<<vgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, rev, res, excess, height, vm, vn,
                                                 csum, esnap, ctl);
         k_gf_sweep_c<<<ggrid(c, Tc), 256, 0, c.stream>>>(nc, N, sw, off, head, rev, res, excess, height, ctl);
       }

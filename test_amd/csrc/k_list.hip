@@ -545,19 +545,30 @@ __global__ __launch_bounds__(256) void k_class_fill(const uint32_t *doff, uint32
   }
 }
 
-// incidence index: inc[incoff[v] + i] = id of the i-th clique containing v
-__global__ void k_incidence(const uint32_t *cl, uint64_t T, uint32_t k, const ull *incoff, uint32_t *fillc,
-                            uint32_t *inc) {
-  // lanes take consecutive cliques: their roots and hub members repeat, so
-  // the slot counters are bumped once per group of equal vertices
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t c0 = (uint64_t)blockIdx.x * blockDim.x; c0 < T; c0 += stride) {
-    const uint64_t c = c0 + threadIdx.x;
-    const bool valid = c < T;
-    for (uint32_t t = 0; t < k; ++t) {
-      const uint32_t v = valid ? cl[c * k + t] : 0u;
-      const uint32_t slot = wave_slot_by_key<uint32_t>(valid, v, fillc);
-      if (valid && incoff[v] + slot < incoff[v + 1]) inc[incoff[v] + slot] = (uint32_t)c;
+// incidence index: inc[incoff[v] + i] = id of the i-th clique containing v.
+// A round of the workgroup takes INC_E * 256 consecutive clique entries and
+// reserves each vertex's slots with one atomic per round (hub vertices repeat
+// across thousands of consecutive cliques).
+constexpr int INC_E = 8, INC_TB = 4096;
+
+__global__ __launch_bounds__(256) void k_incidence(const uint32_t *cl, uint64_t T, uint32_t k, const ull *incoff,
+                                                   uint32_t *fillc, uint32_t *inc) {
+  __shared__ KeyTable<INC_TB> tb;
+  const uint64_t Tk = T * k, per = (uint64_t)INC_E * 256;
+  for (uint64_t r0 = (uint64_t)blockIdx.x * per; r0 < Tk; r0 += (uint64_t)gridDim.x * per) {
+    bool want[INC_E];
+    uint32_t v[INC_E], slot[INC_E];
+#pragma unroll
+    for (int e = 0; e < INC_E; ++e) {
+      const uint64_t i = r0 + (uint64_t)e * 256 + threadIdx.x;
+      want[e] = i < Tk;
+      v[e] = want[e] ? cl[i] : 0u;
+    }
+    block_slot_by_key<INC_E, INC_TB, uint32_t>(tb, want, v, slot, fillc);
+#pragma unroll
+    for (int e = 0; e < INC_E; ++e) {
+      const uint64_t i = r0 + (uint64_t)e * 256 + threadIdx.x;
+      if (want[e] && incoff[v[e]] + slot[e] < incoff[v[e] + 1]) inc[incoff[v[e]] + slot[e]] = (uint32_t)(i / k);
     }
   }
 }
@@ -866,7 +877,8 @@ void list_materialize(Ctx &c, uint32_t k) {
   buf_reserve(c, c.tmp3, sizeof(uint32_t) * ((size_t)n + 1));
   HIP_CHECK(hipMemsetAsync(c.tmp3.p, 0, sizeof(uint32_t) * ((size_t)n + 1), c.stream));
   if (T) {
-    uint32_t grid = (uint32_t)std::min<uint64_t>((T + 255) / 256, 65535ull * 16);
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>((T * k + INC_E * 256 - 1) / (INC_E * 256), (uint64_t)c.num_cus * 2));
     k_incidence<<<grid, 256, 0, c.stream>>>(c.cliques.as<uint32_t>(), T, k, c.incoff.as<ull>(),
                                             c.tmp3.as<uint32_t>(), c.inc.as<uint32_t>());
     KCDS_LAUNCH_CHECK();

@@ -761,7 +761,9 @@ void peel_run(Ctx &c) {
   const ull *incoff = c.incoff.as<ull>();
   int32_t *core = c.core.as<int32_t>(), *status = c.status.as<int32_t>();
   uint32_t *tl = c.tlist.as<uint32_t>();
-  const uint32_t sgrid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, c.num_cus * 2ull));
+  // at least a workgroup per CU: the work of a step is the frontier's
+  // incidence pieces (up to k*T), not its vertices
+  const uint32_t sgrid = (uint32_t)std::max<uint64_t>(c.num_cus, std::min<uint64_t>((n + 255) / 256, c.num_cus * 2ull));
   // the number of steps is a property of the graph: start from the last count
   uint32_t batch = c.peel_steps ? (uint32_t)std::min<uint64_t>(c.peel_steps, 1u << 20) : 32;
   uint64_t rounds = 0;

@@ -198,6 +198,53 @@ __device__ __forceinline__ void stc_u64(ull *p, ull v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Per-key slot assignment aggregated over a workgroup round: every thread
+// brings E (want, key) pairs, and each distinct key of the round takes its
+// slots in base[key] with one device atomic.  A hub key that repeats across
+// the whole round (thousands of cliques of one vertex) therefore costs one
+// atomic per round instead of one per wave, which keeps it from turning
+// into one hot L2 line.  The table (LDS, 2 * blockDim.x * E <= TB slots,
+// open addressing) is cleared by the call; every thread of the workgroup
+// must call it.  slot[e] is valid where want[e].
+template <int TB> struct KeyTable {
+  uint32_t key[TB];
+  uint32_t cnt[TB];
+  ull base[TB];
+};
+
+template <int E, int TB, typename T>
+__device__ void block_slot_by_key(KeyTable<TB> &tb, const bool *want, const uint32_t *key, T *slot, T *base) {
+  const uint32_t tid = threadIdx.x, nt = blockDim.x;
+  for (uint32_t t = tid; t < TB; t += nt) {
+    tb.key[t] = 0xFFFFFFFFu;
+    tb.cnt[t] = 0;
+  }
+  __syncthreads();
+  uint32_t h[E], loc[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    h[e] = 0;
+    loc[e] = 0;
+    if (!want[e]) continue;
+    uint32_t x = ((uint32_t)__umul24(key[e] ^ (key[e] >> 24), 0x9E3779u) >> 8) & (TB - 1);
+    while (true) {
+      const uint32_t prev = atomicCAS(&tb.key[x], 0xFFFFFFFFu, key[e]);
+      if (prev == 0xFFFFFFFFu || prev == key[e]) break;
+      x = (x + 1) & (TB - 1);
+    }
+    h[e] = x;
+    loc[e] = atomicAdd(&tb.cnt[x], 1u);
+  }
+  __syncthreads();
+  for (uint32_t t = tid; t < TB; t += nt)
+    if (tb.cnt[t]) tb.base[t] = (ull)atomicAdd(&base[tb.key[t]], (T)tb.cnt[t]);
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    if (want[e]) slot[e] = (T)tb.base[h[e]] + (T)loc[e];
+  __syncthreads();  // the table is reused by the next round
+}
+
 // Slot of this lane in base[key] for every lane with want: base[key] grows by
 // the number of such lanes.  Lanes that share a key (a hub vertex, the same
 // component) are served by one device atomic per group for the first eight
