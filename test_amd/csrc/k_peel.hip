@@ -32,7 +32,8 @@ namespace kcds {
 namespace {
 
 constexpr uint32_t PSEG = 4;        // frontier segments / counter shards
-constexpr uint32_t PPIECE = 1024;   // incidence entries per work item
+constexpr uint32_t PPIECE = 256;    // incidence entries per work item
+constexpr uint32_t KMAX = 8;        // largest k
 
 // Counters 64 words apart (same-address device atomics are serialised).
 struct PeelCtl {
@@ -252,22 +253,29 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
           mine = status[c] == -1 && atomicCAS(&status[c], -1, (int32_t)L) == -1;
         }
         destroyed += mine;
-        for (uint32_t t = 0; t < k; ++t) {
-          uint32_t u = 0;
-          bool dec = false;
-          if (mine) {
-            u = cl[(ull)c * k + t];
-            dec = u != v && core[u] == -1;
-          }
+        // the k members' ids and states are loaded together, then every
+        // decrement is issued before any result is used
+        uint32_t u[KMAX];
+        bool dec[KMAX];
+#pragma unroll
+        for (uint32_t t = 0; t < KMAX; ++t) u[t] = mine && t < k ? cl[(ull)c * k + t] : v;
+#pragma unroll
+        for (uint32_t t = 0; t < KMAX; ++t) dec[t] = u[t] != v && core[u[t]] == -1;
+        ull old[KMAX];
+        uint32_t rank[KMAX], leader[KMAX];
+#pragma unroll
+        for (uint32_t t = 0; t < KMAX; ++t) {
           // one atomic per group of lanes decrementing the same (hub) vertex;
           // each lane's own old value is the leader's minus its rank
-          uint32_t rank, leader;
-          const uint32_t gsize = wave_group(dec, u, rank, leader);
-          ull old = 0;
-          if (gsize) old = atomicSub(&cnt[u], (ull)gsize);
-          const ull lold = __shfl(old, leader, 64);
-          const bool join = dec && lold - rank == L + 1;
-          pl_append(join, u, incoff, nv + (size_t)seg * seg_v, &ctl->fsize[cur ^ 1][seg][0],
+          const uint32_t gsize = wave_group(dec[t], u[t], rank[t], leader[t]);
+          old[t] = 0;
+          if (gsize) old[t] = atomicSub(&cnt[u[t]], (ull)gsize);
+        }
+#pragma unroll
+        for (uint32_t t = 0; t < KMAX; ++t) {
+          const ull lold = __shfl(old[t], leader[t], 64);
+          const bool join = dec[t] && lold - rank[t] == L + 1;
+          pl_append(join, u[t], incoff, nv + (size_t)seg * seg_v, &ctl->fsize[cur ^ 1][seg][0],
                     np + (size_t)seg * seg_p, &ctl->psize[cur ^ 1][seg][0]);
         }
       }
