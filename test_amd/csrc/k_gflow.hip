@@ -469,7 +469,7 @@ __global__ __launch_bounds__(256) void k_gf_vpush(uint32_t N, uint32_t nch, uint
     i64 left = esnap[u] - before;
     if (left <= 0 || csum[ch] == 0) continue;
     const uint32_t a0 = off[u] + (ch - c0) * VCH, a1 = min(a0 + VCH, off[u + 1]);
-    i64 pushed = 0;
+    i64 pushed = 0, to_t = 0;
     for (uint32_t b = a0; b < a1 && left > 0; b += VU * KCDS_WAVE) {
       i64 r[VU];
       uint32_t x[VU];
@@ -492,7 +492,10 @@ __global__ __launch_bounds__(256) void k_gf_vpush(uint32_t N, uint32_t nch, uint
         if (d > 0) {
           add64(&res[a], -d);
           add64(&res[rev[a]], d);
-          add64(&excess[x[q]], d);
+          if (x[q] == N - 1)
+            to_t += d;  // every vertex pushes to t: summed per wave below
+          else
+            add64(&excess[x[q]], d);
         }
         const i64 tot = (i64)__shfl(incl, 63, 64);
         const i64 used = min(tot, left);
@@ -500,6 +503,8 @@ __global__ __launch_bounds__(256) void k_gf_vpush(uint32_t N, uint32_t nch, uint
         pushed += used;
       }
     }
+    to_t = (i64)wave_sum_u64((ull)to_t);
+    if (lane == 0 && to_t) add64(&excess[N - 1], to_t);
     if (pushed > 0) {
       if (lane == 0) add64(&excess[u], -pushed);
       ++changed;
@@ -508,11 +513,25 @@ __global__ __launch_bounds__(256) void k_gf_vpush(uint32_t N, uint32_t nch, uint
   if (lane == 0 && changed) atomicAdd(&ctl->changed[sweep + 1][0], changed);
 }
 
-// Clique nodes, thread per node (k arcs each).
+// Clique nodes, thread per node (k arcs each).  Their pushes raise vertex
+// excesses; a hub vertex receives from most cliques, so the raises are
+// summed per workgroup in an LDS table keyed by vertex (one device atomic
+// per vertex per workgroup, direct atomics when the table is crowded).
+// Deferring the raises to the end of the workgroup is safe: vertex nodes do
+// not act in this kernel.
+constexpr uint32_t SC_TB = 1024;
+
 __global__ __launch_bounds__(256) void k_gf_sweep_c(uint32_t nc, uint32_t N, uint32_t sweep, const uint32_t *off,
                                                     const uint32_t *head, const uint32_t *rev, i64 *res, i64 *excess,
                                                     uint32_t *height, GFCtl *ctl) {
+  __shared__ uint32_t s_key[SC_TB];
+  __shared__ ull s_val[SC_TB];
   if (ctl->changed[sweep][0] == 0) return;  // (also skips when the vertex kernels of this sweep did)
+  for (uint32_t t = threadIdx.x; t < SC_TB; t += blockDim.x) {
+    s_key[t] = 0xFFFFFFFFu;
+    s_val[t] = 0;
+  }
+  __syncthreads();
   uint32_t changed = 0;
   const uint32_t s = N - 2;
   for (uint64_t u = nc + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < s; u += (uint64_t)gridDim.x * blockDim.x) {
@@ -537,7 +556,17 @@ __global__ __launch_bounds__(256) void k_gf_sweep_c(uint32_t nc, uint32_t N, uin
       const i64 d = min(r, left);
       add64(&res[a], -d);
       add64(&res[rev[a]], d);
-      add64(&excess[x], d);
+      uint32_t h = ((uint32_t)__umul24(x ^ (x >> 24), 0x9E3779u) >> 8) & (SC_TB - 1);
+      bool done = false;
+      for (int probe = 0; probe < 8 && !done; ++probe) {
+        const uint32_t k0 = atomicCAS(&s_key[h], 0xFFFFFFFFu, x);
+        if (k0 == 0xFFFFFFFFu || k0 == x) {
+          atomicAdd(&s_val[h], (ull)d);
+          done = true;
+        }
+        h = (h + 1) & (SC_TB - 1);
+      }
+      if (!done) add64(&excess[x], d);
       left -= d;
     }
     if (e > left) {
@@ -547,6 +576,9 @@ __global__ __launch_bounds__(256) void k_gf_sweep_c(uint32_t nc, uint32_t N, uin
   }
   changed = wave_sum_u32(changed);
   if (lane_id() == 0 && changed) atomicAdd(&ctl->changed[sweep + 1][0], changed);
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < SC_TB; t += blockDim.x)
+    if (s_key[t] != 0xFFFFFFFFu && s_val[t]) add64(&excess[s_key[t]], (i64)s_val[t]);
 }
 
 // Source side of the maximal min cut: vertices that cannot reach t, and the
