@@ -210,16 +210,19 @@ __global__ __launch_bounds__(256) void k_gf_bfs_seed(uint32_t nc, const uint32_t
 }
 
 // The last workgroup of a launch (two-level ticket) returns true.
+// The control step reads only the frontier-size counters, whose atomics
+// returned their values before the appends, so LDS barriers suffice.
 __device__ bool last_block(GFCtl *ctl, uint32_t *s_last) {
-  block_sync();
+  lds_sync();
   if (threadIdx.x == 0) {
+    wave_drain();
     const uint32_t seg = blockIdx.x % NSEG;
     const uint32_t in_seg = (gridDim.x - seg + NSEG - 1) / NSEG;
     uint32_t last = 0;
     if (atomicAdd(&ctl->gblocks[seg][0], 1u) == in_seg - 1) last = atomicAdd(&ctl->blocks[0], 1u) == min(gridDim.x, NSEG) - 1;
     *s_last = last;
   }
-  block_sync();
+  lds_sync();
   return *s_last != 0;
 }
 
@@ -233,18 +236,19 @@ __global__ __launch_bounds__(256) void k_gf_bfs_level(uint32_t N, uint32_t nv_no
                                                       const uint32_t *head, const uint32_t *rev, const i64 *res,
                                                       uint32_t *height, uint32_t *fr0, uint32_t *fr1,
                                                       uint32_t seg_cap, GFCtl *ctl) {
-  __shared__ uint32_t s_hdr[3], s_pre[NSEG + 1], s_last;
+  __shared__ uint32_t s_hdr[3], s_pre[NSEG + 1], s_last, s_sz[2][NSEG];
   const uint32_t tid = threadIdx.x, lane = lane_id(), seg = blockIdx.x % NSEG;
+  // the header and both frontiers' segment sizes in one round trip
   if (tid < 3) s_hdr[tid] = (&ctl->dist)[tid];
-  block_sync();
+  if (tid >= 64 && tid < 64 + 2 * NSEG) s_sz[(tid - 64) / NSEG][(tid - 64) % NSEG] = ctl->fsize[(tid - 64) / NSEG][(tid - 64) % NSEG][0];
+  lds_sync();
   const uint32_t dist = s_hdr[0], cur = s_hdr[1], done = s_hdr[2];
   if (done) return;
-  if (tid < NSEG) s_pre[tid + 1] = ctl->fsize[cur][tid][0];
-  if (tid == 0) s_pre[0] = 0;
-  block_sync();
-  if (tid == 0)
-    for (uint32_t g = 0; g < NSEG; ++g) s_pre[g + 1] += s_pre[g];
-  block_sync();
+  if (tid == 0) {
+    s_pre[0] = 0;
+    for (uint32_t g = 0; g < NSEG; ++g) s_pre[g + 1] = s_pre[g] + s_sz[cur][g];
+  }
+  lds_sync();
   const uint32_t fs = s_pre[NSEG];
   const uint32_t *fr = cur ? fr1 : fr0;
   uint32_t *nx = cur ? fr0 : fr1;
@@ -278,17 +282,21 @@ __global__ __launch_bounds__(256) void k_gf_bfs_level(uint32_t N, uint32_t nv_no
       }
     }
   }
-  if (last_block(ctl, &s_last) && tid == 0) {
-    uint32_t nnext = 0;
-    for (uint32_t g2 = 0; g2 < NSEG; ++g2) {
-      nnext += atomicAdd(&ctl->fsize[cur ^ 1][g2][0], 0u);
-      ctl->fsize[cur][g2][0] = 0;
-      ctl->gblocks[g2][0] = 0;
+  // wave 0 of the last workgroup: lanes g < NSEG read and reset segment g
+  if (last_block(ctl, &s_last) && tid < KCDS_WAVE) {
+    uint32_t nn = 0;
+    if (lane < NSEG) {
+      nn = ldc_u32(&ctl->fsize[cur ^ 1][lane][0]);
+      ctl->fsize[cur][lane][0] = 0;
+      ctl->gblocks[lane][0] = 0;
     }
-    ctl->blocks[0] = 0;
-    ctl->cur = cur ^ 1;
-    ctl->dist = dist + 1;
-    if (nnext == 0) ctl->done = 1;
+    const uint32_t nnext = wave_sum_u32(nn);
+    if (lane == 0) {
+      ctl->blocks[0] = 0;
+      ctl->cur = cur ^ 1;
+      ctl->dist = dist + 1;
+      if (nnext == 0) ctl->done = 1;
+    }
   }
 }
 

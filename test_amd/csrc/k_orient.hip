@@ -223,32 +223,40 @@ __device__ __forceinline__ void wave_append(bool want, uint32_t v, const uint32_
 // Every workgroup's counter atomics completed before it took its ticket; the
 // counters are read back with read-modify-write atomics, which are performed
 // where the other workgroups' atomics were.
-__device__ void order_control(OrderCtl *ctl, uint32_t n) {
-  const uint32_t cur = ctl->cur, phase = ctl->phase;
-  uint32_t fcur = 0, fnext = 0, mn = DEAD;
-  for (uint32_t g = 0; g < NSEG; ++g) {
-    fcur += atomicAdd(&ctl->fsize[cur][g][0], 0u);
-    fnext += atomicAdd(&ctl->fsize[cur ^ 1][g][0], 0u);
-    mn = min(mn, atomicExch(&ctl->gmin[g][0], DEAD));
-    ctl->gblocks[g][0] = 0;
+// Control step, run by wave 0 of the last workgroup: lanes g < NSEG read
+// and reset segment g's counters together (one round trip, not one per
+// segment); the header values are those this step started from (only the
+// control step writes them).
+__device__ void order_control(OrderCtl *ctl, uint32_t n, uint32_t level, uint32_t target, uint32_t next_rank,
+                              uint32_t cur, uint32_t phase) {
+  const uint32_t lane = lane_id();
+  uint32_t fc = 0, fn = 0, mn = DEAD;
+  if (lane < NSEG) {
+    fc = ldc_u32(&ctl->fsize[cur][lane][0]);
+    fn = ldc_u32(&ctl->fsize[cur ^ 1][lane][0]);
+    mn = atomicExch(&ctl->gmin[lane][0], DEAD);
+    ctl->gblocks[lane][0] = 0;
+    if (phase == 1) {
+      ctl->fsize[cur][lane][0] = 0;
+      ctl->psize[cur][lane][0] = 0;
+    }
   }
+  const uint32_t fcur = wave_sum_u32(fc), fnext = wave_sum_u32(fn);
+  mn = wave_min_u32(mn);
+  if (lane != 0) return;
   ctl->blocks[0] = 0;
   ++ctl->steps;
   if (phase == 1) {
-    ctl->next_rank += fcur;
-    for (uint32_t g = 0; g < NSEG; ++g) {
-      ctl->fsize[cur][g][0] = 0;
-      ctl->psize[cur][g][0] = 0;
-    }
+    ctl->next_rank = next_rank + fcur;
     ctl->cur = cur ^ 1;
-    if (ctl->next_rank >= n) {
+    if (next_rank + fcur >= n) {
       ctl->done = 1;
     } else if (fnext == 0) {
       ctl->phase = 0;
-      ctl->target = ctl->level + 1;
+      ctl->target = level + 1;
     }
   } else if (fcur > 0) {
-    ctl->level = ctl->target;
+    ctl->level = target;
     ctl->phase = 1;
   } else if (mn == DEAD) {
     ctl->done = 1;
@@ -266,10 +274,15 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
                                                  uint32_t *fv1, ull *fp0, ull *fp1, uint32_t seg_v, uint32_t seg_p,
                                                  uint32_t *rank, uint32_t *inv) {
   __shared__ uint32_t s_f[NSEG + 1], s_p[NSEG + 1], s_min, s_last;
-  __shared__ uint32_t s_hdr[6];
+  __shared__ uint32_t s_hdr[6], s_sz[2][2][NSEG];
   const uint32_t tid = threadIdx.x, lane = lane_id(), seg = blockIdx.x % NSEG;
+  // the header and both frontiers' segment sizes in one round trip
   if (tid < 6) s_hdr[tid] = (&ctl->level)[tid];
-  block_sync();
+  if (tid >= 64 && tid < 64 + 4 * NSEG) {
+    const uint32_t i = tid - 64, f = i / (2 * NSEG), kind = (i / NSEG) & 1, g = i % NSEG;
+    s_sz[f][kind][g] = kind ? ctl->psize[f][g][0] : ctl->fsize[f][g][0];
+  }
+  lds_sync();
   const uint32_t level = s_hdr[0], target = s_hdr[1], base = s_hdr[2], cur = s_hdr[3], done = s_hdr[4],
                  phase = s_hdr[5];
   if (done) return;
@@ -280,7 +293,7 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
   const uint32_t gtid = blockIdx.x * blockDim.x + tid, gstride = gridDim.x * blockDim.x;
   if (phase == 0) {
     if (tid == 0) s_min = DEAD;
-    block_sync();
+    lds_sync();
     uint32_t m = DEAD;
     for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < n; v0 += gstride) {
       const uint32_t v = v0 + tid;
@@ -292,26 +305,19 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
     }
     m = wave_min_u32(m);
     if (lane == 0 && m != DEAD) atomicMin(&s_min, m);
-    block_sync();
+    lds_sync();
     if (tid == 0 && s_min != DEAD) atomicMin(&ctl->gmin[seg][0], s_min);
   } else {
     // segment sizes and prefixes of the current frontier
-    if (tid < NSEG) {
-      s_f[tid + 1] = ctl->fsize[cur][tid][0];
-      s_p[tid + 1] = ctl->psize[cur][tid][0];
-    }
     if (tid == 0) {
       s_f[0] = 0;
       s_p[0] = 0;
-    }
-    block_sync();
-    if (tid == 0) {
       for (uint32_t g = 0; g < NSEG; ++g) {
-        s_f[g + 1] += s_f[g];
-        s_p[g + 1] += s_p[g];
+        s_f[g + 1] = s_f[g] + s_sz[cur][0][g];
+        s_p[g + 1] = s_p[g] + s_sz[cur][1][g];
       }
     }
-    block_sync();
+    lds_sync();
     const uint32_t fs = s_f[NSEG], ps = s_p[NSEG];
     for (uint32_t i = gtid; i < fs; i += gstride) {  // the frontier takes the next ranks and leaves
       uint32_t g = 0;
@@ -346,9 +352,14 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
                     np + (size_t)seg * seg_p, &ctl->psize[cur ^ 1][seg][0]);
     }
   }
-  // two-level ticket: the last workgroup of a segment, then the last segment
-  block_sync();  // this workgroup's atomics have completed
+  // two-level ticket: the last workgroup of a segment, then the last segment.
+  // What the control step reads of this workgroup is complete by now: the
+  // frontier-size atomics returned their values before their appends, and
+  // thread 0 waits for its own minimum atomic below.  Stores and the
+  // decrements' side effects are read only by later launches.
+  lds_sync();
   if (tid == 0) {
+    wave_drain();
     const uint32_t in_seg = (gridDim.x - seg + NSEG - 1) / NSEG;  // workgroups with this residue
     uint32_t last = 0;
     if (atomicAdd(&ctl->gblocks[seg][0], 1u) == in_seg - 1) {
@@ -357,8 +368,8 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
     }
     s_last = last;
   }
-  block_sync();
-  if (s_last && tid == 0) order_control(ctl, n);
+  lds_sync();
+  if (s_last && tid < KCDS_WAVE) order_control(ctl, n, level, target, base, cur, phase);
 }
 
 __global__ void k_init_deg(const uint32_t *off, uint32_t n, uint32_t *deg) {

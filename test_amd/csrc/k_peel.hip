@@ -123,9 +123,14 @@ __device__ __forceinline__ uint32_t wave_group(bool want, uint32_t key, uint32_t
 }
 
 // The last workgroup of a launch (two-level ticket) returns true.
+// What the control step reads of a workgroup (the frontier-size counters,
+// whose atomics returned their values before the appends, and thread 0's
+// minimum atomic, drained below) is complete at the ticket; the rest of
+// the step's writes are read only by later launches.
 __device__ bool pl_last_block(PeelCtl *ctl, uint32_t *s_last) {
-  block_sync();
+  lds_sync();
   if (threadIdx.x == 0) {
+    wave_drain();
     const uint32_t seg = blockIdx.x % PSEG;
     const uint32_t in_seg = (gridDim.x - seg + PSEG - 1) / PSEG;
     uint32_t last = 0;
@@ -133,7 +138,7 @@ __device__ bool pl_last_block(PeelCtl *ctl, uint32_t *s_last) {
       last = atomicAdd(&ctl->blocks[0], 1u) == min(gridDim.x, PSEG) - 1;
     *s_last = last;
   }
-  block_sync();
+  lds_sync();
   return *s_last != 0;
 }
 
@@ -160,12 +165,17 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
                                                  PItem *fp0, PItem *fp1, uint32_t seg_v, uint32_t seg_p,
                                                  GraphPeel *gp, const uint32_t *__restrict__ gid, uint32_t *tlist) {
   __shared__ ull s_hdr[2];
-  __shared__ uint32_t s_hdr2[3], s_f[PSEG + 1], s_p[PSEG + 1], s_last;
+  __shared__ uint32_t s_hdr2[3], s_f[PSEG + 1], s_p[PSEG + 1], s_last, s_sz[2][2][PSEG];
   __shared__ ull s_min;
   const uint32_t tid = threadIdx.x, lane = lane_id(), seg = blockIdx.x % PSEG;
+  // the header and both frontiers' segment sizes in one round trip
   if (tid < 2) s_hdr[tid] = (&ctl->level)[tid];
   if (tid < 3) s_hdr2[tid] = (&ctl->cur)[tid];
-  block_sync();
+  if (tid >= 64 && tid < 64 + 4 * PSEG) {
+    const uint32_t i = tid - 64, f = i / (2 * PSEG), kind = (i / PSEG) & 1, g = i % PSEG;
+    s_sz[f][kind][g] = kind ? ctl->psize[f][g][0] : ctl->fsize[f][g][0];
+  }
+  lds_sync();
   const ull L = s_hdr[0], target = s_hdr[1];
   const uint32_t cur = s_hdr2[0], done = s_hdr2[1], phase = s_hdr2[2];
   if (done) return;
@@ -185,7 +195,7 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
       P.touch = 0;
     }
     if (tid == 0) s_min = ~0ull;
-    block_sync();
+    lds_sync();
     ull m = ~0ull;
     for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < n; v0 += gstride) {
       const uint32_t v = v0 + tid;
@@ -198,24 +208,18 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = min(m, (ull)__shfl_xor(m, o, 64));
     if (lane == 0 && m != ~0ull) atomicMin(&s_min, m);
-    block_sync();
+    lds_sync();
     if (tid == 0 && s_min != ~0ull) atomicMin(&ctl->gmin[seg][0], s_min);
   } else {
-    if (tid < PSEG) {
-      s_f[tid + 1] = ctl->fsize[cur][tid][0];
-      s_p[tid + 1] = ctl->psize[cur][tid][0];
-    }
     if (tid == 0) {
       s_f[0] = 0;
       s_p[0] = 0;
-    }
-    block_sync();
-    if (tid == 0)
       for (uint32_t g = 0; g < PSEG; ++g) {
-        s_f[g + 1] += s_f[g];
-        s_p[g + 1] += s_p[g];
+        s_f[g + 1] = s_f[g] + s_sz[cur][0][g];
+        s_p[g + 1] = s_p[g] + s_sz[cur][1][g];
       }
-    block_sync();
+    }
+    lds_sync();
     const uint32_t fs = s_f[PSEG], ps = s_p[PSEG];
     // mark: core numbers, remaining vertices and the touched list per graph
     for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < fs; i0 += gstride) {
@@ -284,37 +288,45 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
     }
   }
   // control step: the last workgroup to finish
-  if (pl_last_block(ctl, &s_last) && tid == 0) {
-    uint32_t fcur = 0, fnext = 0;
+  // wave 0 of the last workgroup: lanes g < PSEG read and reset segment g's
+  // counters together; L, target, cur and phase are this step's header
+  // (only the control step writes them)
+  if (pl_last_block(ctl, &s_last) && tid < KCDS_WAVE) {
+    uint32_t fc = 0, fn = 0;
     ull mn = ~0ull;
-    for (uint32_t g = 0; g < PSEG; ++g) {
-      fcur += atomicAdd(&ctl->fsize[cur][g][0], 0u);
-      fnext += atomicAdd(&ctl->fsize[cur ^ 1][g][0], 0u);
-      mn = min(mn, (ull)atomicExch(&ctl->gmin[g][0], ~0ull));
-      ctl->gblocks[g][0] = 0;
+    if (lane < PSEG) {
+      fc = ldc_u32(&ctl->fsize[cur][lane][0]);
+      fn = ldc_u32(&ctl->fsize[cur ^ 1][lane][0]);
+      mn = atomicExch(&ctl->gmin[lane][0], ~0ull);
+      ctl->gblocks[lane][0] = 0;
+      if (phase == 1) {
+        ctl->fsize[cur][lane][0] = 0;
+        ctl->psize[cur][lane][0] = 0;
+      }
     }
-    ctl->blocks[0] = 0;
-    ++ctl->steps;
-    if (phase == 1) {
-      for (uint32_t g = 0; g < PSEG; ++g) {
-        ctl->fsize[cur][g][0] = 0;
-        ctl->psize[cur][g][0] = 0;
-      }
-      ctl->cur = cur ^ 1;
-      if (fnext == 0) {
-        ctl->phase = 0;
-        ctl->target = L + 1;
-      }
-    } else {
-      ctl->ntouch[0] = 0;  // the densities were recorded by this step
-      if (fcur > 0) {
-        ctl->level = target;
-        ctl->phase = 1;
-        ++ctl->levels;
-      } else if (mn == ~0ull) {
-        ctl->done = 1;
+    const uint32_t fcur = wave_sum_u32(fc), fnext = wave_sum_u32(fn);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mn = min(mn, (ull)__shfl_xor(mn, o, 64));
+    if (lane == 0) {
+      ctl->blocks[0] = 0;
+      ++ctl->steps;
+      if (phase == 1) {
+        ctl->cur = cur ^ 1;
+        if (fnext == 0) {
+          ctl->phase = 0;
+          ctl->target = L + 1;
+        }
       } else {
-        ctl->target = mn;
+        ctl->ntouch[0] = 0;  // the densities were recorded by this step
+        if (fcur > 0) {
+          ctl->level = target;
+          ctl->phase = 1;
+          ++ctl->levels;
+        } else if (mn == ~0ull) {
+          ctl->done = 1;
+        } else {
+          ctl->target = mn;
+        }
       }
     }
   }

@@ -28,6 +28,14 @@ __device__ __forceinline__ void block_sync() {
   __syncthreads();
 }
 
+// Waits until this wave's memory operations so far have completed (the
+// compiler may not move memory operations across it).
+__device__ __forceinline__ void wave_drain() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // Workgroup barrier for data exchanged through LDS only: it waits for the
 // wave's LDS operations but leaves its global loads, stores and atomics in
 // flight (a plain __syncthreads also drains those).
