@@ -73,7 +73,8 @@ const char *kcds_last_error(const kcds_ctx *ctx);
 int kcds_upload_csr(kcds_ctx *ctx, uint32_t n, const uint32_t *offset, const uint32_t *nbr);
 
 /* Builds the DAG used for listing.  listing_order == NULL: degeneracy order
- * computed on the GPU.  Otherwise listing_order[v] in 1..n is the reference's
+ * computed on the GPU (with slack unless KCDS_TUNE_EXACT_ORDER is set; see
+ * kcds_set_tuning).  Otherwise listing_order[v] in 1..n is the reference's
  * listing order (src/graph.cpp:78-80: 1 + position in corePeelSequence); arcs
  * point from the vertex peeled earlier to the one peeled later.  Every
  * acyclic orientation yields the same cliques. */
@@ -160,6 +161,15 @@ int kcds_sync(kcds_ctx *ctx);
 /*   KCDS_TUNE_GRID_FLOW  solve every component with the grid-wide max-flow
  *                        (normally only components of >= 65,536 cliques). */
 #define KCDS_TUNE_GRID_FLOW 2u
+/*   KCDS_TUNE_EXACT_ORDER  orient by the exact degeneracy order.  By default
+ *                        the GPU order peels with each threshold the minimum
+ *                        live degree rounded up by a quarter (x -> x + x/4):
+ *                        out-degrees stay within 5/4 of the degeneracy and
+ *                        the peel takes far fewer steps; when that leaves an out-list wider than
+ *                        512 the exact order is used instead.  Every acyclic
+ *                        orientation lists each clique exactly once, so all
+ *                        results are the same either way. */
+#define KCDS_TUNE_EXACT_ORDER 4u
 int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
 
 /* Listing launch shape: waves per root (workgroup size / 64, 1..8) of each

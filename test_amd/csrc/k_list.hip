@@ -41,7 +41,7 @@ namespace kcds {
 namespace {
 
 constexpr int NCLASS = 4;  // W = 1, 2, 4, 8 words per row (out-degree <= 64, 128, 256, 512)
-constexpr uint32_t MAX_LDS_DEG = 512;
+constexpr uint32_t MAX_LDS_DEG = LIST_MAX_OUTDEG;
 constexpr uint32_t MAX_WAVES = 8;  // waves per workgroup (launch bound 512)
 constexpr uint32_t EMPTY_KEY = 0xFFFFFFFFu;
 

@@ -227,8 +227,14 @@ __device__ __forceinline__ void wave_append(bool want, uint32_t v, const uint32_
 // and reset segment g's counters together (one round trip, not one per
 // segment); the header values are those this step started from (only the
 // control step writes them).
+// Threshold from a minimum live degree x: x itself for the exact order,
+// x + x>>sh with slack (clamped below the dead marks).
+__device__ __forceinline__ uint32_t with_slack(uint32_t x, uint32_t sh) {
+  return sh < 32 ? (uint32_t)min<ull>((ull)x + (x >> sh), DEAD_MIN - 1) : x;
+}
+
 __device__ void order_control(OrderCtl *ctl, uint32_t n, uint32_t level, uint32_t target, uint32_t next_rank,
-                              uint32_t cur, uint32_t phase) {
+                              uint32_t cur, uint32_t phase, uint32_t sh) {
   const uint32_t lane = lane_id();
   uint32_t fc = 0, fn = 0, mn = DEAD;
   if (lane < NSEG) {
@@ -253,7 +259,11 @@ __device__ void order_control(OrderCtl *ctl, uint32_t n, uint32_t level, uint32_
       ctl->done = 1;
     } else if (fnext == 0) {
       ctl->phase = 0;
-      ctl->target = level + 1;
+      // exact: every live degree now exceeds level, so try level + 1.  Slack:
+      // find the true minimum first (nothing is <= level), so every threshold
+      // is a slack of a minimum live degree, which never exceeds the
+      // degeneracy: out-degrees stay <= kmax + kmax/4
+      ctl->target = sh < 32 ? level : level + 1;
     }
   } else if (fcur > 0) {
     ctl->level = target;
@@ -261,7 +271,7 @@ __device__ void order_control(OrderCtl *ctl, uint32_t n, uint32_t level, uint32_
   } else if (mn == DEAD) {
     ctl->done = 1;
   } else {
-    ctl->target = mn;
+    ctl->target = with_slack(mn, sh);
   }
 }
 
@@ -272,7 +282,7 @@ __device__ void order_control(OrderCtl *ctl, uint32_t n, uint32_t level, uint32_
 __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, const uint32_t *__restrict__ off,
                                                  const uint32_t *__restrict__ nbr, uint32_t *deg, uint32_t *fv0,
                                                  uint32_t *fv1, ull *fp0, ull *fp1, uint32_t seg_v, uint32_t seg_p,
-                                                 uint32_t *rank, uint32_t *inv) {
+                                                 uint32_t *rank, uint32_t *inv, uint32_t sh) {
   __shared__ uint32_t s_f[NSEG + 1], s_p[NSEG + 1], s_min, s_last;
   __shared__ uint32_t s_hdr[6], s_sz[2][2][NSEG];
   const uint32_t tid = threadIdx.x, lane = lane_id(), seg = blockIdx.x % NSEG;
@@ -369,7 +379,7 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
     s_last = last;
   }
   lds_sync();
-  if (s_last && tid < KCDS_WAVE) order_control(ctl, n, level, target, base, cur, phase);
+  if (s_last && tid < KCDS_WAVE) order_control(ctl, n, level, target, base, cur, phase, sh);
 }
 
 __global__ void k_init_deg(const uint32_t *off, uint32_t n, uint32_t *deg) {
@@ -379,7 +389,7 @@ __global__ void k_init_deg(const uint32_t *off, uint32_t n, uint32_t *deg) {
 
 }  // namespace
 
-void degeneracy_rank_gpu(Ctx &c) {
+void degeneracy_rank_gpu(Ctx &c, uint32_t slack_shift) {
   const uint32_t n = c.n;
   buf_reserve(c, c.rank, sizeof(uint32_t) * (size_t)n + 4);
   buf_reserve(c, c.inv, sizeof(uint32_t) * (size_t)n + 4);
@@ -413,7 +423,8 @@ void degeneracy_rank_gpu(Ctx &c) {
   uint64_t rounds = 0;
   while (true) {
     for (uint32_t r = 0; r < batch; ++r) {
-      k_ko_step<<<vgrid, 256, 0, c.stream>>>(ctl, n, off, nbr, deg, fv0, fv1, fp0, fp1, seg_v, seg_p, rank, inv);
+      k_ko_step<<<vgrid, 256, 0, c.stream>>>(ctl, n, off, nbr, deg, fv0, fv1, fp0, fp1, seg_v, seg_p, rank, inv,
+                                             slack_shift);
       KCDS_LAUNCH_CHECK();
     }
     rounds += batch;
@@ -429,7 +440,7 @@ void degeneracy_rank_gpu(Ctx &c) {
   c.order_steps = ((OrderCtl *)hc)->steps;
 }
 
-void orient_from_rank(Ctx &c) {
+bool orient_from_rank(Ctx &c, bool approx) {
   const uint32_t n = c.n;
   buf_reserve(c, c.doff, sizeof(uint32_t) * ((size_t)n + 1));
   buf_reserve(c, c.tmp1, sizeof(uint32_t) * ((size_t)n + 1));
@@ -458,6 +469,7 @@ void orient_from_rank(Ctx &c) {
   c.sync();
   c.darcs = h[0];
   c.dmax = h[1];
+  if (approx && c.dmax > LIST_MAX_OUTDEG) return false;  // the caller orders exactly instead
   if (c.dmax > SORT_WAVE)
     fail(KCDS_E_LIMIT, "DAG out-degree %u exceeds %u (orientation not degeneracy-bounded)", c.dmax, SORT_WAVE);
   buf_reserve(c, c.dnbr, sizeof(uint32_t) * (c.darcs + 1));
@@ -468,6 +480,22 @@ void orient_from_rank(Ctx &c) {
     KCDS_LAUNCH_CHECK();
   }
   c.w2 = 0;  // the listing pass reports the out-wedge sum it stages
+  return true;
+}
+
+// GPU orientation: the slack order (each threshold is the minimum live
+// degree rounded up by a quarter, so out-degrees stay within 5/4 of the
+// degeneracy, in a fraction of the steps: C2 104 instead of 609), or the
+// exact degeneracy order when KCDS_TUNE_EXACT_ORDER is set or the slack order
+// leaves an out-list wider than the listing's LDS rows.
+void orient_gpu(Ctx &c) {
+  const bool approx = !(c.tune_flags & KCDS_TUNE_EXACT_ORDER);
+  degeneracy_rank_gpu(c, approx ? ORDER_SLACK_SHIFT : NO_SLACK);
+  c.order_exact = !approx;
+  if (orient_from_rank(c, approx)) return;
+  degeneracy_rank_gpu(c, NO_SLACK);
+  c.order_exact = true;
+  orient_from_rank(c, false);
 }
 
 void rank_from_listing_order(Ctx &c, const uint32_t *lo_dev) {

@@ -199,10 +199,10 @@ int kcds_orient(kcds_ctx *ctx, const uint32_t *listing_order) {
       buf_reserve(c, c.tmp0, sizeof(uint32_t) * (size_t)n + 4);
       HIP_CHECK(hipMemcpyAsync(c.tmp0.p, listing_order, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c.stream));
       rank_from_listing_order(c, c.tmp0.as<uint32_t>());
+      orient_from_rank(c);
     } else {
-      degeneracy_rank_gpu(c);
+      orient_gpu(c);
     }
-    orient_from_rank(c);
     c.oriented = true;
   });
 }

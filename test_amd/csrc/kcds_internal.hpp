@@ -100,6 +100,7 @@ struct Ctx {
   uint32_t list_waves_per_root[4] = {0, 0, 0, 0};  // 0 = default of the class
   float tune_ms[4] = {0, 0, 0, 0};
   uint64_t order_steps = 0;  // work/control steps of the last GPU ordering
+  bool order_exact = true;   // the last GPU ordering was the exact degeneracy order
   uint64_t peel_steps = 0;   // steps of the last grid peel
   uint64_t graph_budget = 0; // per-graph cap on T * (8k + 4) bytes (0 = none)
 
@@ -183,8 +184,13 @@ ull sum_u64_host(Ctx &c, const ull *in, uint64_t len);
 ull max_u64_host(Ctx &c, const ull *in, uint64_t len);
 
 // ---- orientation (k_orient.hip) ----
-void orient_from_rank(Ctx &c);       // rank/inv set -> doff, dnbr, dmax, darcs, w2
-void degeneracy_rank_gpu(Ctx &c);    // fills rank/inv from a GPU k-core peel
+// rank/inv set -> doff, dnbr, dmax, darcs; false (nothing failed) when approx
+// and an out-list is wider than LIST_MAX_OUTDEG
+bool orient_from_rank(Ctx &c, bool approx = false);
+// fills rank/inv from a GPU k-core peel; thresholds x -> x + (x >> slack_shift)
+// (NO_SLACK: the exact degeneracy order)
+void degeneracy_rank_gpu(Ctx &c, uint32_t slack_shift);
+void orient_gpu(Ctx &c);  // degeneracy_rank_gpu + orient_from_rank, slack order first
 
 // ---- listing (k_list.hip) ----
 void list_count(Ctx &c, uint32_t k, ull cap = 0);  // cnt, T (cap: per-graph admission cap, 0 = none)
@@ -223,6 +229,8 @@ void dinkelbach_blocks(Ctx &c, const std::vector<BlockTask> &in, std::vector<Blo
 void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t max_iters);
 // Components with at least this many cliques are solved by the grid solver.
 constexpr uint64_t GRID_FLOW_MIN_T = 1u << 16;
+constexpr uint32_t NO_SLACK = 32, ORDER_SLACK_SHIFT = 2;  // slack order: thresholds x + x/4
+constexpr uint32_t LIST_MAX_OUTDEG = 512;                  // widest out-list the listing stages in LDS
 // All tasks: large components one after another on the whole GPU, the rest in
 // one workgroup-per-component launch.
 void dinkelbach_components(Ctx &c, const std::vector<BlockTask> &in, std::vector<BlockResult> &out,

@@ -134,8 +134,7 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
   Timer total(c.stream);
   {
     Timer t(c.stream);
-    degeneracy_rank_gpu(c);
-    orient_from_rank(c);
+    orient_gpu(c);
     c.oriented = true;
     R.ms_orient = t.stop();
   }

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from test_amd import gen
+from test_amd import gen, kcds
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -124,6 +124,31 @@ def test_config3_full_counts_match_oracle_golden(solver):
     assert int(pv.max()) == ref["max"]
     for v, c in ref["sample"].items():
         assert int(pv[int(v)]) == c
+
+
+def test_exact_and_slack_orders(solver):
+    """The exact GPU degeneracy order's widest out-list is the degeneracy (the
+    first vertex the peel takes from the kmax-core has exactly kmax live
+    neighbours); the slack order stays within kmax + kmax/4.  Both give
+    the oracle's counts."""
+    for i, (n, ef, cl) in enumerate(((3000, 12.0, (40,)), (5000, 6.0, (25, 25)), (800, 30.0, ()))):
+        g = gen.rmat(int(np.ceil(np.log2(n))), ef, cl, seed=70 + i, n=n)
+        _, _, _, kmax = O.core_decompose(g.offset, g.nbr)
+        Tr, pvr = O.count(g.offset, g.nbr, 4)
+        for flags in (kcds.TUNE_EXACT_ORDER, 0):
+            solver.set_tuning(0, flags)
+            solver.upload(g.offset, g.nbr)
+            solver.orient(None)
+            T, pv = solver.count(4)
+            mo = solver.list_info().max_outdeg
+            print(f"  n={n} kmax={kmax} flags={flags} max_outdeg={mo} T={T}", flush=True)
+            assert T == Tr
+            np.testing.assert_array_equal(pv, pvr)
+            if flags:
+                assert mo == kmax
+            else:
+                assert mo <= kmax + kmax // 4
+    solver.set_tuning(0, 0)
 
 
 def test_materialize_matches_oracle(solver):
