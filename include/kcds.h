@@ -163,8 +163,8 @@ int kcds_sync(kcds_ctx *ctx);
 #define KCDS_TUNE_GRID_FLOW 2u
 /*   KCDS_TUNE_EXACT_ORDER  orient by the exact degeneracy order.  By default
  *                        the GPU order peels with each threshold the minimum
- *                        live degree rounded up by a quarter (x -> x + x/4):
- *                        out-degrees stay within 5/4 of the degeneracy and
+ *                        live degree rounded up by a half (x -> x + x/2):
+ *                        out-degrees stay within 3/2 of the degeneracy and
  *                        the peel takes far fewer steps; when that leaves an out-list wider than
  *                        512 the exact order is used instead.  Every acyclic
  *                        orientation lists each clique exactly once, so all

@@ -262,7 +262,7 @@ __device__ void order_control(OrderCtl *ctl, uint32_t n, uint32_t level, uint32_
       // exact: every live degree now exceeds level, so try level + 1.  Slack:
       // find the true minimum first (nothing is <= level), so every threshold
       // is a slack of a minimum live degree, which never exceeds the
-      // degeneracy: out-degrees stay <= kmax + kmax/4
+      // degeneracy: out-degrees stay <= kmax + kmax/2
       ctl->target = sh < 32 ? level : level + 1;
     }
   } else if (fcur > 0) {
@@ -484,8 +484,8 @@ bool orient_from_rank(Ctx &c, bool approx) {
 }
 
 // GPU orientation: the slack order (each threshold is the minimum live
-// degree rounded up by a quarter, so out-degrees stay within 5/4 of the
-// degeneracy, in a fraction of the steps: C2 104 instead of 609), or the
+// degree rounded up by a half, so out-degrees stay within 3/2 of the
+// degeneracy, in a fraction of the steps: C2 60 instead of 609), or the
 // exact degeneracy order when KCDS_TUNE_EXACT_ORDER is set or the slack order
 // leaves an out-list wider than the listing's LDS rows.
 void orient_gpu(Ctx &c) {
