@@ -675,12 +675,13 @@ void classify_roots(Ctx &c, uint32_t k) {
 // One launch of class cls over its roots; counters accumulate into the
 // ncopy copies of gcnt, rootT and the sharded totals.
 void launch_class(Ctx &c, int cls, uint32_t k, uint32_t nw, uint32_t *queue, ull *gcnt, ull *rootT, ull *totals,
-                  ull cap = 0) {
+                  ull cap = 0, hipStream_t st = nullptr) {
+  if (!st) st = c.stream;
   const uint32_t nr = c.class_count[cls];
   const uint32_t *roots = c.roots.as<uint32_t>() + c.class_start[cls];
   const uint32_t *gid = c.ngraphs > 1 ? c.gid.as<uint32_t>() : nullptr;
   const ull theta = cap / 64 + 1;
-  hipLaunchKernelGGL(class_count_fn(cls, k), dim3(wg_grid(c, cls, nr, nw)), dim3(64 * nw), 0, c.stream,
+  hipLaunchKernelGGL(class_count_fn(cls, k), dim3(wg_grid(c, cls, nr, nw)), dim3(64 * nw), 0, st,
                      c.doff.as<uint32_t>(), c.dnbr.as<uint32_t>(), roots, nr, queue, gcnt, copy_stride(c.n),
                      count_copies(c.n), rootT, totals, gid, cap ? c.gsum.as<ull>() : nullptr, cap, theta);
   KCDS_LAUNCH_CHECK();
@@ -776,11 +777,28 @@ void list_count(Ctx &c, uint32_t k, ull cap) {
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
   HIP_CHECK(hipEventRecord(e0, c.stream));
+  // The classes share only atomic counters, so they run at once: the widest
+  // on the context stream, the others on three more streams (default
+  // priority), each class filling the others' tails.
+  if (!c.lfork) {
+    HIP_CHECK(hipEventCreateWithFlags(&c.lfork, hipEventDisableTiming));
+    for (int i = 0; i < 3; ++i) {
+      HIP_CHECK(hipStreamCreateWithFlags(&c.lstream[i], hipStreamNonBlocking));
+      HIP_CHECK(hipEventCreateWithFlags(&c.ljoin[i], hipEventDisableTiming));
+    }
+  }
+  HIP_CHECK(hipEventRecord(c.lfork, c.stream));
   uint32_t launches = 0;
   for (int cls = NCLASS - 1; cls >= 0; --cls) {
     if (!c.class_count[cls]) continue;
+    hipStream_t st = cls == NCLASS - 1 ? c.stream : c.lstream[cls];
+    if (st != c.stream) HIP_CHECK(hipStreamWaitEvent(st, c.lfork, 0));
     launch_class(c, cls, k, list_waves_per_root(c, cls, k), count_queue(c, cls), c.cnt.as<ull>(), c.rootT.as<ull>(),
-                 totals, cap);
+                 totals, cap, st);
+    if (st != c.stream) {
+      HIP_CHECK(hipEventRecord(c.ljoin[cls], st));
+      HIP_CHECK(hipStreamWaitEvent(c.stream, c.ljoin[cls], 0));
+    }
     ++launches;
   }
   if (ncopy > 1 && n) {

@@ -142,6 +142,11 @@ void kcds_destroy(kcds_ctx *ctx) {
                     &f.g_vmin, &f.g_csum, &f.g_esnap, &f.b_off, &f.b_excess, &f.b_height, &f.b_fr0, &f.b_fr1, &f.b_head, &f.b_rev, &f.b_res, &f.b_tasks};
   for (DevBuf *b : bufs) buf_free(c, *b);
   if (c.pinned) (void)hipHostFree(c.pinned);
+  for (int i = 0; i < 3; ++i) {
+    if (c.lstream[i]) (void)hipStreamDestroy(c.lstream[i]);
+    if (c.ljoin[i]) (void)hipEventDestroy(c.ljoin[i]);
+  }
+  if (c.lfork) (void)hipEventDestroy(c.lfork);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete ctx;
 }

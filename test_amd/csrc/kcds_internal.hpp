@@ -92,6 +92,8 @@ struct FlowBufs {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t lstream[3] = {nullptr, nullptr, nullptr};  // listing classes run beside the context stream
+  hipEvent_t lfork = nullptr, ljoin[3] = {nullptr, nullptr, nullptr};
   uint64_t budget = 0, used = 0;
   int num_cus = 0;
   std::string err;
