@@ -217,7 +217,7 @@ def main():
         traffic, traffic_src = tj.get("traffic_bytes"), tj.get("source")
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
-                "kernel": "k_list_count<W,4> (one launch per width class W=1,2,4,8, back to back)",
+                "kernel": "k_list_count<W,4> (one launch per width class W=1,2,4,8, concurrently on four streams)",
                 "kernel_ms": kms, "bytes_per_pass": bytes_list}
 
     cpu = None

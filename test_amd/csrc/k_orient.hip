@@ -173,6 +173,19 @@ constexpr uint32_t NSEG = 8;  // frontier segments / counter shards (one per wor
 
 // Counters are 64 words apart so that no two share a cache line (same-address
 // device atomics are serialised).
+// Degree counters in a scattered layout: vertex v's counter sits at slot
+// (v * a) mod K, K = n rounded up to a power of two and a odd, so the map is
+// a bijection with inverse p -> (p * ainv) mod K.  R-MAT-like graphs number
+// their hubs close together; packed, the dense core's counters fill a few
+// cache lines, and the decrements of a step that removes much of that core
+// at once queue on those lines.  Scattered, each hot counter has a line to
+// itself.  Padding slots (vertex >= n) hold DEAD.
+struct Scatter {
+  uint32_t mask, a, ainv;
+};
+__device__ __forceinline__ uint32_t slot_of(const Scatter &sc, uint32_t v) { return (v * sc.a) & sc.mask; }
+__device__ __forceinline__ uint32_t vertex_of(const Scatter &sc, uint32_t p) { return (p * sc.ainv) & sc.mask; }
+
 struct OrderCtl {
   uint32_t level, target, next_rank, cur, done, phase;
   uint32_t steps;  // steps taken (sizes the next call's first batch)
@@ -282,7 +295,7 @@ __device__ void order_control(OrderCtl *ctl, uint32_t n, uint32_t level, uint32_
 __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, const uint32_t *__restrict__ off,
                                                  const uint32_t *__restrict__ nbr, uint32_t *deg, uint32_t *fv0,
                                                  uint32_t *fv1, ull *fp0, ull *fp1, uint32_t seg_v, uint32_t seg_p,
-                                                 uint32_t *rank, uint32_t *inv, uint32_t sh) {
+                                                 uint32_t *rank, uint32_t *inv, uint32_t sh, Scatter sc) {
   __shared__ uint32_t s_f[NSEG + 1], s_p[NSEG + 1], s_min, s_last;
   __shared__ uint32_t s_hdr[6], s_sz[2][2][NSEG];
   const uint32_t tid = threadIdx.x, lane = lane_id(), seg = blockIdx.x % NSEG;
@@ -305,9 +318,11 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
     if (tid == 0) s_min = DEAD;
     lds_sync();
     uint32_t m = DEAD;
-    for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < n; v0 += gstride) {
-      const uint32_t v = v0 + tid;
-      const uint32_t d = v < n ? deg[v] : DEAD;
+    const uint32_t K = sc.mask + 1;
+    for (uint32_t p0 = blockIdx.x * blockDim.x; p0 < K; p0 += gstride) {  // slots in order: reads coalesce
+      const uint32_t p = p0 + tid;
+      const uint32_t d = p < K ? deg[p] : DEAD;
+      const uint32_t v = vertex_of(sc, p);
       const bool live = d < DEAD_MIN;
       wave_append(live && d <= target, v, off, fv + (size_t)seg * seg_v, &ctl->fsize[cur][seg][0],
                   fp + (size_t)seg * seg_p, &ctl->psize[cur][seg][0]);
@@ -335,7 +350,7 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
       const uint32_t v = fv[(size_t)g * seg_v + (i - s_f[g])];
       rank[v] = base + i;
       inv[base + i] = v;
-      atomicExch(&deg[v], DEAD);
+      atomicExch(&deg[slot_of(sc, v)], DEAD);
     }
     // wave per adjacency piece of the frontier.  Decrementing a vertex that
     // already left keeps its value >= DEAD_MIN (it loses at most its degree).
@@ -354,7 +369,7 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
 #pragma unroll
       for (uint32_t x = 0; x < PUNROLL; ++x) {
         const uint32_t j = st + x * KCDS_WAVE + lane;
-        dv[x] = j < e ? atomicSub(&deg[u[x]], 1u) : DEAD;
+        dv[x] = j < e ? atomicSub(&deg[slot_of(sc, u[x])], 1u) : DEAD;
       }
 #pragma unroll
       for (uint32_t x = 0; x < PUNROLL; ++x)
@@ -382,9 +397,11 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
   if (s_last && tid < KCDS_WAVE) order_control(ctl, n, level, target, base, cur, phase, sh);
 }
 
-__global__ void k_init_deg(const uint32_t *off, uint32_t n, uint32_t *deg) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
-    deg[v] = off[v + 1] - off[v];
+__global__ void k_init_deg(const uint32_t *off, uint32_t n, Scatter sc, uint32_t *deg) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p <= sc.mask; p += gridDim.x * blockDim.x) {
+    const uint32_t v = vertex_of(sc, p);
+    deg[p] = v < n ? off[v + 1] - off[v] : DEAD;
+  }
 }
 
 }  // namespace
@@ -399,16 +416,25 @@ void degeneracy_rank_gpu(Ctx &c, uint32_t slack_shift) {
   const uint64_t seg_p64 = (((uint64_t)n + c.nnz / PIECE + 64 + 63) / 64) * 64;
   if (seg_p64 * NSEG >= (1ull << 32)) fail(KCDS_E_LIMIT, "graph too large for the ordering frontiers");
   const uint32_t seg_p = (uint32_t)seg_p64;
-  const size_t bytes = sizeof(uint32_t) * ((size_t)n + 2 * NSEG * (size_t)seg_v) +
+  Scatter sc;
+  uint32_t K = 1;
+  while (K < n) K <<= 1;
+  sc.mask = K - 1;
+  sc.a = 0x9E3779B1u;
+  sc.ainv = sc.a;  // Newton iteration for the inverse mod 2^32
+  for (int i = 0; i < 5; ++i) sc.ainv *= 2u - sc.a * sc.ainv;
+  const size_t kslots = ((size_t)K + 63) / 64 * 64;
+  const size_t bytes = sizeof(uint32_t) * (kslots + 2 * NSEG * (size_t)seg_v) +
                        sizeof(ull) * 2 * NSEG * (size_t)seg_p + sizeof(OrderCtl) + 1024;
   buf_reserve(c, c.tmp0, bytes);
   uint32_t *deg = c.tmp0.as<uint32_t>();
-  uint32_t *fv0 = deg + ((n + 63) / 64) * 64, *fv1 = fv0 + (size_t)NSEG * seg_v;
+  uint32_t *fv0 = deg + kslots, *fv1 = fv0 + (size_t)NSEG * seg_v;
   ull *fp0 = (ull *)(((uintptr_t)(fv1 + (size_t)NSEG * seg_v) + 255) & ~(uintptr_t)255);
   ull *fp1 = fp0 + (size_t)NSEG * seg_p;
   OrderCtl *ctl = (OrderCtl *)(((uintptr_t)(fp1 + (size_t)NSEG * seg_p) + 255) & ~(uintptr_t)255);
   const uint32_t vgrid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, c.num_cus));
-  k_init_deg<<<(n + 255) / 256, 256, 0, c.stream>>>(c.off.as<uint32_t>(), n, deg);
+  k_init_deg<<<(uint32_t)std::min<uint64_t>((K + 255) / 256, (uint64_t)c.num_cus * 16), 256, 0, c.stream>>>(
+      c.off.as<uint32_t>(), n, sc, deg);
   KCDS_LAUNCH_CHECK();
   OrderCtl *h0 = (OrderCtl *)c.host_scratch(sizeof(OrderCtl));
   memset(h0, 0, sizeof(OrderCtl));
@@ -424,7 +450,7 @@ void degeneracy_rank_gpu(Ctx &c, uint32_t slack_shift) {
   while (true) {
     for (uint32_t r = 0; r < batch; ++r) {
       k_ko_step<<<vgrid, 256, 0, c.stream>>>(ctl, n, off, nbr, deg, fv0, fv1, fp0, fp1, seg_v, seg_p, rank, inv,
-                                             slack_shift);
+                                             slack_shift, sc);
       KCDS_LAUNCH_CHECK();
     }
     rounds += batch;
