@@ -173,19 +173,11 @@ constexpr uint32_t NSEG = 8;  // frontier segments / counter shards (one per wor
 
 // Counters are 64 words apart so that no two share a cache line (same-address
 // device atomics are serialised).
-// Degree counters in a scattered layout: vertex v's counter sits at slot
-// (v * a) mod K, K = n rounded up to a power of two and a odd, so the map is
-// a bijection with inverse p -> (p * ainv) mod K.  R-MAT-like graphs number
-// their hubs close together; packed, the dense core's counters fill a few
-// cache lines, and the decrements of a step that removes much of that core
-// at once queue on those lines.  Scattered, each hot counter has a line to
-// itself.  Padding slots (vertex >= n) hold DEAD.
-struct Scatter {
-  uint32_t mask, a, ainv;
-};
-__device__ __forceinline__ uint32_t slot_of(const Scatter &sc, uint32_t v) { return (v * sc.a) & sc.mask; }
-__device__ __forceinline__ uint32_t vertex_of(const Scatter &sc, uint32_t p) { return (p * sc.ainv) & sc.mask; }
-
+// The degree counters use the scattered layout of kcds_device.hpp (Scatter):
+// R-MAT-like graphs number their hubs close together, and packed, the dense
+// core's counters would fill a few cache lines on which the decrements of a
+// step that removes much of that core at once would queue.  Padding slots
+// (vertex >= n) hold DEAD.
 struct OrderCtl {
   uint32_t level, target, next_rank, cur, done, phase;
   uint32_t steps;  // steps taken (sizes the next call's first batch)
@@ -416,13 +408,8 @@ void degeneracy_rank_gpu(Ctx &c, uint32_t slack_shift) {
   const uint64_t seg_p64 = (((uint64_t)n + c.nnz / PIECE + 64 + 63) / 64) * 64;
   if (seg_p64 * NSEG >= (1ull << 32)) fail(KCDS_E_LIMIT, "graph too large for the ordering frontiers");
   const uint32_t seg_p = (uint32_t)seg_p64;
-  Scatter sc;
-  uint32_t K = 1;
-  while (K < n) K <<= 1;
-  sc.mask = K - 1;
-  sc.a = 0x9E3779B1u;
-  sc.ainv = sc.a;  // Newton iteration for the inverse mod 2^32
-  for (int i = 0; i < 5; ++i) sc.ainv *= 2u - sc.a * sc.ainv;
+  const Scatter sc = make_scatter(n);
+  const uint32_t K = sc.mask + 1;
   const size_t kslots = ((size_t)K + 63) / 64 * 64;
   const size_t bytes = sizeof(uint32_t) * (kslots + 2 * NSEG * (size_t)seg_v) +
                        sizeof(ull) * 2 * NSEG * (size_t)seg_p + sizeof(OrderCtl) + 1024;

@@ -163,7 +163,8 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
                                                  const ull *__restrict__ incoff, const uint32_t *__restrict__ inc,
                                                  ull *cnt, int32_t *core, int32_t *status, uint32_t *fv0, uint32_t *fv1,
                                                  PItem *fp0, PItem *fp1, uint32_t seg_v, uint32_t seg_p,
-                                                 GraphPeel *gp, const uint32_t *__restrict__ gid, uint32_t *tlist) {
+                                                 GraphPeel *gp, const uint32_t *__restrict__ gid, uint32_t *tlist,
+                                                 Scatter sc) {
   __shared__ ull s_hdr[2];
   __shared__ uint32_t s_hdr2[3], s_f[PSEG + 1], s_p[PSEG + 1], s_last, s_sz[2][2][PSEG];
   __shared__ ull s_min;
@@ -197,10 +198,11 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
     if (tid == 0) s_min = ~0ull;
     lds_sync();
     ull m = ~0ull;
-    for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < n; v0 += gstride) {
-      const uint32_t v = v0 + tid;
-      const bool live = v < n && core[v] == -1;
-      const ull cv = live ? cnt[v] : ~0ull;
+    const uint32_t K = sc.mask + 1;
+    for (uint32_t p0 = blockIdx.x * blockDim.x; p0 < K; p0 += gstride) {  // counter slots in order
+      const uint32_t p = p0 + tid, v = vertex_of(sc, p);
+      const bool live = p < K && v < n && core[v] == -1;
+      const ull cv = live ? cnt[p] : ~0ull;
       pl_append(live && cv <= target, v, incoff, fv + (size_t)seg * seg_v, &ctl->fsize[cur][seg][0],
                 fp + (size_t)seg * seg_p, &ctl->psize[cur][seg][0]);
       if (live && cv > target) m = min(m, cv);
@@ -273,7 +275,7 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
           // each lane's own old value is the leader's minus its rank
           const uint32_t gsize = wave_group(dec[t], u[t], rank[t], leader[t]);
           old[t] = 0;
-          if (gsize) old[t] = atomicSub(&cnt[u[t]], (ull)gsize);
+          if (gsize) old[t] = atomicSub(&cnt[slot_of(sc, u[t])], (ull)gsize);
         }
 #pragma unroll
         for (uint32_t t = 0; t < KMAX; ++t) {
@@ -332,10 +334,13 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
   }
 }
 
-__global__ void k_peel_init(uint32_t n, uint64_t T, const ull *cnt0, ull *cnt, int32_t *core, int32_t *status) {
+// cnt: the working counts, in the scattered layout of sc when scattered
+// (grid peel), else by vertex
+__global__ void k_peel_init(uint32_t n, uint64_t T, const ull *cnt0, ull *cnt, int32_t *core, int32_t *status,
+                            Scatter sc, uint32_t scattered) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    cnt[i] = cnt0[i];
+    cnt[scattered ? slot_of(sc, (uint32_t)i) : i] = cnt0[i];
     core[i] = -1;
   }
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += stride) status[i] = -1;
@@ -689,7 +694,8 @@ void peel_blocks(Ctx &c, const std::vector<uint32_t> &ids) {
   HIP_CHECK(hipMemsetAsync(queue, 0, QUEUE_WORDS * sizeof(uint32_t), c.stream));
   HIP_CHECK(hipMemcpyAsync(glist, list.data(), sizeof(uint32_t) * L, hipMemcpyHostToDevice, c.stream));
   if (ids.empty() && T) {  // every graph: reset all clique statuses at once
-    k_peel_init<<<grid1d(T), 256, 0, c.stream>>>(0, T, nullptr, nullptr, nullptr, c.status.as<int32_t>());
+    k_peel_init<<<grid1d(T), 256, 0, c.stream>>>(0, T, nullptr, nullptr, nullptr, c.status.as<int32_t>(),
+                                                 make_scatter(1), 0u);
     KCDS_LAUNCH_CHECK();
   }
   const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(L, (uint32_t)c.num_cus * 2));
@@ -747,11 +753,13 @@ void peel_run(Ctx &c) {
   const uint64_t seg_p64 = (((uint64_t)n + (T * k) / PPIECE + 64 + 63) / 64) * 64;
   if (seg_p64 * PSEG >= (1ull << 32)) fail(KCDS_E_LIMIT, "too many cliques for the peeling frontiers");
   const uint32_t seg_p = (uint32_t)seg_p64;
-  const size_t bytes = sizeof(ull) * (size_t)seg_v + sizeof(uint32_t) * 2 * PSEG * (size_t)seg_v +
+  const Scatter sc = make_scatter(n);  // working counts: hub counters on lines of their own
+  const size_t kslots = ((size_t)sc.mask + 1 + 63) / 64 * 64;
+  const size_t bytes = sizeof(ull) * kslots + sizeof(uint32_t) * 2 * PSEG * (size_t)seg_v +
                        sizeof(PItem) * 2 * PSEG * (size_t)seg_p + sizeof(PeelCtl) + 1024;
   buf_reserve(c, c.tmp0, bytes);
   ull *cnt = c.tmp0.as<ull>();
-  uint32_t *fv0 = (uint32_t *)(cnt + seg_v), *fv1 = fv0 + (size_t)PSEG * seg_v;
+  uint32_t *fv0 = (uint32_t *)(cnt + kslots), *fv1 = fv0 + (size_t)PSEG * seg_v;
   PItem *fp0 = (PItem *)(((uintptr_t)(fv1 + (size_t)PSEG * seg_v) + 255) & ~(uintptr_t)255);
   PItem *fp1 = fp0 + (size_t)PSEG * seg_p;
   PeelCtl *ctl = (PeelCtl *)(((uintptr_t)(fp1 + (size_t)PSEG * seg_p) + 255) & ~(uintptr_t)255);
@@ -762,7 +770,8 @@ void peel_run(Ctx &c) {
   HIP_CHECK(hipMemcpyAsync(ctl, h0, sizeof(PeelCtl), hipMemcpyHostToDevice, c.stream));
   c.sync();  // h0 is reused below
   const uint32_t grid = grid1d(std::max<uint64_t>(n, T));
-  k_peel_init<<<grid, 256, 0, c.stream>>>(n, T, c.cnt.as<ull>(), cnt, c.core.as<int32_t>(), c.status.as<int32_t>());
+  k_peel_init<<<grid, 256, 0, c.stream>>>(n, T, c.cnt.as<ull>(), cnt, c.core.as<int32_t>(), c.status.as<int32_t>(),
+                                          sc, 1u);
   KCDS_LAUNCH_CHECK();
   if (!small_ids.empty() && n) {
     buf_reserve(c, c.tmp3, (size_t)G + 8);
@@ -790,7 +799,7 @@ void peel_run(Ctx &c) {
   while (n) {
     for (uint32_t r = 0; r < batch; ++r) {
       k_pl_step<<<sgrid, 256, 0, c.stream>>>(ctl, n, k, cl, incoff, inc, cnt, core, status, fv0, fv1, fp0, fp1, seg_v,
-                                             seg_p, gp, gidp, tl);
+                                             seg_p, gp, gidp, tl, sc);
       KCDS_LAUNCH_CHECK();
     }
     rounds += batch;

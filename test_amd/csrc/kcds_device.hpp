@@ -36,6 +36,27 @@ __device__ __forceinline__ void wave_drain() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// Scattered layout for per-vertex counters that take many atomics: vertex
+// v's counter sits at slot (v * a) mod K (K = n rounded up to a power of two,
+// a odd, so the map is a bijection; inverse p -> (p * ainv) mod K).  Hot
+// vertices with close ids then get cache lines of their own instead of
+// queueing their atomics on a few shared lines.
+struct Scatter {
+  uint32_t mask, a, ainv;
+};
+__device__ __forceinline__ uint32_t slot_of(const Scatter &sc, uint32_t v) { return (v * sc.a) & sc.mask; }
+__device__ __forceinline__ uint32_t vertex_of(const Scatter &sc, uint32_t p) { return (p * sc.ainv) & sc.mask; }
+inline Scatter make_scatter(uint32_t n) {
+  Scatter sc;
+  uint32_t K = 1;
+  while (K < n) K <<= 1;
+  sc.mask = K - 1;
+  sc.a = 0x9E3779B1u;
+  sc.ainv = sc.a;  // Newton iteration for the inverse mod 2^32
+  for (int i = 0; i < 5; ++i) sc.ainv *= 2u - sc.a * sc.ainv;
+  return sc;
+}
+
 // Workgroup barrier for data exchanged through LDS only: it waits for the
 // wave's LDS operations but leaves its global loads, stores and atomics in
 // flight (a plain __syncthreads also drains those).
