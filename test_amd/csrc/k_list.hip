@@ -342,7 +342,7 @@ __device__ __forceinline__ bool admit(ull *gsum, uint32_t g, ull E, ull &pend, u
 template <int W, int K>
 __global__ __launch_bounds__(64 * MAX_WAVES) void k_list_count(const uint32_t *__restrict__ doff,
                                                                const uint32_t *__restrict__ dnbr,
-                                                               const uint32_t *__restrict__ roots, uint32_t nroots,
+                                                               const uint4 *__restrict__ roots, uint32_t nroots,
                                                                uint32_t *queue, ull *gcnt, uint64_t cstride,
                                                                uint32_t ncopy, ull *rootT, ull *gtotal,
                                                                const uint32_t *gid, ull *gsum, ull cap, ull theta) {
@@ -366,7 +366,7 @@ __global__ __launch_bounds__(64 * MAX_WAVES) void k_list_count(const uint32_t *_
     if (tid == 0) {
       f.root = pre;
       uint32_t sk = 0;
-      if (cap && pre < nroots) sk = ldc_u64(&gsum[gid ? gid[roots[pre]] : 0u]) > cap;
+      if (cap && pre < nroots) sk = ldc_u64(&gsum[gid ? gid[roots[pre].x] : 0u]) > cap;
       f.skip = sk;
     }
     lds_sync();
@@ -375,7 +375,8 @@ __global__ __launch_bounds__(64 * MAX_WAVES) void k_list_count(const uint32_t *_
     if (idx >= nroots) break;
     if (tid == 0) pre = queue_next(queue, nroots, qc);  // in flight while this root is processed
     if (skip) continue;
-    const uint32_t r = roots[idx];
+    const uint4 rec = roots[idx];  // root, its out-list start and out-degree: one round trip
+    const uint32_t r = rec.x, base = rec.y, d = rec.z;
     uint32_t g = 0;
     if (cap) {
       g = gid ? gid[r] : 0u;
@@ -384,7 +385,6 @@ __global__ __launch_bounds__(64 * MAX_WAVES) void k_list_count(const uint32_t *_
         pg = g;
       }
     }
-    const uint32_t base = doff[r], d = doff[r + 1] - base;
     w2 += build_rows<W>(f, doff, dnbr, base, d);
     ull cnt[W];
 #pragma unroll
@@ -459,7 +459,7 @@ __global__ void k_sum_copies(ull *cnt, uint64_t n, uint64_t cstride, uint32_t nc
 template <int W, int K>
 __global__ __launch_bounds__(64) void k_list_emit(const uint32_t *__restrict__ doff,
                                                   const uint32_t *__restrict__ dnbr,
-                                                  const uint32_t *__restrict__ roots, uint32_t nroots,
+                                                  const uint4 *__restrict__ roots, uint32_t nroots,
                                                   uint32_t *queue, const ull *rootOff, uint32_t *out) {
   __shared__ Frame<W> f;
   QueueCursor qc = queue_start(blockIdx.x);
@@ -468,11 +468,11 @@ __global__ __launch_bounds__(64) void k_list_emit(const uint32_t *__restrict__ d
     const uint32_t idx = share_root(f, pre);
     if (idx >= nroots) break;
     if (threadIdx.x == 0) pre = queue_next(queue, nroots, qc);
-    const uint32_t r = roots[idx];
+    const uint4 rec = roots[idx];
+    const uint32_t r = rec.x, base = rec.y, d = rec.z;
     ull pos = rootOff[r];
     const ull lim = rootOff[r + 1];
     if (lim == pos) continue;
-    const uint32_t base = doff[r], d = doff[r + 1] - base;
     build_rows<W>(f, doff, dnbr, base, d);
     ull S[W];
     all_bits<W>(S, d);
@@ -513,14 +513,16 @@ __global__ __launch_bounds__(256) void k_class_count(const uint32_t *doff, uint3
 // Writes each root into its class's range: one global atomic per class per
 // 256 roots.
 __global__ __launch_bounds__(256) void k_class_fill(const uint32_t *doff, uint32_t n, uint32_t k,
-                                                    const uint32_t *cstart, uint32_t *cfill, uint32_t *roots) {
+                                                    const uint32_t *cstart, uint32_t *cfill, uint4 *roots) {
   __shared__ uint32_t wc[4][NCLASS], bbase[NCLASS];
   const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
   for (uint32_t r0 = blockIdx.x * 256; r0 < n; r0 += gridDim.x * 256) {
     const uint32_t r = r0 + threadIdx.x;
     int c = -1;
+    uint32_t b = 0, d = 0;
     if (r < n) {
-      const uint32_t d = doff[r + 1] - doff[r];
+      b = doff[r];
+      d = doff[r + 1] - b;
       if (d + 1 >= k) c = width_class(d);
     }
     ull bal[NCLASS];
@@ -539,7 +541,7 @@ __global__ __launch_bounds__(256) void k_class_fill(const uint32_t *doff, uint32
     if (c >= 0 && c < NCLASS) {
       uint32_t at = bbase[c] + lanes_below(bal[c]);
       for (uint32_t v = 0; v < wv; ++v) at += wc[v][c];
-      roots[at] = r;
+      roots[at] = make_uint4(r, b, d, 0u);
     }
     __syncthreads();
   }
@@ -573,9 +575,9 @@ __global__ __launch_bounds__(256) void k_incidence(const uint32_t *cl, uint64_t 
   }
 }
 
-typedef void (*CountFn)(const uint32_t *, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, ull *, uint64_t,
+typedef void (*CountFn)(const uint32_t *, const uint32_t *, const uint4 *, uint32_t, uint32_t *, ull *, uint64_t,
                         uint32_t, ull *, ull *, const uint32_t *, ull *, ull, ull);
-typedef void (*EmitFn)(const uint32_t *, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, const ull *,
+typedef void (*EmitFn)(const uint32_t *, const uint32_t *, const uint4 *, uint32_t, uint32_t *, const ull *,
                        uint32_t *);
 
 template <int W> CountFn count_fn(uint32_t k) {
@@ -642,7 +644,7 @@ uint64_t copy_stride(uint32_t n) { return (((uint64_t)n + 1 + 63) / 64) * 64; }
 
 void classify_roots(Ctx &c, uint32_t k) {
   const uint32_t n = c.n;
-  buf_reserve(c, c.roots, sizeof(uint32_t) * ((size_t)n + 1));
+  buf_reserve(c, c.roots, sizeof(uint4) * ((size_t)n + 1));
   buf_reserve(c, c.qctr, QCTR_WORDS * sizeof(uint32_t));
   uint32_t *q = c.qctr.as<uint32_t>();
   HIP_CHECK(hipMemsetAsync(q, 0, QCTR_WORDS * sizeof(uint32_t), c.stream));
@@ -666,7 +668,7 @@ void classify_roots(Ctx &c, uint32_t k) {
   }
   HIP_CHECK(hipMemcpyAsync(cstart, hs, 8 * sizeof(uint32_t), hipMemcpyHostToDevice, c.stream));
   if (n && s) {
-    k_class_fill<<<grid, 256, 0, c.stream>>>(c.doff.as<uint32_t>(), n, k, cstart, cfill, c.roots.as<uint32_t>());
+    k_class_fill<<<grid, 256, 0, c.stream>>>(c.doff.as<uint32_t>(), n, k, cstart, cfill, c.roots.as<uint4>());
     KCDS_LAUNCH_CHECK();
   }
   c.sync();  // hs lives on the host stack
@@ -678,7 +680,7 @@ void launch_class(Ctx &c, int cls, uint32_t k, uint32_t nw, uint32_t *queue, ull
                   ull cap = 0, hipStream_t st = nullptr) {
   if (!st) st = c.stream;
   const uint32_t nr = c.class_count[cls];
-  const uint32_t *roots = c.roots.as<uint32_t>() + c.class_start[cls];
+  const uint4 *roots = c.roots.as<uint4>() + c.class_start[cls];
   const uint32_t *gid = c.ngraphs > 1 ? c.gid.as<uint32_t>() : nullptr;
   const ull theta = cap / 64 + 1;
   hipLaunchKernelGGL(class_count_fn(cls, k), dim3(wg_grid(c, cls, nr, nw)), dim3(64 * nw), 0, st,
@@ -887,7 +889,7 @@ void list_materialize(Ctx &c, uint32_t k) {
     const uint32_t nr = c.class_count[cls];
     if (!nr) continue;
     EmitFn fn = cls == 0 ? emit_fn<1>(k) : cls == 1 ? emit_fn<2>(k) : cls == 2 ? emit_fn<4>(k) : emit_fn<8>(k);
-    const uint32_t *roots = c.roots.as<uint32_t>() + c.class_start[cls];
+    const uint4 *roots = c.roots.as<uint4>() + c.class_start[cls];
     hipLaunchKernelGGL(fn, dim3(wg_grid(c, cls, nr, 1)), dim3(64), 0, c.stream, c.doff.as<uint32_t>(),
                        c.dnbr.as<uint32_t>(), roots, nr, emit_queue(c, cls), rootOff, c.cliques.as<uint32_t>());
     KCDS_LAUNCH_CHECK();
