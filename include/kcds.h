@@ -170,6 +170,10 @@ int kcds_sync(kcds_ctx *ctx);
  *                        orientation lists each clique exactly once, so all
  *                        results are the same either way. */
 #define KCDS_TUNE_EXACT_ORDER 4u
+/*   KCDS_TUNE_NO_GRAPHS  launch the grid-wide max-flow's relabel and sweep
+ *                        sequences one kernel at a time instead of replaying
+ *                        them as captured HIP graphs (same results). */
+#define KCDS_TUNE_NO_GRAPHS 8u
 int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
 
 /* Listing launch shape: waves per root (workgroup size / 64, 1..8) of each

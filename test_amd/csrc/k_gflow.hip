@@ -39,6 +39,7 @@ typedef long long i64;
 
 constexpr uint32_t NSEG = 8;
 constexpr uint32_t SWEEPS = 24;
+constexpr uint32_t RELABEL_BATCH = 16;  // BFS levels per host check
 constexpr uint32_t MAX_PHASES = 1u << 14;
 constexpr uint32_t NO_ARC = 0xFFFFFFFFu;
 constexpr uint32_t VCH = 1024;  // arcs of a vertex node per work item (a wave, 4 x 64 arcs per step)
@@ -711,6 +712,46 @@ void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t m
   uint32_t flows = 0;
   i64 flow = 0;
   ull sn = 0, sT = 0;
+  // The relabel and sweep sequences of a phase are the same launches every
+  // phase (all state lives on the device), so they are captured once as HIP
+  // graphs and replayed: 36 launches per relabel batch and 97 per sweep
+  // batch become one graph launch each.  No memset node is recorded; the
+  // active counters are cleared by a fill kernel.
+  const auto relabel_batch = [&](bool first) {
+    if (first) {
+      k_gf_bfs_init<<<ggrid(c, Nn), 256, 0, c.stream>>>(N, height, ctl);
+      k_gf_bfs_seed<<<ggrid(c, nc), 256, 0, c.stream>>>(nc, off, res, height, fr0, seg_cap, ctl);
+    } else {  // the active count of the unfinished batch was taken too early
+      k_gf_fill_u32<<<1, 64, 0, c.stream>>>((uint32_t *)&ctl->active[0][0], 0u, sizeof(ctl->active) / 4);
+    }
+    for (uint32_t i = 0; i < RELABEL_BATCH; ++i) {
+      k_gf_bfs_vchunks<<<vgrid, 256, 0, c.stream>>>(N, nch, choff, chv, off, head, rev, res, height, fr0, fr1,
+                                                    seg_cap, ctl);
+      k_gf_bfs_level<<<bgrid, 256, 0, c.stream>>>(N, nc, k, off, head, rev, res, height, fr0, fr1, seg_cap, ctl);
+    }
+    k_gf_active<<<ggrid(c, Nn), 256, 0, c.stream>>>(N, excess, height, ctl);
+    KCDS_LAUNCH_CHECK();
+  };
+  const auto sweeps = [&]() {
+    k_gf_fill_u32<<<ggrid(c, 2ull * nc), 256, 0, c.stream>>>(vmin, NO_ARC, 2ull * nc);
+    for (uint32_t sw = 0; sw < SWEEPS; ++sw) {
+      uint32_t *vm = vmin + (size_t)(sw & 1) * nc, *vn = vmin + (size_t)((sw + 1) & 1) * nc;
+      k_gf_vmin<<<vgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, res, excess, height, vm, ctl);
+      k_gf_vsum<<<vgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, res, excess, height, vm, csum, esnap,
+                                             ctl);
+      k_gf_vpush<<<vgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, rev, res, excess, height, vm, vn,
+                                              csum, esnap, ctl);
+      k_gf_sweep_c<<<ggrid(c, Tc), 256, 0, c.stream>>>(nc, N, sw, off, head, rev, res, excess, height, ctl);
+    }
+    KCDS_LAUNCH_CHECK();
+  };
+  const bool graphs = !(c.tune_flags & KCDS_TUNE_NO_GRAPHS);
+  StreamGraph g_first(c), g_next(c), g_sweeps(c);
+  if (graphs) {
+    g_first.capture([&]() { relabel_batch(true); });
+    g_next.capture([&]() { relabel_batch(false); });
+    g_sweeps.capture(sweeps);
+  }
   for (uint32_t it = 0;; ++it) {
     k_gf_init<<<ggrid(c, std::max<uint64_t>(Nn, Tc * k)), 256, 0, c.stream>>>(nc, Tc, k, (i64)p, (i64)q, off, rev, res,
                                                                              excess);
@@ -718,42 +759,25 @@ void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t m
     uint32_t phase = 0;
     for (;; ++phase) {
       if (phase >= MAX_PHASES) fail(KCDS_E_LIMIT, "grid max-flow of component %u exceeded %u phases", d, MAX_PHASES);
-      k_gf_bfs_init<<<ggrid(c, Nn), 256, 0, c.stream>>>(N, height, ctl);
-      k_gf_bfs_seed<<<ggrid(c, nc), 256, 0, c.stream>>>(nc, off, res, height, fr0, seg_cap, ctl);
-      KCDS_LAUNCH_CHECK();
       uint32_t levels = 0;
-      while (true) {
-        const uint32_t batch = 16;
-        for (uint32_t i = 0; i < batch; ++i) {
-          k_gf_bfs_vchunks<<<vgrid, 256, 0, c.stream>>>(N, nch, choff, chv, off, head, rev, res, height, fr0, fr1,
-                                                        seg_cap, ctl);
-          k_gf_bfs_level<<<bgrid, 256, 0, c.stream>>>(N, nc, k, off, head, rev, res, height, fr0, fr1, seg_cap, ctl);
-        }
-        KCDS_LAUNCH_CHECK();
-        levels += batch;
-        k_gf_active<<<ggrid(c, Nn), 256, 0, c.stream>>>(N, excess, height, ctl);
-        KCDS_LAUNCH_CHECK();
+      for (bool first = true;; first = false) {
+        if (graphs)
+          (first ? g_first : g_next).launch();
+        else
+          relabel_batch(first);
+        levels += RELABEL_BATCH;
         HIP_CHECK(hipMemcpyAsync(h, ctl, sizeof(GFCtl), hipMemcpyDeviceToHost, c.stream));
         c.sync();
         if (h->done) break;
         if (levels > N + 16) fail(KCDS_E_HIP, "relabel BFS of component %u did not finish", d);
-        // not finished: the active count was taken too early; clear it
-        HIP_CHECK(hipMemsetAsync(&ctl->active[0][0], 0, sizeof(ctl->active), c.stream));
       }
       ull active = 0;
       for (uint32_t s = 0; s < NSEG; ++s) active += h->active[s][0];
       if (active == 0) break;
-      k_gf_fill_u32<<<ggrid(c, 2ull * nc), 256, 0, c.stream>>>(vmin, NO_ARC, 2ull * nc);
-      for (uint32_t sw = 0; sw < SWEEPS; ++sw) {
-        uint32_t *vm = vmin + (size_t)(sw & 1) * nc, *vn = vmin + (size_t)((sw + 1) & 1) * nc;
-        k_gf_vmin<<<vgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, res, excess, height, vm, ctl);
-        k_gf_vsum<<<vgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, res, excess, height, vm, csum, esnap,
-                                               ctl);
-        k_gf_vpush<<<vgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, rev, res, excess, height, vm, vn,
-                                                csum, esnap, ctl);
-        k_gf_sweep_c<<<ggrid(c, Tc), 256, 0, c.stream>>>(nc, N, sw, off, head, rev, res, excess, height, ctl);
-      }
-      KCDS_LAUNCH_CHECK();
+      if (graphs)
+        g_sweeps.launch();
+      else
+        sweeps();
     }
     ++flows;
     HIP_CHECK(hipMemsetAsync(&ctl->sums[0][0], 0, sizeof(ctl->sums), c.stream));

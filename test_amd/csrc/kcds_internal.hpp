@@ -46,6 +46,17 @@ struct DevBuf {
 };
 
 struct Ctx;
+// A launch sequence on the context stream captured once as a HIP graph and
+// replayed; freed with the object.  capture() ends the capture (and
+// rethrows) if a launch in it fails.
+struct StreamGraph {
+  Ctx &c;
+  hipGraphExec_t exec = nullptr;
+  explicit StreamGraph(Ctx &cc) : c(cc) {}
+  ~StreamGraph();
+  template <typename F> void capture(F &&launches);
+  void launch();
+};
 void buf_reserve(Ctx &c, DevBuf &b, size_t bytes);  // grows, never shrinks; contents lost on growth
 void buf_free(Ctx &c, DevBuf &b);
 
@@ -240,6 +251,22 @@ void dinkelbach_components(Ctx &c, const std::vector<BlockTask> &in, std::vector
 void graph_ids(Ctx &c);
 void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_graph, uint8_t *in_set);
 void graph_ids(Ctx &c);
+
+template <typename F> void StreamGraph::capture(F &&launches) {
+  HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  hipGraph_t g = nullptr;
+  try {
+    launches();
+  } catch (...) {
+    (void)hipStreamEndCapture(c.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  HIP_CHECK(hipStreamEndCapture(c.stream, &g));
+  const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIP_CHECK(e);
+}
 
 }  // namespace kcds
 

@@ -27,6 +27,13 @@
 
 namespace kcds {
 
+StreamGraph::~StreamGraph() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+}
+
+void StreamGraph::launch() { HIP_CHECK(hipGraphLaunch(exec, c.stream)); }
+
+
 void check_partition(Ctx &c);
 
 namespace {

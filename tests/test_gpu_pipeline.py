@@ -87,11 +87,18 @@ def _peel_and_components(solver, name, off, nbr, ks):
         assert ncomp == len(set(int(x) for x in comp_r if x != 0xFFFFFFFF))
 
 
-@pytest.mark.parametrize("grid_flow", [False, True], ids=["block_flow", "grid_flow"])
+# block_flow: the workgroup max-flow; grid_flow: the grid-wide one, its
+# relabel and sweep sequences replayed as captured HIP graphs (many replays
+# per flow, with host reads and syncs between them); grid_flow_eager: the
+# same launches one at a time
+FLOWS = {"block_flow": 0, "grid_flow": 2, "grid_flow_eager": 2 | 8}
+
+
+@pytest.mark.parametrize("flow", list(FLOWS))
 @pytest.mark.parametrize("name,off,nbr,ks", GRAPHS, ids=[g[0] for g in GRAPHS])
-def test_maxflow_values(solver, name, off, nbr, ks, grid_flow):
+def test_maxflow_values(solver, name, off, nbr, ks, flow):
     """Max-flow values and maximal min-cut sides of the workgroup and the grid solver."""
-    solver.set_tuning(0, 2 if grid_flow else 0)
+    solver.set_tuning(0, FLOWS[flow])
     try:
         _maxflow_values(solver, name, off, nbr, ks)
     finally:
@@ -124,10 +131,10 @@ def _maxflow_values(solver, name, off, nbr, ks):
             assert got == exp, (cid, p, q)
 
 
-@pytest.mark.parametrize("grid_flow", [False, True], ids=["block_flow", "grid_flow"])
+@pytest.mark.parametrize("flow", list(FLOWS))
 @pytest.mark.parametrize("name,off,nbr,ks", GRAPHS, ids=[g[0] for g in GRAPHS])
-def test_solve_matches_oracle(solver, name, off, nbr, ks, grid_flow):
-    solver.set_tuning(0, 2 if grid_flow else 0)
+def test_solve_matches_oracle(solver, name, off, nbr, ks, flow):
+    solver.set_tuning(0, FLOWS[flow])
     try:
         _solve_matches(solver, name, off, nbr, ks)
     finally:
