@@ -34,7 +34,7 @@ typedef struct kcds_ctx kcds_ctx;
 
 typedef enum kcds_status {
   KCDS_OK = 0,
-  KCDS_E_INVALID = -1, /* bad argument: NULL pointer, k < 3, not a permutation, ... */
+  KCDS_E_INVALID = -1, /* bad argument: NULL pointer, k outside 2..8, not a permutation, ... */
   KCDS_E_HIP = -2,     /* HIP runtime error (message in kcds_last_error) */
   KCDS_E_NOMEM = -3,   /* HBM budget of the context exceeded or allocation failed */
   KCDS_E_STATE = -4,   /* call out of order, e.g. kcds_count before kcds_orient */

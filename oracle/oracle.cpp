@@ -611,7 +611,7 @@ int64_t oracle_clique_maxflow(uint32_t nc, uint32_t k, uint64_t Tc, const uint32
 int oracle_solve(uint32_t n, const uint32_t *offset, const uint32_t *nbr, uint32_t k,
                  oracle_result *res, int64_t *per_vertex, int32_t *core_out, uint32_t *comp_out,
                  uint8_t *in_set) {
-  if (k < 3) return -1;
+  if (k < 2) return -1;  // k = 2: edge density (main.cu:1365-1423)
   oracle_result R;
   memset(&R, 0, sizeof R);
   // listing (main.cu:93-327)

@@ -575,6 +575,46 @@ __global__ __launch_bounds__(256) void k_incidence(const uint32_t *cl, uint64_t 
   }
 }
 
+// k = 2 (edge density: main.cu:1365-1423 with writeEdgeCliques,
+// helpers.cu:552-585): the 2-cliques are the DAG arcs.  cnt[r] = degree,
+// rootT[r] = out-degree, cliques row a = (r, dnbr[a]) for arc a of root r.
+__global__ void k_edge_out(const uint32_t *doff, uint32_t n, ull *cnt, ull *rootT) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const ull d = doff[r + 1] - doff[r];
+    cnt[r] = d;
+    rootT[r] = d;
+  }
+}
+
+// in-degrees: rounds of INC_E * 256 arcs, one atomic per distinct head per round
+__global__ __launch_bounds__(256) void k_edge_in(const uint32_t *dnbr, uint64_t arcs, ull *cnt) {
+  __shared__ KeyTable<INC_TB> tb;
+  const uint64_t per = (uint64_t)INC_E * 256;
+  for (uint64_t r0 = (uint64_t)blockIdx.x * per; r0 < arcs; r0 += (uint64_t)gridDim.x * per) {
+    bool want[INC_E];
+    uint32_t v[INC_E];
+    ull slot[INC_E];
+#pragma unroll
+    for (int e = 0; e < INC_E; ++e) {
+      const uint64_t i = r0 + (uint64_t)e * 256 + threadIdx.x;
+      want[e] = i < arcs;
+      v[e] = want[e] ? dnbr[i] : 0u;
+    }
+    block_slot_by_key<INC_E, INC_TB, ull>(tb, want, v, slot, cnt);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_edge_emit(const uint32_t *doff, const uint32_t *dnbr, uint32_t n,
+                                                   uint32_t *cl) {
+  const uint32_t lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwv = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t r = gw; r < n; r += nwv)
+    for (uint32_t a = doff[r] + lane; a < doff[r + 1]; a += KCDS_WAVE) {
+      cl[2ull * a] = r;
+      cl[2ull * a + 1] = dnbr[a];
+    }
+}
+
 typedef void (*CountFn)(const uint32_t *, const uint32_t *, const uint4 *, uint32_t, uint32_t *, ull *, uint64_t,
                         uint32_t, ull *, ull *, const uint32_t *, ull *, ull, ull);
 typedef void (*EmitFn)(const uint32_t *, const uint32_t *, const uint4 *, uint32_t, uint32_t *, const ull *,
@@ -715,7 +755,11 @@ uint32_t list_waves_per_root(const Ctx &c, int cls, uint32_t k) {
 // current graph (results of the trial runs go to scratch buffers).
 void list_autotune(Ctx &c, uint32_t k) {
   if (!c.oriented) fail(KCDS_E_STATE, "autotune before kcds_orient");
-  if (k < 3 || k > 8) fail(KCDS_E_INVALID, "k = %u outside 3..8", k);
+  if (k < 2 || k > 8) fail(KCDS_E_INVALID, "k = %u outside 2..8", k);
+  if (k == 2) {  // no listing kernels to shape
+    for (int cls = 0; cls < NCLASS; ++cls) c.tune_ms[cls] = 0;
+    return;
+  }
   const uint32_t n = c.n;
   classify_roots(c, k);
   const uint64_t cw = copy_stride(n) * count_copies(n);
@@ -756,9 +800,53 @@ void list_autotune(Ctx &c, uint32_t k) {
   HIP_CHECK(hipEventDestroy(e1));
 }
 
+// k = 2: T = arcs, per-vertex counts = degrees (no listing kernels)
+void list_count_edges(Ctx &c) {
+  const uint32_t n = c.n;
+  buf_reserve(c, c.cnt, sizeof(ull) * ((size_t)n + 1));
+  buf_reserve(c, c.rootT, sizeof(ull) * ((size_t)n + 2));
+  HIP_CHECK(hipMemsetAsync(c.rootT.p, 0, sizeof(ull) * ((size_t)n + 2), c.stream));
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  HIP_CHECK(hipEventRecord(e0, c.stream));
+  if (n) {
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)c.num_cus * 8);
+    k_edge_out<<<grid, 256, 0, c.stream>>>(c.doff.as<uint32_t>(), n, c.cnt.as<ull>(), c.rootT.as<ull>());
+    KCDS_LAUNCH_CHECK();
+  }
+  if (c.darcs) {
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>((c.darcs + INC_E * 256 - 1) / (INC_E * 256), (uint64_t)c.num_cus * 2));
+    k_edge_in<<<grid, 256, 0, c.stream>>>(c.dnbr.as<uint32_t>(), c.darcs, c.cnt.as<ull>());
+    KCDS_LAUNCH_CHECK();
+  }
+  HIP_CHECK(hipEventRecord(e1, c.stream));
+  HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  HIP_CHECK(hipEventDestroy(e0));
+  HIP_CHECK(hipEventDestroy(e1));
+  for (int i = 0; i < NCLASS; ++i) c.class_count[i] = 0;
+  c.list_kernel_ms = ms;
+  c.list_kernel_launches = 2;
+  c.count_cap = 0;
+  c.T = c.darcs;
+  c.w2 = 0;
+  c.k = 2;
+  c.counted = true;
+  c.listed = false;
+  c.peeled = false;
+  c.have_comp = false;
+}
+
 void list_count(Ctx &c, uint32_t k, ull cap) {
   if (!c.oriented) fail(KCDS_E_STATE, "kcds_count/kcds_list before kcds_orient");
-  if (k < 3 || k > 8) fail(KCDS_E_INVALID, "k = %u outside 3..8", k);
+  if (k < 2 || k > 8) fail(KCDS_E_INVALID, "k = %u outside 2..8", k);
+  if (k == 2) {
+    list_count_edges(c);
+    return;
+  }
   const uint32_t n = c.n;
   classify_roots(c, k);
   const uint32_t ncopy = count_copies(n);
@@ -884,8 +972,14 @@ void list_materialize(Ctx &c, uint32_t k) {
   scan_u64_to_u64(c, c.cnt.as<ull>(), c.incoff.as<ull>(), n);
   buf_reserve(c, c.cliques, sizeof(uint32_t) * (T * k + 1));
   buf_reserve(c, c.inc, sizeof(uint32_t) * (T * k + 1));
-  HIP_CHECK(hipMemsetAsync(emit_queue(c, 0), 0, NCLASS * QUEUE_WORDS * sizeof(uint32_t), c.stream));
-  for (int cls = NCLASS - 1; cls >= 0; --cls) {
+  if (k == 2 && n) {  // the arcs themselves, at rootOff[r] = doff[r]
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 3) / 4, (uint64_t)c.num_cus * 16);
+    k_edge_emit<<<grid, 256, 0, c.stream>>>(c.doff.as<uint32_t>(), c.dnbr.as<uint32_t>(), n,
+                                            c.cliques.as<uint32_t>());
+    KCDS_LAUNCH_CHECK();
+  }
+  if (k > 2) HIP_CHECK(hipMemsetAsync(emit_queue(c, 0), 0, NCLASS * QUEUE_WORDS * sizeof(uint32_t), c.stream));
+  for (int cls = NCLASS - 1; cls >= 0 && k > 2; --cls) {
     const uint32_t nr = c.class_count[cls];
     if (!nr) continue;
     EmitFn fn = cls == 0 ? emit_fn<1>(k) : cls == 1 ? emit_fn<2>(k) : cls == 2 ? emit_fn<4>(k) : emit_fn<8>(k);

@@ -294,7 +294,7 @@ int kcds_maxflow(kcds_ctx *ctx, uint32_t comp_id, uint64_t p, uint64_t q, int64_
 int kcds_solve(kcds_ctx *ctx, uint32_t k, kcds_result *res, uint8_t *in_set) {
   return guarded(ctx, [&](Ctx &c) {
     if (!c.have_graph) fail(KCDS_E_STATE, "kcds_solve before kcds_upload_csr");
-    if (k < 3 || k > 8) fail(KCDS_E_INVALID, "k = %u outside 3..8", k);
+    if (k < 2 || k > 8) fail(KCDS_E_INVALID, "k = %u outside 2..8", k);
     c.ngraphs = 1;
     c.h_gvoff.assign({0u, c.n});
     solve_graphs(c, k, res, nullptr, in_set);
@@ -305,7 +305,7 @@ int kcds_solve_batch(kcds_ctx *ctx, uint32_t k, uint32_t ngraphs, const uint32_t
                      kcds_graph_result *results, uint8_t *in_set) {
   return guarded(ctx, [&](Ctx &c) {
     if (!c.have_graph) fail(KCDS_E_STATE, "kcds_solve_batch before kcds_upload_csr");
-    if (k < 3 || k > 8) fail(KCDS_E_INVALID, "k = %u outside 3..8", k);
+    if (k < 2 || k > 8) fail(KCDS_E_INVALID, "k = %u outside 2..8", k);
     if (!graph_voff || !results || ngraphs == 0) fail(KCDS_E_INVALID, "null batch partition or results");
     if (graph_voff[0] != 0 || graph_voff[ngraphs] != c.n)
       fail(KCDS_E_INVALID, "graph_voff must run from 0 to n");

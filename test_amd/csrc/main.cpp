@@ -24,8 +24,8 @@ int main(int argc, const char *argv[]) {
   }
   const std::string path = argv[1];
   const long k = std::atol(argv[2]);
-  if (k < 3 || k > 8) {
-    std::cout << "k must be in 3..8" << std::endl;
+  if (k < 2 || k > 8) {
+    std::cout << "k must be in 2..8" << std::endl;
     return 1;
   }
   Graph graph;

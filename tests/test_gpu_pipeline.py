@@ -42,13 +42,15 @@ def disjoint_cliques(sizes, extra_edges=()):
 
 def graphs():
     off, nbr, _ = O.load_adjlist(os.path.join(GOLD, "ref_data_test.txt"))
-    yield "ref_data_test", off, nbr, (3, 4)
+    yield "ref_data_test", off, nbr, (3, 4, 2)
     g = gen.er(400, 0.04, 18, seed=21)
-    yield "er400", g.offset, g.nbr, (3, 4, 5)
+    yield "er400", g.offset, g.nbr, (3, 4, 5, 2)
     g = gen.rmat(10, 8, (12,), seed=9)
-    yield "rmat10", g.offset, g.nbr, (3, 4)
+    yield "rmat10", g.offset, g.nbr, (3, 4, 2)
     off, nbr = disjoint_cliques([6, 6, 5], extra_edges=[(0, 6), (11, 12)])
-    yield "tied_cliques", off, nbr, (3, 4)
+    yield "tied_cliques", off, nbr, (3, 4, 2)
+    g = gen.rmat(9, 6, (), seed=13)  # edge density only (k = 2 first: the max-flow test uses ks[0])
+    yield "rmat9_edges", g.offset, g.nbr, (2, 3)
 
 
 GRAPHS = list(graphs())

@@ -42,9 +42,9 @@ def test_cli_usage_errors():
     assert r.returncode == 1 and "usage" in r.stdout
     r = subprocess.run([exe, "/nonexistent/graph.txt", "3"], stdout=subprocess.PIPE, text=True)
     assert r.returncode == 1 and "Open Failed" in r.stdout
-    r = subprocess.run([exe, os.path.join(ROOT, "tests", "golden", "ref_data_test.txt"), "2"],
+    r = subprocess.run([exe, os.path.join(ROOT, "tests", "golden", "ref_data_test.txt"), "9"],
                        stdout=subprocess.PIPE, text=True)
-    assert r.returncode == 1
+    assert r.returncode == 1 and "2..8" in r.stdout
 
 
 def test_concat_graphs():

@@ -135,7 +135,7 @@ def test_count_vs_brute_force(seed):
     n = int(rng.integers(8, 20))
     g = gen.er(n, float(rng.uniform(0.3, 0.8)), 0, seed=seed)
     adj = adjsets(g.offset, g.nbr)
-    for k in (3, 4, 5):
+    for k in (2, 3, 4, 5):
         cl = brute_cliques(n, adj, k)
         T, pv = O.count(g.offset, g.nbr, k)
         assert T == len(cl)
@@ -306,7 +306,7 @@ def test_solve_vs_brute_force(seed):
     n = int(rng.integers(7, 13))
     g = gen.er(n, float(rng.uniform(0.35, 0.85)), 0, seed=200 + seed)
     adj = adjsets(g.offset, g.nbr)
-    for k in (3, 4):
+    for k in (2, 3, 4):  # k = 2: edge density (main.cu:1365-1423)
         cl = brute_cliques(n, adj, k)
         r, pv, core, comp, ins = O.solve(g.offset, g.nbr, k)
         assert r.T == len(cl)
