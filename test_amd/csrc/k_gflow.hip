@@ -38,7 +38,7 @@ namespace {
 typedef long long i64;
 
 constexpr uint32_t NSEG = 8;
-constexpr uint32_t SWEEPS = 24;
+constexpr uint32_t SWEEPS = 12;
 constexpr uint32_t RELABEL_BATCH = 16;  // BFS levels per host check
 constexpr uint32_t MAX_PHASES = 1u << 14;
 constexpr uint32_t NO_ARC = 0xFFFFFFFFu;

@@ -32,7 +32,10 @@ namespace kcds {
 namespace {
 
 constexpr uint32_t PSEG = 4;        // frontier segments / counter shards
-constexpr uint32_t PPIECE = 256;    // incidence entries per work item
+// Incidence entries per work item: small pieces keep a level's critical path
+// short (a piece is one wave's chain of dependent memory steps), larger ones
+// bound the piece frontiers when T is large (peel_piece()).
+constexpr uint32_t PPIECE_MIN = 64;
 constexpr uint32_t KMAX = 8;        // largest k
 
 // Counters 64 words apart (same-address device atomics are serialised).
@@ -45,6 +48,8 @@ struct PeelCtl {
   uint32_t gblocks[PSEG][64];
   uint32_t blocks[64];
   uint32_t ntouch[64];          // graphs touched by the current level
+  ull dmin[PSEG][8];            // minimum count above the level left by this level's decrements (PROCESS)
+  ull nextmin[8];               // minimum live count above the level when the level opened
 };
 
 struct PItem {
@@ -60,10 +65,10 @@ __device__ __forceinline__ bool ratio_ge(ull a, ull b, ull c, ull d) {
 }
 
 // Appends v (where want) to a vertex frontier segment and its incidence list,
-// cut in PPIECE-entry pieces, to a piece segment; one counter atomic of each
+// cut in piece-entry pieces, to a piece segment; one counter atomic of each
 // kind per wave.  Every lane calls it.
 __device__ __forceinline__ void pl_append(bool want, uint32_t v, const ull *__restrict__ incoff, uint32_t *fv,
-                                          uint32_t *fvsize, PItem *fp, uint32_t *fpsize) {
+                                          uint32_t *fvsize, PItem *fp, uint32_t *fpsize, uint32_t piece) {
   const ull bal = __ballot(want);
   if (!bal) return;
   const uint32_t lane = lane_id(), lead = (uint32_t)__builtin_ctzll(bal);
@@ -72,7 +77,7 @@ __device__ __forceinline__ void pl_append(bool want, uint32_t v, const ull *__re
   if (want) {
     b = incoff[v];
     e = incoff[v + 1];
-    np = (uint32_t)((e - b + PPIECE - 1) / PPIECE);
+    np = (uint32_t)((e - b + piece - 1) / piece);
   }
   const uint32_t incl = wave_incl_scan_u32(np);
   const uint32_t tot = __shfl(incl, 63, 64);
@@ -86,10 +91,10 @@ __device__ __forceinline__ void pl_append(bool want, uint32_t v, const ull *__re
   if (want) {
     fv[vb + lanes_below(bal)] = v;
     uint32_t p = pb + incl - np;
-    for (ull st = b; st < e; st += PPIECE) {
+    for (ull st = b; st < e; st += piece) {
       PItem it;
       it.st = st;
-      it.en = min(st + PPIECE, e);
+      it.en = min(st + piece, e);
       it.v = v;
       it.pad = 0;
       fp[p++] = it;
@@ -164,7 +169,7 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
                                                  ull *cnt, int32_t *core, int32_t *status, uint32_t *fv0, uint32_t *fv1,
                                                  PItem *fp0, PItem *fp1, uint32_t seg_v, uint32_t seg_p,
                                                  GraphPeel *gp, const uint32_t *__restrict__ gid, uint32_t *tlist,
-                                                 Scatter sc) {
+                                                 Scatter sc, uint32_t piece) {
   __shared__ ull s_hdr[2];
   __shared__ uint32_t s_hdr2[3], s_f[PSEG + 1], s_p[PSEG + 1], s_last, s_sz[2][2][PSEG];
   __shared__ ull s_min;
@@ -204,7 +209,7 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
       const bool live = p < K && v < n && core[v] == -1;
       const ull cv = live ? cnt[p] : ~0ull;
       pl_append(live && cv <= target, v, incoff, fv + (size_t)seg * seg_v, &ctl->fsize[cur][seg][0],
-                fp + (size_t)seg * seg_p, &ctl->psize[cur][seg][0]);
+                fp + (size_t)seg * seg_p, &ctl->psize[cur][seg][0], piece);
       if (live && cv > target) m = min(m, cv);
     }
 #pragma unroll
@@ -242,7 +247,10 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
         if (atomicExch(&gp[g].touch, 1u) == 0u) tlist[atomicAdd(&ctl->ntouch[0], 1u)] = g;
       }
     }
-    // incidence pieces, wave per piece
+    // incidence pieces, wave per piece.  dm: the lowest count above L that a
+    // decrement leaves (the next level starts at min(dm, the minimum when
+    // this level opened), so it needs no failed OPEN step to find it)
+    ull dm = ~0ull;
     const uint32_t wid = gtid / KCDS_WAVE, nw = gstride / KCDS_WAVE;
     for (uint32_t it = wid; it < ps; it += nw) {
       uint32_t sg = 0;
@@ -281,13 +289,22 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
         for (uint32_t t = 0; t < KMAX; ++t) {
           const ull lold = __shfl(old[t], leader[t], 64);
           const bool join = dec[t] && lold - rank[t] == L + 1;
+          const ull now = lold - rank[t] - 1;  // this lane's count after its decrement
+          if (dec[t] && now > L) dm = min(dm, now);
           pl_append(join, u[t], incoff, nv + (size_t)seg * seg_v, &ctl->fsize[cur ^ 1][seg][0],
-                    np + (size_t)seg * seg_p, &ctl->psize[cur ^ 1][seg][0]);
+                    np + (size_t)seg * seg_p, &ctl->psize[cur ^ 1][seg][0], piece);
         }
       }
       destroyed = wave_sum_u32(destroyed);
       if (lane == 0 && destroyed) atomicSub(&gp[gid[v]].live, (ull)destroyed);
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dm = min(dm, (ull)__shfl_xor(dm, o, 64));
+    if (tid == 0) s_min = ~0ull;
+    lds_sync();
+    if (lane == 0 && dm != ~0ull) atomicMin(&s_min, dm);
+    lds_sync();
+    if (tid == 0 && s_min != ~0ull) atomicMin(&ctl->dmin[seg][0], s_min);
   }
   // control step: the last workgroup to finish
   // wave 0 of the last workgroup: lanes g < PSEG read and reset segment g's
@@ -309,20 +326,31 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
     const uint32_t fcur = wave_sum_u32(fc), fnext = wave_sum_u32(fn);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mn = min(mn, (ull)__shfl_xor(mn, o, 64));
+    // end of a level: the lowest count its decrements left above it
+    ull dn = ~0ull;
+    if (phase == 1 && fnext == 0 && lane < PSEG) dn = atomicExch(&ctl->dmin[lane][0], ~0ull);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dn = min(dn, (ull)__shfl_xor(dn, o, 64));
     if (lane == 0) {
       ctl->blocks[0] = 0;
       ++ctl->steps;
       if (phase == 1) {
         ctl->cur = cur ^ 1;
         if (fnext == 0) {
+          // every live count is >= the minimum when the level opened (untouched
+          // vertices) or >= the lowest a decrement left (touched ones): start
+          // the next level there.  A stale minimum (its vertex left in the
+          // cascade) only costs one OPEN that finds nothing and the true one.
+          const ull nm = min(ctl->nextmin[0], dn);
           ctl->phase = 0;
-          ctl->target = L + 1;
+          ctl->target = nm == ~0ull ? L + 1 : max(nm, L + 1);
         }
       } else {
         ctl->ntouch[0] = 0;  // the densities were recorded by this step
         if (fcur > 0) {
           ctl->level = target;
           ctl->phase = 1;
+          ctl->nextmin[0] = mn;
           ++ctl->levels;
         } else if (mn == ~0ull) {
           ctl->done = 1;
@@ -750,7 +778,11 @@ void peel_run(Ctx &c) {
   // working counts, two frontiers (vertices, incidence pieces) in PSEG
   // segments that can each hold any frontier, and the control block
   const uint32_t seg_v = ((n + 63) / 64) * 64;
-  const uint64_t seg_p64 = (((uint64_t)n + (T * k) / PPIECE + 64 + 63) / 64) * 64;
+  // pieces of 64 incidences while the piece frontiers stay under ~2^24
+  // entries per segment, larger for very large T
+  uint32_t piece = PPIECE_MIN;
+  while ((T * k) / piece > (1ull << 24)) piece *= 2;
+  const uint64_t seg_p64 = (((uint64_t)n + (T * k) / piece + 64 + 63) / 64) * 64;
   if (seg_p64 * PSEG >= (1ull << 32)) fail(KCDS_E_LIMIT, "too many cliques for the peeling frontiers");
   const uint32_t seg_p = (uint32_t)seg_p64;
   const Scatter sc = make_scatter(n);  // working counts: hub counters on lines of their own
@@ -765,7 +797,11 @@ void peel_run(Ctx &c) {
   PeelCtl *ctl = (PeelCtl *)(((uintptr_t)(fp1 + (size_t)PSEG * seg_p) + 255) & ~(uintptr_t)255);
   PeelCtl *h0 = (PeelCtl *)c.host_scratch(sizeof(PeelCtl));
   memset(h0, 0, sizeof(PeelCtl));
-  for (uint32_t g = 0; g < PSEG; ++g) h0->gmin[g][0] = ~0ull;
+  for (uint32_t g = 0; g < PSEG; ++g) {
+    h0->gmin[g][0] = ~0ull;
+    h0->dmin[g][0] = ~0ull;
+  }
+  h0->nextmin[0] = ~0ull;
   h0->done = n == 0;
   HIP_CHECK(hipMemcpyAsync(ctl, h0, sizeof(PeelCtl), hipMemcpyHostToDevice, c.stream));
   c.sync();  // h0 is reused below
@@ -799,7 +835,7 @@ void peel_run(Ctx &c) {
   while (n) {
     for (uint32_t r = 0; r < batch; ++r) {
       k_pl_step<<<sgrid, 256, 0, c.stream>>>(ctl, n, k, cl, incoff, inc, cnt, core, status, fv0, fv1, fp0, fp1, seg_v,
-                                             seg_p, gp, gidp, tl, sc);
+                                             seg_p, gp, gidp, tl, sc, piece);
       KCDS_LAUNCH_CHECK();
     }
     rounds += batch;
