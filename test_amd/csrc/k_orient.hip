@@ -54,45 +54,200 @@ __device__ __forceinline__ uint32_t rank_in(const uint32_t *a, uint32_t cnt, uin
 
 constexpr uint32_t HEAVY_DEG = 2048;   // vertices of higher degree are gathered by a whole workgroup
 constexpr uint32_t HEAVY_THREADS = 1024;
+constexpr uint32_t SMALL_DEG = 32;     // vertices of degree <= SMALL_DEG are gathered 64 to a wave
+constexpr uint32_t TILE_CAP = 1024;    // arcs a tile pass holds in LDS (>= SMALL_DEG)
+constexpr uint32_t GUNROLL = 4;        // 64-arc chunks whose loads are in flight together
 
-// Wave per vertex v (4 waves per block, 4 KB of LDS each): gathers the new
-// ids of its later neighbours, sorts them by rank counting (position = number
-// of smaller entries; entries are distinct) and writes them to
-// tmp[off[v] ..), the start of v's undirected list; dout[rank[v]] = count.
-// Vertices of degree above HEAVY_DEG are only listed in heavy[] for
-// k_dag_gather_heavy.  A list longer than SORT_WAVE is counted but not
-// written (the caller fails).
+// Appends v (where want) to list[] with one counter atomic per wave.
+__device__ __forceinline__ void wave_push(bool want, uint32_t v, uint32_t *list, uint32_t *len) {
+  const ull bal = __ballot(want);
+  if (!bal) return;
+  const uint32_t lead = (uint32_t)__builtin_ctzll(bal);
+  uint32_t at = 0;
+  if (lane_id() == lead) at = atomicAdd(len, (uint32_t)__popcll(bal));
+  at = __shfl(at, lead, 64);
+  if (want) list[at + lanes_below(bal)] = v;
+}
+
+struct TileLds {
+  uint32_t buf[TILE_CAP];  // kept new ids of the pass, in arc order
+  uint8_t sg[TILE_CAP];    // lane (vertex of the tile) each kept id belongs to
+  uint32_t excl[64], b[64], r[64], kst[64], ken[64];
+};
+
+// Wave per tile of 64 consecutive vertices, one per lane.  The arcs of the
+// tile's vertices of degree <= SMALL_DEG form one flat run (row after row);
+// the wave walks it 64 arcs at a time (GUNROLL chunks of loads in flight),
+// finds each arc's row by binary search over the row starts, keeps the new
+// ids above the row's own and compacts them in LDS, which keeps rows
+// contiguous.  A kept id's place in its sorted out-list is the number of
+// smaller kept ids of the same row; it is written to tmp[off[v] ..), the
+// start of v's undirected list, and dout[rank[v]] = the row's count.  Runs
+// longer than TILE_CAP are split at row boundaries into passes.  Vertices of
+// higher degree go to mid[] (<= HEAVY_DEG) or heavy[].
+__global__ __launch_bounds__(256) void k_dag_gather_tiles(
+    const uint32_t *__restrict__ off, const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ rank, uint32_t n,
+    uint32_t *__restrict__ dout, uint32_t *__restrict__ tmp, uint32_t *__restrict__ mid, uint32_t *nmid,
+    uint32_t *__restrict__ heavy, uint32_t *nheavy) {
+  __shared__ TileLds sl[4];
+  const uint32_t lane = lane_id(), wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  TileLds &W = sl[wv];
+  const uint32_t ntiles = (n + 63) / 64;
+  for (uint32_t tile = blockIdx.x * 4 + wv; tile < ntiles; tile += gridDim.x * 4) {
+    const uint32_t v = tile * 64 + lane;
+    uint32_t b = 0, e = 0, r = 0;
+    if (v < n) {
+      b = off[v];
+      e = off[v + 1];
+      r = rank[v];
+    }
+    const uint32_t d = e - b;
+    const bool small = v < n && d <= SMALL_DEG;
+    wave_push(v < n && d > SMALL_DEG && d <= HEAVY_DEG, v, mid, nmid);
+    wave_push(v < n && d > HEAVY_DEG, v, heavy, nheavy);
+    const uint32_t ds = small ? d : 0u;
+    const uint32_t incl = wave_incl_scan_u32(ds), excl = incl - ds;
+    W.excl[lane] = excl;
+    W.b[lane] = b;
+    W.r[lane] = r;
+    wave_sync();
+    const uint32_t nl = min(64u, n - tile * 64);
+    uint32_t a = 0;
+    while (a < nl) {
+      // pass: rows [a, bnd), whose arcs [F0, F1) fit TILE_CAP (row a alone always does)
+      const uint32_t F0 = __shfl(excl, a, 64);
+      const uint32_t bnd = (uint32_t)__popcll(__ballot(incl <= F0 + TILE_CAP));
+      const uint32_t F1 = __shfl(incl, bnd - 1, 64);
+      if (lane >= a && lane < bnd) {
+        W.kst[lane] = 0;
+        W.ken[lane] = 0;
+      }
+      uint32_t cnt = 0;
+      for (uint32_t f0 = F0; f0 < F1; f0 += GUNROLL * KCDS_WAVE) {
+        uint32_t u[GUNROLL], sr[GUNROLL];
+#pragma unroll
+        for (uint32_t q = 0; q < GUNROLL; ++q) {
+          const uint32_t f = f0 + q * KCDS_WAVE + lane;
+          uint32_t row = a;
+          if (f < F1) {
+#pragma unroll
+            for (uint32_t step = 32; step; step >>= 1)
+              if (row + step < bnd && W.excl[row + step] <= f) row += step;
+          }
+          sr[q] = row;
+          u[q] = f < F1 ? nbr[W.b[row] + (f - W.excl[row])] : 0u;
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < GUNROLL; ++q) u[q] = (f0 + q * KCDS_WAVE + lane < F1) ? rank[u[q]] : 0u;
+#pragma unroll
+        for (uint32_t q = 0; q < GUNROLL; ++q) {
+          const bool keep = (f0 + q * KCDS_WAVE + lane < F1) && u[q] > W.r[sr[q]];
+          const ull bal = __ballot(keep);
+          if (keep) {
+            const uint32_t at = cnt + lanes_below(bal);
+            W.buf[at] = u[q];
+            W.sg[at] = (uint8_t)sr[q];
+          }
+          cnt += (uint32_t)__popcll(bal);
+        }
+      }
+      wave_sync();
+      for (uint32_t i = lane; i < cnt; i += KCDS_WAVE) {  // row ranges of the kept list
+        const uint32_t s = W.sg[i];
+        if (i == 0 || W.sg[i - 1] != s) W.kst[s] = i;
+        if (i + 1 == cnt || W.sg[i + 1] != s) W.ken[s] = i + 1;
+      }
+      wave_sync();
+      for (uint32_t i = lane; i < cnt; i += KCDS_WAVE) {
+        const uint32_t s = W.sg[i], x = W.buf[i], en = W.ken[s];
+        uint32_t j = W.kst[s], p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+        for (; j + 4 <= en; j += 4) {
+          p0 += W.buf[j] < x;
+          p1 += W.buf[j + 1] < x;
+          p2 += W.buf[j + 2] < x;
+          p3 += W.buf[j + 3] < x;
+        }
+        for (; j < en; ++j) p0 += W.buf[j] < x;
+        tmp[W.b[s] + p0 + p1 + p2 + p3] = x;
+      }
+      if (small && lane >= a && lane < bnd) dout[r] = W.ken[lane] - W.kst[lane];
+      wave_sync();
+      a = bnd;
+    }
+  }
+}
+
+// Sorts the distinct entries buf[0..cnt) (16-byte aligned, padded with ~0 to
+// a multiple of 4) into dst[0..cnt): each lane holds M of them in registers
+// and counts the smaller entries of the whole list, read once by broadcast.
+template <uint32_t M>
+__device__ __forceinline__ void wave_rank_sort(const uint32_t *buf, uint32_t cnt, uint32_t *__restrict__ dst) {
+  const uint32_t lane = lane_id();
+  uint32_t x[M], p[M];
+#pragma unroll
+  for (uint32_t q = 0; q < M; ++q) {
+    const uint32_t i = lane + q * KCDS_WAVE;
+    x[q] = i < cnt ? buf[i] : 0xFFFFFFFFu;
+    p[q] = 0;
+  }
+  const uint4 *a4 = reinterpret_cast<const uint4 *>(buf);
+  const uint32_t n4 = (cnt + 3) / 4;
+  for (uint32_t j = 0; j < n4; ++j) {
+    const uint4 u = a4[j];
+#pragma unroll
+    for (uint32_t q = 0; q < M; ++q) p[q] += (u.x < x[q]) + (u.y < x[q]) + (u.z < x[q]) + (u.w < x[q]);
+  }
+#pragma unroll
+  for (uint32_t q = 0; q < M; ++q)
+    if (lane + q * KCDS_WAVE < cnt) dst[p[q]] = x[q];
+}
+
+constexpr uint32_t MUNROLL = 8;  // 64-arc chunks in flight per mid vertex
+
+// Wave per vertex of mid[] (degree in (SMALL_DEG, HEAVY_DEG]; 4 waves per
+// block, 4 KB of LDS each): gathers the new ids of its later neighbours with
+// MUNROLL chunks of loads in flight, sorts them (wave_rank_sort) and writes
+// them to tmp[off[v] ..); dout[rank[v]] = count.  A list longer than
+// SORT_WAVE is counted but not written (the caller fails).
 __global__ __launch_bounds__(256) void k_dag_gather(const uint32_t *__restrict__ off,
                                                     const uint32_t *__restrict__ nbr,
-                                                    const uint32_t *__restrict__ rank, uint32_t n,
-                                                    uint32_t *__restrict__ dout, uint32_t *__restrict__ tmp,
-                                                    uint32_t *__restrict__ heavy, uint32_t *nheavy) {
+                                                    const uint32_t *__restrict__ rank,
+                                                    const uint32_t *__restrict__ mid, const uint32_t *nmid,
+                                                    uint32_t *__restrict__ dout, uint32_t *__restrict__ tmp) {
   __shared__ __align__(16) uint32_t sbuf[4][SORT_WAVE + 4];
   const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
   uint32_t *buf = sbuf[wv];
-  const uint32_t gw = blockIdx.x * 4 + wv, nwv = gridDim.x * 4;
-  for (uint32_t v = gw; v < n; v += nwv) {
-    const uint32_t b = off[v], e = off[v + 1];
-    if (e - b > HEAVY_DEG) {
-      if (lane == 0) heavy[atomicAdd(nheavy, 1u)] = v;
-      continue;
-    }
-    const uint32_t r = rank[v];
+  const uint32_t gw = blockIdx.x * 4 + wv, nwv = gridDim.x * 4, nm = *nmid;
+  for (uint32_t h = gw; h < nm; h += nwv) {
+    const uint32_t v = mid[h];
+    const uint32_t b = off[v], e = off[v + 1], r = rank[v];
     uint32_t cnt = 0;
-    for (uint32_t i0 = b; i0 < e; i0 += KCDS_WAVE) {
-      const uint32_t i = i0 + lane;
-      const uint32_t x = (i < e) ? rank[nbr[i]] : 0u;
-      const bool keep = (i < e) && x > r;
-      const ull bal = __ballot(keep);
-      if (keep && cnt + lanes_below(bal) < SORT_WAVE) buf[cnt + lanes_below(bal)] = x;
-      cnt += (uint32_t)__popcll(bal);
+    for (uint32_t i0 = b; i0 < e; i0 += MUNROLL * KCDS_WAVE) {
+      uint32_t x[MUNROLL];
+#pragma unroll
+      for (uint32_t q = 0; q < MUNROLL; ++q) {
+        const uint32_t i = i0 + q * KCDS_WAVE + lane;
+        x[q] = i < e ? nbr[i] : 0u;
+      }
+#pragma unroll
+      for (uint32_t q = 0; q < MUNROLL; ++q) x[q] = (i0 + q * KCDS_WAVE + lane < e) ? rank[x[q]] : 0u;
+#pragma unroll
+      for (uint32_t q = 0; q < MUNROLL; ++q) {
+        const bool keep = (i0 + q * KCDS_WAVE + lane < e) && x[q] > r;
+        const ull bal = __ballot(keep);
+        if (keep && cnt + lanes_below(bal) < SORT_WAVE) buf[cnt + lanes_below(bal)] = x[q];
+        cnt += (uint32_t)__popcll(bal);
+      }
     }
     if (lane == 0) dout[r] = cnt;
     if (lane < 4 && cnt + lane < SORT_WAVE) buf[cnt + lane] = 0xFFFFFFFFu;  // pad to whole 4-entry reads
     wave_sync();
-    if (cnt <= SORT_WAVE) {
-      for (uint32_t i = lane; i < cnt; i += KCDS_WAVE) tmp[b + rank_in(buf, cnt, buf[i])] = buf[i];
-    }
+    uint32_t *dst = tmp + b;
+    if (cnt <= 64) wave_rank_sort<1>(buf, cnt, dst);
+    else if (cnt <= 128) wave_rank_sort<2>(buf, cnt, dst);
+    else if (cnt <= 256) wave_rank_sort<4>(buf, cnt, dst);
+    else if (cnt <= 512) wave_rank_sort<8>(buf, cnt, dst);
+    else if (cnt <= SORT_WAVE) wave_rank_sort<16>(buf, cnt, dst);
     wave_sync();
   }
 }
@@ -137,15 +292,16 @@ __global__ __launch_bounds__(HEAVY_THREADS) void k_dag_gather_heavy(
   }
 }
 
-// Wave per new id r: copies the sorted out-list of v = inv[r] into the DAG.
+// 16 lanes per new id r (mean out-degree is small): copies the sorted
+// out-list of v = inv[r] into the DAG.
 __global__ __launch_bounds__(256) void k_dag_copy(const uint32_t *__restrict__ off, const uint32_t *__restrict__ inv,
                                                   uint32_t n, const uint32_t *__restrict__ doff,
                                                   const uint32_t *__restrict__ tmp, uint32_t *__restrict__ dnbr) {
-  const uint32_t lane = lane_id();
-  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwv = (gridDim.x * blockDim.x) >> 6;
-  for (uint32_t r = gw; r < n; r += nwv) {
+  const uint32_t sub = threadIdx.x & 15;
+  const uint32_t gq = (blockIdx.x * blockDim.x + threadIdx.x) >> 4, nq = (gridDim.x * blockDim.x) >> 4;
+  for (uint32_t r = gq; r < n; r += nq) {
     const uint32_t src = off[inv[r]], dst = doff[r], d = doff[r + 1] - dst;
-    for (uint32_t i = lane; i < d; i += KCDS_WAVE) dnbr[dst + i] = tmp[src + i];
+    for (uint32_t i = sub; i < d; i += 16) dnbr[dst + i] = tmp[src + i];
   }
 }
 
@@ -161,8 +317,9 @@ __global__ __launch_bounds__(256) void k_dag_copy(const uint32_t *__restrict__ o
 //           the graph, decrementing its live neighbours; a neighbour whose
 //           degree falls from L+1 to L joins the next sub-round of level L.
 // A frontier carries each vertex's adjacency cut into PIECE-neighbour
-// pieces, and the decrements run wave per piece, so a hub of degree 25,000
-// is spread over 50 waves instead of one.
+// pieces, and the decrements run PLANES lanes per piece (a wave takes
+// 64 / PLANES pieces at once), so a hub of degree 25,000 is spread over 49
+// waves and a level of short lists still fills its waves.
 // A vertex leaves by having its degree replaced with DEAD; decrements that
 // race with that replacement keep the value >= DEAD_MIN, which every reader
 // treats as dead.  Kernel boundaries order the steps (the next step reads
@@ -191,12 +348,17 @@ struct OrderCtl {
 
 constexpr uint32_t DEAD = 0xFFFFFFFFu;
 constexpr uint32_t DEAD_MIN = 0x80000000u;
-constexpr uint32_t PIECE = 512;   // neighbours per decrement work item
-constexpr uint32_t PUNROLL = PIECE / KCDS_WAVE;
+constexpr uint32_t PIECE = 64;    // neighbours per decrement work item
+constexpr uint32_t PLANES = 8;    // lanes per piece
+constexpr uint32_t PUNROLL = PIECE / PLANES;
+constexpr uint32_t SMALL_FRONT = 16;  // frontier vertices of at most this degree decrement lane per vertex
+constexpr uint32_t DEC_TB = 4096;     // LDS table of a workgroup round's decrements (2 * 256 * PUNROLL)
+constexpr uint32_t AGG_MIN_PIECES = 2048;  // steps with fewer pieces decrement directly
 
-// Appends v (where want) to a vertex frontier segment fv and its adjacency,
-// cut in PIECE-neighbour pieces (start << 32 | end), to the piece segment
-// fp; one counter atomic of each kind per wave.  Every lane calls it.
+// Appends v (where want) to a vertex frontier segment fv and, when its
+// degree exceeds SMALL_FRONT, its adjacency cut in PIECE-neighbour pieces
+// (start << 32 | end) to the piece segment fp; one counter atomic of each
+// kind per wave.  Every lane calls it.
 __device__ __forceinline__ void wave_append(bool want, uint32_t v, const uint32_t *__restrict__ off, uint32_t *fv,
                                             uint32_t *fvsize, ull *fp, uint32_t *fpsize) {
   const ull bal = __ballot(want);
@@ -206,7 +368,7 @@ __device__ __forceinline__ void wave_append(bool want, uint32_t v, const uint32_
   if (want) {
     b = off[v];
     e = off[v + 1];
-    np = (e - b + PIECE - 1) / PIECE;
+    np = e - b > SMALL_FRONT ? (e - b + PIECE - 1) / PIECE : 0u;
   }
   const uint32_t incl = wave_incl_scan_u32(np);
   const uint32_t tot = __shfl(incl, 63, 64);
@@ -220,7 +382,8 @@ __device__ __forceinline__ void wave_append(bool want, uint32_t v, const uint32_
   if (want) {
     fv[vb + lanes_below(bal)] = v;
     uint32_t p = pb + incl - np;
-    for (uint32_t st = b; st < e; st += PIECE) fp[p++] = ((ull)st << 32) | min(st + PIECE, e);
+    const uint32_t pe = np ? e : b;  // no pieces for a vertex of degree <= SMALL_FRONT
+    for (uint32_t st = b; st < pe; st += PIECE) fp[p++] = ((ull)st << 32) | min(st + PIECE, e);
   }
 }
 
@@ -287,9 +450,11 @@ __device__ void order_control(OrderCtl *ctl, uint32_t n, uint32_t level, uint32_
 __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, const uint32_t *__restrict__ off,
                                                  const uint32_t *__restrict__ nbr, uint32_t *deg, uint32_t *fv0,
                                                  uint32_t *fv1, ull *fp0, ull *fp1, uint32_t seg_v, uint32_t seg_p,
-                                                 uint32_t *rank, uint32_t *inv, uint32_t sh, Scatter sc) {
+                                                 uint32_t *rank, uint32_t *inv, uint32_t sh, Scatter sc,
+                                                 uint32_t *gone) {
   __shared__ uint32_t s_f[NSEG + 1], s_p[NSEG + 1], s_min, s_last;
   __shared__ uint32_t s_hdr[6], s_sz[2][2][NSEG];
+  __shared__ uint32_t t_key[DEC_TB], t_cnt[DEC_TB], t_used[DEC_TB], t_nused[2];
   const uint32_t tid = threadIdx.x, lane = lane_id(), seg = blockIdx.x % NSEG;
   // the header and both frontiers' segment sizes in one round trip
   if (tid < 6) s_hdr[tid] = (&ctl->level)[tid];
@@ -336,37 +501,138 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
     }
     lds_sync();
     const uint32_t fs = s_f[NSEG], ps = s_p[NSEG];
-    for (uint32_t i = gtid; i < fs; i += gstride) {  // the frontier takes the next ranks and leaves
-      uint32_t g = 0;
-      while (i >= s_f[g + 1]) ++g;
-      const uint32_t v = fv[(size_t)g * seg_v + (i - s_f[g])];
-      rank[v] = base + i;
-      inv[base + i] = v;
-      atomicExch(&deg[slot_of(sc, v)], DEAD);
+    // The frontier takes the next ranks and leaves; a vertex of degree <=
+    // SMALL_FRONT (most of an early level) decrements its neighbours from its
+    // own lane, its arcs' loads and atomics in flight together.
+    for (uint32_t i0 = gtid - lane; i0 < fs; i0 += gstride) {
+      const uint32_t i = i0 + lane;
+      uint32_t b = 0, e = 0;
+      if (i < fs) {
+        uint32_t g = 0;
+        while (i >= s_f[g + 1]) ++g;
+        const uint32_t v = fv[(size_t)g * seg_v + (i - s_f[g])];
+        rank[v] = base + i;
+        inv[base + i] = v;
+        atomicExch(&deg[slot_of(sc, v)], DEAD);
+        atomicOr(&gone[v >> 5], 1u << (v & 31));
+        b = off[v];
+        e = off[v + 1];
+        if (e - b > SMALL_FRONT) e = b;  // its pieces are in fp
+      }
+      const uint32_t dmax = wave_max_u32(e - b);
+      uint32_t u[SMALL_FRONT], dv[SMALL_FRONT];
+#pragma unroll
+      for (uint32_t x = 0; x < SMALL_FRONT; ++x)
+        if (x < dmax) u[x] = b + x < e ? nbr[b + x] : 0u;
+#pragma unroll
+      for (uint32_t x = 0; x < SMALL_FRONT; ++x)
+        if (x < dmax) dv[x] = b + x < e ? atomicSub(&deg[slot_of(sc, u[x])], 1u) : DEAD;
+#pragma unroll
+      for (uint32_t x = 0; x < SMALL_FRONT; ++x)
+        if (x < dmax)
+          wave_append(dv[x] == level + 1, u[x], off, nv + (size_t)seg * seg_v, &ctl->fsize[cur ^ 1][seg][0],
+                      np + (size_t)seg * seg_p, &ctl->psize[cur ^ 1][seg][0]);
     }
-    // wave per adjacency piece of the frontier.  Decrementing a vertex that
-    // already left keeps its value >= DEAD_MIN (it loses at most its degree).
-    const uint32_t wid = gtid / KCDS_WAVE, nw = gstride / KCDS_WAVE;
-    for (uint32_t it = wid; it < ps; it += nw) {
-      uint32_t g = 0;
-      while (it >= s_p[g + 1]) ++g;
-      const ull item = fp[(size_t)g * seg_p + (it - s_p[g])];
-      const uint32_t st = (uint32_t)(item >> 32), e = (uint32_t)item;
-      uint32_t u[PUNROLL], dv[PUNROLL];
+    // PLANES lanes per adjacency piece of the frontier.  Decrementing a vertex
+    // that already left keeps its value >= DEAD_MIN (it loses at most its
+    // degree).  The wide vertices a late level removes have mostly departed
+    // neighbours: the departed bitmap (n/8 bytes, cache resident; bits are
+    // only ever set, so a stale word only costs an atomic) screens them out.
+    // A large level's decrements pile onto few counters (at a mid level of
+    // an R-MAT graph, 700,000 decrements onto 15,000 vertices, thousands onto
+    // each hub), and same-address device atomics serialise.  In a step of at
+    // least AGG_MIN_PIECES pieces a workgroup round (4 waves x 64 / PLANES
+    // pieces) therefore sums its decrements per counter in an LDS table and
+    // applies each sum with one atomic: the counter crossed from above L to L
+    // or below exactly when old > L >= old - sum, which one atomic observes
+    // (departed counters, >= DEAD_MIN, never satisfy it).
+    constexpr uint32_t PW = KCDS_WAVE / PLANES;  // pieces per wave
+    constexpr uint32_t BPW = 4 * PW;             // pieces per workgroup round (256 threads)
+    const uint32_t sub = lane % PLANES, wv = tid / KCDS_WAVE;
+    const bool agg = ps >= AGG_MIN_PIECES;       // uniform over the grid
+    if (agg && ps > blockIdx.x * BPW) {
+      for (uint32_t t = tid; t < DEC_TB; t += 256) {
+        t_key[t] = 0xFFFFFFFFu;
+        t_cnt[t] = 0;
+      }
+      if (tid < 2) t_nused[tid] = 0;
+      lds_sync();
+    }
+    uint32_t par = 0;
+    for (uint32_t it0 = blockIdx.x * BPW; it0 < ps; it0 += gridDim.x * BPW, par ^= 1) {
+      const uint32_t it = it0 + wv * PW + lane / PLANES;
+      uint32_t st = 0, e = 0;
+      if (it < ps) {
+        uint32_t g = 0;
+        while (it >= s_p[g + 1]) ++g;
+        const ull item = fp[(size_t)g * seg_p + (it - s_p[g])];
+        st = (uint32_t)(item >> 32);
+        e = (uint32_t)item;
+      }
+      uint32_t u[PUNROLL], gw[PUNROLL];
 #pragma unroll
       for (uint32_t x = 0; x < PUNROLL; ++x) {
-        const uint32_t j = st + x * KCDS_WAVE + lane;
+        const uint32_t j = st + x * PLANES + sub;
         u[x] = j < e ? nbr[j] : 0u;
       }
 #pragma unroll
       for (uint32_t x = 0; x < PUNROLL; ++x) {
-        const uint32_t j = st + x * KCDS_WAVE + lane;
-        dv[x] = j < e ? atomicSub(&deg[slot_of(sc, u[x])], 1u) : DEAD;
+        const uint32_t j = st + x * PLANES + sub;
+        gw[x] = j < e ? gone[u[x] >> 5] : 0xFFFFFFFFu;
+      }
+      if (!agg) {
+        uint32_t dv[PUNROLL];
+#pragma unroll
+        for (uint32_t x = 0; x < PUNROLL; ++x)
+          dv[x] = (gw[x] >> (u[x] & 31)) & 1u ? DEAD : atomicSub(&deg[slot_of(sc, u[x])], 1u);
+#pragma unroll
+        for (uint32_t x = 0; x < PUNROLL; ++x)
+          wave_append(dv[x] == level + 1, u[x], off, nv + (size_t)seg * seg_v, &ctl->fsize[cur ^ 1][seg][0],
+                      np + (size_t)seg * seg_p, &ctl->psize[cur ^ 1][seg][0]);
+        continue;
       }
 #pragma unroll
-      for (uint32_t x = 0; x < PUNROLL; ++x)
-        wave_append(dv[x] == level + 1, u[x], off, nv + (size_t)seg * seg_v, &ctl->fsize[cur ^ 1][seg][0],
-                    np + (size_t)seg * seg_p, &ctl->psize[cur ^ 1][seg][0]);
+      for (uint32_t x = 0; x < PUNROLL; ++x) {
+        if ((gw[x] >> (u[x] & 31)) & 1u) continue;
+        const uint32_t key = slot_of(sc, u[x]);
+        uint32_t h = (key * 0x9E3779B1u) >> (32 - 12);
+        while (true) {
+          const uint32_t prev = atomicCAS(&t_key[h], 0xFFFFFFFFu, key);
+          if (prev == 0xFFFFFFFFu) t_used[atomicAdd(&t_nused[par], 1u)] = h;
+          if (prev == 0xFFFFFFFFu || prev == key) break;
+          h = (h + 1) & (DEC_TB - 1);
+        }
+        atomicAdd(&t_cnt[h], 1u);
+      }
+      lds_sync();
+      // apply the used entries (one atomic each) and clear them; the other
+      // parity's counter is free again for the next round
+      const uint32_t nu = t_nused[par];
+      if (tid == 0) t_nused[par ^ 1] = 0;
+      for (uint32_t i0 = 0; i0 < nu; i0 += 4 * 256) {
+        uint32_t ck[4], cc[4], old[4];
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+          const uint32_t i = i0 + q * 256 + tid;
+          ck[q] = 0xFFFFFFFFu;
+          cc[q] = 0;
+          if (i < nu) {
+            const uint32_t h = t_used[i];
+            ck[q] = t_key[h];
+            cc[q] = t_cnt[h];
+            t_key[h] = 0xFFFFFFFFu;
+            t_cnt[h] = 0;
+          }
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) old[q] = cc[q] ? atomicSub(&deg[ck[q]], cc[q]) : 0u;
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q)
+          wave_append(cc[q] && old[q] > level && old[q] - cc[q] <= level, vertex_of(sc, ck[q]), off,
+                      nv + (size_t)seg * seg_v, &ctl->fsize[cur ^ 1][seg][0], np + (size_t)seg * seg_p,
+                      &ctl->psize[cur ^ 1][seg][0]);
+      }
+      lds_sync();
     }
   }
   // two-level ticket: the last workgroup of a segment, then the last segment.
@@ -389,11 +655,12 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
   if (s_last && tid < KCDS_WAVE) order_control(ctl, n, level, target, base, cur, phase, sh);
 }
 
-__global__ void k_init_deg(const uint32_t *off, uint32_t n, Scatter sc, uint32_t *deg) {
+__global__ void k_init_deg(const uint32_t *off, uint32_t n, Scatter sc, uint32_t *deg, uint32_t *gone) {
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p <= sc.mask; p += gridDim.x * blockDim.x) {
     const uint32_t v = vertex_of(sc, p);
     deg[p] = v < n ? off[v + 1] - off[v] : DEAD;
   }
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < (n + 31) / 32; w += gridDim.x * blockDim.x) gone[w] = 0;
 }
 
 }  // namespace
@@ -412,16 +679,18 @@ void degeneracy_rank_gpu(Ctx &c, uint32_t slack_shift) {
   const uint32_t K = sc.mask + 1;
   const size_t kslots = ((size_t)K + 63) / 64 * 64;
   const size_t bytes = sizeof(uint32_t) * (kslots + 2 * NSEG * (size_t)seg_v) +
-                       sizeof(ull) * 2 * NSEG * (size_t)seg_p + sizeof(OrderCtl) + 1024;
+                       sizeof(ull) * 2 * NSEG * (size_t)seg_p + sizeof(OrderCtl) + 1024 +
+                       sizeof(uint32_t) * (((size_t)n + 31) / 32) + 256;
   buf_reserve(c, c.tmp0, bytes);
   uint32_t *deg = c.tmp0.as<uint32_t>();
   uint32_t *fv0 = deg + kslots, *fv1 = fv0 + (size_t)NSEG * seg_v;
   ull *fp0 = (ull *)(((uintptr_t)(fv1 + (size_t)NSEG * seg_v) + 255) & ~(uintptr_t)255);
   ull *fp1 = fp0 + (size_t)NSEG * seg_p;
   OrderCtl *ctl = (OrderCtl *)(((uintptr_t)(fp1 + (size_t)NSEG * seg_p) + 255) & ~(uintptr_t)255);
-  const uint32_t vgrid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, c.num_cus));
+  uint32_t *gone = (uint32_t *)(((uintptr_t)(ctl + 1) + 255) & ~(uintptr_t)255);
+  const uint32_t vgrid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, (uint64_t)c.num_cus * 2));
   k_init_deg<<<(uint32_t)std::min<uint64_t>((K + 255) / 256, (uint64_t)c.num_cus * 16), 256, 0, c.stream>>>(
-      c.off.as<uint32_t>(), n, sc, deg);
+      c.off.as<uint32_t>(), n, sc, deg, gone);
   KCDS_LAUNCH_CHECK();
   OrderCtl *h0 = (OrderCtl *)c.host_scratch(sizeof(OrderCtl));
   memset(h0, 0, sizeof(OrderCtl));
@@ -437,7 +706,7 @@ void degeneracy_rank_gpu(Ctx &c, uint32_t slack_shift) {
   while (true) {
     for (uint32_t r = 0; r < batch; ++r) {
       k_ko_step<<<vgrid, 256, 0, c.stream>>>(ctl, n, off, nbr, deg, fv0, fv1, fp0, fp1, seg_v, seg_p, rank, inv,
-                                             slack_shift, sc);
+                                             slack_shift, sc, gone);
       KCDS_LAUNCH_CHECK();
     }
     rounds += batch;
@@ -458,14 +727,18 @@ bool orient_from_rank(Ctx &c, bool approx) {
   buf_reserve(c, c.doff, sizeof(uint32_t) * ((size_t)n + 1));
   buf_reserve(c, c.tmp1, sizeof(uint32_t) * ((size_t)n + 1));
   buf_reserve(c, c.tmp0, sizeof(uint32_t) * ((size_t)c.nnz + 1));
-  buf_reserve(c, c.tmp2, sizeof(uint32_t) * ((size_t)n + 64));
+  buf_reserve(c, c.tmp2, sizeof(uint32_t) * (2 * (size_t)n + 128));
   uint32_t *dout = c.tmp1.as<uint32_t>(), *tmp = c.tmp0.as<uint32_t>();
-  uint32_t *nheavy = c.tmp2.as<uint32_t>(), *heavy = nheavy + 64;
+  uint32_t *nheavy = c.tmp2.as<uint32_t>(), *nmid = nheavy + 1, *heavy = nheavy + 64, *mid = heavy + n + 64;
   if (n) {
-    fill_u32(c, nheavy, 0, 1);
-    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 3) / 4, (uint64_t)c.num_cus * 40);
-    k_dag_gather<<<grid, 256, 0, c.stream>>>(c.off.as<uint32_t>(), c.nbr.as<uint32_t>(), c.rank.as<uint32_t>(), n,
-                                             dout, tmp, heavy, nheavy);
+    fill_u32(c, nheavy, 0, 2);
+    const uint32_t tiles = (n + 63) / 64;
+    const uint32_t tgrid = (uint32_t)std::min<uint64_t>((tiles + 3) / 4, (uint64_t)c.num_cus * 8);
+    k_dag_gather_tiles<<<tgrid, 256, 0, c.stream>>>(c.off.as<uint32_t>(), c.nbr.as<uint32_t>(),
+                                                    c.rank.as<uint32_t>(), n, dout, tmp, mid, nmid, heavy, nheavy);
+    KCDS_LAUNCH_CHECK();
+    k_dag_gather<<<c.num_cus * 8, 256, 0, c.stream>>>(c.off.as<uint32_t>(), c.nbr.as<uint32_t>(),
+                                                      c.rank.as<uint32_t>(), mid, nmid, dout, tmp);
     KCDS_LAUNCH_CHECK();
     k_dag_gather_heavy<<<c.num_cus, HEAVY_THREADS, 0, c.stream>>>(c.off.as<uint32_t>(), c.nbr.as<uint32_t>(),
                                                                    c.rank.as<uint32_t>(), heavy, nheavy, dout, tmp);
@@ -487,7 +760,7 @@ bool orient_from_rank(Ctx &c, bool approx) {
     fail(KCDS_E_LIMIT, "DAG out-degree %u exceeds %u (orientation not degeneracy-bounded)", c.dmax, SORT_WAVE);
   buf_reserve(c, c.dnbr, sizeof(uint32_t) * (c.darcs + 1));
   if (n) {
-    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 3) / 4, (uint64_t)c.num_cus * 40);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 15) / 16, (uint64_t)c.num_cus * 40);
     k_dag_copy<<<grid, 256, 0, c.stream>>>(c.off.as<uint32_t>(), c.inv.as<uint32_t>(), n, c.doff.as<uint32_t>(),
                                            tmp, c.dnbr.as<uint32_t>());
     KCDS_LAUNCH_CHECK();
