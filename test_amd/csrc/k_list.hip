@@ -510,10 +510,11 @@ __global__ __launch_bounds__(256) void k_class_count(const uint32_t *doff, uint3
   if (threadIdx.x < 5 && sc[threadIdx.x]) atomicAdd(&ccount[threadIdx.x], sc[threadIdx.x]);
 }
 
-// Writes each root into its class's range: one global atomic per class per
+// Writes each root into its class's range (classes in order, each starting
+// where the counts of the earlier ones end): one global atomic per class per
 // 256 roots.
 __global__ __launch_bounds__(256) void k_class_fill(const uint32_t *doff, uint32_t n, uint32_t k,
-                                                    const uint32_t *cstart, uint32_t *cfill, uint4 *roots) {
+                                                    const uint32_t *ccount, uint32_t *cfill, uint4 *roots) {
   __shared__ uint32_t wc[4][NCLASS], bbase[NCLASS];
   const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
   for (uint32_t r0 = blockIdx.x * 256; r0 < n; r0 += gridDim.x * 256) {
@@ -535,7 +536,9 @@ __global__ __launch_bounds__(256) void k_class_fill(const uint32_t *doff, uint32
     if (threadIdx.x < NCLASS) {
       const uint32_t i = threadIdx.x;
       const uint32_t tot = wc[0][i] + wc[1][i] + wc[2][i] + wc[3][i];
-      bbase[i] = tot ? cstart[i] + atomicAdd(&cfill[i], tot) : 0u;
+      uint32_t start = 0;
+      for (uint32_t j = 0; j < i; ++j) start += ccount[j];
+      bbase[i] = tot ? start + atomicAdd(&cfill[i], tot) : 0u;
     }
     __syncthreads();
     if (c >= 0 && c < NCLASS) {
@@ -688,7 +691,7 @@ void classify_roots(Ctx &c, uint32_t k) {
   buf_reserve(c, c.qctr, QCTR_WORDS * sizeof(uint32_t));
   uint32_t *q = c.qctr.as<uint32_t>();
   HIP_CHECK(hipMemsetAsync(q, 0, QCTR_WORDS * sizeof(uint32_t), c.stream));
-  uint32_t *ccount = q + 16, *cstart = q + 24, *cfill = q + 32;
+  uint32_t *ccount = q + 16, *cfill = q + 32;
   const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((n + 255) / 256, 256));
   if (n) {
     k_class_count<<<grid, 256, 0, c.stream>>>(c.doff.as<uint32_t>(), n, k, ccount);
@@ -699,19 +702,15 @@ void classify_roots(Ctx &c, uint32_t k) {
   c.sync();
   if (h[4]) fail(KCDS_E_LIMIT, "%u roots have out-degree above %u (the LDS row limit)", h[4], MAX_LDS_DEG);
   uint32_t s = 0;
-  uint32_t hs[8] = {0};
   for (int i = 0; i < NCLASS; ++i) {
     c.class_count[i] = h[i];
     c.class_start[i] = s;
-    hs[i] = s;
     s += h[i];
   }
-  HIP_CHECK(hipMemcpyAsync(cstart, hs, 8 * sizeof(uint32_t), hipMemcpyHostToDevice, c.stream));
   if (n && s) {
-    k_class_fill<<<grid, 256, 0, c.stream>>>(c.doff.as<uint32_t>(), n, k, cstart, cfill, c.roots.as<uint4>());
+    k_class_fill<<<grid, 256, 0, c.stream>>>(c.doff.as<uint32_t>(), n, k, ccount, cfill, c.roots.as<uint4>());
     KCDS_LAUNCH_CHECK();
   }
-  c.sync();  // hs lives on the host stack
 }
 
 // One launch of class cls over its roots; counters accumulate into the
@@ -854,18 +853,16 @@ void list_count(Ctx &c, uint32_t k, ull cap) {
   buf_reserve(c, c.cnt, sizeof(ull) * cstride * ncopy);
   buf_reserve(c, c.rootT, sizeof(ull) * ((size_t)n + 2));
   buf_reserve(c, c.lsum, sizeof(ull) * QSHARDS * 8 * 2);
-  HIP_CHECK(hipMemsetAsync(c.cnt.p, 0, sizeof(ull) * cstride * ncopy, c.stream));
-  HIP_CHECK(hipMemsetAsync(c.rootT.p, 0, sizeof(ull) * ((size_t)n + 2), c.stream));
   ull *totals = c.lsum.as<ull>();
-  HIP_CHECK(hipMemsetAsync(totals, 0, sizeof(ull) * QSHARDS * 8, c.stream));
-  if (cap) {
-    buf_reserve(c, c.gsum, sizeof(ull) * ((size_t)c.ngraphs + 1));
-    HIP_CHECK(hipMemsetAsync(c.gsum.p, 0, sizeof(ull) * c.ngraphs, c.stream));
-  }
+  if (cap) buf_reserve(c, c.gsum, sizeof(ull) * ((size_t)c.ngraphs + 1));
+  zero_u64(c, c.cnt.as<ull>(), cstride * ncopy, c.rootT.as<ull>(), (uint64_t)n + 2, totals, (uint64_t)QSHARDS * 8,
+           cap ? c.gsum.as<ull>() : nullptr, cap ? c.ngraphs : 0);
   c.count_cap = cap;
-  hipEvent_t e0, e1;
-  HIP_CHECK(hipEventCreate(&e0));
-  HIP_CHECK(hipEventCreate(&e1));
+  if (!c.ltime[0]) {
+    HIP_CHECK(hipEventCreate(&c.ltime[0]));
+    HIP_CHECK(hipEventCreate(&c.ltime[1]));
+  }
+  hipEvent_t e0 = c.ltime[0], e1 = c.ltime[1];
   HIP_CHECK(hipEventRecord(e0, c.stream));
   // The classes share only atomic counters, so they run at once: the widest
   // on the context stream, the others on three more streams (default
@@ -902,8 +899,6 @@ void list_count(Ctx &c, uint32_t k, ull cap) {
   read_totals(c, totals, T, w2);
   float ms = 0;
   HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-  HIP_CHECK(hipEventDestroy(e0));
-  HIP_CHECK(hipEventDestroy(e1));
   c.list_kernel_ms = ms;
   c.list_kernel_launches = launches;
   c.T = T;

@@ -78,7 +78,16 @@ template <typename In, typename Out> __global__ void k_last_total(const In *in, 
   out[len] = out[len - 1] + (Out)in[len - 1];
 }
 
-template <typename In, typename Out> void scan_impl(Ctx &c, const In *in, Out *out, uint64_t len) {
+// Words of block sums a scan of len entries needs at every level below it.
+uint64_t scan_ws_words(uint64_t len) {
+  uint64_t w = 0;
+  for (uint64_t nb = (len + SCAN_CHUNK - 1) / SCAN_CHUNK; nb > 1; nb = (nb + SCAN_CHUNK - 1) / SCAN_CHUNK)
+    w += 2 * nb + 1;
+  return w;
+}
+
+// ws: scan_ws_words(len) entries of block sums (bsum, then boff, per level).
+template <typename In, typename Out> void scan_impl(Ctx &c, const In *in, Out *out, uint64_t len, Out *ws) {
   if (len == 0) {
     HIP_CHECK(hipMemsetAsync(out, 0, sizeof(Out), c.stream));
     return;
@@ -92,18 +101,30 @@ template <typename In, typename Out> void scan_impl(Ctx &c, const In *in, Out *o
     KCDS_LAUNCH_CHECK();
     return;
   }
-  Out *bsum = nullptr, *boff = nullptr;
-  HIP_CHECK(hipMallocAsync((void **)&bsum, nb * sizeof(Out), c.stream));
-  HIP_CHECK(hipMallocAsync((void **)&boff, (nb + 1) * sizeof(Out), c.stream));
+  Out *bsum = ws, *boff = ws + nb;
   k_chunk_sums<In, Out><<<(uint32_t)nb, SCAN_THREADS, 0, c.stream>>>(in, len, bsum);
   KCDS_LAUNCH_CHECK();
-  scan_impl<Out, Out>(c, bsum, boff, nb);
+  scan_impl<Out, Out>(c, bsum, boff, nb, ws + 2 * nb + 1);
   k_chunk_apply<In, Out><<<(uint32_t)nb, SCAN_THREADS, 0, c.stream>>>(in, out, len, boff);
   KCDS_LAUNCH_CHECK();
   k_write_total<Out><<<1, 1, 0, c.stream>>>(out, len, boff, nb);
   KCDS_LAUNCH_CHECK();
-  HIP_CHECK(hipFreeAsync(bsum, c.stream));
-  HIP_CHECK(hipFreeAsync(boff, c.stream));
+}
+
+template <typename In, typename Out> void scan_top(Ctx &c, const In *in, Out *out, uint64_t len) {
+  const uint64_t w = scan_ws_words(len);
+  if (w) buf_reserve(c, c.scan_ws, sizeof(Out) * w);
+  scan_impl<In, Out>(c, in, out, len, w ? c.scan_ws.as<Out>() : nullptr);
+}
+
+__global__ void k_zero4(ull *p0, uint64_t n0, ull *p1, uint64_t n1, ull *p2, uint64_t n2, ull *p3, uint64_t n3) {
+  const uint64_t tot = n0 + n1 + n2 + n3, stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += stride) {
+    if (i < n0) p0[i] = 0;
+    else if (i < n0 + n1) p1[i - n0] = 0;
+    else if (i < n0 + n1 + n2) p2[i - n0 - n1] = 0;
+    else p3[i - n0 - n1 - n2] = 0;
+  }
 }
 
 template <typename T> __global__ void k_fill(T *p, T v, uint64_t len) {
@@ -199,9 +220,20 @@ void gather_by_rank_u32(Ctx &c, const uint32_t *src_new, uint32_t *dst_orig) {
 }
 
 void scan_u32_to_u32(Ctx &c, const uint32_t *in, uint32_t *out, uint64_t len) {
-  scan_impl<uint32_t, uint32_t>(c, in, out, len);
+  scan_top<uint32_t, uint32_t>(c, in, out, len);
 }
-void scan_u64_to_u64(Ctx &c, const ull *in, ull *out, uint64_t len) { scan_impl<ull, ull>(c, in, out, len); }
+void scan_u64_to_u64(Ctx &c, const ull *in, ull *out, uint64_t len) { scan_top<ull, ull>(c, in, out, len); }
+
+void zero_u64(Ctx &c, ull *p0, uint64_t n0, ull *p1, uint64_t n1, ull *p2, uint64_t n2, ull *p3, uint64_t n3) {
+  if (!p0) n0 = 0;
+  if (!p1) n1 = 0;
+  if (!p2) n2 = 0;
+  if (!p3) n3 = 0;
+  const uint64_t tot = n0 + n1 + n2 + n3;
+  if (!tot) return;
+  k_zero4<<<grid_for(tot, 256), 256, 0, c.stream>>>(p0, n0, p1, n1, p2, n2, p3, n3);
+  KCDS_LAUNCH_CHECK();
+}
 
 void fill_u32(Ctx &c, uint32_t *p, uint32_t v, uint64_t len) {
   if (!len) return;

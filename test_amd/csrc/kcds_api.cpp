@@ -147,6 +147,8 @@ void kcds_destroy(kcds_ctx *ctx) {
     if (c.ljoin[i]) (void)hipEventDestroy(c.ljoin[i]);
   }
   if (c.lfork) (void)hipEventDestroy(c.lfork);
+  for (int i = 0; i < 2; ++i)
+    if (c.ltime[i]) (void)hipEventDestroy(c.ltime[i]);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete ctx;
 }

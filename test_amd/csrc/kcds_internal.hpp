@@ -105,6 +105,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   hipStream_t lstream[3] = {nullptr, nullptr, nullptr};  // listing classes run beside the context stream
   hipEvent_t lfork = nullptr, ljoin[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ltime[2] = {nullptr, nullptr};  // around the listing pass
   uint64_t budget = 0, used = 0;
   int num_cus = 0;
   std::string err;
@@ -177,6 +178,7 @@ struct Ctx {
 
   // generic temporaries
   DevBuf tmp0, tmp1, tmp2, tmp3;
+  DevBuf scan_ws;  // block sums of the multi-level scans
 
   // last listing kernel timing
   float list_kernel_ms = 0;
@@ -191,6 +193,9 @@ struct Ctx {
 void scan_u32_to_u32(Ctx &c, const uint32_t *in, uint32_t *out, uint64_t len);
 void scan_u64_to_u64(Ctx &c, const ull *in, ull *out, uint64_t len);
 void fill_u32(Ctx &c, uint32_t *p, uint32_t v, uint64_t len);
+// Zeroes up to four ull ranges with one launch (null or empty ranges skipped).
+void zero_u64(Ctx &c, ull *p0, uint64_t n0, ull *p1 = nullptr, uint64_t n1 = 0, ull *p2 = nullptr, uint64_t n2 = 0,
+              ull *p3 = nullptr, uint64_t n3 = 0);
 void fill_u64(Ctx &c, ull *p, ull v, uint64_t len);
 void max_u32(Ctx &c, const uint32_t *in, uint64_t len, uint32_t *out_dev);
 ull sum_u64_host(Ctx &c, const ull *in, uint64_t len);
