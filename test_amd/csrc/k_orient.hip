@@ -773,7 +773,7 @@ bool orient_from_rank(Ctx &c, bool approx) {
 
 // GPU orientation: the slack order (each threshold is the minimum live
 // degree rounded up by a half, so out-degrees stay within 3/2 of the
-// degeneracy, in a fraction of the steps: C2 60 instead of 609), or the
+// degeneracy, in a fraction of the steps: C2 53 instead of 609), or the
 // exact degeneracy order when KCDS_TUNE_EXACT_ORDER is set or the slack order
 // leaves an out-list wider than the listing's LDS rows.
 void orient_gpu(Ctx &c) {
