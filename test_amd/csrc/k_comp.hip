@@ -36,6 +36,19 @@ __device__ uint32_t uf_find(uint32_t *par, uint32_t x) {
   }
 }
 
+// Root of x followed through plain, cacheable loads.  Every value par[y] ever
+// held is y or a vertex of smaller key (hooks point down in key, path halving
+// to a grandparent), so the walk ends, at a vertex that was a root of x's set
+// when it was read; sets never split, so equal results for two vertices mean
+// one set.  (Unequal results may be stale: the caller then unites exactly.)
+__device__ uint32_t uf_find_ro(const uint32_t *par, uint32_t x) {
+  while (true) {
+    const uint32_t p = par[x];
+    if (p == x) return x;
+    x = p;
+  }
+}
+
 __device__ void uf_unite(uint32_t *par, const uint32_t *key, uint32_t a, uint32_t b) {
   while (true) {
     a = uf_find(par, a);
@@ -55,24 +68,44 @@ __global__ void k_uf_init(uint32_t n, uint32_t *par) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) par[v] = v;
 }
 
+// Points every vertex straight at its root (no unions run meanwhile).
+__global__ void k_uf_compress(uint32_t n, uint32_t *par) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const uint32_t r = uf_find(par, v);
+    if (r != v) __hip_atomic_store(&par[v], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // a clique is in its graph's k'-core iff its status (min member core) >= k'
 __device__ __forceinline__ bool core_clique(uint64_t c, uint32_t k, const uint32_t *cl, const int32_t *status,
                                             const GraphPeel *gp, const uint32_t *gid) {
   return status[c] >= (int32_t)gp[gid[cl[c * k]]].k_prime;
 }
 
-__global__ void k_uf_hook(uint64_t T, uint32_t k, const uint32_t *cl, const int32_t *status, const GraphPeel *gp,
-                          const uint32_t *gid, uint32_t *par, const uint32_t *key) {
-  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
-       c += (uint64_t)gridDim.x * blockDim.x) {
+// Unites the members of every stride-th clique.
+__global__ void k_uf_hook(uint64_t T, uint64_t stride, uint32_t k, const uint32_t *cl, const int32_t *status,
+                          const GraphPeel *gp, const uint32_t *gid, uint32_t *par, const uint32_t *key) {
+  const uint64_t ns = (T + stride - 1) / stride;
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t c = s * stride;
     if (!core_clique(c, k, cl, status, gp, gid)) continue;
     const uint32_t a = cl[c * k];
     // members with one common parent are in one set already: most cliques,
     // once the big components have formed, and this test reads no root (a
-    // root's entry is read by every find and would be one hot L2 line)
-    const uint32_t pa = ld(&par[a]);
+    // root's entry is read by every find and would be one hot L2 line).  The
+    // test uses plain, cacheable loads: every value par[x] ever held is an
+    // ancestor of x at that time and sets never split, so equal parents seen
+    // in any (possibly stale) copies still mean one set; a stale copy can
+    // only send the clique to the exact unite below.
+    const uint32_t pa = par[a];
     bool same = true;
-    for (uint32_t t = 1; t < k; ++t) same = same && ld(&par[cl[c * k + t]]) == pa;
+    for (uint32_t t = 1; t < k; ++t) same = same && par[cl[c * k + t]] == pa;
+    if (same) continue;
+    // then common roots through plain loads (a hub core makes the parents of
+    // members differ inside one set long before they all point at the root)
+    const uint32_t ra = uf_find_ro(par, a);
+    same = true;
+    for (uint32_t t = 1; t < k; ++t) same = same && uf_find_ro(par, cl[c * k + t]) == ra;
     if (same) continue;
     for (uint32_t t = 1; t < k; ++t) uf_unite(par, key, a, cl[c * k + t]);
   }
@@ -159,6 +192,8 @@ uint32_t grid1d(uint64_t len) {
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((len + 255) / 256, 65535ull * 16));
 }
 
+constexpr uint64_t UF_STRIDE0 = 4096, UF_STRIDE1 = 64;  // union-find sample passes
+
 }  // namespace
 
 void check_partition(Ctx &c) {
@@ -200,10 +235,22 @@ void components_run(Ctx &c) {
   const uint32_t *key = c.inv.as<uint32_t>();
   k_uf_init<<<grid1d(n), 256, 0, c.stream>>>(n, par);
   KCDS_LAUNCH_CHECK();
-  if (T) {
-    k_uf_hook<<<grid1d(T), 256, 0, c.stream>>>(T, k, c.cliques.as<uint32_t>(), c.status.as<int32_t>(), gp, gidp, par,
-                                               key);
+  // Every 4096th clique, then every 64th, then all, with every path
+  // compressed after each sample pass.  The samples, spread over the whole
+  // clique range, form the big components with few competing unions; after
+  // them a member's parent is its root, and almost every clique of the full
+  // pass leaves at the common-parent test without a find.  (One launch over
+  // all 6M cliques of a 697-vertex core had hundreds of thousands of threads
+  // in finds and root CASes at once: 1.8-2.1 ms.)
+  for (uint64_t stride : {UF_STRIDE0, UF_STRIDE1, (uint64_t)1}) {
+    if (stride > 1 && T < stride * 64) continue;  // small T: the full pass alone
+    k_uf_hook<<<grid1d((T + stride - 1) / stride), 256, 0, c.stream>>>(T, stride, k, c.cliques.as<uint32_t>(),
+                                                                        c.status.as<int32_t>(), gp, gidp, par, key);
     KCDS_LAUNCH_CHECK();
+    if (stride > 1) {
+      k_uf_compress<<<grid1d(n), 256, 0, c.stream>>>(n, par);
+      KCDS_LAUNCH_CHECK();
+    }
   }
   k_uf_label<<<grid1d(n), 256, 0, c.stream>>>(n, c.core.as<int32_t>(), gp, gidp, par, key, c.comp.as<uint32_t>(),
                                               g.flag.as<uint32_t>());
