@@ -22,6 +22,9 @@ namespace kcds {
 
 namespace {
 
+constexpr int CC_E = 8;        // cliques per thread in a component-count round
+constexpr int CC_TB = 4096;    // its LDS key table (2 * 256 * CC_E slots)
+
 __device__ __forceinline__ uint32_t ld(const uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -147,30 +150,56 @@ __global__ void k_scatter_vertices(uint32_t n, const uint32_t *cid, const uint32
   }
 }
 
-__global__ void k_count_comp_cliques(uint64_t T, uint32_t k, const uint32_t *cl, const int32_t *status,
-                                     const GraphPeel *gp, const uint32_t *gid, const uint32_t *cid, ull *ccount) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t c0 = (uint64_t)blockIdx.x * blockDim.x; c0 < T; c0 += stride) {
-    const uint64_t c = c0 + threadIdx.x;
-    const bool in = c < T && core_clique(c, k, cl, status, gp, gid);
-    wave_slot_by_key<ull>(in, in ? cid[cl[c * k]] : 0u, ccount);
+// Core cliques per component.  A batch has few components and a graph
+// usually one, so the cliques of a workgroup round (CC_E per thread) take
+// their component's count with one atomic per distinct component.
+__global__ __launch_bounds__(256) void k_count_comp_cliques(uint64_t T, uint32_t k, const uint32_t *cl,
+                                                            const int32_t *status, const GraphPeel *gp,
+                                                            const uint32_t *gid, const uint32_t *cid, ull *ccount) {
+  __shared__ KeyTable<CC_TB> tb;
+  const uint64_t round = 256ull * CC_E, stride = (uint64_t)gridDim.x * round;
+  for (uint64_t c0 = (uint64_t)blockIdx.x * round; c0 < T; c0 += stride) {
+    bool want[CC_E];
+    uint32_t key[CC_E];
+    ull slot[CC_E];
+#pragma unroll
+    for (int e = 0; e < CC_E; ++e) {
+      const uint64_t c = c0 + (uint64_t)e * 256 + threadIdx.x;
+      want[e] = c < T && core_clique(c, k, cl, status, gp, gid);
+      key[e] = want[e] ? cid[cl[c * k]] : 0u;
+    }
+    block_slot_by_key<CC_E, CC_TB, ull>(tb, want, key, slot, ccount);
   }
 }
 
 // core cliques grouped by component, members as local vertex ids
-__global__ void k_scatter_cliques(uint64_t T, uint32_t k, const uint32_t *cl, const int32_t *status,
-                                  const GraphPeel *gp, const uint32_t *gid, const uint32_t *cid, const uint32_t *local,
-                                  const ull *coff, ull *cfill, uint32_t *ccl) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t c0 = (uint64_t)blockIdx.x * blockDim.x; c0 < T; c0 += stride) {
-    const uint64_t c = c0 + threadIdx.x;
-    const bool in = c < T && core_clique(c, k, cl, status, gp, gid);
-    const uint32_t d = in ? cid[cl[c * k]] : 0u;
-    const ull slot = wave_slot_by_key<ull>(in, d, cfill);
-    if (!in) continue;
-    const ull at = coff[d] + slot;
-    if (at >= coff[d + 1]) continue;  // never taken: counts and scatter agree
-    for (uint32_t t = 0; t < k; ++t) ccl[at * k + t] = local[cl[c * k + t]];
+__global__ __launch_bounds__(256) void k_scatter_cliques(uint64_t T, uint32_t k, const uint32_t *cl,
+                                                         const int32_t *status, const GraphPeel *gp,
+                                                         const uint32_t *gid, const uint32_t *cid,
+                                                         const uint32_t *local, const ull *coff, ull *cfill,
+                                                         uint32_t *ccl) {
+  __shared__ KeyTable<CC_TB> tb;
+  const uint64_t round = 256ull * CC_E, stride = (uint64_t)gridDim.x * round;
+  for (uint64_t c0 = (uint64_t)blockIdx.x * round; c0 < T; c0 += stride) {
+    bool want[CC_E];
+    uint32_t key[CC_E];
+    ull slot[CC_E];
+#pragma unroll
+    for (int e = 0; e < CC_E; ++e) {
+      const uint64_t c = c0 + (uint64_t)e * 256 + threadIdx.x;
+      want[e] = c < T && core_clique(c, k, cl, status, gp, gid);
+      key[e] = want[e] ? cid[cl[c * k]] : 0u;
+    }
+    block_slot_by_key<CC_E, CC_TB, ull>(tb, want, key, slot, cfill);
+#pragma unroll
+    for (int e = 0; e < CC_E; ++e) {
+      if (!want[e]) continue;
+      const uint64_t c = c0 + (uint64_t)e * 256 + threadIdx.x;
+      const uint32_t d = key[e];
+      const ull at = coff[d] + slot[e];
+      if (at >= coff[d + 1]) continue;  // never taken: counts and scatter agree
+      for (uint32_t t = 0; t < k; ++t) ccl[at * k + t] = local[cl[c * k + t]];
+    }
   }
 }
 
@@ -285,14 +314,14 @@ void components_run(Ctx &c) {
   HIP_CHECK(hipMemsetAsync(g.ccount.p, 0, sizeof(ull) * ((size_t)C + 2), c.stream));
   HIP_CHECK(hipMemsetAsync(g.cfill.p, 0, sizeof(ull) * ((size_t)C + 2), c.stream));
   if (T) {
-    k_count_comp_cliques<<<grid1d(T), 256, 0, c.stream>>>(T, k, c.cliques.as<uint32_t>(), c.status.as<int32_t>(),
+    k_count_comp_cliques<<<grid1d((T + CC_E - 1) / CC_E), 256, 0, c.stream>>>(T, k, c.cliques.as<uint32_t>(), c.status.as<int32_t>(),
                                                           gp, gidp, g.cid.as<uint32_t>(), g.ccount.as<ull>());
     KCDS_LAUNCH_CHECK();
   }
   scan_u64_to_u64(c, g.ccount.as<ull>(), g.coff.as<ull>(), C);
   buf_reserve(c, g.ccl, sizeof(uint32_t) * (core_cliques * k + 1));
   if (T) {
-    k_scatter_cliques<<<grid1d(T), 256, 0, c.stream>>>(T, k, c.cliques.as<uint32_t>(), c.status.as<int32_t>(), gp,
+    k_scatter_cliques<<<grid1d((T + CC_E - 1) / CC_E), 256, 0, c.stream>>>(T, k, c.cliques.as<uint32_t>(), c.status.as<int32_t>(), gp,
                                                        gidp, g.cid.as<uint32_t>(), g.local.as<uint32_t>(),
                                                        g.coff.as<ull>(), g.cfill.as<ull>(), g.ccl.as<uint32_t>());
     KCDS_LAUNCH_CHECK();

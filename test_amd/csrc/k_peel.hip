@@ -448,6 +448,11 @@ __global__ void k_graph_summary(uint32_t n, uint64_t T, uint32_t k, const uint32
       if (cv > 0) atomicMax(&gp[g].max_core, (uint32_t)cv);
     }
   }
+  // core cliques: a wave whose cliques all lie in one graph adds to a
+  // running count and flushes it when the graph changes (a lone graph's
+  // count then takes one atomic per wave, not one per 64 cliques)
+  uint32_t acc_g = 0xFFFFFFFFu;
+  ull acc = 0;
   for (uint64_t c0 = (uint64_t)blockIdx.x * blockDim.x; c0 < T; c0 += stride) {
     const uint64_t c = c0 + threadIdx.x;
     const bool valid = c < T;
@@ -456,11 +461,17 @@ __global__ void k_graph_summary(uint32_t n, uint64_t T, uint32_t k, const uint32
     const uint32_t g0 = __shfl(g, 0, 64);
     if (__ballot(valid && g != g0) == 0ull) {
       const uint32_t cc = (uint32_t)__popcll(__ballot(in));
-      if (lane == 0 && cc) atomicAdd(&gp[g0].core_cliques, (ull)cc);
+      if (g0 != acc_g) {
+        if (lane == 0 && acc) atomicAdd(&gp[acc_g].core_cliques, acc);
+        acc_g = g0;
+        acc = 0;
+      }
+      acc += cc;
     } else if (in) {
       atomicAdd(&gp[g].core_cliques, 1ull);
     }
   }
+  if (lane == 0 && acc) atomicAdd(&gp[acc_g].core_cliques, acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -851,7 +862,8 @@ void peel_run(Ctx &c) {
   c.peel_steps = h0->steps;
   k_graph_kprime<<<grid1d(G), 256, 0, c.stream>>>(G, gp);
   KCDS_LAUNCH_CHECK();
-  k_graph_summary<<<grid, 256, 0, c.stream>>>(n, T, k, gidp, core, status, cl, gp);
+  k_graph_summary<<<std::min<uint32_t>(grid, c.num_cus * 8), 256, 0, c.stream>>>(n, T, k, gidp, core, status, cl,
+                                                                                gp);
   KCDS_LAUNCH_CHECK();
   PeelCtl *hs = (PeelCtl *)c.host_scratch(sizeof(PeelCtl) + 64);
   HIP_CHECK(hipMemcpyAsync(hs, ctl, sizeof(PeelCtl), hipMemcpyDeviceToHost, c.stream));
