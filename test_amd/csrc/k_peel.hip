@@ -264,17 +264,21 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
         uint32_t c = 0;
         if (j < item.en) {
           c = inc[j];
-          mine = status[c] == -1 && atomicCAS(&status[c], -1, (int32_t)L) == -1;
+          mine = atomicCAS(&status[c], -1, (int32_t)L) == -1;
         }
         destroyed += mine;
-        // the k members' ids and states are loaded together, then every
-        // decrement is issued before any result is used
+        // the k members' ids are loaded together, then every decrement is
+        // issued before any result is used.  A clique still live when claimed
+        // has no member removed in an earlier sub-round (removing a vertex
+        // destroys all its live cliques), so a member's state needs no read:
+        // one in the current frontier has count <= L, and its decrement can
+        // neither join nor lower the next level's start
         uint32_t u[KMAX];
         bool dec[KMAX];
 #pragma unroll
         for (uint32_t t = 0; t < KMAX; ++t) u[t] = mine && t < k ? cl[(ull)c * k + t] : v;
 #pragma unroll
-        for (uint32_t t = 0; t < KMAX; ++t) dec[t] = u[t] != v && core[u[t]] == -1;
+        for (uint32_t t = 0; t < KMAX; ++t) dec[t] = u[t] != v;
         ull old[KMAX];
         uint32_t rank[KMAX], leader[KMAX];
 #pragma unroll
@@ -312,10 +316,14 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
   // (only the control step writes them)
   if (pl_last_block(ctl, &s_last) && tid < KCDS_WAVE) {
     uint32_t fc = 0, fn = 0;
-    ull mn = ~0ull;
+    ull mn = ~0ull, dn = ~0ull;
     if (lane < PSEG) {
       fc = ldc_u32(&ctl->fsize[cur][lane][0]);
       fn = ldc_u32(&ctl->fsize[cur ^ 1][lane][0]);
+      // the other workgroups' atomics on these completed before their
+      // tickets, so dmin is read here with the rest and reset below only
+      // when the level ends (no second round trip after fnext is known)
+      if (phase == 1) dn = ldc_u64(&ctl->dmin[lane][0]);
       mn = atomicExch(&ctl->gmin[lane][0], ~0ull);
       ctl->gblocks[lane][0] = 0;
       if (phase == 1) {
@@ -327,8 +335,8 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mn = min(mn, (ull)__shfl_xor(mn, o, 64));
     // end of a level: the lowest count its decrements left above it
-    ull dn = ~0ull;
-    if (phase == 1 && fnext == 0 && lane < PSEG) dn = atomicExch(&ctl->dmin[lane][0], ~0ull);
+    if (!(phase == 1 && fnext == 0)) dn = ~0ull;
+    else if (lane < PSEG) stc_u64(&ctl->dmin[lane][0], ~0ull);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) dn = min(dn, (ull)__shfl_xor(dn, o, 64));
     if (lane == 0) {
