@@ -245,8 +245,9 @@ struct BlockResult {
 };
 void dinkelbach_blocks(Ctx &c, const std::vector<BlockTask> &in, std::vector<BlockResult> &out, uint32_t max_iters);
 void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t max_iters);
-// Components with at least this many cliques are solved by the grid solver.
-constexpr uint64_t GRID_FLOW_MIN_T = 1u << 16;
+// Components with at least this many cliques are solved by the grid solver;
+// with fewer components than a quarter of the CUs, from GRID_FLOW_ALONE_T up.
+constexpr uint64_t GRID_FLOW_MIN_T = 1u << 16, GRID_FLOW_ALONE_T = 1u << 12;
 constexpr uint32_t NO_SLACK = 32, ORDER_SLACK_SHIFT = 1;  // slack order: thresholds x + x/2
 constexpr uint32_t LIST_MAX_OUTDEG = 512;                  // widest out-list the listing stages in LDS
 // All tasks: large components one after another on the whole GPU, the rest in

@@ -94,11 +94,17 @@ void dinkelbach_components(Ctx &c, const std::vector<BlockTask> &in, std::vector
     HIP_CHECK(hipMemsetAsync(f.cside.p, 0, (size_t)g.h_voff.back() + 8, c.stream));
   }
   const bool all_grid = (c.tune_flags & KCDS_TUNE_GRID_FLOW) != 0;
+  // A batch of many components fills the GPU with one workgroup each; a few
+  // components would leave most of it idle, so there the grid solver takes
+  // them from GRID_FLOW_ALONE_T cliques up (a lone 36,000-clique component:
+  // 39 ms in one workgroup, 7 ms on the grid; the first 100 C5 requests'
+  // flow stage 522 -> 441 ms).
+  const ull grid_min = in.size() * 4 < (size_t)c.num_cus ? GRID_FLOW_ALONE_T : GRID_FLOW_MIN_T;
   std::vector<BlockTask> small;
   std::vector<size_t> small_at;
   for (size_t i = 0; i < in.size(); ++i) {
     const ull Tc = g.h_coff[in[i].d + 1] - g.h_coff[in[i].d];
-    if (all_grid || Tc >= GRID_FLOW_MIN_T) {
+    if (all_grid || Tc >= grid_min) {
       dinkelbach_grid(c, in[i], out[i], max_iters);
     } else {
       small.push_back(in[i]);
