@@ -176,6 +176,10 @@ int kcds_sync(kcds_ctx *ctx);
  *                        sequences one kernel at a time instead of replaying
  *                        them as captured HIP graphs (same results). */
 #define KCDS_TUNE_NO_GRAPHS 8u
+/*   KCDS_TUNE_BLOCK_FLOW  solve every component with the workgroup max-flow
+ *                        (same results; used to test one solver against the
+ *                        other).  KCDS_TUNE_GRID_FLOW takes precedence. */
+#define KCDS_TUNE_BLOCK_FLOW 16u
 int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
 
 /* Listing launch shape: waves per root (workgroup size / 64, 1..8) of each

@@ -99,7 +99,9 @@ void dinkelbach_components(Ctx &c, const std::vector<BlockTask> &in, std::vector
   // them from GRID_FLOW_ALONE_T cliques up (a lone 36,000-clique component:
   // 39 ms in one workgroup, 7 ms on the grid; the first 100 C5 requests'
   // flow stage 522 -> 441 ms).
-  const ull grid_min = in.size() * 4 < (size_t)c.num_cus ? GRID_FLOW_ALONE_T : GRID_FLOW_MIN_T;
+  const ull grid_min = (c.tune_flags & KCDS_TUNE_BLOCK_FLOW) ? ~0ull
+                       : in.size() * 4 < (size_t)c.num_cus ? GRID_FLOW_ALONE_T
+                                                          : GRID_FLOW_MIN_T;
   std::vector<BlockTask> small;
   std::vector<size_t> small_at;
   for (size_t i = 0; i < in.size(); ++i) {

@@ -29,7 +29,7 @@ EXPORTS = ("kcds_create", "kcds_destroy", "kcds_last_error", "kcds_upload_csr", 
            "kcds_autotune_list", "kcds_set_graph_budget")
 
 GRAPH_SOLVED, GRAPH_OVER_BUDGET = 0, 1
-TUNE_GRID_PEEL, TUNE_GRID_FLOW, TUNE_EXACT_ORDER, TUNE_NO_GRAPHS = 1, 2, 4, 8
+TUNE_GRID_PEEL, TUNE_GRID_FLOW, TUNE_EXACT_ORDER, TUNE_NO_GRAPHS, TUNE_BLOCK_FLOW = 1, 2, 4, 8, 16
 E_INVALID, E_HIP, E_NOMEM, E_STATE, E_LIMIT = -1, -2, -3, -4, -5
 
 

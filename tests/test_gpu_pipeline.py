@@ -89,11 +89,12 @@ def _peel_and_components(solver, name, off, nbr, ks):
         assert ncomp == len(set(int(x) for x in comp_r if x != 0xFFFFFFFF))
 
 
-# block_flow: the workgroup max-flow; grid_flow: the grid-wide one, its
-# relabel and sweep sequences replayed as captured HIP graphs (many replays
-# per flow, with host reads and syncs between them); grid_flow_eager: the
-# same launches one at a time
-FLOWS = {"block_flow": 0, "grid_flow": 2, "grid_flow_eager": 2 | 8}
+# block_flow: the workgroup max-flow for every component; default: each
+# component to the solver the library picks; grid_flow: the grid-wide one,
+# its relabel and sweep sequences replayed as captured HIP graphs (many
+# replays per flow, with host reads and syncs between them);
+# grid_flow_eager: the same launches one at a time
+FLOWS = {"block_flow": 16, "default_flow": 0, "grid_flow": 2, "grid_flow_eager": 2 | 8}
 
 
 @pytest.mark.parametrize("flow", list(FLOWS))
