@@ -54,6 +54,8 @@ struct GFCtl {
   uint32_t changed[SWEEPS + 1][64];
   ull active[NSEG][8];
   ull sums[2][8];  // side vertices, side cliques
+  ull udeg[8];       // arcs of the vertex nodes the BFS has not reached
+  ull udeg_snap[8];  // udeg when the current level started
 };
 
 __device__ __forceinline__ void add64(i64 *p, i64 v) { atomicAdd((ull *)p, (ull)v); }
@@ -168,7 +170,7 @@ __global__ void k_gf_init(uint32_t nc, uint64_t Tc, uint32_t k, i64 p, i64 q, co
 // heights = N except t = 0; phase counters cleared; the frontier starts at
 // level 1 (k_gf_bfs_seed), so it never holds t, whose nc arcs would be one
 // node's work.
-__global__ void k_gf_bfs_init(uint32_t N, uint32_t *height, GFCtl *ctl) {
+__global__ void k_gf_bfs_init(uint32_t N, uint32_t nc, const uint32_t *off, uint32_t *height, GFCtl *ctl) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < N; x += stride)
     height[x] = x == N - 1 ? 0u : N;
@@ -186,6 +188,8 @@ __global__ void k_gf_bfs_init(uint32_t N, uint32_t *height, GFCtl *ctl) {
       ctl->dist = 1;
       ctl->cur = 0;
       ctl->done = 0;
+      ctl->udeg[0] = off[nc];  // every vertex arc (the clique arcs start there)
+      ctl->udeg_snap[0] = off[nc];
     }
   }
 }
@@ -199,6 +203,8 @@ __global__ __launch_bounds__(256) void k_gf_bfs_seed(uint32_t nc, const uint32_t
     const uint32_t v = v0 + threadIdx.x;
     const bool add = v < nc && res[off[v]] > 0;
     if (add) height[v] = 1;
+    const ull gone = wave_sum_u64(add ? (ull)(off[v + 1] - off[v]) : 0ull);
+    if (lane == 0 && gone) atomicSub(&ctl->udeg[0], gone);
     const ull bal = __ballot(add);
     if (bal) {
       const uint32_t lead = (uint32_t)__builtin_ctzll(bal);
@@ -225,6 +231,56 @@ __device__ bool last_block(GFCtl *ctl, uint32_t *s_last) {
   }
   lds_sync();
   return *s_last != 0;
+}
+
+// Levels alternate: vertex nodes at odd distances (t is at 0), clique nodes at
+// even ones.  A clique level of fs nodes costs fs * k arc checks pushed from
+// the frontier; pulled from the vertex side it costs at most the arcs of the
+// vertices not reached yet, and usually far less (a vertex stops at its first
+// frontier clique).  Pull when the first bound is the larger.  Every workgroup
+// of k_gf_bfs_vchunks and k_gf_bfs_level decides from the same values (the
+// frontier and the arc sum when the level started), so the level is done by
+// exactly one of them.
+__device__ __forceinline__ bool bfs_pull_level(uint32_t dist, uint32_t fs, uint32_t k, const GFCtl *ctl) {
+  return (dist & 1u) == 0 && fs > 0 && (ull)fs * k > ldc_u64(&ctl->udeg_snap[0]);
+}
+
+// BFS clique level from the vertex side (when bfs_pull_level): wave per
+// chunk of an unreached vertex node u; u gets height dist + 1 at its first
+// arc u -> y with residual capacity and y at height dist (the condition the
+// push form checks from y).  Arcs to t and s never match (t is at 0, s at N).
+__device__ __forceinline__ void bfs_pull(uint32_t N, uint32_t dist, uint32_t nch, const uint32_t *choff,
+                                         const uint32_t *chv, const uint32_t *off, const uint32_t *head,
+                                         const i64 *res, uint32_t *height, uint32_t *nx, uint32_t *nxsize,
+                                         uint32_t seg_cap, GFCtl *ctl) {
+  const uint32_t lane = lane_id(), seg = blockIdx.x % NSEG;
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KCDS_WAVE, nw = gridDim.x * blockDim.x / KCDS_WAVE;
+  for (uint32_t ch = wid; ch < nch; ch += nw) {
+    const uint32_t u = chv[ch];
+    if (height[u] != N) continue;
+    const uint32_t a0 = off[u] + (ch - choff[u]) * VCH, a1 = min(a0 + VCH, off[u + 1]);
+    for (uint32_t b = a0; b < a1; b += VU * KCDS_WAVE) {
+      uint32_t y[VU];
+      i64 rr[VU];
+#pragma unroll
+      for (uint32_t q = 0; q < VU; ++q) {
+        const uint32_t a = b + q * KCDS_WAVE + lane;
+        y[q] = a < a1 ? head[a] : N - 1;
+        rr[q] = a < a1 ? res[a] : 0;
+      }
+      bool hit = false;
+#pragma unroll
+      for (uint32_t q = 0; q < VU; ++q) hit = hit || (rr[q] > 0 && height[y[q]] == dist);
+      if (__ballot(hit)) {
+        if (lane == 0 && atomicCAS(&height[u], N, dist + 1) == N) {
+          nx[(size_t)seg * seg_cap + atomicAdd(nxsize, 1u)] = u;
+          atomicSub(&ctl->udeg[0], (ull)(off[u + 1] - off[u]));
+        }
+        break;
+      }
+      if (__shfl(height[u], 0, 64) != N) break;  // another chunk of u found it
+    }
+  }
 }
 
 // One BFS level, clique nodes of the frontier: a thread per node y (k arcs);
@@ -254,7 +310,10 @@ __global__ __launch_bounds__(256) void k_gf_bfs_level(uint32_t N, uint32_t nv_no
   const uint32_t *fr = cur ? fr1 : fr0;
   uint32_t *nx = cur ? fr0 : fr1;
   const uint32_t s = N - 2;
-  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < fs; i0 += gridDim.x * blockDim.x) {
+  // a clique level whose frontier arcs outnumber the unreached vertices'
+  // arcs was pulled from the vertex side by k_gf_bfs_vchunks
+  const bool pulled = bfs_pull_level(dist, fs, k, ctl);
+  for (uint32_t i0 = blockIdx.x * blockDim.x; !pulled && i0 < fs; i0 += gridDim.x * blockDim.x) {
     const uint32_t i = i0 + tid;
     uint32_t y = 0;
     bool clique = false;
@@ -273,6 +332,8 @@ __global__ __launch_bounds__(256) void k_gf_bfs_level(uint32_t N, uint32_t nv_no
         x = head[a];
         if (res[rev[a]] > 0 && height[x] == N) add = atomicCAS(&height[x], N, dist + 1) == N;
       }
+      const ull gone = wave_sum_u64(add ? (ull)(off[x + 1] - off[x]) : 0ull);
+      if (lane == 0 && gone) atomicSub(&ctl->udeg[0], gone);
       const ull bal = __ballot(add);
       if (bal) {
         const uint32_t lead = (uint32_t)__builtin_ctzll(bal);
@@ -296,6 +357,7 @@ __global__ __launch_bounds__(256) void k_gf_bfs_level(uint32_t N, uint32_t nv_no
       ctl->blocks[0] = 0;
       ctl->cur = cur ^ 1;
       ctl->dist = dist + 1;
+      ctl->udeg_snap[0] = ldc_u64(&ctl->udeg[0]);
       if (nnext == 0) ctl->done = 1;
     }
   }
@@ -329,10 +391,12 @@ __global__ void k_gf_chv(uint32_t nc, const uint32_t *choff, uint32_t *chv) {
     for (uint32_t ch = choff[u]; ch < choff[u + 1]; ++ch) chv[ch] = u;
 }
 
-// BFS level, vertex nodes of the frontier (those at height dist): wave per
-// chunk of their arcs.  Runs before k_gf_bfs_level of the same level, which
-// handles the other frontier nodes and advances the level.
-__global__ __launch_bounds__(256) void k_gf_bfs_vchunks(uint32_t N, uint32_t nch, const uint32_t *choff,
+// BFS level, vertex nodes of the frontier (those at height dist, odd): wave
+// per chunk of their arcs.  At a clique level (even dist) it pulls instead
+// when bfs_pull_level says so, and otherwise has nothing to do.  Runs before
+// k_gf_bfs_level of the same level, which handles the clique frontier (unless
+// pulled) and advances the level.
+__global__ __launch_bounds__(256) void k_gf_bfs_vchunks(uint32_t N, uint32_t k, uint32_t nch, const uint32_t *choff,
                                                         const uint32_t *chv, const uint32_t *off, const uint32_t *head,
                                                         const uint32_t *rev, const i64 *res, uint32_t *height,
                                                         uint32_t *fr0, uint32_t *fr1, uint32_t seg_cap, GFCtl *ctl) {
@@ -340,6 +404,13 @@ __global__ __launch_bounds__(256) void k_gf_bfs_vchunks(uint32_t N, uint32_t nch
   if (ctl->done) return;
   uint32_t *nx = cur ? fr0 : fr1;
   const uint32_t lane = lane_id(), seg = blockIdx.x % NSEG, s = N - 2;
+  if ((dist & 1u) == 0) {  // a clique level: no vertex node is at this height
+    uint32_t fs = 0;
+    for (uint32_t g = 0; g < NSEG; ++g) fs += ctl->fsize[cur][g][0];
+    if (bfs_pull_level(dist, fs, k, ctl))
+      bfs_pull(N, dist, nch, choff, chv, off, head, res, height, nx, &ctl->fsize[cur ^ 1][seg][0], seg_cap, ctl);
+    return;
+  }
   const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KCDS_WAVE, nw = gridDim.x * blockDim.x / KCDS_WAVE;
   for (uint32_t ch = wid; ch < nch; ch += nw) {
     const uint32_t u = chv[ch];
@@ -719,13 +790,13 @@ void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t m
   // active counters are cleared by a fill kernel.
   const auto relabel_batch = [&](bool first) {
     if (first) {
-      k_gf_bfs_init<<<ggrid(c, Nn), 256, 0, c.stream>>>(N, height, ctl);
+      k_gf_bfs_init<<<ggrid(c, Nn), 256, 0, c.stream>>>(N, nc, off, height, ctl);
       k_gf_bfs_seed<<<ggrid(c, nc), 256, 0, c.stream>>>(nc, off, res, height, fr0, seg_cap, ctl);
     } else {  // the active count of the unfinished batch was taken too early
       k_gf_fill_u32<<<1, 64, 0, c.stream>>>((uint32_t *)&ctl->active[0][0], 0u, sizeof(ctl->active) / 4);
     }
     for (uint32_t i = 0; i < RELABEL_BATCH; ++i) {
-      k_gf_bfs_vchunks<<<vgrid, 256, 0, c.stream>>>(N, nch, choff, chv, off, head, rev, res, height, fr0, fr1,
+      k_gf_bfs_vchunks<<<vgrid, 256, 0, c.stream>>>(N, k, nch, choff, chv, off, head, rev, res, height, fr0, fr1,
                                                     seg_cap, ctl);
       k_gf_bfs_level<<<bgrid, 256, 0, c.stream>>>(N, nc, k, off, head, rev, res, height, fr0, fr1, seg_cap, ctl);
     }
