@@ -446,53 +446,27 @@ __global__ __launch_bounds__(256) void k_gf_bfs_vchunks(uint32_t N, uint32_t k, 
 // Vertex discharge in three kernels (wave per chunk).  Vertex nodes only
 // push to clique nodes and t, whose heights and the vertices' own excess do
 // not change while these run.
-//   A: vmin[u] = lowest head height over u's residual arcs
+//   A: per chunk, the lowest head height over its residual arcs (cmin) and
+//      the residual to that height (csum), from one pass with a running
+//      (lowest, sum) pair per lane; vmin[u] = the lowest over u's chunks.
+//      Chunk 0 of u also records u's excess (esnap) and height (hsnap): B
+//      pushes from those, since its own first chunk may relabel u meanwhile.
+//      Every chunk writes its record, so none survives from an earlier sweep.
 __global__ __launch_bounds__(256) void k_gf_vmin(uint32_t N, uint32_t nch, uint32_t sweep, const uint32_t *choff,
                                                  const uint32_t *chv, const uint32_t *off, const uint32_t *head,
                                                  const i64 *res, const i64 *excess, const uint32_t *height,
-                                                 uint32_t *vmin, const GFCtl *ctl) {
-  if (ctl->changed[sweep][0] == 0) return;
-  const uint32_t lane = lane_id();
-  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KCDS_WAVE, nw = gridDim.x * blockDim.x / KCDS_WAVE;
-  for (uint32_t ch = wid; ch < nch; ch += nw) {
-    const uint32_t u = chv[ch];
-    if (excess[u] <= 0 || height[u] >= N) continue;
-    const uint32_t a0 = off[u] + (ch - choff[u]) * VCH, a1 = min(a0 + VCH, off[u + 1]);
-    uint32_t m = NO_ARC;
-    for (uint32_t b = a0; b < a1; b += VU * KCDS_WAVE) {
-      i64 r[VU];
-      uint32_t x[VU];
-#pragma unroll
-      for (uint32_t q = 0; q < VU; ++q) {
-        const uint32_t a = b + q * KCDS_WAVE + lane;
-        r[q] = a < a1 ? res[a] : 0;
-      }
-#pragma unroll
-      for (uint32_t q = 0; q < VU; ++q) x[q] = r[q] > 0 ? head[b + q * KCDS_WAVE + lane] : 0u;
-#pragma unroll
-      for (uint32_t q = 0; q < VU; ++q)
-        if (r[q] > 0) m = min(m, height[x[q]]);
-    }
-    m = wave_min_u32(m);
-    if (lane == 0 && m != NO_ARC) atomicMin(&vmin[u], m);
-  }
-}
-
-//   B: csum[chunk] = residual on the chunk's arcs to height vmin[u] when u
-//      pushes (height[u] > vmin[u]); esnap[u] = excess[u]
-__global__ __launch_bounds__(256) void k_gf_vsum(uint32_t N, uint32_t nch, uint32_t sweep, const uint32_t *choff,
-                                                 const uint32_t *chv, const uint32_t *off, const uint32_t *head,
-                                                 const i64 *res, const i64 *excess, const uint32_t *height,
-                                                 const uint32_t *vmin, i64 *csum, i64 *esnap, const GFCtl *ctl) {
+                                                 uint32_t *vmin, uint32_t *cmin, i64 *csum, i64 *esnap,
+                                                 uint32_t *hsnap, const GFCtl *ctl) {
   if (ctl->changed[sweep][0] == 0) return;
   const uint32_t lane = lane_id();
   const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KCDS_WAVE, nw = gridDim.x * blockDim.x / KCDS_WAVE;
   for (uint32_t ch = wid; ch < nch; ch += nw) {
     const uint32_t u = chv[ch];
     const i64 e = excess[u];
-    const uint32_t hm = vmin[u];
+    const uint32_t hu = height[u];
+    uint32_t m = NO_ARC;
     i64 sum = 0;
-    if (e > 0 && height[u] < N && hm != NO_ARC && height[u] > hm) {
+    if (e > 0 && hu < N) {
       const uint32_t a0 = off[u] + (ch - choff[u]) * VCH, a1 = min(a0 + VCH, off[u + 1]);
       for (uint32_t b = a0; b < a1; b += VU * KCDS_WAVE) {
         i64 r[VU];
@@ -506,13 +480,28 @@ __global__ __launch_bounds__(256) void k_gf_vsum(uint32_t N, uint32_t nch, uint3
         for (uint32_t q = 0; q < VU; ++q) x[q] = r[q] > 0 ? head[b + q * KCDS_WAVE + lane] : 0u;
 #pragma unroll
         for (uint32_t q = 0; q < VU; ++q)
-          if (r[q] > 0 && height[x[q]] == hm) sum += r[q];
+          if (r[q] > 0) {
+            const uint32_t h = height[x[q]];
+            if (h < m) {
+              m = h;
+              sum = r[q];
+            } else if (h == m) {
+              sum += r[q];
+            }
+          }
       }
-      sum = (i64)wave_sum_u64((ull)sum);
+      const uint32_t wm = wave_min_u32(m);
+      sum = (i64)wave_sum_u64(m == wm && wm != NO_ARC ? (ull)sum : 0ull);
+      m = wm;
+      if (lane == 0 && m != NO_ARC) atomicMin(&vmin[u], m);
     }
     if (lane == 0) {
+      cmin[ch] = m;
       csum[ch] = sum;
-      if (ch == choff[u]) esnap[u] = e;
+      if (ch == choff[u]) {
+        esnap[u] = e;
+        hsnap[u] = hu;
+      }
     }
   }
 }
@@ -524,8 +513,9 @@ __global__ __launch_bounds__(256) void k_gf_vsum(uint32_t N, uint32_t nch, uint3
 __global__ __launch_bounds__(256) void k_gf_vpush(uint32_t N, uint32_t nch, uint32_t sweep, const uint32_t *choff,
                                                   const uint32_t *chv, const uint32_t *off, const uint32_t *head,
                                                   const uint32_t *rev, i64 *res, i64 *excess, uint32_t *height,
-                                                  const uint32_t *vmin, uint32_t *vnext, const i64 *csum,
-                                                  const i64 *esnap, GFCtl *ctl) {
+                                                  const uint32_t *vmin, uint32_t *vnext, const uint32_t *cmin,
+                                                  const i64 *csum, const i64 *esnap, const uint32_t *hsnap,
+                                                  GFCtl *ctl) {
   if (ctl->changed[sweep][0] == 0) return;
   const uint32_t lane = lane_id();
   const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KCDS_WAVE, nw = gridDim.x * blockDim.x / KCDS_WAVE;
@@ -543,11 +533,14 @@ __global__ __launch_bounds__(256) void k_gf_vpush(uint32_t N, uint32_t nch, uint
       }
       continue;
     }
+    // the chunks push only if u stood above hm when A ran (a chunk that sees
+    // u after chunk 0 relabelled it must not push from A's sums)
+    if (hsnap[u] <= hm) continue;
     i64 before = 0;  // the earlier chunks' residual to hm, summed across the lanes
-    for (uint32_t q = c0 + lane; q < ch; q += KCDS_WAVE) before += csum[q];
+    for (uint32_t q = c0 + lane; q < ch; q += KCDS_WAVE) before += cmin[q] == hm ? csum[q] : 0;
     before = (i64)wave_sum_u64((ull)before);
     i64 left = esnap[u] - before;
-    if (left <= 0 || csum[ch] == 0) continue;
+    if (left <= 0 || cmin[ch] != hm || csum[ch] == 0) continue;
     const uint32_t a0 = off[u] + (ch - c0) * VCH, a1 = min(a0 + VCH, off[u + 1]);
     i64 pushed = 0, to_t = 0;
     for (uint32_t b = a0; b < a1 && left > 0; b += VU * KCDS_WAVE) {
@@ -772,8 +765,11 @@ void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t m
   c.sync();
   buf_reserve(c, f.g_chv, sizeof(uint32_t) * (nch + 1));
   buf_reserve(c, f.g_csum, sizeof(i64) * (nch + 1));
+  buf_reserve(c, f.g_cmin, sizeof(uint32_t) * (nch + 1));
+  buf_reserve(c, f.g_hsnap, sizeof(uint32_t) * (nc + 1));
   uint32_t *chv = f.g_chv.as<uint32_t>(), *vmin = f.g_vmin.as<uint32_t>();
   i64 *csum = f.g_csum.as<i64>(), *esnap = f.g_esnap.as<i64>();
+  uint32_t *cmin = f.g_cmin.as<uint32_t>(), *hsnap = f.g_hsnap.as<uint32_t>();
   k_gf_chv<<<ggrid(c, nc), 256, 0, c.stream>>>(nc, choff, chv);
   KCDS_LAUNCH_CHECK();
   GFCtl *h = (GFCtl *)c.host_scratch(sizeof(GFCtl));
@@ -785,7 +781,7 @@ void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t m
   ull sn = 0, sT = 0;
   // The relabel and sweep sequences of a phase are the same launches every
   // phase (all state lives on the device), so they are captured once as HIP
-  // graphs and replayed: 36 launches per relabel batch and 97 per sweep
+  // graphs and replayed: 35-36 launches per relabel batch and 37 per sweep
   // batch become one graph launch each.  No memset node is recorded; the
   // active counters are cleared by a fill kernel.
   const auto relabel_batch = [&](bool first) {
@@ -807,11 +803,10 @@ void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t m
     k_gf_fill_u32<<<ggrid(c, 2ull * nc), 256, 0, c.stream>>>(vmin, NO_ARC, 2ull * nc);
     for (uint32_t sw = 0; sw < SWEEPS; ++sw) {
       uint32_t *vm = vmin + (size_t)(sw & 1) * nc, *vn = vmin + (size_t)((sw + 1) & 1) * nc;
-      k_gf_vmin<<<vgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, res, excess, height, vm, ctl);
-      k_gf_vsum<<<vgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, res, excess, height, vm, csum, esnap,
-                                             ctl);
+      k_gf_vmin<<<vgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, res, excess, height, vm, cmin, csum,
+                                             esnap, hsnap, ctl);
       k_gf_vpush<<<vgrid, 256, 0, c.stream>>>(N, nch, sw, choff, chv, off, head, rev, res, excess, height, vm, vn,
-                                              csum, esnap, ctl);
+                                              cmin, csum, esnap, hsnap, ctl);
       k_gf_sweep_c<<<ggrid(c, Tc), 256, 0, c.stream>>>(nc, N, sw, off, head, rev, res, excess, height, ctl);
     }
     KCDS_LAUNCH_CHECK();

@@ -139,7 +139,7 @@ void kcds_destroy(kcds_ctx *ctx) {
                     &g.par, &g.flag, &g.dense, &g.cid, &g.local, &g.vcount, &g.voff, &g.verts, &g.ccount,
                     &g.coff, &g.cfill, &g.ccl, &f.scal, &f.cside, &f.g_off, &f.g_head, &f.g_rev, &f.g_res,
                     &f.g_excess, &f.g_height, &f.g_fr, &f.g_ctl, &c.gsum, &f.g_nch, &f.g_choff, &f.g_chv,
-                    &f.g_vmin, &f.g_csum, &f.g_esnap, &f.b_off, &f.b_excess, &f.b_height, &f.b_fr0, &f.b_fr1, &f.b_head, &f.b_rev, &f.b_res, &f.b_tasks};
+                    &f.g_vmin, &f.g_csum, &f.g_esnap, &f.g_cmin, &f.g_hsnap, &f.b_off, &f.b_excess, &f.b_height, &f.b_fr0, &f.b_fr1, &f.b_head, &f.b_rev, &f.b_res, &f.b_tasks};
   for (DevBuf *b : bufs) buf_free(c, *b);
   if (c.pinned) (void)hipHostFree(c.pinned);
   for (int i = 0; i < 3; ++i) {

@@ -97,7 +97,7 @@ struct FlowBufs {
   DevBuf cside;  // uint8 per core vertex (component-grouped): min-cut side of each solved component
   DevBuf b_off, b_excess, b_height, b_fr0, b_fr1, b_head, b_rev, b_res, b_tasks;  // block solver slices
   DevBuf g_off, g_head, g_rev, g_res, g_excess, g_height, g_fr, g_ctl;           // grid solver (one component)
-  DevBuf g_nch, g_choff, g_chv, g_vmin, g_csum, g_esnap;                         // grid solver: vertex arc chunks
+  DevBuf g_nch, g_choff, g_chv, g_vmin, g_csum, g_esnap, g_cmin, g_hsnap;        // grid solver: vertex arc chunks
 };
 
 struct Ctx {
