@@ -33,6 +33,8 @@
 // side of the maximal minimum cut.  A component's arrays belong to one
 // workgroup, so its memory operations use workgroup scope; workgroup
 // barriers wait for each wave's own memory operations first.
+#include <algorithm>
+
 #include "kcds_device.hpp"
 #include "kcds_internal.hpp"
 
@@ -447,7 +449,27 @@ __global__ void k_mark_cside(uint32_t nc, const uint32_t *verts, const uint8_t *
     if (cside[v]) inset[verts[v]] = 1;
 }
 
+// the same for every core vertex of the components flagged in sel
+__global__ void k_mark_csides(uint32_t nv, const uint32_t *verts, const uint32_t *cid, const uint8_t *sel,
+                              const uint8_t *cside, uint8_t *inset) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x)
+    if (cside[i] && sel[cid[verts[i]]]) inset[verts[i]] = 1;
+}
+
 }  // namespace
+
+void mark_component_sides(Ctx &c, const std::vector<uint8_t> &sel) {
+  CompData &g = c.cd;
+  const uint32_t nv = g.h_voff.back();
+  if (!nv) return;
+  buf_reserve(c, c.tmp3, sel.size() + 8);
+  HIP_CHECK(hipMemcpyAsync(c.tmp3.p, sel.data(), sel.size(), hipMemcpyHostToDevice, c.stream));
+  k_mark_csides<<<grid1d(nv), 256, 0, c.stream>>>(nv, g.verts.as<uint32_t>(), g.cid.as<uint32_t>(),
+                                                  c.tmp3.as<uint8_t>(), c.fb.cside.as<uint8_t>(),
+                                                  c.inset.as<uint8_t>());
+  KCDS_LAUNCH_CHECK();
+  c.sync();  // sel lives on the host
+}
 
 void mark_component_side(Ctx &c, uint32_t d) {
   CompData &g = c.cd;
@@ -468,11 +490,20 @@ void dinkelbach_blocks(Ctx &c, const std::vector<BlockTask> &in, std::vector<Blo
     buf_reserve(c, f.cside, (size_t)g.h_voff.back() + 8);
     HIP_CHECK(hipMemsetAsync(f.cside.p, 0, (size_t)g.h_voff.back() + 8, c.stream));
   }
+  // largest components first: the launch lasts as long as its last task, and
+  // a big component taken late would end far behind the rest (C4: 126 ms for
+  // 4,096 components in component order)
+  std::vector<uint32_t> order(in.size());
+  for (size_t i = 0; i < in.size(); ++i) order[i] = (uint32_t)i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return g.h_coff[in[a].d + 1] - g.h_coff[in[a].d] > g.h_coff[in[b].d + 1] - g.h_coff[in[b].d];
+  });
   std::vector<CompTask> tasks(in.size());
   ull nodes = 0, arcs = 0;
-  for (size_t i = 0; i < in.size(); ++i) {
+  for (size_t j = 0; j < in.size(); ++j) {
+    const size_t i = order[j];
     const uint32_t d = in[i].d;
-    CompTask &t = tasks[i];
+    CompTask &t = tasks[j];
     t.nc = g.h_voff[d + 1] - g.h_voff[d];
     t.voff = g.h_voff[d];
     t.Tc = g.h_coff[d + 1] - g.h_coff[d];
@@ -517,15 +548,16 @@ void dinkelbach_blocks(Ctx &c, const std::vector<BlockTask> &in, std::vector<Blo
   std::vector<CompOut> h(tasks.size());
   HIP_CHECK(hipMemcpyAsync(h.data(), dout, sizeof(CompOut) * tasks.size(), hipMemcpyDeviceToHost, c.stream));
   c.sync();
-  for (size_t i = 0; i < h.size(); ++i) {
-    if (h[i].status != 1)
-      fail(KCDS_E_LIMIT, "block solver did not converge on component %u (status %u)", in[i].d, h[i].status);
-    out[i].p = h[i].p;
-    out[i].q = h[i].q;
-    out[i].side_n = h[i].side_n;
-    out[i].side_T = h[i].side_T;
-    out[i].flow = h[i].flow;
-    out[i].flows = h[i].flows;
+  for (size_t j = 0; j < h.size(); ++j) {
+    const size_t i = order[j];
+    if (h[j].status != 1)
+      fail(KCDS_E_LIMIT, "block solver did not converge on component %u (status %u)", in[i].d, h[j].status);
+    out[i].p = h[j].p;
+    out[i].q = h[j].q;
+    out[i].side_n = h[j].side_n;
+    out[i].side_T = h[j].side_T;
+    out[i].flow = h[j].flow;
+    out[i].flows = h[j].flows;
   }
 }
 

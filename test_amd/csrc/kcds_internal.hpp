@@ -230,6 +230,8 @@ void components_run(Ctx &c);
 // ---- flow (k_flow.hip) ----
 // Marks component d's slice of c.fb.cside (its last min-cut side) in c.inset.
 void mark_component_side(Ctx &c, uint32_t d);
+// The same for every component d with sel[d] (one launch).
+void mark_component_sides(Ctx &c, const std::vector<uint8_t> &sel);
 
 // Exact search (or one max-flow when max_iters == 1) of many components at
 // once, one workgroup each.

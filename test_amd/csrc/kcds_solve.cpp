@@ -278,14 +278,18 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
   }
   buf_reserve(c, c.inset, (size_t)c.n + 8);
   HIP_CHECK(hipMemsetAsync(c.inset.p, 0, c.n, c.stream));
+  std::vector<uint8_t> sel(C, 0);
+  bool any = false;
   for (uint32_t d = 0; d < C; ++d) {
     if (!solved[d] || res[d].side_n == 0) continue;
     const uint32_t gi = cg[d];
     if ((u128)res[d].p * bd[gi] != (u128)bn[gi] * res[d].q) continue;
-    mark_component_side(c, d);
+    sel[d] = 1;
+    any = true;
     out[gi].rho_num += res[d].side_T;
     out[gi].rho_den += res[d].side_n;
   }
+  if (any) mark_component_sides(c, sel);
   for (uint32_t gi = 0; gi < G; ++gi) {
     const GraphPeel &P = c.h_gpeel[gi];
     out[gi].T = P.T;
