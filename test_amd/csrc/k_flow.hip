@@ -480,8 +480,12 @@ void mark_component_side(Ctx &c, uint32_t d) {
   KCDS_LAUNCH_CHECK();
 }
 
-void dinkelbach_blocks(Ctx &c, const std::vector<BlockTask> &in, std::vector<BlockResult> &out, uint32_t max_iters) {
-  out.assign(in.size(), BlockResult{});
+// Enqueues the workgroup-per-component launch on st (after the context
+// stream's work so far); block_finish waits for it and fills the results.
+void block_launch(Ctx &c, const std::vector<BlockTask> &in, uint32_t max_iters, hipStream_t st, BlockRun &run) {
+  run.order.clear();
+  run.dout = nullptr;
+  run.st = st;
   if (in.empty()) return;
   CompData &g = c.cd;
   FlowBufs &f = c.fb;
@@ -493,7 +497,8 @@ void dinkelbach_blocks(Ctx &c, const std::vector<BlockTask> &in, std::vector<Blo
   // largest components first: the launch lasts as long as its last task, and
   // a big component taken late would end far behind the rest (C4: 126 ms for
   // 4,096 components in component order)
-  std::vector<uint32_t> order(in.size());
+  std::vector<uint32_t> &order = run.order;
+  order.resize(in.size());
   for (size_t i = 0; i < in.size(); ++i) order[i] = (uint32_t)i;
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
     return g.h_coff[in[a].d + 1] - g.h_coff[in[a].d] > g.h_coff[in[b].d + 1] - g.h_coff[in[b].d];
@@ -529,25 +534,39 @@ void dinkelbach_blocks(Ctx &c, const std::vector<BlockTask> &in, std::vector<Blo
   buf_reserve(c, f.b_rev, sizeof(uint32_t) * (arcs + 1));
   buf_reserve(c, f.b_res, sizeof(i64) * (arcs + 1));
   buf_reserve(c, f.b_tasks, sizeof(CompTask) * tasks.size() + sizeof(CompOut) * tasks.size() + 256);
+  buf_reserve(c, f.scal, QUEUE_WORDS * sizeof(uint32_t));
   CompTask *dt = f.b_tasks.as<CompTask>();
   CompOut *dout = (CompOut *)(((uintptr_t)(dt + tasks.size()) + 63) & ~(uintptr_t)63);
-  HIP_CHECK(hipMemcpyAsync(dt, tasks.data(), sizeof(CompTask) * tasks.size(), hipMemcpyHostToDevice, c.stream));
-  buf_reserve(c, f.scal, QUEUE_WORDS * sizeof(uint32_t));
+  run.dout = dout;
+  if (st != c.stream) {  // after everything enqueued on the context stream so far
+    if (!c.ffork) HIP_CHECK(hipEventCreateWithFlags(&c.ffork, hipEventDisableTiming));
+    HIP_CHECK(hipEventRecord(c.ffork, c.stream));
+    HIP_CHECK(hipStreamWaitEvent(st, c.ffork, 0));
+  }
+  // tasks live on the host until the copy completes
+  HIP_CHECK(hipMemcpyAsync(dt, tasks.data(), sizeof(CompTask) * tasks.size(), hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipStreamSynchronize(st));
   uint32_t *qctr = (uint32_t *)f.scal.p;
-  HIP_CHECK(hipMemsetAsync(qctr, 0, QUEUE_WORDS * sizeof(uint32_t), c.stream));
+  HIP_CHECK(hipMemsetAsync(qctr, 0, QUEUE_WORDS * sizeof(uint32_t), st));
   int per_cu = 0;
   HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_dinkelbach_block, 1024, 0));
   if (per_cu < 1) fail(KCDS_E_HIP, "k_dinkelbach_block cannot be resident");
   const uint32_t grid =
       (uint32_t)std::min<uint64_t>(tasks.size(), (uint64_t)c.num_cus * (uint64_t)std::min(per_cu, 2));
-  k_dinkelbach_block<<<grid, 1024, 0, c.stream>>>(
+  k_dinkelbach_block<<<grid, 1024, 0, st>>>(
       dt, (uint32_t)tasks.size(), qctr, k, max_iters, g.ccl.as<uint32_t>(), f.b_off.as<uint32_t>(),
       f.b_excess.as<i64>(), f.b_height.as<uint32_t>(), f.b_fr0.as<uint32_t>(), f.b_fr1.as<uint32_t>(),
       f.b_head.as<uint32_t>(), f.b_rev.as<uint32_t>(), f.b_res.as<i64>(), f.cside.as<uint8_t>(), dout);
   KCDS_LAUNCH_CHECK();
-  std::vector<CompOut> h(tasks.size());
-  HIP_CHECK(hipMemcpyAsync(h.data(), dout, sizeof(CompOut) * tasks.size(), hipMemcpyDeviceToHost, c.stream));
-  c.sync();
+}
+
+void block_finish(Ctx &c, const std::vector<BlockTask> &in, BlockRun &run, std::vector<BlockResult> &out) {
+  out.assign(in.size(), BlockResult{});
+  if (in.empty()) return;
+  const std::vector<uint32_t> &order = run.order;
+  std::vector<CompOut> h(in.size());
+  HIP_CHECK(hipMemcpyAsync(h.data(), run.dout, sizeof(CompOut) * in.size(), hipMemcpyDeviceToHost, run.st));
+  HIP_CHECK(hipStreamSynchronize(run.st));
   for (size_t j = 0; j < h.size(); ++j) {
     const size_t i = order[j];
     if (h[j].status != 1)
@@ -559,6 +578,12 @@ void dinkelbach_blocks(Ctx &c, const std::vector<BlockTask> &in, std::vector<Blo
     out[i].flow = h[j].flow;
     out[i].flows = h[j].flows;
   }
+}
+
+void dinkelbach_blocks(Ctx &c, const std::vector<BlockTask> &in, std::vector<BlockResult> &out, uint32_t max_iters) {
+  BlockRun run;
+  block_launch(c, in, max_iters, c.stream, run);
+  block_finish(c, in, run, out);
 }
 
 }  // namespace kcds

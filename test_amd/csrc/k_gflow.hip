@@ -713,6 +713,29 @@ ull gcd_u64(ull a, ull b) {
 // Dinkelbach search on component d with grid-wide max-flows (as the
 // workgroup solver: at most max_iters flows; the final alpha, the maximal
 // min-cut side at it and the last flow value).
+void grid_reserve(Ctx &c, uint32_t nc, uint64_t Tc) {
+  FlowBufs &f = c.fb;
+  const uint64_t Nn = (uint64_t)nc + Tc + 2, A = 4ull * nc + 2ull * c.k * Tc;
+  const uint64_t seg_cap = ((Nn + 63) / 64) * 64, nch = A / VCH + nc + 1;
+  buf_reserve(c, f.g_off, sizeof(uint32_t) * (Nn + 2));
+  buf_reserve(c, f.g_head, sizeof(uint32_t) * (A + 1));
+  buf_reserve(c, f.g_rev, sizeof(uint32_t) * (A + 1));
+  buf_reserve(c, f.g_res, sizeof(i64) * (A + 1));
+  buf_reserve(c, f.g_excess, sizeof(i64) * (Nn + 1));
+  buf_reserve(c, f.g_height, sizeof(uint32_t) * (Nn + 1));
+  buf_reserve(c, f.g_fr, sizeof(uint32_t) * 2ull * NSEG * seg_cap);
+  buf_reserve(c, f.g_ctl, sizeof(GFCtl) + 256);
+  buf_reserve(c, f.g_nch, sizeof(uint32_t) * (nc + 1));
+  buf_reserve(c, f.g_choff, sizeof(uint32_t) * (nc + 1));
+  buf_reserve(c, f.g_vmin, sizeof(uint32_t) * 2ull * nc);
+  buf_reserve(c, f.g_esnap, sizeof(i64) * nc);
+  buf_reserve(c, f.g_chv, sizeof(uint32_t) * (nch + 1));
+  buf_reserve(c, f.g_csum, sizeof(i64) * (nch + 1));
+  buf_reserve(c, f.g_cmin, sizeof(uint32_t) * (nch + 1));
+  buf_reserve(c, f.g_hsnap, sizeof(uint32_t) * (nc + 1));
+  (void)c.host_scratch(sizeof(GFCtl) + 64);
+}
+
 void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t max_iters) {
   CompData &g = c.cd;
   FlowBufs &f = c.fb;

@@ -870,7 +870,7 @@ void list_count(Ctx &c, uint32_t k, ull cap) {
   if (!c.lfork) {
     HIP_CHECK(hipEventCreateWithFlags(&c.lfork, hipEventDisableTiming));
     for (int i = 0; i < 3; ++i) {
-      HIP_CHECK(hipStreamCreateWithFlags(&c.lstream[i], hipStreamNonBlocking));
+      if (!c.lstream[i]) HIP_CHECK(hipStreamCreateWithFlags(&c.lstream[i], hipStreamNonBlocking));
       HIP_CHECK(hipEventCreateWithFlags(&c.ljoin[i], hipEventDisableTiming));
     }
   }

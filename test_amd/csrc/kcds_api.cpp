@@ -149,6 +149,7 @@ void kcds_destroy(kcds_ctx *ctx) {
   if (c.lfork) (void)hipEventDestroy(c.lfork);
   for (int i = 0; i < 2; ++i)
     if (c.ltime[i]) (void)hipEventDestroy(c.ltime[i]);
+  if (c.ffork) (void)hipEventDestroy(c.ffork);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete ctx;
 }

@@ -106,6 +106,7 @@ struct Ctx {
   hipStream_t lstream[3] = {nullptr, nullptr, nullptr};  // listing classes run beside the context stream
   hipEvent_t lfork = nullptr, ljoin[3] = {nullptr, nullptr, nullptr};
   hipEvent_t ltime[2] = {nullptr, nullptr};  // around the listing pass
+  hipEvent_t ffork = nullptr;                // workgroup max-flow launch beside the grid solver
   uint64_t budget = 0, used = 0;
   int num_cus = 0;
   std::string err;
@@ -246,7 +247,18 @@ struct BlockResult {
   uint32_t flows = 0;
 };
 void dinkelbach_blocks(Ctx &c, const std::vector<BlockTask> &in, std::vector<BlockResult> &out, uint32_t max_iters);
+// The same in two halves: the launch on stream st, then the wait and results.
+struct BlockRun {
+  std::vector<uint32_t> order;  // task j solves in[order[j]]
+  void *dout = nullptr;
+  hipStream_t st = nullptr;
+};
+void block_launch(Ctx &c, const std::vector<BlockTask> &in, uint32_t max_iters, hipStream_t st, BlockRun &run);
+void block_finish(Ctx &c, const std::vector<BlockTask> &in, BlockRun &run, std::vector<BlockResult> &out);
 void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t max_iters);
+// Reserves the grid solver's buffers for components of up to nc vertices and
+// Tc cliques, so that solving them frees and allocates nothing.
+void grid_reserve(Ctx &c, uint32_t nc, uint64_t Tc);
 // Components with at least this many cliques are solved by the grid solver;
 // with fewer components than a quarter of the CUs, from GRID_FLOW_ALONE_T up.
 constexpr uint64_t GRID_FLOW_MIN_T = 1u << 16, GRID_FLOW_ALONE_T = 1u << 12;

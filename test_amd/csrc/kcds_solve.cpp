@@ -103,18 +103,38 @@ void dinkelbach_components(Ctx &c, const std::vector<BlockTask> &in, std::vector
                        : in.size() * 4 < (size_t)c.num_cus ? GRID_FLOW_ALONE_T
                                                           : GRID_FLOW_MIN_T;
   std::vector<BlockTask> small;
-  std::vector<size_t> small_at;
+  std::vector<size_t> small_at, big_at;
+  uint32_t big_nc = 0;
+  ull big_T = 0;
   for (size_t i = 0; i < in.size(); ++i) {
-    const ull Tc = g.h_coff[in[i].d + 1] - g.h_coff[in[i].d];
+    const uint32_t d = in[i].d;
+    const ull Tc = g.h_coff[d + 1] - g.h_coff[d];
     if (all_grid || Tc >= grid_min) {
-      dinkelbach_grid(c, in[i], out[i], max_iters);
+      big_at.push_back(i);
+      big_nc = std::max(big_nc, g.h_voff[d + 1] - g.h_voff[d]);
+      big_T = std::max(big_T, Tc);
     } else {
       small.push_back(in[i]);
       small_at.push_back(i);
     }
   }
+  // Both kinds: the workgroup launch runs on a stream of its own while the
+  // grid solver takes the big components one after another; its buffers are
+  // reserved at their largest first, so nothing is freed (which would wait for
+  // the whole device) meanwhile.  The workgroup launch's tail is its largest
+  // component, which leaves most CUs to the grid solver.
+  const bool overlap = !small.empty() && !big_at.empty();
+  hipStream_t bst = c.stream;
+  if (overlap) {
+    grid_reserve(c, big_nc, big_T);
+    if (!c.lstream[0]) HIP_CHECK(hipStreamCreateWithFlags(&c.lstream[0], hipStreamNonBlocking));
+    bst = c.lstream[0];
+  }
+  BlockRun run;
+  block_launch(c, small, max_iters, bst, run);
+  for (size_t i : big_at) dinkelbach_grid(c, in[i], out[i], max_iters);
   std::vector<BlockResult> r;
-  dinkelbach_blocks(c, small, r, max_iters);
+  block_finish(c, small, run, r);
   for (size_t j = 0; j < small.size(); ++j) out[small_at[j]] = r[j];
 }
 

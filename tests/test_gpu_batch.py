@@ -105,3 +105,22 @@ def test_batch_mixed_peels(solver):
         assert (res[i].rho_num, res[i].rho_den) == (r.rho_num, r.rho_den)
         np.testing.assert_array_equal(sets[i], ins_r)
     assert res[-1].T > (1 << 18)
+
+
+@pytest.mark.parametrize("count", [40, 80])
+def test_batch_grid_and_workgroup_flows_together(solver, count):
+    """A batch whose big components go to the grid max-flow while the rest
+    run in the workgroup launch on a stream of its own.  Graph 546 of the
+    4,096-graph batch has 100,529 4-cliques; with 41 components every
+    component of 4,096+ core cliques takes the grid solver, with 81 the
+    65,536 threshold applies.  Every graph's result equals its own solve
+    (which the oracle tests pin)."""
+    specs = gen.batch_specs()
+    gs = [gen.batch_graph(s) for s in specs[:count]] + [gen.batch_graph(specs[546])]
+    res, sets = solver.solve_batch([(g.offset, g.nbr) for g in gs], 4)
+    for i, g in enumerate(gs):
+        solver.upload(g.offset, g.nbr)
+        solo, ins = solver.solve(4)
+        assert (solo.T, solo.rho_num, solo.rho_den) == (res[i].T, res[i].rho_num, res[i].rho_den), i
+        np.testing.assert_array_equal(ins, sets[i])
+    assert res[-1].T == 100529
