@@ -159,7 +159,7 @@ int kcds_sync(kcds_ctx *ctx);
  *                        the same results; used to test one against the other). */
 #define KCDS_TUNE_GRID_PEEL 1u
 /*   KCDS_TUNE_GRID_FLOW  solve every component with the grid-wide max-flow
- *                        (normally only components of >= 65,536 cliques,
+ *                        (normally only components of >= 49,152 cliques,
  *                        or >= 4,096 when a solve has fewer components than
  *                        a quarter of the GPU's compute units). */
 #define KCDS_TUNE_GRID_FLOW 2u

@@ -261,7 +261,9 @@ void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t m
 void grid_reserve(Ctx &c, uint32_t nc, uint64_t Tc);
 // Components with at least this many cliques are solved by the grid solver;
 // with fewer components than a quarter of the CUs, from GRID_FLOW_ALONE_T up.
-constexpr uint64_t GRID_FLOW_MIN_T = 1u << 16, GRID_FLOW_ALONE_T = 1u << 12;
+// (49,152: with the grid solves beside the workgroup launch, the C4 batch
+// took 145-151 ms against 166-188 ms at 65,536 and 170-204 ms at 32-40K.)
+constexpr uint64_t GRID_FLOW_MIN_T = 3u << 14, GRID_FLOW_ALONE_T = 1u << 12;
 constexpr uint32_t NO_SLACK = 32, ORDER_SLACK_SHIFT = 1;  // slack order: thresholds x + x/2
 constexpr uint32_t LIST_MAX_OUTDEG = 512;                  // widest out-list the listing stages in LDS
 // All tasks: large components one after another on the whole GPU, the rest in

@@ -113,7 +113,7 @@ def test_batch_grid_and_workgroup_flows_together(solver, count):
     run in the workgroup launch on a stream of its own.  Graph 546 of the
     4,096-graph batch has 100,529 4-cliques; with 41 components every
     component of 4,096+ core cliques takes the grid solver, with 81 the
-    65,536 threshold applies.  Every graph's result equals its own solve
+    49,152 threshold applies.  Every graph's result equals its own solve
     (which the oracle tests pin)."""
     specs = gen.batch_specs()
     gs = [gen.batch_graph(s) for s in specs[:count]] + [gen.batch_graph(specs[546])]
