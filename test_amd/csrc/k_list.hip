@@ -741,6 +741,34 @@ void read_totals(Ctx &c, const ull *totals, ull &T, ull &w2) {
   }
 }
 
+// The width classes with waves per root nw[cls], all at once: the widest on
+// the context stream, the others on three more streams (default priority),
+// each class filling the others' tails (they share only atomic counters).
+// Returns the number of launches; the context stream waits for all.
+uint32_t run_classes(Ctx &c, uint32_t k, const uint32_t nw[NCLASS], ull *gcnt, ull *rootT, ull *totals, ull cap) {
+  if (!c.lfork) {
+    HIP_CHECK(hipEventCreateWithFlags(&c.lfork, hipEventDisableTiming));
+    for (int i = 0; i < 3; ++i) {
+      if (!c.lstream[i]) HIP_CHECK(hipStreamCreateWithFlags(&c.lstream[i], hipStreamNonBlocking));
+      HIP_CHECK(hipEventCreateWithFlags(&c.ljoin[i], hipEventDisableTiming));
+    }
+  }
+  HIP_CHECK(hipEventRecord(c.lfork, c.stream));
+  uint32_t launches = 0;
+  for (int cls = NCLASS - 1; cls >= 0; --cls) {
+    if (!c.class_count[cls]) continue;
+    hipStream_t st = cls == NCLASS - 1 ? c.stream : c.lstream[cls];
+    if (st != c.stream) HIP_CHECK(hipStreamWaitEvent(st, c.lfork, 0));
+    launch_class(c, cls, k, nw[cls], count_queue(c, cls), gcnt, rootT, totals, cap, st);
+    if (st != c.stream) {
+      HIP_CHECK(hipEventRecord(c.ljoin[cls], st));
+      HIP_CHECK(hipStreamWaitEvent(c.stream, c.ljoin[cls], 0));
+    }
+    ++launches;
+  }
+  return launches;
+}
+
 }  // namespace
 
 uint32_t list_waves_per_root(const Ctx &c, int cls, uint32_t k) {
@@ -770,12 +798,17 @@ void list_autotune(Ctx &c, uint32_t k) {
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
+  auto allowed = [&](int cls, uint32_t nw) { return !(k == 3 && nw > (1u << cls)); };
+  // 1. each class alone: the best waves per root as a starting point (and
+  //    the isolated times reported)
+  float iso[NCLASS][4] = {};
+  uint32_t shape[NCLASS];
   for (int cls = 0; cls < NCLASS; ++cls) {
+    shape[cls] = kWavesPerRoot[cls];
     if (!c.class_count[cls]) continue;
     float best = 1e30f;
-    uint32_t best_nw = kWavesPerRoot[cls];
-    for (uint32_t nw = 1; nw <= MAX_WAVES; nw *= 2) {
-      if (k == 3 && nw > (1u << cls)) break;
+    for (uint32_t nw = 1, i = 0; nw <= MAX_WAVES; nw *= 2, ++i) {
+      if (!allowed(cls, nw)) break;
       float tsum = 0;
       for (int rep = 0; rep < 2; ++rep) {
         HIP_CHECK(hipMemsetAsync(q, 0, QUEUE_WORDS * sizeof(uint32_t), c.stream));
@@ -787,13 +820,54 @@ void list_autotune(Ctx &c, uint32_t k) {
         HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
         if (rep) tsum += ms;  // first run warms caches
       }
+      iso[cls][i] = tsum;
       if (tsum < best) {
         best = tsum;
-        best_nw = nw;
+        shape[cls] = nw;
       }
     }
-    c.list_waves_per_root[cls] = best_nw;
-    c.tune_ms[cls] = best;
+  }
+  // 2. the classes run at once and share the GPU, so the best shape of the
+  //    pass is not the classes' best alone (C2: [4,4,8,8] from the isolated
+  //    times, 1.86 ms; [2,2,4,8], 1.66 ms): coordinate descent on the time
+  //    of the whole concurrent pass, two rounds over the classes
+  auto pass_ms = [&](const uint32_t *sh) {
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+      HIP_CHECK(hipMemsetAsync(count_queue(c, 0), 0, NCLASS * QUEUE_WORDS * sizeof(uint32_t), c.stream));
+      HIP_CHECK(hipEventRecord(e0, c.stream));
+      run_classes(c, k, sh, gcnt, rootT, totals, 0);
+      HIP_CHECK(hipEventRecord(e1, c.stream));
+      HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0;
+      HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (rep) best = std::min(best, ms);  // first run warms caches
+    }
+    return best;
+  };
+  float cur = pass_ms(shape);
+  for (int round = 0; round < 2; ++round) {
+    for (int cls = 0; cls < NCLASS; ++cls) {
+      if (!c.class_count[cls]) continue;
+      for (uint32_t nw = 1; nw <= MAX_WAVES; nw *= 2) {
+        if (!allowed(cls, nw)) break;
+        if (nw == shape[cls]) continue;
+        uint32_t trial[NCLASS];
+        for (int i = 0; i < NCLASS; ++i) trial[i] = shape[i];
+        trial[cls] = nw;
+        const float t = pass_ms(trial);
+        if (t < cur) {
+          cur = t;
+          shape[cls] = nw;
+        }
+      }
+    }
+  }
+  for (int cls = 0; cls < NCLASS; ++cls) {
+    c.list_waves_per_root[cls] = shape[cls];
+    uint32_t i = 0;
+    while ((1u << i) < shape[cls]) ++i;
+    c.tune_ms[cls] = c.class_count[cls] ? iso[cls][i] : 0.f;
   }
   HIP_CHECK(hipEventDestroy(e0));
   HIP_CHECK(hipEventDestroy(e1));
@@ -864,30 +938,9 @@ void list_count(Ctx &c, uint32_t k, ull cap) {
   }
   hipEvent_t e0 = c.ltime[0], e1 = c.ltime[1];
   HIP_CHECK(hipEventRecord(e0, c.stream));
-  // The classes share only atomic counters, so they run at once: the widest
-  // on the context stream, the others on three more streams (default
-  // priority), each class filling the others' tails.
-  if (!c.lfork) {
-    HIP_CHECK(hipEventCreateWithFlags(&c.lfork, hipEventDisableTiming));
-    for (int i = 0; i < 3; ++i) {
-      if (!c.lstream[i]) HIP_CHECK(hipStreamCreateWithFlags(&c.lstream[i], hipStreamNonBlocking));
-      HIP_CHECK(hipEventCreateWithFlags(&c.ljoin[i], hipEventDisableTiming));
-    }
-  }
-  HIP_CHECK(hipEventRecord(c.lfork, c.stream));
-  uint32_t launches = 0;
-  for (int cls = NCLASS - 1; cls >= 0; --cls) {
-    if (!c.class_count[cls]) continue;
-    hipStream_t st = cls == NCLASS - 1 ? c.stream : c.lstream[cls];
-    if (st != c.stream) HIP_CHECK(hipStreamWaitEvent(st, c.lfork, 0));
-    launch_class(c, cls, k, list_waves_per_root(c, cls, k), count_queue(c, cls), c.cnt.as<ull>(), c.rootT.as<ull>(),
-                 totals, cap, st);
-    if (st != c.stream) {
-      HIP_CHECK(hipEventRecord(c.ljoin[cls], st));
-      HIP_CHECK(hipStreamWaitEvent(c.stream, c.ljoin[cls], 0));
-    }
-    ++launches;
-  }
+  uint32_t shape[NCLASS];
+  for (int cls = 0; cls < NCLASS; ++cls) shape[cls] = list_waves_per_root(c, cls, k);
+  uint32_t launches = run_classes(c, k, shape, c.cnt.as<ull>(), c.rootT.as<ull>(), totals, cap);
   if (ncopy > 1 && n) {
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)c.num_cus * 8);
     k_sum_copies<<<grid, 256, 0, c.stream>>>(c.cnt.as<ull>(), n, cstride, ncopy);
