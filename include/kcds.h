@@ -190,8 +190,10 @@ int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
 int kcds_set_list_shape(kcds_ctx *ctx, const uint32_t *waves_per_root /* 4, or NULL */);
 
 /* Autotuner: on the oriented graph, times every waves-per-root choice of
- * every width class at clique size k, keeps the fastest and returns it in
- * waves_per_root[4] and the class times (ms) in class_ms[4] (either may be
+ * every width class alone at clique size k, then refines the shape by
+ * coordinate descent on the time of the whole pass with the classes running
+ * at once; keeps the result and returns it in waves_per_root[4] and each
+ * class's time alone at its chosen shape (ms) in class_ms[4] (either may be
  * NULL).  No counterpart in the reference (its launch shape is fixed). */
 int kcds_autotune_list(kcds_ctx *ctx, uint32_t k, uint32_t *waves_per_root, float *class_ms);
 
