@@ -319,7 +319,7 @@ __global__ __launch_bounds__(256) void k_dag_copy(const uint32_t *__restrict__ o
 // A frontier carries each vertex's adjacency cut into PIECE-neighbour
 // pieces, and the decrements run PLANES lanes per piece (a wave takes
 // 64 / PLANES pieces at once), so a hub of degree 25,000 is spread over 49
-// waves and a level of short lists still fills its waves.
+// waves.
 // A vertex leaves by having its degree replaced with DEAD; decrements that
 // race with that replacement keep the value >= DEAD_MIN, which every reader
 // treats as dead.  Kernel boundaries order the steps (the next step reads
@@ -348,8 +348,8 @@ struct OrderCtl {
 
 constexpr uint32_t DEAD = 0xFFFFFFFFu;
 constexpr uint32_t DEAD_MIN = 0x80000000u;
-constexpr uint32_t PIECE = 64;    // neighbours per decrement work item
-constexpr uint32_t PLANES = 8;    // lanes per piece
+constexpr uint32_t PIECE = 256;   // neighbours per decrement work item
+constexpr uint32_t PLANES = 32;   // lanes per piece
 constexpr uint32_t PUNROLL = PIECE / PLANES;
 constexpr uint32_t SMALL_FRONT = 16;  // frontier vertices of at most this degree decrement lane per vertex
 constexpr uint32_t DEC_TB = 4096;     // LDS table of a workgroup round's decrements (2 * 256 * PUNROLL)
