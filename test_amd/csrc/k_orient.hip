@@ -351,7 +351,7 @@ constexpr uint32_t DEAD_MIN = 0x80000000u;
 constexpr uint32_t PIECE = 256;   // neighbours per decrement work item
 constexpr uint32_t PLANES = 32;   // lanes per piece
 constexpr uint32_t PUNROLL = PIECE / PLANES;
-constexpr uint32_t SMALL_FRONT = 16;  // frontier vertices of at most this degree decrement lane per vertex
+constexpr uint32_t SMALL_FRONT = 4;  // frontier vertices of at most this degree decrement lane per vertex
 constexpr uint32_t DEC_TB = 4096;     // LDS table of a workgroup round's decrements (2 * 256 * PUNROLL)
 constexpr uint32_t AGG_MIN_PIECES = 2048;  // steps with fewer pieces decrement directly
 
@@ -504,7 +504,7 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
     lds_sync();
     const uint32_t fs = s_f[NSEG], ps = s_p[NSEG];
     // The frontier takes the next ranks and leaves; a vertex of degree <=
-    // SMALL_FRONT (most of an early level) decrements its neighbours from its
+    // SMALL_FRONT (many of an early level) decrements its neighbours from its
     // own lane, its arcs' loads and atomics in flight together.
     for (uint32_t i0 = gtid - lane; i0 < fs; i0 += gstride) {
       const uint32_t i = i0 + lane;
