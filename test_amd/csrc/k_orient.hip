@@ -312,7 +312,8 @@ __global__ __launch_bounds__(256) void k_dag_copy(const uint32_t *__restrict__ o
 //   OPEN    collect every live vertex of degree <= target into the frontier
 //           and take the minimum degree of the others; the control step
 //           either starts level = target or, when nothing was collected,
-//           retries with target = that minimum (empty levels cost one pass).
+//           retries with target = that minimum (its slack in the slack
+//           order; empty levels cost one pass).
 //   PROCESS the frontier takes the next ranks (in frontier order) and leaves
 //           the graph, decrementing its live neighbours; a neighbour whose
 //           degree falls from L+1 to L joins the next sub-round of level L.
