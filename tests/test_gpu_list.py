@@ -165,3 +165,25 @@ def test_materialize_matches_oracle(solver):
         ref = O.list_cliques(g.offset, g.nbr, k)
         assert T == len(ref)
         np.testing.assert_array_equal(cl, ref)
+
+
+@pytest.mark.parametrize("k", [3, 4])
+def test_autotune_then_count_matches_oracle(solver, k):
+    """The autotuner's trial passes (each class alone, then all classes at
+    once) run into scratch buffers; the shape it keeps is a valid one (k = 3
+    splits rows by 64-row word, so class W takes at most W waves), and counts
+    taken with it are the oracle's."""
+    g = gen.rmat(14, 12.0, (30,), seed=91)
+    Tr, pvr = O.count(g.offset, g.nbr, k)
+    solver.upload(g.offset, g.nbr)
+    solver.orient(None)
+    shape, ms = solver.autotune_list(k)
+    assert all(w in (1, 2, 4, 8) for w in shape), shape
+    if k == 3:
+        assert all(w <= (1 << c) for c, w in enumerate(shape)), shape
+    assert all(t >= 0 for t in ms)
+    T, pv = solver.count(k)
+    assert T == Tr
+    np.testing.assert_array_equal(pv, pvr)
+    assert list(solver.list_info().waves_per_root) == shape
+    solver.set_list_shape(None)
