@@ -130,12 +130,13 @@ def test_exact_and_slack_orders(solver):
     """The exact GPU degeneracy order's widest out-list is the degeneracy (the
     first vertex the peel takes from the kmax-core has exactly kmax live
     neighbours); the slack order stays within kmax + kmax/2.  Both give
-    the oracle's counts.  The 32,768-vertex graph has ordering steps of
-    2,400 (exact) and 4,000 (slack) adjacency pieces, enough for the
-    workgroup-aggregated decrements, so the exact bound checks that path."""
+    the oracle's counts.  The 65,536-vertex graph has ordering steps of
+    2,321 (exact) and 4,458 (slack) adjacency pieces (256 arcs each, for
+    vertices of degree > 4), enough for the workgroup-aggregated decrements
+    (2,048 pieces and up), so the exact bound checks that path."""
     for i, (n, ef, cl) in enumerate(((3000, 12.0, (40,)), (5000, 6.0, (25, 25)), (800, 30.0, ()),
-                                     (32768, 16.0, (40,)))):
-        seed = 77 if n == 32768 else 70 + i
+                                     (65536, 12.0, (40,)))):
+        seed = 77 if n == 65536 else 70 + i
         g = gen.rmat(int(np.ceil(np.log2(n))), ef, cl, seed=seed, n=n)
         _, _, _, kmax = O.core_decompose(g.offset, g.nbr)
         Tr, pvr = O.count(g.offset, g.nbr, 4)
