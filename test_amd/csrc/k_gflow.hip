@@ -42,7 +42,7 @@ constexpr uint32_t SWEEPS = 12;
 constexpr uint32_t RELABEL_BATCH = 16;  // BFS levels per host check
 constexpr uint32_t MAX_PHASES = 1u << 14;
 constexpr uint32_t NO_ARC = 0xFFFFFFFFu;
-constexpr uint32_t VCH = 1024;  // arcs of a vertex node per work item (a wave, 4 x 64 arcs per step)
+constexpr uint32_t VCH = 512;   // arcs of a vertex node per work item (a wave, 4 x 64 arcs per step)
 constexpr uint32_t VU = 4;
 
 // Counters 64 words apart (same-address device atomics are serialised).
