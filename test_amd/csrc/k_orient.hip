@@ -353,7 +353,10 @@ constexpr uint32_t PIECE = 256;   // neighbours per decrement work item
 constexpr uint32_t PLANES = 32;   // lanes per piece
 constexpr uint32_t PUNROLL = PIECE / PLANES;
 constexpr uint32_t SMALL_FRONT = 4;  // frontier vertices of at most this degree decrement lane per vertex
-constexpr uint32_t DEC_TB = 4096;     // LDS table of a workgroup round's decrements (2 * 256 * PUNROLL)
+constexpr uint32_t DEC_TB = 4096;     // LDS table of a workgroup's pending decrements
+constexpr uint32_t DEC_ROUND = 256 * PUNROLL;              // most new keys one workgroup round inserts
+constexpr uint32_t DEC_FLUSH = DEC_TB * 3 / 4 - DEC_ROUND;  // flush above this many keys (load stays <= 3/4)
+static_assert(DEC_TB * 3 / 4 > DEC_ROUND, "a round must fit the decrement table");
 constexpr uint32_t AGG_MIN_PIECES = 2048;  // steps with fewer pieces decrement directly
 
 // Appends v (where want) to a vertex frontier segment fv and, when its
@@ -544,11 +547,12 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
     // A large level's decrements pile onto few counters (at a mid level of
     // an R-MAT graph, 700,000 decrements onto 15,000 vertices, thousands onto
     // each hub), and same-address device atomics serialise.  In a step of at
-    // least AGG_MIN_PIECES pieces a workgroup round (4 waves x 64 / PLANES
-    // pieces) therefore sums its decrements per counter in an LDS table and
-    // applies each sum with one atomic: the counter crossed from above L to L
-    // or below exactly when old > L >= old - sum, which one atomic observes
-    // (departed counters, >= DEAD_MIN, never satisfy it).
+    // least AGG_MIN_PIECES pieces a workgroup therefore sums its decrements
+    // per counter in an LDS table, over its rounds of 4 waves x 64 / PLANES
+    // pieces, and applies each sum with one atomic: the counter crossed from
+    // above L to L or below exactly when old > L >= old - sum, which one
+    // atomic observes whatever the sum covers (departed counters, >=
+    // DEAD_MIN, never satisfy it).
     constexpr uint32_t PW = KCDS_WAVE / PLANES;  // pieces per wave
     constexpr uint32_t BPW = 4 * PW;             // pieces per workgroup round (256 threads)
     const uint32_t sub = lane % PLANES, wv = tid / KCDS_WAVE;
@@ -562,7 +566,7 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
       lds_sync();
     }
     uint32_t par = 0;
-    for (uint32_t it0 = blockIdx.x * BPW; it0 < ps; it0 += gridDim.x * BPW, par ^= 1) {
+    for (uint32_t it0 = blockIdx.x * BPW; it0 < ps; it0 += gridDim.x * BPW) {
       const uint32_t it = it0 + wv * PW + lane / PLANES;
       uint32_t st = 0, e = 0;
       if (it < ps) {
@@ -608,9 +612,15 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
         atomicAdd(&t_cnt[h], 1u);
       }
       lds_sync();
-      // apply the used entries (one atomic each) and clear them; the other
-      // parity's counter is free again for the next round
+      // The sums stay in the table across rounds (a hub counter then takes one
+      // atomic per workgroup, not one per workgroup round) until the next
+      // round could fill it past 3/4, or this was the workgroup's last round.
+      // Then the used entries are applied (one atomic each) and cleared; the
+      // other parity's key counter, last read by the previous flush, is free
+      // again for the keys that follow.
       const uint32_t nu = t_nused[par];
+      lds_sync();  // every wave has read nu before any inserts more keys
+      if (nu <= DEC_FLUSH && it0 + gridDim.x * BPW < ps) continue;  // uniform over the workgroup
       if (tid == 0) t_nused[par ^ 1] = 0;
       for (uint32_t i0 = 0; i0 < nu; i0 += 4 * 256) {
         uint32_t ck[4], cc[4], old[4];
@@ -636,6 +646,7 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
                       &ctl->psize[cur ^ 1][seg][0]);
       }
       lds_sync();
+      par ^= 1;
     }
   }
   // two-level ticket: the last workgroup of a segment, then the last segment.
