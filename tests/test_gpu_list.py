@@ -99,6 +99,31 @@ def test_config2_full(solver):
     assert np.all(pv[g.planted[:60]] >= math.comb(59, 3))
 
 
+def test_config2_exact_order_bound(solver):
+    """The exact GPU degeneracy order of the C2 graph: its widest out-list is
+    the oracle's degeneracy, 373.  Its aggregated ordering steps reach 8,513
+    adjacency pieces, up to 3 rounds per workgroup on a 256-CU device (512
+    workgroups of 8 pieces per round), so decrement sums that stay in LDS
+    across rounds are held to that exact bound.  The slack order stays within
+    kmax + kmax/2; both give C2's clique count."""
+    g, k = gen.config(2)
+    _, _, _, kmax = O.core_decompose(g.offset, g.nbr)
+    assert kmax == 373
+    for flags in (kcds.TUNE_EXACT_ORDER, 0):
+        solver.set_tuning(0, flags)
+        solver.upload(g.offset, g.nbr)
+        solver.orient(None)
+        T, _ = solver.count(k, per_vertex=False)
+        mo = solver.list_info().max_outdeg
+        print(f"  C2 flags={flags} max_outdeg={mo} T={T}", flush=True)
+        assert T == 2281884915
+        if flags:
+            assert mo == kmax
+        else:
+            assert mo <= kmax + kmax // 2
+    solver.set_tuning(0, 0)
+
+
 @pytest.mark.slow
 def test_config3_full_counts_match_oracle_golden(solver):
     """C3 at full size (n = 1,000,000, ~10M edges, k = 5): T = 7.3e10 5-cliques
