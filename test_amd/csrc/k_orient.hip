@@ -566,16 +566,19 @@ __global__ __launch_bounds__(256) void k_ko_step(OrderCtl *ctl, uint32_t n, cons
       lds_sync();
     }
     uint32_t par = 0;
+    // a lane's piece record for a round is loaded during the round before
+    auto piece_at = [&](uint32_t it) -> ull {
+      if (it >= ps) return 0ull;
+      uint32_t g = 0;
+      while (it >= s_p[g + 1]) ++g;
+      return fp[(size_t)g * seg_p + (it - s_p[g])];
+    };
+    const uint32_t my = wv * PW + lane / PLANES;
+    ull item_next = piece_at(blockIdx.x * BPW + my);
     for (uint32_t it0 = blockIdx.x * BPW; it0 < ps; it0 += gridDim.x * BPW) {
-      const uint32_t it = it0 + wv * PW + lane / PLANES;
-      uint32_t st = 0, e = 0;
-      if (it < ps) {
-        uint32_t g = 0;
-        while (it >= s_p[g + 1]) ++g;
-        const ull item = fp[(size_t)g * seg_p + (it - s_p[g])];
-        st = (uint32_t)(item >> 32);
-        e = (uint32_t)item;
-      }
+      const ull item = item_next;
+      item_next = piece_at(it0 + gridDim.x * BPW + my);
+      const uint32_t st = (uint32_t)(item >> 32), e = (uint32_t)item;
       uint32_t u[PUNROLL], gw[PUNROLL];
 #pragma unroll
       for (uint32_t x = 0; x < PUNROLL; ++x) {
