@@ -91,8 +91,11 @@ int kcds_list(kcds_ctx *ctx, uint32_t k, uint64_t *T);
  * lexicographically) to the host; *T receives the total. */
 int kcds_download_cliques(kcds_ctx *ctx, uint32_t *out, uint64_t cap, uint64_t *T);
 
-/* k-clique-core peeling over the materialised cliques.  clique_core[n]
- * (original ids) may be NULL. */
+/* k-clique-core peeling.  After kcds_list it runs over the materialised
+ * cliques; after kcds_count alone (or with KCDS_TUNE_ENUM_PEEL) it enumerates
+ * each removed vertex's live cliques from its neighbourhood instead, which
+ * needs O(n + m) memory whatever T is.  Both give the same core numbers and
+ * statistics.  clique_core[n] (original ids) may be NULL. */
 int kcds_peel(kcds_ctx *ctx, int32_t *clique_core, kcds_peel_stats *stats);
 
 /* Densest core, edge pruning and connected components.  comp[n] receives the
@@ -180,6 +183,13 @@ int kcds_sync(kcds_ctx *ctx);
  *                        (same results; used to test one solver against the
  *                        other).  KCDS_TUNE_GRID_FLOW takes precedence. */
 #define KCDS_TUNE_BLOCK_FLOW 16u
+/*   KCDS_TUNE_ENUM_PEEL  peel by enumerating each removed vertex's live
+ *                        cliques from its neighbourhood (no clique list), even
+ *                        when the clique list would fit; the solve then lists
+ *                        only the k'-core's cliques.  Without the flag a solve
+ *                        takes this path by itself when the clique list would
+ *                        not fit in HBM (T * (8k + 4) bytes, or T >= 2^32). */
+#define KCDS_TUNE_ENUM_PEEL 32u
 int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
 
 /* Listing launch shape: waves per root (workgroup size / 64, 1..8) of each

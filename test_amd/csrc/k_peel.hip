@@ -458,7 +458,9 @@ __global__ void k_graph_summary(uint32_t n, uint64_t T, uint32_t k, const uint32
   }
   // core cliques: a wave whose cliques all lie in one graph adds to a
   // running count and flushes it when the graph changes (a lone graph's
-  // count then takes one atomic per wave, not one per 64 cliques)
+  // count then takes one atomic per wave, not one per 64 cliques).  Without
+  // a clique list (enumerating peel) they are counted later, on the core.
+  if (!cl) return;
   uint32_t acc_g = 0xFFFFFFFFu;
   ull acc = 0;
   for (uint64_t c0 = (uint64_t)blockIdx.x * blockDim.x; c0 < T; c0 += stride) {
@@ -884,6 +886,36 @@ void peel_run(Ctx &c) {
     HIP_CHECK(hipMemcpyAsync(c.h_gpeel.data(), gp, sizeof(GraphPeel) * G, hipMemcpyDeviceToHost, c.stream));
     c.sync();
   }
+  set_peel_stats(c, levels);
+}
+
+// Shared by the enumerating peel (k_epeel.hip): per-graph live cliques and
+// vertices from the per-vertex counts before a peel, and k', the k'-core
+// size and the largest core after it (cl = nullptr: no clique list, so the
+// core cliques are left at 0).
+void peel_graphs_begin(Ctx &c, GraphPeel *gp) {
+  const uint32_t n = c.n, G = c.ngraphs;
+  HIP_CHECK(hipMemsetAsync(gp, 0, sizeof(GraphPeel) * G, c.stream));
+  if (n) {
+    k_graph_accum<<<grid1d(n), 256, 0, c.stream>>>(n, c.gid.as<uint32_t>(), c.cnt.as<ull>(), gp);
+    KCDS_LAUNCH_CHECK();
+  }
+  k_graph_init<<<grid1d(G), 256, 0, c.stream>>>(G, c.k, gp);
+  KCDS_LAUNCH_CHECK();
+}
+
+void peel_graphs_end(Ctx &c, GraphPeel *gp, uint32_t levels) {
+  const uint32_t n = c.n, G = c.ngraphs;
+  k_graph_kprime<<<grid1d(G), 256, 0, c.stream>>>(G, gp);
+  KCDS_LAUNCH_CHECK();
+  if (n) {
+    k_graph_summary<<<std::min<uint32_t>(grid1d(n), c.num_cus * 8), 256, 0, c.stream>>>(
+        n, 0, c.k, c.gid.as<uint32_t>(), c.core.as<int32_t>(), nullptr, nullptr, gp);
+    KCDS_LAUNCH_CHECK();
+  }
+  c.h_gpeel.resize(G);
+  HIP_CHECK(hipMemcpyAsync(c.h_gpeel.data(), gp, sizeof(GraphPeel) * G, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
   set_peel_stats(c, levels);
 }
 

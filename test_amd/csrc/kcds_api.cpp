@@ -134,7 +134,7 @@ void kcds_destroy(kcds_ctx *ctx) {
   CompData &g = c.cd;
   FlowBufs &f = c.fb;
   DevBuf *bufs[] = {&c.off, &c.nbr, &c.rank, &c.inv, &c.doff, &c.dnbr, &c.roots, &c.qctr, &c.cnt, &c.scal,
-                    &c.rootT, &c.lsum, &c.cliques, &c.incoff, &c.inc, &c.core, &c.status, &c.comp, &c.tmp0,
+                    &c.rootT, &c.lsum, &c.cliques, &c.incoff, &c.inc, &c.core, &c.status, &c.foff, &c.fnbr, &c.erem, &c.kdoff, &c.kdnbr, &c.comp, &c.tmp0,
                     &c.tmp1, &c.tmp2, &c.tmp3, &c.gvoff, &c.gid, &c.gpeel, &c.tlist, &c.inset,
                     &g.par, &g.flag, &g.dense, &g.cid, &g.local, &g.vcount, &g.voff, &g.verts, &g.ccount,
                     &g.coff, &g.cfill, &g.ccl, &f.scal, &f.cside, &f.g_off, &f.g_head, &f.g_rev, &f.g_res,
@@ -260,8 +260,14 @@ int kcds_download_cliques(kcds_ctx *ctx, uint32_t *out, uint64_t cap, uint64_t *
 
 int kcds_peel(kcds_ctx *ctx, int32_t *clique_core, kcds_peel_stats *stats) {
   return guarded(ctx, [&](Ctx &c) {
-    if (!c.listed) fail(KCDS_E_STATE, "kcds_peel before kcds_list");
-    peel_run(c);
+    if (!c.counted) fail(KCDS_E_STATE, "kcds_peel before kcds_count or kcds_list");
+    if (c.listed && !(c.tune_flags & KCDS_TUNE_ENUM_PEEL)) {
+      c.enum_peel = false;
+      peel_run(c);
+    } else {
+      epeel_run(c);
+      core_materialize(c);  // the k'-core's cliques (the statistics report their number)
+    }
     if (stats) *stats = c.peel;
     if (clique_core && c.n) {
       buf_reserve(c, c.tmp1, sizeof(int32_t) * (size_t)c.n);
@@ -275,6 +281,10 @@ int kcds_peel(kcds_ctx *ctx, int32_t *clique_core, kcds_peel_stats *stats) {
 int kcds_components(kcds_ctx *ctx, uint32_t *comp, uint32_t *ncomp) {
   return guarded(ctx, [&](Ctx &c) {
     if (!c.peeled) fail(KCDS_E_STATE, "kcds_components before kcds_peel");
+    if (c.enum_peel && !c.core_counted) core_materialize(c);
+    if (c.enum_peel && !c.core_listed)
+      fail(KCDS_E_LIMIT, "the k'-core holds %llu cliques, too many to list in HBM",
+           (unsigned long long)c.peel.core_cliques);
     components_run(c);
     if (ncomp) *ncomp = c.ncomp;
     if (comp && c.n) {

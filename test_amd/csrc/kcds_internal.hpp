@@ -159,6 +159,13 @@ struct Ctx {
   // peeling
   DevBuf core;    // int32[n] new-id space
   DevBuf status;  // int32[T]
+  // enumerating peel: full adjacency over new ids (in-list, then out-list)
+  // and the step at which each vertex leaves (0xFFFFFFFF while live)
+  DevBuf foff, fnbr, erem;
+  DevBuf kdoff, kdnbr;      // the k'-core's DAG (listing of the core cliques)
+  bool enum_peel = false;   // the last peel enumerated cliques (no clique list)
+  bool core_counted = false; // ... and the k'-cores' cliques have been counted since
+  bool core_listed = false;  // ... and materialised (they fit in HBM)
   kcds_peel_stats peel{};
   bool peeled = false;
 
@@ -222,8 +229,17 @@ ull drop_graphs(Ctx &c, const std::vector<uint8_t> &drop, const std::vector<ull>
 uint32_t list_waves_per_root(const Ctx &c, int cls, uint32_t k);  // effective waves per root of a class
 void list_materialize(Ctx &c, uint32_t k);  // cliques, incoff, inc (requires list_count)
 
-// ---- peeling (k_peel.hip) ----
+// ---- peeling (k_peel.hip, k_epeel.hip) ----
 void peel_run(Ctx &c);
+void peel_graphs_begin(Ctx &c, GraphPeel *gp);
+void peel_graphs_end(Ctx &c, GraphPeel *gp, uint32_t levels);
+// Clique-core peel from the per-vertex counts of the count pass alone: each
+// removed vertex's live cliques are enumerated from its neighbourhood.
+void epeel_run(Ctx &c);
+// After epeel_run: counts the k'-cores' cliques (status >= k') and, when they
+// fit, materialises them, so the components and flow stages run as after a
+// materialised peel.
+void core_materialize(Ctx &c);
 
 // ---- components (k_comp.hip) ----
 void components_run(Ctx &c);

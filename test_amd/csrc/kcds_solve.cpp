@@ -179,6 +179,7 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
   }
   std::vector<ull> gT;
   std::vector<uint8_t> drop(G, 0);
+  bool enumerate = false;
   {
     Timer t(c.stream);
     // per-graph HBM admission on the count pass: cliques, incidence and
@@ -202,7 +203,13 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
            (double)gT[0] * (8.0 * k + 4.0), (unsigned long long)c.graph_budget);
     }
     if (ndrop) drop_graphs(c, drop, gT);
-    if (c.T) list_materialize(c, k);
+    // the clique list, its incidence and status take T * (8k + 4) bytes and
+    // 32-bit clique ids; when that does not fit, the peel enumerates instead
+    // and only the k'-cores' cliques are listed afterwards
+    const size_t avail = c.budget > c.used ? c.budget - c.used : 0;
+    enumerate = (c.tune_flags & KCDS_TUNE_ENUM_PEEL) ||
+                (c.T && (c.T >= 0xFFFFFFFFull || (double)c.T * (8.0 * k + 4.0) > 0.85 * (double)avail));
+    if (c.T && !enumerate) list_materialize(c, k);
     R.ms_list = t.stop();
   }
   if (in_set && c.n) memset(in_set, 0, c.n);
@@ -224,7 +231,12 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
   }
   {
     Timer t(c.stream);
-    peel_run(c);
+    if (enumerate) {
+      epeel_run(c);
+    } else {
+      c.enum_peel = false;
+      peel_run(c);
+    }
     R.peel = c.peel;
     R.k_prime = c.peel.k_prime;
     R.core_vertices = c.peel.core_size;
@@ -232,6 +244,13 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
   }
   {
     Timer t(c.stream);
+    if (enumerate) {
+      core_materialize(c);
+      R.peel.core_cliques = c.peel.core_cliques;
+      if (!c.core_listed)
+        fail(KCDS_E_LIMIT, "the k'-core holds %llu cliques, too many to list in HBM",
+             (unsigned long long)c.peel.core_cliques);
+    }
     components_run(c);
     R.ncomp = c.ncomp;
     R.ms_components = t.stop();
