@@ -60,6 +60,8 @@ typedef struct kcds_result {
   uint32_t flow_calls;   /* max-flow solves */
   kcds_peel_stats peel;
   float ms_orient, ms_list, ms_peel, ms_components, ms_flow, ms_total; /* stage times (hipEvents) */
+  uint32_t dense_passes; /* greedy passes of the dense exact search (0: clique flow networks were used) */
+  uint32_t pad;
 } kcds_result;
 
 /* Creates a context on `device` with an HBM budget (0 = 90% of free memory). */
@@ -190,6 +192,15 @@ int kcds_sync(kcds_ctx *ctx);
  *                        takes this path by itself when the clique list would
  *                        not fit in HBM (T * (8k + 4) bytes, or T >= 2^32). */
 #define KCDS_TUNE_ENUM_PEEL 32u
+/*   KCDS_TUNE_DENSE      take the dense exact search for a single graph whose
+ *                        k'-core has at most 1,024 vertices, whatever its
+ *                        clique count (by default only from 2^24 core cliques
+ *                        up, where a clique flow network stops fitting): the
+ *                        core is held as a bitmask matrix, greedy clique
+ *                        splits (one listing pass each) are combined by
+ *                        Frank-Wolfe, and the answer is certified by an exact
+ *                        integer max-flow over vertex pairs. */
+#define KCDS_TUNE_DENSE 64u
 int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
 
 /* Listing launch shape: waves per root (workgroup size / 64, 1..8) of each

@@ -969,9 +969,10 @@ void epeel_run(Ctx &c) {
 // records each graph's count; when they fit in HBM, materialises them with
 // their incidence and status = INT32_MAX, so that the components and flow
 // stages run unchanged.  c.T becomes the number of core cliques.
-void core_materialize(Ctx &c) {
-  if (!c.peeled || !c.enum_peel) fail(KCDS_E_STATE, "core cliques before the enumerating peel");
-  const uint32_t n = c.n, k = c.k, G = c.ngraphs;
+// The DAG restricted to every graph's k'-core (c.kdoff, c.kdnbr); returns
+// its arcs.
+uint64_t build_core_dag(Ctx &c) {
+  const uint32_t n = c.n;
   buf_reserve(c, c.tmp1, sizeof(uint32_t) * ((size_t)n + 2));
   buf_reserve(c, c.kdoff, sizeof(uint32_t) * ((size_t)n + 2));
   uint32_t *kdeg = c.tmp1.as<uint32_t>();
@@ -995,6 +996,13 @@ void core_materialize(Ctx &c) {
                                                                c.kdnbr.as<uint32_t>());
     KCDS_LAUNCH_CHECK();
   }
+  return karcs;
+}
+
+void core_materialize(Ctx &c, bool materialize) {
+  if (!c.peeled || !c.enum_peel) fail(KCDS_E_STATE, "core cliques before the enumerating peel");
+  const uint32_t k = c.k, G = c.ngraphs;
+  const uint64_t karcs = build_core_dag(c);
   // list on the core DAG (the full DAG is swapped back afterwards)
   const kcds_peel_stats keep_stats = c.peel;
   const uint64_t keep_w2 = c.w2, keep_arcs = c.darcs;
@@ -1017,7 +1025,7 @@ void core_materialize(Ctx &c) {
   const uint64_t Tc = c.T;
   // the core cliques, their incidence and status: Tc * (8k + 4) bytes
   const size_t avail = c.budget > c.used ? c.budget - c.used : 0;
-  const bool fits = Tc < 0xFFFFFFFFull && (double)Tc * (8.0 * k + 4.0) <= 0.85 * (double)avail;
+  const bool fits = materialize && Tc < 0xFFFFFFFFull && (double)Tc * (8.0 * k + 4.0) <= 0.85 * (double)avail;
   if (Tc && fits) list_materialize(c, k);
   if (fits) {
     buf_reserve(c, c.status, sizeof(int32_t) * (Tc + 1));

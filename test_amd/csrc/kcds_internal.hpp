@@ -239,7 +239,16 @@ void epeel_run(Ctx &c);
 // After epeel_run: counts the k'-cores' cliques (status >= k') and, when they
 // fit, materialises them, so the components and flow stages run as after a
 // materialised peel.
-void core_materialize(Ctx &c);
+void core_materialize(Ctx &c, bool materialize = true);
+uint64_t build_core_dag(Ctx &c);  // c.kdoff/c.kdnbr: the DAG restricted to the k'-cores
+// Exact search on graph 0's k'-core when it has at most 1,024 vertices (false
+// otherwise, or when the certificate is not reached within max_iters greedy
+// passes): the Frank-Wolfe / integer-certificate search of k_dense.hip.
+// num/den = cliques in / size of the maximal densest set, marked in c.inset.
+bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, uint32_t &passes,
+                      uint32_t max_iters);
+constexpr ull DENSE_MIN_T = 1ull << 24;  // k'-cores with this many cliques take the dense search
+constexpr uint32_t DENSE_MAX_ITERS = 12000;  // major cycles of the minimum-norm-point search
 
 // ---- components (k_comp.hip) ----
 void components_run(Ctx &c);

@@ -242,6 +242,56 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
     R.core_vertices = c.peel.core_size;
     R.ms_peel = t.stop();
   }
+  // A single graph whose k'-core is small in vertices but huge in cliques
+  // (from DENSE_MIN_T up, or always with KCDS_TUNE_DENSE) takes the dense
+  // exact search: no clique list, no clique flow network (k_dense.hip).
+  const bool dense_try = G == 1 && k >= 3 && c.peel.core_size > 0 && c.peel.core_size <= 1024 &&
+                         ((c.tune_flags & KCDS_TUNE_DENSE) || c.peel.core_cliques >= DENSE_MIN_T || enumerate);
+  if (dense_try) {
+    Timer t(c.stream);
+    if (enumerate) {
+      core_materialize(c, false);  // counts the core's cliques, lists nothing
+      R.peel.core_cliques = c.peel.core_cliques;
+    } else {
+      build_core_dag(c);
+    }
+    if ((c.tune_flags & KCDS_TUNE_DENSE) || c.peel.core_cliques >= DENSE_MIN_T) {
+      ull num = 0, den = 0;
+      uint32_t ncomp = 0, passes = 0;
+      if (dense_core_solve(c, k, num, den, ncomp, passes, DENSE_MAX_ITERS)) {
+        R.ms_components = t.stop();
+        Timer tf(c.stream);
+        R.ncomp = ncomp;
+        R.dense_passes = passes;
+        R.rho_num = num;
+        R.rho_den = den;
+        out[0].T = c.h_gpeel[0].T;
+        out[0].rho_num = num;
+        out[0].rho_den = den;
+        out[0].k_prime = c.h_gpeel[0].k_prime;
+        out[0].core_vertices = c.h_gpeel[0].core_size;
+        out[0].ncomp = ncomp;
+        if (in_set && c.n) {
+          std::vector<uint8_t> tmp(c.n);
+          std::vector<uint32_t> rank(c.n);
+          HIP_CHECK(hipMemcpyAsync(tmp.data(), c.inset.p, c.n, hipMemcpyDeviceToHost, c.stream));
+          HIP_CHECK(hipMemcpyAsync(rank.data(), c.rank.p, sizeof(uint32_t) * c.n, hipMemcpyDeviceToHost,
+                                   c.stream));
+          c.sync();
+          for (uint32_t v = 0; v < c.n; ++v) in_set[v] = tmp[rank[v]];
+        }
+        R.ms_flow = tf.stop();
+        R.ms_total = total.stop();
+        if (agg) *agg = R;
+        if (per_graph) memcpy(per_graph, out.data(), sizeof(kcds_graph_result) * G);
+        return;
+      }
+      if (c.peel.core_cliques >= 0xFFFFFFFFull / (2ull * k))
+        fail(KCDS_E_LIMIT, "dense exact search on the %u-vertex k'-core (%llu cliques) not certified within %u "
+             "greedy passes", c.peel.core_size, (unsigned long long)c.peel.core_cliques, passes);
+    }
+    (void)t.stop();
+  }
   {
     Timer t(c.stream);
     if (enumerate) {

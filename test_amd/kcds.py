@@ -30,7 +30,7 @@ EXPORTS = ("kcds_create", "kcds_destroy", "kcds_last_error", "kcds_upload_csr", 
 
 GRAPH_SOLVED, GRAPH_OVER_BUDGET = 0, 1
 TUNE_GRID_PEEL, TUNE_GRID_FLOW, TUNE_EXACT_ORDER, TUNE_NO_GRAPHS, TUNE_BLOCK_FLOW = 1, 2, 4, 8, 16
-TUNE_ENUM_PEEL = 32
+TUNE_ENUM_PEEL, TUNE_DENSE = 32, 64
 E_INVALID, E_HIP, E_NOMEM, E_STATE, E_LIMIT = -1, -2, -3, -4, -5
 
 
@@ -45,7 +45,8 @@ class Result(ctypes.Structure):
                 ("k_prime", ctypes.c_uint32), ("core_vertices", ctypes.c_uint32), ("ncomp", ctypes.c_uint32),
                 ("flow_calls", ctypes.c_uint32), ("peel", PeelStats),
                 ("ms_orient", ctypes.c_float), ("ms_list", ctypes.c_float), ("ms_peel", ctypes.c_float),
-                ("ms_components", ctypes.c_float), ("ms_flow", ctypes.c_float), ("ms_total", ctypes.c_float)]
+                ("ms_components", ctypes.c_float), ("ms_flow", ctypes.c_float), ("ms_total", ctypes.c_float),
+                ("dense_passes", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
 
 
 class GraphResult(ctypes.Structure):

@@ -1,0 +1,130 @@
+"""GPU parity of the dense exact search (k_dense.hip): the k'-core held as a
+bitmask matrix, greedy clique splits combined by Frank-Wolfe, and the answer
+certified by an exact integer max-flow over vertex pairs.
+
+The search is forced (KCDS_TUNE_DENSE) on graphs whose k'-core has at most
+1,024 vertices, after either peel, and must return the oracle's density
+rational and maximal densest set, including ties that span components.
+"""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from test_amd import gen, kcds
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def disjoint_cliques(sizes, extra_edges=()):
+    n = sum(sizes)
+    adj = [set() for _ in range(n)]
+    at = 0
+    for s in sizes:
+        for i in range(at, at + s):
+            adj[i].update(j for j in range(at, at + s) if j != i)
+        at += s
+    for u, v in extra_edges:
+        adj[u].add(v)
+        adj[v].add(u)
+    off = np.zeros(n + 1, np.uint32)
+    for v in range(n):
+        off[v + 1] = off[v] + len(adj[v])
+    nbr = np.array([u for v in range(n) for u in sorted(adj[v])], np.uint32)
+    return off, nbr
+
+
+def graphs():
+    off, nbr, _ = O.load_adjlist(os.path.join(GOLD, "ref_data_test.txt"))
+    yield "ref_data_test", off, nbr, (3, 4)
+    g = gen.er(400, 0.04, 18, seed=21)
+    yield "er400", g.offset, g.nbr, (3, 4, 5, 6)
+    g = gen.rmat(10, 8, (12,), seed=9)
+    yield "rmat10", g.offset, g.nbr, (3, 4)
+    off, nbr = disjoint_cliques([6, 6, 5], extra_edges=[(0, 6), (11, 12)])
+    yield "tied_cliques", off, nbr, (3, 4)
+    g = gen.rmat(12, 16, (30, 20), seed=5)
+    yield "rmat12_hubs", g.offset, g.nbr, (3,)
+    g = gen.er(1500, 0.02, 30, seed=3)
+    yield "er1500", g.offset, g.nbr, (3, 4)
+    g = gen.chunglu(20000, 2.1, 2.0, 900.0, 120000, 25, 0.95, seed=17)
+    yield "chunglu20k", g.offset, g.nbr, (4,)
+
+
+GRAPHS = list(graphs())
+PEELS = {"listed_peel": kcds.TUNE_DENSE, "enum_peel": kcds.TUNE_DENSE | kcds.TUNE_ENUM_PEEL}
+
+
+@pytest.mark.parametrize("peel", list(PEELS))
+@pytest.mark.parametrize("name,off,nbr,ks", GRAPHS, ids=[g[0] for g in GRAPHS])
+def test_dense_solve_matches_oracle(solver, name, off, nbr, ks, peel):
+    solver.set_tuning(0, PEELS[peel])
+    try:
+        for k in ks:
+            solver.upload(off, nbr)
+            res, ins = solver.solve(k)
+            r, _, _, comp_r, ins_r = O.solve(off, nbr, k)
+            print(f"  {name} k={k} rho={res.rho_num}/{res.rho_den} core={res.core_vertices} "
+                  f"passes={res.dense_passes} ncomp={res.ncomp}", flush=True)
+            assert res.dense_passes > 0  # the dense search ran
+            assert res.T == r.T
+            assert (res.rho_num, res.rho_den) == (r.rho_num, r.rho_den)
+            assert (res.k_prime, res.core_vertices) == (r.k_prime, r.core_vertices)
+            assert res.ncomp == len(set(int(x) for x in comp_r if x != 0xFFFFFFFF))
+            np.testing.assert_array_equal(ins, ins_r)
+    finally:
+        solver.set_tuning(0, 0)
+
+
+def test_dense_tied_components_union(solver):
+    """Two disjoint K6 and a K5: the maximal densest set is both K6."""
+    off, nbr = disjoint_cliques([6, 6, 5])
+    solver.set_tuning(0, kcds.TUNE_DENSE)
+    try:
+        solver.upload(off, nbr)
+        res, ins = solver.solve(3)
+    finally:
+        solver.set_tuning(0, 0)
+    assert res.dense_passes > 0
+    assert (res.rho_num, res.rho_den) == (40, 12)
+    assert set(np.flatnonzero(ins).tolist()) == set(range(12))
+
+
+def brute_densest(n, cl):
+    best, union = (0, 1), 0
+    sets = []
+    for m in range(1, 1 << n):
+        inside = sum(1 for c in cl if all((m >> v) & 1 for v in c))
+        size = bin(m).count("1")
+        sets.append((m, inside, size))
+        if inside * best[1] > best[0] * size:
+            best = (inside, size)
+    for m, inside, size in sets:
+        if inside and inside * best[1] == best[0] * size:
+            union |= m
+    return best, union
+
+
+def test_dense_brute_force_small(solver):
+    rng = np.random.default_rng(11)
+    solver.set_tuning(0, kcds.TUNE_DENSE)
+    try:
+        for trial in range(8):
+            n = int(rng.integers(7, 12))
+            g = gen.er(n, float(rng.uniform(0.45, 0.85)), 0, seed=700 + trial)
+            adj = [set(g.nbr[g.offset[v]:g.offset[v + 1]].tolist()) for v in range(n)]
+            for k in (3, 4):
+                cl = [c for c in itertools.combinations(range(n), k)
+                      if all(b in adj[a] for a, b in itertools.combinations(c, 2))]
+                if not cl:
+                    continue
+                solver.upload(g.offset, g.nbr)
+                res, ins = solver.solve(k)
+                best, union = brute_densest(n, cl)
+                assert res.rho_num * best[1] == best[0] * res.rho_den
+                assert int(sum(1 << int(v) for v in np.flatnonzero(ins))) == union
+    finally:
+        solver.set_tuning(0, 0)
