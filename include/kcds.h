@@ -38,15 +38,16 @@ typedef enum kcds_status {
   KCDS_E_HIP = -2,     /* HIP runtime error (message in kcds_last_error) */
   KCDS_E_NOMEM = -3,   /* HBM budget of the context exceeded or allocation failed */
   KCDS_E_STATE = -4,   /* call out of order, e.g. kcds_count before kcds_orient */
-  KCDS_E_LIMIT = -5    /* size beyond a supported limit (message names it) */
+  KCDS_E_LIMIT = -5,   /* size beyond a supported limit (message names it) */
+  KCDS_E_PENDING = -6  /* kcds_poll: the request has not finished yet */
 } kcds_status;
 
 typedef struct kcds_peel_stats {
   uint64_t max_num, max_den; /* maxDensity of the densest clique-core, exact (main.cu:398-413) */
-  uint32_t k_prime;          /* ceil(maxDensity) (main.cu:1476) */
-  uint32_t core_size;        /* |{v : core(v) >= k_prime}| (main.cu:1482) */
+  uint64_t k_prime;          /* ceil(maxDensity) (main.cu:1476) */
   uint64_t core_cliques;     /* cliques with every member in the k'-core */
-  int32_t max_core;          /* largest clique-core number */
+  int64_t max_core;          /* largest clique-core number */
+  uint32_t core_size;        /* |{v : core(v) >= k_prime}| (main.cu:1482) */
   uint32_t levels;           /* non-empty peeling levels processed */
 } kcds_peel_stats;
 
@@ -54,7 +55,7 @@ typedef struct kcds_result {
   uint64_t T;            /* k-cliques in the graph */
   uint64_t rho_num;      /* cliques inside the answer set S */
   uint64_t rho_den;      /* |S|: density = rho_num / rho_den exactly */
-  uint32_t k_prime;
+  uint64_t k_prime;
   uint32_t core_vertices;
   uint32_t ncomp;        /* components of the pruned k'-core */
   uint32_t flow_calls;   /* max-flow solves */
@@ -97,8 +98,10 @@ int kcds_download_cliques(kcds_ctx *ctx, uint32_t *out, uint64_t cap, uint64_t *
  * cliques; after kcds_count alone (or with KCDS_TUNE_ENUM_PEEL) it enumerates
  * each removed vertex's live cliques from its neighbourhood instead, which
  * needs O(n + m) memory whatever T is.  Both give the same core numbers and
- * statistics.  clique_core[n] (original ids) may be NULL. */
-int kcds_peel(kcds_ctx *ctx, int32_t *clique_core, kcds_peel_stats *stats);
+ * statistics.  Core numbers are 64-bit: a vertex's count can exceed 2^31 (the
+ * reference's cliqueCore is a 32-bit int, main.cu:329-420).  clique_core[n]
+ * (original ids) may be NULL. */
+int kcds_peel(kcds_ctx *ctx, int64_t *clique_core, kcds_peel_stats *stats);
 
 /* Densest core, edge pruning and connected components.  comp[n] receives the
  * canonical component id of each vertex (smallest original vertex id in its
@@ -122,7 +125,7 @@ typedef struct kcds_graph_result {
   uint64_t T;          /* k-cliques of the graph */
   uint64_t rho_num;    /* cliques inside the graph's maximal densest set */
   uint64_t rho_den;    /* its size (0 when the graph has no k-clique) */
-  uint32_t k_prime;
+  uint64_t k_prime;
   uint32_t core_vertices;
   uint32_t ncomp;
   uint32_t flow_calls;
@@ -153,6 +156,60 @@ int kcds_set_graph_budget(kcds_ctx *ctx, uint64_t bytes);
  * counterpart in the reference, which solves one graph per process. */
 int kcds_solve_batch(kcds_ctx *ctx, uint32_t k, uint32_t ngraphs, const uint32_t *graph_voff,
                      kcds_graph_result *results, uint8_t *in_set);
+
+/* ---- Serving: continuous batching behind submit / poll ----------------------
+ * No counterpart in the reference (one graph per process).  A server owns
+ * `slots` batch contexts and `large_slots` large-lane contexts, each with
+ * its own host thread and HIP stream.  Requests wait in per-k queues; a free
+ * batch slot takes the k whose oldest request has waited longest and solves
+ * that k's waiting requests, oldest first, up to max_batch graphs and
+ * batch_edges adjacency entries, in one kcds_solve_batch.  Requests submitted
+ * meanwhile join the next batch, and batches of different slots overlap on
+ * the device.  A graph whose clique list exceeds graph_budget bytes (T *
+ * (8k + 4)) moves to the large lane, which solves one graph at a time
+ * without a clique list; one with more than max_cliques cliques is answered
+ * KCDS_GRAPH_OVER_BUDGET.  kcds_submit copies the CSR.  Results equal
+ * kcds_solve on the graph alone. */
+typedef struct kcds_server kcds_server;
+
+typedef struct kcds_server_config {
+  int32_t device;
+  uint32_t slots;        /* batch slots (>= 1) */
+  uint32_t large_slots;  /* large-lane slots; 0: graphs over graph_budget are answered over budget */
+  uint32_t max_batch;    /* graphs per batch (0 = 512) */
+  uint64_t batch_edges;  /* adjacency entries per batch (0 = 4,000,000) */
+  uint64_t graph_budget; /* clique-list bytes per graph in a batch (0 = no per-graph cap) */
+  uint64_t max_cliques;  /* graphs with more cliques are answered over budget (0 = no cap) */
+  uint64_t hbm_budget;   /* bytes shared by all contexts (0 = each context 90% of free memory) */
+  uint32_t want_sets;    /* keep each answer's vertex set for kcds_poll */
+  uint32_t pad;
+} kcds_server_config;
+
+typedef struct kcds_served {
+  kcds_graph_result result;
+  double t_submit, t_done; /* kcds_server_clock() seconds */
+  uint32_t batch;          /* graphs in the batch that answered it (1 in the large lane) */
+  uint32_t lane;           /* 0 batch slot, 1 large lane */
+  int32_t code;            /* KCDS_OK, or the status that left it unsolved (KCDS_E_NOMEM: over budget) */
+  uint32_t pad;
+} kcds_served;
+
+int kcds_server_create(kcds_server **srv, const kcds_server_config *cfg);
+/* Queues a request (undirected CSR as for kcds_upload_csr) and returns its ticket. */
+int kcds_submit(kcds_server *srv, uint32_t n, const uint32_t *offset, const uint32_t *nbr, uint32_t k,
+                uint64_t *ticket);
+/* Waits up to wait_ms (0: no wait, < 0: until done) for the request; KCDS_OK
+ * fills out (and in_set[n], if both it and want_sets are set) and forgets the
+ * ticket, KCDS_E_PENDING if it has not finished. */
+int kcds_poll(kcds_server *srv, uint64_t ticket, int32_t wait_ms, kcds_served *out, uint8_t *in_set);
+/* Seconds since the server was created (the clock of t_submit / t_done). */
+double kcds_server_clock(const kcds_server *srv);
+/* Answers every request that no slot has started yet with code
+ * KCDS_E_STATE (status KCDS_GRAPH_OVER_BUDGET, nothing solved); requests in
+ * flight finish normally.  Returns how many were cancelled. */
+int kcds_server_cancel_pending(kcds_server *srv);
+/* Serves every queued request, then stops the threads and frees the contexts. */
+void kcds_server_destroy(kcds_server *srv);
 
 /* Blocks until all work queued on the context's stream has finished. */
 int kcds_sync(kcds_ctx *ctx);

@@ -82,7 +82,7 @@ __global__ void k_uf_compress(uint32_t n, uint32_t *par) {
 // a clique is in its graph's k'-core iff its status (min member core) >= k'
 __device__ __forceinline__ bool core_clique(uint64_t c, uint32_t k, const uint32_t *cl, const int32_t *status,
                                             const GraphPeel *gp, const uint32_t *gid) {
-  return status[c] >= (int32_t)gp[gid[cl[c * k]]].k_prime;
+  return (long long)status[c] >= (long long)gp[gid[cl[c * k]]].k_prime;
 }
 
 // Unites the members of every stride-th clique.
@@ -116,10 +116,10 @@ __global__ void k_uf_hook(uint64_t T, uint64_t stride, uint32_t k, const uint32_
 
 // comp[r] = key of the root for core vertices, ~0 elsewhere; roots flagged in
 // original-id order for dense numbering
-__global__ void k_uf_label(uint32_t n, const int32_t *core, const GraphPeel *gp, const uint32_t *gid, uint32_t *par,
+__global__ void k_uf_label(uint32_t n, const long long *core, const GraphPeel *gp, const uint32_t *gid, uint32_t *par,
                            const uint32_t *key, uint32_t *comp, uint32_t *isroot_orig) {
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-    const bool in = core[r] >= (int32_t)gp[gid[r]].k_prime;
+    const bool in = core[r] >= (long long)gp[gid[r]].k_prime;
     const uint32_t root = uf_find(par, r);
     comp[r] = in ? key[root] : 0xFFFFFFFFu;
     isroot_orig[key[r]] = (in && root == r) ? 1u : 0u;
@@ -281,7 +281,7 @@ void components_run(Ctx &c) {
       KCDS_LAUNCH_CHECK();
     }
   }
-  k_uf_label<<<grid1d(n), 256, 0, c.stream>>>(n, c.core.as<int32_t>(), gp, gidp, par, key, c.comp.as<uint32_t>(),
+  k_uf_label<<<grid1d(n), 256, 0, c.stream>>>(n, c.core.as<long long>(), gp, gidp, par, key, c.comp.as<uint32_t>(),
                                               g.flag.as<uint32_t>());
   KCDS_LAUNCH_CHECK();
   // dense[orig] = number of component roots with a smaller original id

@@ -140,8 +140,8 @@ template <int W, bool WP> struct DCount<W, 1, WP> {
 // lowest-rank member is i; with WP, P[i*n + j] = those of them containing j.
 // A wave takes (root i, candidate word w0) tasks from a ticket counter.
 template <int W, int K, bool WP>
-__global__ __launch_bounds__(DN_THREADS) void k_dn_pass(const ull *__restrict__ Mr, uint32_t n, uint32_t *ticket,
-                                                        ull *q, ull *P) {
+__global__ __launch_bounds__(DN_THREADS) void k_dn_pass(const ull *__restrict__ Mr, uint32_t n, uint32_t split,
+                                                        uint32_t *ticket, ull *q, ull *P) {
   constexpr uint32_t NP = 64u * W;
   __shared__ ull A[W * NP];
   for (uint32_t e = threadIdx.x; e < W * NP; e += blockDim.x) {
@@ -150,13 +150,17 @@ __global__ __launch_bounds__(DN_THREADS) void k_dn_pass(const ull *__restrict__ 
   }
   __syncthreads();
   const uint32_t lane = lane_id();
-  const uint32_t ntask = n * W;
+  // task (i, w0, sub): root i's candidates in word w0 whose bit index is in
+  // the sub-th of `split` equal ranges (small cores need many tasks to fill
+  // the GPU: a dense core's low-rank roots carry most of the work)
+  const uint32_t ntask = n * W * split, bits = 64 / split;
   while (true) {
     uint32_t t = 0;
     if (lane == 0) t = atomicAdd(ticket, 1u);
     t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
     if (t >= ntask) break;
-    const uint32_t i = t / W, w0 = t % W;
+    const uint32_t sub = t % split, iw = t / split;
+    const uint32_t i = iw / W, w0 = iw % W;
     if (w0 < (i >> 6)) continue;
     ull Sr[W];
 #pragma unroll
@@ -165,6 +169,7 @@ __global__ __launch_bounds__(DN_THREADS) void k_dn_pass(const ull *__restrict__ 
 #pragma unroll
     for (int w = 0; w < W; ++w)
       if ((uint32_t)w == w0) cand = Sr[w];
+    cand &= bits == 64 ? ~0ull : (((1ull << bits) - 1ull) << (sub * bits));
     if (!cand) continue;
     ull cr[W];
 #pragma unroll
@@ -198,7 +203,7 @@ __global__ __launch_bounds__(DN_THREADS) void k_dn_pass(const ull *__restrict__ 
 }
 
 // core vertices (new ids) of graph 0: core >= k'
-__global__ void k_dn_collect(uint32_t n, const int32_t *core, int32_t kp, uint32_t *cnt, uint32_t *verts) {
+__global__ void k_dn_collect(uint32_t n, const long long *core, long long kp, uint32_t *cnt, uint32_t *verts) {
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
     if (core[r] >= kp) {
       const uint32_t at = atomicAdd(cnt, 1u);
@@ -281,7 +286,7 @@ uint32_t dgrid(uint64_t len) {
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((len + 255) / 256, 65535ull * 16));
 }
 
-typedef void (*PassFn)(const ull *, uint32_t, uint32_t *, ull *, ull *);
+typedef void (*PassFn)(const ull *, uint32_t, uint32_t, uint32_t *, ull *, ull *);
 
 template <int W, bool WP> PassFn pass_fn_k(uint32_t k) {
   switch (k) {
@@ -484,8 +489,11 @@ struct DenseSolver {
     KCDS_LAUNCH_CHECK();
     PassFn fn = want_P ? pass_fn<true>(W, k) : pass_fn<false>(W, k);
     if (!fn) fail(KCDS_E_INVALID, "dense pass: k = %u, %u words", k, W);
-    hipLaunchKernelGGL(fn, dim3(c.num_cus), dim3(DN_THREADS), 0, c.stream, Mr.as<ull>(), n, tick.as<uint32_t>(),
-                       qd.as<ull>(), want_P ? Pd.as<ull>() : nullptr);
+    // at least ~16 tasks per wave of the launch
+    uint32_t split = 1;
+    while (split < 64 && (uint64_t)n * W * split < 16ull * c.num_cus * (DN_THREADS / 64)) split *= 2;
+    hipLaunchKernelGGL(fn, dim3(c.num_cus), dim3(DN_THREADS), 0, c.stream, Mr.as<ull>(), n, split,
+                       tick.as<uint32_t>(), qd.as<ull>(), want_P ? Pd.as<ull>() : nullptr);
     KCDS_LAUNCH_CHECK();
     ull *Pl = want_P ? Phist.as<ull>() + (size_t)slot * nn : nullptr;
     k_dn_unrank<<<dgrid(nn), 256, 0, c.stream>>>(n, rankd.as<uint32_t>(), qd.as<ull>(),
@@ -510,11 +518,11 @@ struct DenseSolver {
 bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, uint32_t &passes,
                       uint32_t max_iters) {
   const uint32_t nall = c.n;
-  const int32_t kp = (int32_t)c.h_gpeel[0].k_prime;
+  const long long kp = (long long)c.h_gpeel[0].k_prime;
   buf_reserve(c, c.tmp1, sizeof(uint32_t) * (DN_MAX + 64));
   uint32_t *dcnt = c.tmp1.as<uint32_t>(), *dverts = dcnt + 64;
   HIP_CHECK(hipMemsetAsync(dcnt, 0, 4, c.stream));
-  k_dn_collect<<<dgrid(nall), 256, 0, c.stream>>>(nall, c.core.as<int32_t>(), kp, dcnt, dverts);
+  k_dn_collect<<<dgrid(nall), 256, 0, c.stream>>>(nall, c.core.as<long long>(), kp, dcnt, dverts);
   KCDS_LAUNCH_CHECK();
   uint32_t nc = 0;
   HIP_CHECK(hipMemcpyAsync(&nc, dcnt, 4, hipMemcpyDeviceToHost, c.stream));
@@ -690,6 +698,8 @@ bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, u
     return !S.empty() && certify(S, cS);
   };
   const bool log = getenv("KCDS_LOG") != nullptr;  // progress lines on stderr
+  double next_try_gap = 1e-6;
+  uint32_t next_try_major = 0;
   for (uint32_t major = 1; major <= max_iters; ++major) {
     // linear oracle: the greedy split minimising <x, q> (each clique to its
     // member of smallest load)
@@ -707,9 +717,19 @@ bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, u
     for (uint32_t i = 0; i < Q.m && !again; ++i) again = Q.q[i] == q;
     if (log && (major % 100 == 0 || again))
       fprintf(stderr, "kcds dense: n=%u major=%u corral=%u gap=%.3e\n", n, major, Q.m, (xx - xq) / xx);
-    if (again || xx - xq <= 1e-12 * xx || !Q.add(std::move(q), std::move(sq), sl)) {
+    const double gap = (xx - xq) / xx;
+    if (again || gap <= 1e-12 || !Q.add(std::move(q), std::move(sq), sl)) {
       ok = attempt();  // x is the minimum-norm point to working precision
       break;
+    }
+    // the certificate has slack (vertices whose load is far below their
+    // clique-degree absorb imbalance), so it often holds well before x has
+    // converged: try it from a gap of 1e-6 on, each time the gap has fallen
+    // tenfold and at least every 256 cycles
+    if (gap <= next_try_gap || (gap <= 1e-6 && major >= next_try_major)) {
+      next_try_gap = gap * 0.1;
+      next_try_major = major + 256;
+      if ((ok = attempt())) break;
     }
     free_slots.pop_back();
     // minor cycles: move to the affine minimiser, or as far toward it as the

@@ -29,6 +29,9 @@
 // Levels and sub-rounds run as in the grid peel: one fused kernel per step
 // (OPEN a level or PROCESS a sub-round) whose last workgroup runs the
 // control step; the host enqueues steps in batches.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "kcds_device.hpp"
@@ -602,7 +605,7 @@ template <int K>
 __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, const uint32_t *__restrict__ foff,
                                                      const uint32_t *__restrict__ fnbr,
                                                      const uint32_t *__restrict__ doff,
-                                                     const uint32_t *__restrict__ dnbr, ull *cnt, int32_t *core,
+                                                     const uint32_t *__restrict__ dnbr, ull *cnt, long long *core,
                                                      uint32_t *erem, uint32_t *fv0, uint32_t *fv1, uint4 *fi0,
                                                      uint4 *fi1, uint32_t seg_v, uint32_t seg_i, GraphPeel *gp,
                                                      const uint32_t *__restrict__ gid, uint32_t *tlist, Scatter sc) {
@@ -674,7 +677,7 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
         uint32_t sg = 0;
         while (i >= s_f[sg + 1]) ++sg;
         v = fv[(size_t)sg * seg_v + (i - s_f[sg])];
-        __hip_atomic_store(&core[v], (int32_t)L, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&core[v], (long long)L, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         g = gid[v];
       }
       const uint32_t gsize = e_group(valid, g);
@@ -774,7 +777,7 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
 }
 
 // working counts (scattered layout), core = -1, every vertex live
-__global__ void k_ep_init(uint32_t n, const ull *cnt0, ull *cnt, int32_t *core, uint32_t *erem, Scatter sc) {
+__global__ void k_ep_init(uint32_t n, const ull *cnt0, ull *cnt, long long *core, uint32_t *erem, Scatter sc) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     cnt[slot_of(sc, i)] = cnt0[i];
     core[i] = -1;
@@ -811,11 +814,11 @@ __global__ void k_fadj_fill(uint32_t n, const uint32_t *doff, const uint32_t *dn
 
 // The k'-core's DAG: arcs between vertices whose core reaches their graph's
 // k' (both ends lie in one graph), in the same ascending order.
-__device__ __forceinline__ bool in_core(uint32_t v, const int32_t *core, const uint32_t *gid, const GraphPeel *gp) {
-  return core[v] >= (int32_t)gp[gid[v]].k_prime;
+__device__ __forceinline__ bool in_core(uint32_t v, const long long *core, const uint32_t *gid, const GraphPeel *gp) {
+  return core[v] >= (long long)gp[gid[v]].k_prime;
 }
 
-__global__ void k_core_dag(uint32_t n, const uint32_t *doff, const uint32_t *dnbr, const int32_t *core,
+__global__ void k_core_dag(uint32_t n, const uint32_t *doff, const uint32_t *dnbr, const long long *core,
                            const uint32_t *gid, const GraphPeel *gp, const uint32_t *koff, uint32_t *kdeg,
                            uint32_t *knbr) {
   const uint32_t lane = lane_id();
@@ -840,7 +843,7 @@ __global__ void k_fill_i32(int32_t *p, int32_t v, uint64_t len) {
 }
 
 typedef void (*EpStepFn)(EpCtl *, uint32_t, const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *,
-                         ull *, int32_t *, uint32_t *, uint32_t *, uint32_t *, uint4 *, uint4 *, uint32_t, uint32_t,
+                         ull *, long long *, uint32_t *, uint32_t *, uint32_t *, uint4 *, uint4 *, uint32_t, uint32_t,
                          GraphPeel *, const uint32_t *, uint32_t *, Scatter);
 
 EpStepFn ep_step_fn(uint32_t k) {
@@ -892,11 +895,9 @@ void epeel_run(Ctx &c) {
   if (!c.counted) fail(KCDS_E_STATE, "peel before the count pass");
   const uint32_t n = c.n, k = c.k, G = c.ngraphs;
   if (c.dmax > ED) fail(KCDS_E_LIMIT, "DAG out-degree %u above %u (the enumerating peel's frame)", c.dmax, ED);
-  if (c.T > 0x7FFFFFFFull && max_u64_host(c, c.cnt.as<ull>(), n) > 0x7FFFFFFFull)
-    fail(KCDS_E_LIMIT, "a per-vertex clique count exceeds the int32 clique-core range");
   graph_ids(c);
   build_full_adjacency(c);
-  buf_reserve(c, c.core, sizeof(int32_t) * ((size_t)n + 1));
+  buf_reserve(c, c.core, sizeof(long long) * ((size_t)n + 1));
   buf_reserve(c, c.erem, sizeof(uint32_t) * ((size_t)n + 1));
   buf_reserve(c, c.gpeel, sizeof(GraphPeel) * ((size_t)G + 1));
   buf_reserve(c, c.tlist, sizeof(uint32_t) * ((size_t)G + 1));
@@ -926,7 +927,7 @@ void epeel_run(Ctx &c) {
   HIP_CHECK(hipMemcpyAsync(ctl, h0, sizeof(EpCtl), hipMemcpyHostToDevice, c.stream));
   c.sync();  // h0 is reused below
   if (n) {
-    k_ep_init<<<egrid(n), 256, 0, c.stream>>>(n, c.cnt.as<ull>(), cnt, c.core.as<int32_t>(), c.erem.as<uint32_t>(),
+    k_ep_init<<<egrid(n), 256, 0, c.stream>>>(n, c.cnt.as<ull>(), cnt, c.core.as<long long>(), c.erem.as<uint32_t>(),
                                                sc);
     KCDS_LAUNCH_CHECK();
   }
@@ -937,11 +938,13 @@ void epeel_run(Ctx &c) {
   const uint32_t sgrid = (uint32_t)c.num_cus * 2;
   uint32_t batch = c.peel_steps ? (uint32_t)std::min<uint64_t>(c.peel_steps, 1u << 20) : 32;
   uint64_t rounds = 0;
+  const bool log = getenv("KCDS_LOG") != nullptr;  // progress lines on stderr
+  auto t_start = std::chrono::steady_clock::now(), t_log = t_start;
   while (n) {
     for (uint32_t r = 0; r < batch; ++r) {
       hipLaunchKernelGGL(step, dim3(sgrid), dim3(ETHREADS), 0, c.stream, ctl, n, c.foff.as<uint32_t>(),
                          c.fnbr.as<uint32_t>(), c.doff.as<uint32_t>(), c.dnbr.as<uint32_t>(), cnt,
-                         c.core.as<int32_t>(), c.erem.as<uint32_t>(), fv0, fv1, fi0, fi1, seg_v, seg_i, gp,
+                         c.core.as<long long>(), c.erem.as<uint32_t>(), fv0, fv1, fi0, fi1, seg_v, seg_i, gp,
                          c.gid.as<uint32_t>(), c.tlist.as<uint32_t>(), sc);
       KCDS_LAUNCH_CHECK();
     }
@@ -954,6 +957,12 @@ void epeel_run(Ctx &c) {
     if (rounds > 3ull * n + 8)
       fail(KCDS_E_HIP, "peeling did not finish within %llu steps", (unsigned long long)rounds);
     batch = std::min<uint32_t>(std::max<uint32_t>(batch, 32) * 2, 128);
+    const auto now = std::chrono::steady_clock::now();
+    if (log && std::chrono::duration<double>(now - t_log).count() > 5.0) {
+      t_log = now;
+      fprintf(stderr, "kcds epeel: n=%u k=%u steps=%u levels=%u level=%llu %.1f s\n", n, k, h0->steps, h0->levels,
+              (unsigned long long)h0->level, std::chrono::duration<double>(now - t_start).count());
+    }
   }
   c.peel_steps = h0->steps;
   const uint32_t levels = h0->levels;
@@ -979,7 +988,7 @@ uint64_t build_core_dag(Ctx &c) {
   const GraphPeel *gp = c.gpeel.as<GraphPeel>();
   if (n) {
     k_core_dag<<<egrid((uint64_t)n * 64), 256, 0, c.stream>>>(n, c.doff.as<uint32_t>(), c.dnbr.as<uint32_t>(),
-                                                               c.core.as<int32_t>(), c.gid.as<uint32_t>(), gp,
+                                                               c.core.as<long long>(), c.gid.as<uint32_t>(), gp,
                                                                nullptr, kdeg, nullptr);
     KCDS_LAUNCH_CHECK();
   }
@@ -991,7 +1000,7 @@ uint64_t build_core_dag(Ctx &c) {
   buf_reserve(c, c.kdnbr, sizeof(uint32_t) * (karcs + 1));
   if (n && karcs) {
     k_core_dag<<<egrid((uint64_t)n * 64), 256, 0, c.stream>>>(n, c.doff.as<uint32_t>(), c.dnbr.as<uint32_t>(),
-                                                               c.core.as<int32_t>(), c.gid.as<uint32_t>(), gp,
+                                                               c.core.as<long long>(), c.gid.as<uint32_t>(), gp,
                                                                c.kdoff.as<uint32_t>(), nullptr,
                                                                c.kdnbr.as<uint32_t>());
     KCDS_LAUNCH_CHECK();

@@ -166,7 +166,7 @@ __device__ bool pl_last_block(PeelCtl *ctl, uint32_t *s_last) {
 // else is peeled alongside).
 __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint32_t k, const uint32_t *__restrict__ cl,
                                                  const ull *__restrict__ incoff, const uint32_t *__restrict__ inc,
-                                                 ull *cnt, int32_t *core, int32_t *status, uint32_t *fv0, uint32_t *fv1,
+                                                 ull *cnt, long long *core, int32_t *status, uint32_t *fv0, uint32_t *fv1,
                                                  PItem *fp0, PItem *fp1, uint32_t seg_v, uint32_t seg_p,
                                                  GraphPeel *gp, const uint32_t *__restrict__ gid, uint32_t *tlist,
                                                  Scatter sc, uint32_t piece) {
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
         uint32_t sg = 0;
         while (i >= s_f[sg + 1]) ++sg;
         v = fv[(size_t)sg * seg_v + (i - s_f[sg])];
-        __hip_atomic_store(&core[v], (int32_t)L, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&core[v], (long long)L, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         g = gid[v];
       }
       uint32_t rank, leader;
@@ -372,7 +372,7 @@ __global__ __launch_bounds__(256) void k_pl_step(PeelCtl *ctl, uint32_t n, uint3
 
 // cnt: the working counts, in the scattered layout of sc when scattered
 // (grid peel), else by vertex
-__global__ void k_peel_init(uint32_t n, uint64_t T, const ull *cnt0, ull *cnt, int32_t *core, int32_t *status,
+__global__ void k_peel_init(uint32_t n, uint64_t T, const ull *cnt0, ull *cnt, long long *core, int32_t *status,
                             Scatter sc, uint32_t scattered) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -424,7 +424,7 @@ __global__ void k_graph_init(uint32_t G, uint32_t k, GraphPeel *gp) {
 __global__ void k_graph_kprime(uint32_t G, GraphPeel *gp) {
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
     GraphPeel &P = gp[g];
-    P.k_prime = P.T == 0 ? 0x7FFFFFFFu : (uint32_t)((P.max_num + P.max_den - 1) / P.max_den);
+    P.k_prime = P.T == 0 ? NO_KPRIME : (P.max_num + P.max_den - 1) / P.max_den;
     P.core_size = 0;
     P.core_cliques = 0;
     P.max_core = 0;
@@ -433,7 +433,7 @@ __global__ void k_graph_kprime(uint32_t G, GraphPeel *gp) {
 
 // k'-core size, cliques inside it and the largest core, per graph.  Lanes of
 // one graph are combined first (one atomic per wave when they agree).
-__global__ void k_graph_summary(uint32_t n, uint64_t T, uint32_t k, const uint32_t *gid, const int32_t *core,
+__global__ void k_graph_summary(uint32_t n, uint64_t T, uint32_t k, const uint32_t *gid, const long long *core,
                                 const int32_t *status, const uint32_t *cl, GraphPeel *gp) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint32_t lane = lane_id();
@@ -441,19 +441,21 @@ __global__ void k_graph_summary(uint32_t n, uint64_t T, uint32_t k, const uint32
     const uint64_t i = i0 + threadIdx.x;
     const bool valid = i < n;
     const uint32_t g = valid ? gid[i] : 0u;
-    const int32_t cv = valid ? core[i] : -1;
-    const bool in = valid && cv >= (int32_t)gp[g].k_prime;
+    const long long cv = valid ? core[i] : -1;
+    const bool in = valid && cv >= (long long)gp[g].k_prime;
     const uint32_t g0 = __shfl(g, 0, 64);
     if (__ballot(valid && g != g0) == 0ull) {  // the whole wave is in graph g0
       const uint32_t cs = (uint32_t)__popcll(__ballot(in));
-      const uint32_t mc = wave_max_u32(cv > 0 ? (uint32_t)cv : 0u);
+      ull mc = cv > 0 ? (ull)cv : 0ull;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) mc = max(mc, (ull)__shfl_xor(mc, o, 64));
       if (lane == 0) {
         if (cs) atomicAdd(&gp[g0].core_size, cs);
         if (mc) atomicMax(&gp[g0].max_core, mc);
       }
     } else if (valid) {
       if (in) atomicAdd(&gp[g].core_size, 1u);
-      if (cv > 0) atomicMax(&gp[g].max_core, (uint32_t)cv);
+      if (cv > 0) atomicMax(&gp[g].max_core, (ull)cv);
     }
   }
   // core cliques: a wave whose cliques all lie in one graph adds to a
@@ -467,7 +469,7 @@ __global__ void k_graph_summary(uint32_t n, uint64_t T, uint32_t k, const uint32
     const uint64_t c = c0 + threadIdx.x;
     const bool valid = c < T;
     const uint32_t g = valid ? gid[cl[c * k]] : 0u;
-    const bool in = valid && status[c] >= (int32_t)gp[g].k_prime;
+    const bool in = valid && (long long)status[c] >= (long long)gp[g].k_prime;
     const uint32_t g0 = __shfl(g, 0, 64);
     if (__ballot(valid && g != g0) == 0ull) {
       const uint32_t cc = (uint32_t)__popcll(__ballot(in));
@@ -523,7 +525,7 @@ __device__ __forceinline__ ull block_sum_u64(ull v, ull *slot) {
 __global__ __launch_bounds__(BP_THREADS) void k_peel_block(uint32_t G, const uint32_t *__restrict__ glist,
                                                           const uint32_t *__restrict__ gvoff,
                                                           const uint32_t *__restrict__ rank, uint32_t k,
-                                                          const ull *__restrict__ cnt0, ull *cnt, int32_t *core,
+                                                          const ull *__restrict__ cnt0, ull *cnt, long long *core,
                                                           int32_t *status, const uint32_t *__restrict__ cl,
                                                           const ull *__restrict__ incoff,
                                                           const uint32_t *__restrict__ inc, uint32_t *fr0,
@@ -551,7 +553,7 @@ __global__ __launch_bounds__(BP_THREADS) void k_peel_block(uint32_t G, const uin
       const uint32_t r = rank[vb + i];
       const ull c0 = cnt0[r];
       stc_u64(&cnt[r], c0);
-      stc_i32(&core[r], -1);
+      stc_i64(&core[r], -1);
       mine += c0;
       if (init_status)  // the graph's cliques (when not reset grid-wide before the launch)
         for (ull j = incoff[r]; j < incoff[r + 1]; ++j) stc_i32(&status[inc[j]], -1);
@@ -563,7 +565,7 @@ __global__ __launch_bounds__(BP_THREADS) void k_peel_block(uint32_t G, const uin
       ull m = ~0ull;
       for (uint32_t i = tid; i < ng; i += nt) {
         const uint32_t r = rank[vb + i];
-        if (ldc_i32(&core[r]) == -1) m = min(m, ldc_u64(&cnt[r]));
+        if (ldc_i64(&core[r]) == -1) m = min(m, ldc_u64(&cnt[r]));
       }
       m = block_min_u64(m, &s_red);
       if (m == ~0ull) break;
@@ -577,7 +579,7 @@ __global__ __launch_bounds__(BP_THREADS) void k_peel_block(uint32_t G, const uin
         uint32_t r = 0;
         if (i < ng) {
           r = rank[vb + i];
-          want = ldc_i32(&core[r]) == -1 && ldc_u64(&cnt[r]) <= L;
+          want = ldc_i64(&core[r]) == -1 && ldc_u64(&cnt[r]) <= L;
         }
         const ull bal = __ballot(want);
         uint32_t at = 0;
@@ -591,7 +593,7 @@ __global__ __launch_bounds__(BP_THREADS) void k_peel_block(uint32_t G, const uin
         const uint32_t fs = s_fsize[cur];
         if (fs == 0) break;
         uint32_t *Fc = cur ? F1 : F0, *Fn = cur ? F0 : F1;
-        for (uint32_t i = tid; i < fs; i += nt) stc_i32(&core[ldc_u32(&Fc[i])], (int32_t)L);
+        for (uint32_t i = tid; i < fs; i += nt) stc_i64(&core[ldc_u32(&Fc[i])], (long long)L);
         rem -= fs;
         if (tid == 0) s_fsize[cur ^ 1] = 0;
         block_sync();
@@ -624,7 +626,7 @@ __global__ __launch_bounds__(BP_THREADS) void k_peel_block(uint32_t G, const uin
             ++destroyed;
             for (uint32_t t = 0; t < k; ++t) {
               const uint32_t u = cl[(ull)c * k + t];
-              if (u == v || ldc_i32(&core[u]) != -1) continue;
+              if (u == v || ldc_i64(&core[u]) != -1) continue;
               if (atomicSub(&cnt[u], 1ull) == L + 1) stc_u32(&Fn[atomicAdd(&s_fsize[cur ^ 1], 1u)], u);
             }
           }
@@ -640,14 +642,14 @@ __global__ __launch_bounds__(BP_THREADS) void k_peel_block(uint32_t G, const uin
       }
     }
     // k' = ceil(maxDensity) and the k'-core summary
-    const uint32_t kp = T == 0 ? 0x7FFFFFFFu : (uint32_t)((max_num + max_den - 1) / max_den);
+    const ull kp = T == 0 ? NO_KPRIME : (max_num + max_den - 1) / max_den;
     ull csize = 0, ccl = 0, mc = 0;
     for (uint32_t i = tid; i < ng; i += nt) {
       const uint32_t r = rank[vb + i];
-      const int32_t cv = ldc_i32(&core[r]);
-      if (cv >= (int32_t)kp) csize += 1;
+      const long long cv = ldc_i64(&core[r]);
+      if (cv >= (long long)kp) csize += 1;
       if (cv > 0) mc = max(mc, (ull)cv);
-      for (ull j = incoff[r]; j < incoff[r + 1]; ++j) ccl += ldc_i32(&status[inc[j]]) >= (int32_t)kp;
+      for (ull j = incoff[r]; j < incoff[r + 1]; ++j) ccl += (long long)ldc_i32(&status[inc[j]]) >= (long long)kp;
     }
     csize = block_sum_u64(csize, &s_red);
     ccl = block_sum_u64(ccl, &s_red) / k;  // every clique is seen from its k members
@@ -664,7 +666,7 @@ __global__ __launch_bounds__(BP_THREADS) void k_peel_block(uint32_t G, const uin
       P.n = ng;
       P.k_prime = kp;
       P.core_size = (uint32_t)csize;
-      P.max_core = (uint32_t)mc;
+      P.max_core = mc;
       P.levels = levels;
       P.pad = 0;
       gp[g] = P;
@@ -712,7 +714,7 @@ void set_peel_stats(Ctx &c, uint32_t levels) {
   ps.levels = levels;
   ps.core_size = P.core_size;
   ps.core_cliques = P.core_cliques;
-  ps.max_core = (int32_t)P.max_core;
+  ps.max_core = (int64_t)P.max_core;
   c.peel = ps;
   c.peeled = true;
   c.have_comp = false;
@@ -730,7 +732,7 @@ void peel_blocks(Ctx &c, const std::vector<uint32_t> &ids) {
   }
   const std::vector<uint32_t> &list = ids.empty() ? all : ids;
   const uint32_t L = (uint32_t)list.size();
-  buf_reserve(c, c.core, sizeof(int32_t) * ((size_t)n + 1));
+  buf_reserve(c, c.core, sizeof(long long) * ((size_t)n + 1));
   buf_reserve(c, c.status, sizeof(int32_t) * (T + 1));
   buf_reserve(c, c.gpeel, sizeof(GraphPeel) * ((size_t)G + 1));
   buf_reserve(c, c.gvoff, sizeof(uint32_t) * ((size_t)G + 1));
@@ -749,7 +751,7 @@ void peel_blocks(Ctx &c, const std::vector<uint32_t> &ids) {
   }
   const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(L, (uint32_t)c.num_cus * 2));
   k_peel_block<<<grid, BP_THREADS, 0, c.stream>>>(L, glist, c.gvoff.as<uint32_t>(), c.rank.as<uint32_t>(), k,
-                                                  c.cnt.as<ull>(), cnt, c.core.as<int32_t>(), c.status.as<int32_t>(),
+                                                  c.cnt.as<ull>(), cnt, c.core.as<long long>(), c.status.as<int32_t>(),
                                                   c.cliques.as<uint32_t>(), c.incoff.as<ull>(), c.inc.as<uint32_t>(),
                                                   fr0, fr1, queue, c.gpeel.as<GraphPeel>(), ids.empty() ? 0u : 1u);
   KCDS_LAUNCH_CHECK();
@@ -760,7 +762,7 @@ void peel_blocks(Ctx &c, const std::vector<uint32_t> &ids) {
 
 // vertices of graphs peeled elsewhere take core -2: the grid peel treats
 // them as neither live nor removed
-__global__ void k_exclude(uint32_t n, const uint32_t *gid, const uint8_t *excl, int32_t *core) {
+__global__ void k_exclude(uint32_t n, const uint32_t *gid, const uint8_t *excl, long long *core) {
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
     if (excl[gid[r]]) core[r] = -2;
 }
@@ -769,8 +771,8 @@ void peel_run(Ctx &c) {
   const uint32_t n = c.n, k = c.k, G = c.ngraphs;
   const uint64_t T = c.T;
   const bool multi = G > 1;
-  // core numbers are int32 (as the reference's cliqueCore); they never exceed
-  // the largest per-vertex count
+  // clique statuses (destroy levels) are int32; they never exceed the
+  // largest per-vertex count (larger counts take the enumerating peel)
   if (T > 0x7FFFFFFFull && max_u64_host(c, c.cnt.as<ull>(), n) > 0x7FFFFFFFull)
     fail(KCDS_E_LIMIT, "a per-vertex clique count exceeds the int32 clique-core range");
   graph_ids(c);
@@ -790,7 +792,7 @@ void peel_run(Ctx &c) {
     set_peel_stats(c, c.h_gpeel[0].levels);
     return;
   }
-  buf_reserve(c, c.core, sizeof(int32_t) * ((size_t)n + 1));
+  buf_reserve(c, c.core, sizeof(long long) * ((size_t)n + 1));
   buf_reserve(c, c.status, sizeof(int32_t) * (T + 1));
   buf_reserve(c, c.gpeel, sizeof(GraphPeel) * ((size_t)G + 1));
   buf_reserve(c, c.tlist, sizeof(uint32_t) * ((size_t)G + 1));
@@ -827,13 +829,13 @@ void peel_run(Ctx &c) {
   HIP_CHECK(hipMemcpyAsync(ctl, h0, sizeof(PeelCtl), hipMemcpyHostToDevice, c.stream));
   c.sync();  // h0 is reused below
   const uint32_t grid = grid1d(std::max<uint64_t>(n, T));
-  k_peel_init<<<grid, 256, 0, c.stream>>>(n, T, c.cnt.as<ull>(), cnt, c.core.as<int32_t>(), c.status.as<int32_t>(),
+  k_peel_init<<<grid, 256, 0, c.stream>>>(n, T, c.cnt.as<ull>(), cnt, c.core.as<long long>(), c.status.as<int32_t>(),
                                           sc, 1u);
   KCDS_LAUNCH_CHECK();
   if (!small_ids.empty() && n) {
     buf_reserve(c, c.tmp3, (size_t)G + 8);
     HIP_CHECK(hipMemcpyAsync(c.tmp3.p, small.data(), G, hipMemcpyHostToDevice, c.stream));
-    k_exclude<<<grid1d(n), 256, 0, c.stream>>>(n, c.gid.as<uint32_t>(), c.tmp3.as<uint8_t>(), c.core.as<int32_t>());
+    k_exclude<<<grid1d(n), 256, 0, c.stream>>>(n, c.gid.as<uint32_t>(), c.tmp3.as<uint8_t>(), c.core.as<long long>());
     KCDS_LAUNCH_CHECK();
   }
   const uint32_t *gidp = c.gid.as<uint32_t>();
@@ -845,7 +847,8 @@ void peel_run(Ctx &c) {
   KCDS_LAUNCH_CHECK();
   const uint32_t *cl = c.cliques.as<uint32_t>(), *inc = c.inc.as<uint32_t>();
   const ull *incoff = c.incoff.as<ull>();
-  int32_t *core = c.core.as<int32_t>(), *status = c.status.as<int32_t>();
+  long long *core = c.core.as<long long>();
+  int32_t *status = c.status.as<int32_t>();
   uint32_t *tl = c.tlist.as<uint32_t>();
   // at least a workgroup per CU: the work of a step is the frontier's
   // incidence pieces (up to k*T), not its vertices
@@ -910,7 +913,7 @@ void peel_graphs_end(Ctx &c, GraphPeel *gp, uint32_t levels) {
   KCDS_LAUNCH_CHECK();
   if (n) {
     k_graph_summary<<<std::min<uint32_t>(grid1d(n), c.num_cus * 8), 256, 0, c.stream>>>(
-        n, 0, c.k, c.gid.as<uint32_t>(), c.core.as<int32_t>(), nullptr, nullptr, gp);
+        n, 0, c.k, c.gid.as<uint32_t>(), c.core.as<long long>(), nullptr, nullptr, gp);
     KCDS_LAUNCH_CHECK();
   }
   c.h_gpeel.resize(G);

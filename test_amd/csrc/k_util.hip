@@ -206,12 +206,6 @@ void gather_by_rank_u64(Ctx &c, const ull *src_new, ull *dst_orig) {
   k_gather_by_rank<ull><<<grid_for(c.n, 256), 256, 0, c.stream>>>(src_new, c.rank.as<uint32_t>(), c.n, dst_orig);
   KCDS_LAUNCH_CHECK();
 }
-void gather_by_rank_i32(Ctx &c, const int32_t *src_new, int32_t *dst_orig) {
-  if (!c.n) return;
-  k_gather_by_rank<int32_t><<<grid_for(c.n, 256), 256, 0, c.stream>>>(src_new, c.rank.as<uint32_t>(), c.n,
-                                                                      dst_orig);
-  KCDS_LAUNCH_CHECK();
-}
 void gather_by_rank_u32(Ctx &c, const uint32_t *src_new, uint32_t *dst_orig) {
   if (!c.n) return;
   k_gather_by_rank<uint32_t><<<grid_for(c.n, 256), 256, 0, c.stream>>>(src_new, c.rank.as<uint32_t>(), c.n,

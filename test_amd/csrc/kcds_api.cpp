@@ -64,7 +64,6 @@ void *Ctx::host_scratch(size_t bytes) {
 }
 
 void gather_by_rank_u64(Ctx &c, const ull *src_new, ull *dst_orig);
-void gather_by_rank_i32(Ctx &c, const int32_t *src_new, int32_t *dst_orig);
 void gather_by_rank_u32(Ctx &c, const uint32_t *src_new, uint32_t *dst_orig);
 void rank_from_listing_order(Ctx &c, const uint32_t *lo_dev);
 void maxflow_component(Ctx &c, uint32_t comp_id, uint64_t p, uint64_t q, int64_t *flow, uint8_t *side);
@@ -258,7 +257,7 @@ int kcds_download_cliques(kcds_ctx *ctx, uint32_t *out, uint64_t cap, uint64_t *
   });
 }
 
-int kcds_peel(kcds_ctx *ctx, int32_t *clique_core, kcds_peel_stats *stats) {
+int kcds_peel(kcds_ctx *ctx, int64_t *clique_core, kcds_peel_stats *stats) {
   return guarded(ctx, [&](Ctx &c) {
     if (!c.counted) fail(KCDS_E_STATE, "kcds_peel before kcds_count or kcds_list");
     if (c.listed && !(c.tune_flags & KCDS_TUNE_ENUM_PEEL)) {
@@ -270,9 +269,9 @@ int kcds_peel(kcds_ctx *ctx, int32_t *clique_core, kcds_peel_stats *stats) {
     }
     if (stats) *stats = c.peel;
     if (clique_core && c.n) {
-      buf_reserve(c, c.tmp1, sizeof(int32_t) * (size_t)c.n);
-      gather_by_rank_i32(c, c.core.as<int32_t>(), c.tmp1.as<int32_t>());
-      HIP_CHECK(hipMemcpyAsync(clique_core, c.tmp1.p, sizeof(int32_t) * c.n, hipMemcpyDeviceToHost, c.stream));
+      buf_reserve(c, c.tmp1, sizeof(int64_t) * (size_t)c.n);
+      gather_by_rank_u64(c, c.core.as<ull>(), c.tmp1.as<ull>());  // int64 core numbers, copied bitwise
+      HIP_CHECK(hipMemcpyAsync(clique_core, c.tmp1.p, sizeof(int64_t) * c.n, hipMemcpyDeviceToHost, c.stream));
       c.sync();
     }
   });

@@ -71,15 +71,16 @@ struct GraphPeel {
   ull max_num, max_den;  // maxDensity of the graph's clique-cores
   ull T;
   ull core_cliques;    // cliques with every member in the k'-core
+  ull k_prime;         // ceil(maxDensity); NO_KPRIME for a graph without cliques
+  ull max_core;        // largest clique-core number
   uint32_t rem;        // remaining vertices during the peel
   uint32_t touch;      // touched in the current level
   uint32_t n;
-  uint32_t k_prime;    // ceil(maxDensity); 0x7FFFFFFF for a graph without cliques
   uint32_t core_size;
-  uint32_t max_core;
   uint32_t levels;     // non-empty peeling levels
   uint32_t pad;
 };
+constexpr ull NO_KPRIME = 0x7FFFFFFFFFFFFFFFull;  // k' of a graph without cliques (no vertex reaches it)
 
 // Core vertices and core cliques grouped by component (dense index d in
 // order of the canonical label = smallest original id of the component).

@@ -75,7 +75,7 @@ struct Timer {
 // Upper bound of any set density inside a component with Tc cliques
 // (helpers.cu:1126-1135), widened by a relative 1e-9 so that rounding never
 // skips a component that could hold a set of density >= num/den.
-bool may_reach(uint32_t k, uint32_t max_core, ull Tc, ull num, ull den) {
+bool may_reach(uint32_t k, ull max_core, ull Tc, ull num, ull den) {
   double kf = 1;
   for (uint32_t i = 2; i <= k; ++i) kf *= i;
   const double ub = std::min((double)max_core, std::pow((double)Tc, (k - 1.0) / k) / std::pow(kf, 1.0 / k));
@@ -209,6 +209,9 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
     const size_t avail = c.budget > c.used ? c.budget - c.used : 0;
     enumerate = (c.tune_flags & KCDS_TUNE_ENUM_PEEL) ||
                 (c.T && (c.T >= 0xFFFFFFFFull || (double)c.T * (8.0 * k + 4.0) > 0.85 * (double)avail));
+    // the materialising peel keeps int32 clique statuses (destroy levels, at
+    // most a vertex's count): larger counts take the enumerating peel
+    if (!enumerate && c.T > 0x7FFFFFFFull && max_u64_host(c, c.cnt.as<ull>(), c.n) > 0x7FFFFFFFull) enumerate = true;
     if (c.T && !enumerate) list_materialize(c, k);
     R.ms_list = t.stop();
   }

@@ -26,23 +26,24 @@ U64P = ctypes.POINTER(ctypes.c_uint64)
 EXPORTS = ("kcds_create", "kcds_destroy", "kcds_last_error", "kcds_upload_csr", "kcds_orient", "kcds_count",
            "kcds_list", "kcds_download_cliques", "kcds_peel", "kcds_components", "kcds_maxflow", "kcds_solve",
            "kcds_sync", "kcds_set_tuning", "kcds_get_list_info", "kcds_solve_batch", "kcds_set_list_shape",
-           "kcds_autotune_list", "kcds_set_graph_budget")
+           "kcds_autotune_list", "kcds_set_graph_budget", "kcds_server_create", "kcds_submit", "kcds_poll",
+           "kcds_server_clock", "kcds_server_cancel_pending", "kcds_server_destroy")
 
 GRAPH_SOLVED, GRAPH_OVER_BUDGET = 0, 1
 TUNE_GRID_PEEL, TUNE_GRID_FLOW, TUNE_EXACT_ORDER, TUNE_NO_GRAPHS, TUNE_BLOCK_FLOW = 1, 2, 4, 8, 16
 TUNE_ENUM_PEEL, TUNE_DENSE = 32, 64
-E_INVALID, E_HIP, E_NOMEM, E_STATE, E_LIMIT = -1, -2, -3, -4, -5
+E_INVALID, E_HIP, E_NOMEM, E_STATE, E_LIMIT, E_PENDING = -1, -2, -3, -4, -5, -6
 
 
 class PeelStats(ctypes.Structure):
-    _fields_ = [("max_num", ctypes.c_uint64), ("max_den", ctypes.c_uint64), ("k_prime", ctypes.c_uint32),
-                ("core_size", ctypes.c_uint32), ("core_cliques", ctypes.c_uint64), ("max_core", ctypes.c_int32),
+    _fields_ = [("max_num", ctypes.c_uint64), ("max_den", ctypes.c_uint64), ("k_prime", ctypes.c_uint64),
+                ("core_cliques", ctypes.c_uint64), ("max_core", ctypes.c_int64), ("core_size", ctypes.c_uint32),
                 ("levels", ctypes.c_uint32)]
 
 
 class Result(ctypes.Structure):
     _fields_ = [("T", ctypes.c_uint64), ("rho_num", ctypes.c_uint64), ("rho_den", ctypes.c_uint64),
-                ("k_prime", ctypes.c_uint32), ("core_vertices", ctypes.c_uint32), ("ncomp", ctypes.c_uint32),
+                ("k_prime", ctypes.c_uint64), ("core_vertices", ctypes.c_uint32), ("ncomp", ctypes.c_uint32),
                 ("flow_calls", ctypes.c_uint32), ("peel", PeelStats),
                 ("ms_orient", ctypes.c_float), ("ms_list", ctypes.c_float), ("ms_peel", ctypes.c_float),
                 ("ms_components", ctypes.c_float), ("ms_flow", ctypes.c_float), ("ms_total", ctypes.c_float),
@@ -51,7 +52,7 @@ class Result(ctypes.Structure):
 
 class GraphResult(ctypes.Structure):
     _fields_ = [("T", ctypes.c_uint64), ("rho_num", ctypes.c_uint64), ("rho_den", ctypes.c_uint64),
-                ("k_prime", ctypes.c_uint32), ("core_vertices", ctypes.c_uint32), ("ncomp", ctypes.c_uint32),
+                ("k_prime", ctypes.c_uint64), ("core_vertices", ctypes.c_uint32), ("ncomp", ctypes.c_uint32),
                 ("flow_calls", ctypes.c_uint32), ("status", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
 
 
@@ -85,7 +86,7 @@ def lib():
         L.kcds_count.argtypes = [V, ctypes.c_uint32, U64P, I64P]
         L.kcds_list.argtypes = [V, ctypes.c_uint32, U64P]
         L.kcds_download_cliques.argtypes = [V, U32P, ctypes.c_uint64, U64P]
-        L.kcds_peel.argtypes = [V, I32P, ctypes.POINTER(PeelStats)]
+        L.kcds_peel.argtypes = [V, I64P, ctypes.POINTER(PeelStats)]
         L.kcds_components.argtypes = [V, U32P, U32P]
         L.kcds_maxflow.argtypes = [V, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, I64P, U8P]
         L.kcds_solve.argtypes = [V, ctypes.c_uint32, ctypes.POINTER(Result), U8P]
@@ -164,9 +165,9 @@ class Solver:
         return out[: T.value]
 
     def peel(self):
-        core = np.zeros(self.n, np.int32)
+        core = np.zeros(self.n, np.int64)
         st = PeelStats()
-        self._chk(lib().kcds_peel(self._h, _p(core, I32P), ctypes.byref(st)))
+        self._chk(lib().kcds_peel(self._h, _p(core, I64P), ctypes.byref(st)))
         return core, st
 
     def components(self):

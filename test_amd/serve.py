@@ -1,31 +1,29 @@
 """Serving loop: continuous batching of exact k-CDS requests on one GPU.
 
-A request is (arrival time, CSR graph, k).  Requests enter per-k queues at
-their arrival times.  S worker slots each own one kcds context (one HIP
-stream and its resident, grow-only device buffers: CSR, DAG, cliques, flow
-networks), so batches of different slots run concurrently on the device.  A
-free slot takes the k whose oldest request has waited longest and batches
-every queued request of that k, oldest first, up to an edge budget, into one
-kcds_solve_batch (one orientation, listing, peel and flow launch for the
-whole batch).  Requests that arrive while batches run wait for the next free
-slot and join the next batch: continuous batching, not a queue in front of
-one solver.
+A thin front end of the native server in libkcds.so (kcds_server_create /
+kcds_submit / kcds_poll, include/kcds.h).  The server owns `slots` batch
+contexts and `large_slots` large-lane contexts, each with its own host
+thread and HIP stream, so batches of different slots overlap on the device.
+Requests wait in per-k queues; a free batch slot solves the waiting requests
+of the k whose oldest request has waited longest (oldest first, up to an edge
+budget) in one kcds_solve_batch.  Requests that arrive while batches run join
+the next batch: continuous batching, not a queue in front of one solver.
 
-HBM admission:
-  * a slot's context has an HBM budget (its share of the device budget);
-    a batch that would exceed it is split in halves and retried;
-  * every graph is admitted on its count pass: one that needs more than the
-    per-graph budget for its materialised cliques (T * (8k + 4) bytes) is
-    answered KCDS_GRAPH_OVER_BUDGET without being solved, and the rest of
-    its batch is solved normally.
+Admission:
+  * a graph whose clique list would exceed `graph_budget` bytes
+    (T * (8k + 4)) is not solved in a batch: it moves to the large lane,
+    which solves one graph at a time without a clique list (enumerating peel,
+    dense exact search for a huge k'-core);
+  * a graph with more than `max_cliques` cliques is answered over budget;
+  * a batch over its slot's HBM budget is split in halves.
 
-Latency is measured from a request's scheduled arrival to the moment its
-batch's results are back on the host (queueing included).  The plain form
-for comparison is the same loop with one slot and batches of one graph.
+Latency runs from a request's scheduled arrival to the moment the server
+finished it (queueing included).  The plain form for comparison is the same
+server with one batch slot and batches of one graph.
 
 No counterpart in the reference (it solves one graph per process).
 """
-import heapq
+import ctypes
 import threading
 import time
 from dataclasses import dataclass, field
@@ -43,12 +41,15 @@ class Request:
     nbr: np.ndarray
     k: int
     t_done: float = -1.0
-    status: int = -1  # kcds.GRAPH_SOLVED / kcds.GRAPH_OVER_BUDGET; -2 = failed
+    t_submit: float = -1.0
+    status: int = -1  # kcds.GRAPH_SOLVED / kcds.GRAPH_OVER_BUDGET; -2 = failed; -3 = not answered
+    code: int = 0     # kcds status that left it unsolved (E_NOMEM: over budget, E_STATE: cancelled)
     T: int = 0
     rho_num: int = 0
     rho_den: int = 0
     in_set: np.ndarray = None
     batch: int = 0  # size of the batch that answered it
+    lane: int = 0   # 0 batch slot, 1 large lane
     error: str = ""
 
     @property
@@ -62,41 +63,69 @@ class Request:
 
 @dataclass
 class ServeStats:
-    completed: int
+    offered: int
     solved: int
     over_budget: int
+    cancelled: int
     failed: int
     wall_s: float
     solves_per_s: float
-    p50_ms: float
+    p50_ms: float       # over solved requests
     p99_ms: float
+    p50_all_ms: float   # over every offered request (unanswered ones at the end of the run)
+    p99_all_ms: float
     mean_batch: float
     batches: int
+    large_lane: int
+    max_pending: int
     latencies_ms: list = field(default_factory=list)
+
+    @property
+    def completed(self):
+        return self.solved + self.over_budget + self.failed
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("slots", ctypes.c_uint32), ("large_slots", ctypes.c_uint32),
+                ("max_batch", ctypes.c_uint32), ("batch_edges", ctypes.c_uint64), ("graph_budget", ctypes.c_uint64),
+                ("max_cliques", ctypes.c_uint64), ("hbm_budget", ctypes.c_uint64), ("want_sets", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32)]
+
+
+class Served(ctypes.Structure):
+    _fields_ = [("result", kcds.GraphResult), ("t_submit", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("batch", ctypes.c_uint32), ("lane", ctypes.c_uint32), ("code", ctypes.c_int32),
+                ("pad", ctypes.c_uint32)]
+
+
+def _lib():
+    L = kcds.lib()
+    if not getattr(L, "_serve_bound", False):
+        V = ctypes.c_void_p
+        L.kcds_server_create.argtypes = [ctypes.POINTER(V), ctypes.POINTER(ServerConfig)]
+        L.kcds_submit.argtypes = [V, ctypes.c_uint32, kcds.U32P, kcds.U32P, ctypes.c_uint32,
+                                  ctypes.POINTER(ctypes.c_uint64)]
+        L.kcds_poll.argtypes = [V, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(Served), kcds.U8P]
+        L.kcds_server_clock.argtypes = [V]
+        L.kcds_server_clock.restype = ctypes.c_double
+        L.kcds_server_cancel_pending.argtypes = [V]
+        L.kcds_server_destroy.argtypes = [V]
+        L.kcds_server_destroy.restype = None
+        L._serve_bound = True
+    return L
 
 
 class Server:
-    """Continuous-batching request loop over `slots` device contexts."""
+    """Continuous-batching request loop (the native server of libkcds.so)."""
 
-    def __init__(self, slots=4, batch_edges=4_000_000, max_batch=512, graph_budget=1 << 30, hbm_budget=0,
-                 want_sets=False, device=0):
-        self.slots = slots
-        self.batch_edges = batch_edges
-        self.max_batch = max_batch
-        self.graph_budget = graph_budget
-        self.hbm_budget = hbm_budget
-        self.want_sets = want_sets
-        self.device = device
-        self._solvers = []
-        for _ in range(slots):
-            s = kcds.Solver(device, hbm_budget // slots if hbm_budget else 0)
-            s.set_graph_budget(graph_budget)
-            self._solvers.append(s)
+    def __init__(self, slots=4, large_slots=1, batch_edges=4_000_000, max_batch=512, graph_budget=1 << 30,
+                 max_cliques=0, hbm_budget=0, want_sets=False, device=0):
+        self.cfg = dict(slots=slots, large_slots=large_slots, batch_edges=batch_edges, max_batch=max_batch,
+                        graph_budget=graph_budget, max_cliques=max_cliques, hbm_budget=hbm_budget,
+                        want_sets=want_sets, device=device)
 
     def close(self):
-        for s in self._solvers:
-            s.close()
-        self._solvers = []
+        pass
 
     def __enter__(self):
         return self
@@ -104,114 +133,94 @@ class Server:
     def __exit__(self, *a):
         self.close()
 
-    # -- one batch on one slot -------------------------------------------------
-    def _solve(self, solver, batch):
-        k = batch[0].k
-        try:
-            res, sets = solver.solve_batch([(r.offset, r.nbr) for r in batch], k, want_set=self.want_sets)
-        except kcds.KcdsError as e:
-            if len(batch) > 1:  # over the slot's budget, or one graph beyond a limit: split and retry
-                h = len(batch) // 2
-                return self._solve(solver, batch[:h]) + self._solve(solver, batch[h:])
-            r = batch[0]
-            if e.code == kcds.E_NOMEM:  # a single graph over the slot's budget
-                r.status = kcds.GRAPH_OVER_BUDGET
-            else:
-                r.status = -2
-            r.error = str(e)
-            r.batch = 1
-            return [(r, None)]
-        out = []
-        for i, r in enumerate(batch):
-            g = res[i]
-            r.status, r.T, r.rho_num, r.rho_den = int(g.status), int(g.T), int(g.rho_num), int(g.rho_den)
-            r.batch = len(batch)
-            out.append((r, sets[i] if sets else None))
-        return out
+    def run(self, requests, plain=False, deadline_s=None):
+        """Serves `requests` (ascending t_arrival) in real time.
 
-    # -- the loop ---------------------------------------------------------------
-    def run(self, requests, plain=False):
-        """Serves `requests` (iterable of Request, ascending t_arrival) in real time.
-
-        plain=True serves them one graph per call on one slot (the comparison
-        form).  Returns (requests, ServeStats)."""
+        plain=True: one batch slot, batches of one graph (the comparison form).
+        deadline_s: seconds after the last arrival to wait; requests not
+        started by then are cancelled and the run waits for those in flight.
+        Returns (requests, ServeStats)."""
+        L = _lib()
+        c = self.cfg
+        cfg = ServerConfig(device=c["device"], slots=1 if plain else c["slots"], large_slots=c["large_slots"],
+                           max_batch=1 if plain else c["max_batch"], batch_edges=c["batch_edges"],
+                           graph_budget=c["graph_budget"], max_cliques=c["max_cliques"],
+                           hbm_budget=c["hbm_budget"], want_sets=1 if c["want_sets"] else 0)
+        h = ctypes.c_void_p()
+        rc = L.kcds_server_create(ctypes.byref(h), ctypes.byref(cfg))
+        if rc != 0:
+            raise kcds.KcdsError(rc, "kcds_server_create failed")
         reqs = list(requests)
-        queues = {}  # k -> heap of (t_arrival, rid, request)
-        lock = threading.Condition()
-        state = {"fed": 0, "done": 0, "batches": 0, "batched": 0}
-        nslots = 1 if plain else self.slots
-        t0 = time.perf_counter()
-
-        def feeder():
+        tickets = {}
+        try:
+            t0 = time.perf_counter()
+            c0 = L.kcds_server_clock(h)
             for r in reqs:
                 delay = r.t_arrival - (time.perf_counter() - t0)
                 if delay > 0:
                     time.sleep(delay)
-                with lock:
-                    heapq.heappush(queues.setdefault(r.k, []), (r.t_arrival, r.rid, r))
-                    state["fed"] += 1
-                    lock.notify()
-
-        def take():
-            # the k whose oldest request has waited longest; its requests oldest
-            # first up to the edge budget (always at least one)
-            best = None
-            for k, q in queues.items():
-                if q and (best is None or q[0][0] < queues[best][0][0]):
-                    best = k
-            if best is None:
-                return None
-            q = queues[best]
-            batch, edges = [], 0
-            limit = 1 if plain else self.max_batch
-            while q and len(batch) < limit:
-                r = q[0][2]
-                if batch and edges + r.edges > self.batch_edges:
-                    break
-                heapq.heappop(q)
-                batch.append(r)
-                edges += r.edges
-            return batch
-
-        def worker(slot):
-            solver = self._solvers[slot]
-            while True:
-                with lock:
-                    while True:
-                        batch = take()
-                        if batch or (state["fed"] == len(reqs)):
-                            break
-                        lock.wait(0.05)
-                    if not batch:
-                        return
-                out = self._solve(solver, batch)
-                t = time.perf_counter() - t0
-                with lock:
-                    for r, st in out:
-                        r.t_done = t
-                        r.in_set = st
-                    state["done"] += len(out)
-                    state["batches"] += 1
-                    state["batched"] += len(out)
-                    lock.notify_all()
-
-        threads = [threading.Thread(target=feeder, daemon=True)]
-        threads += [threading.Thread(target=worker, args=(i,), daemon=True) for i in range(nslots)]
-        for th in threads:
-            th.start()
-        for th in threads:
-            th.join()
-        wall = time.perf_counter() - t0
-        lat = [(r.t_done - r.t_arrival) * 1e3 for r in reqs if r.status == kcds.GRAPH_SOLVED]
-        solved = len(lat)
+                off = np.ascontiguousarray(r.offset, np.uint32)
+                nbr = np.ascontiguousarray(r.nbr, np.uint32)
+                t = ctypes.c_uint64()
+                rc = L.kcds_submit(h, r.n, off.ctypes.data_as(kcds.U32P), nbr.ctypes.data_as(kcds.U32P), r.k,
+                                   ctypes.byref(t))
+                if rc != 0:
+                    r.status, r.error = -2, f"submit failed ({rc})"
+                    continue
+                tickets[r.rid] = t.value
+            if deadline_s is not None:
+                # after the deadline, requests no slot has started are cancelled
+                timer = threading.Timer(max(0.0, reqs[-1].t_arrival + deadline_s - (time.perf_counter() - t0))
+                                        if reqs else 0.0, lambda: L.kcds_server_cancel_pending(h))
+                timer.start()
+            out = Served()
+            for r in reqs:
+                if r.rid not in tickets:
+                    continue
+                ins = np.zeros(max(r.n, 1), np.uint8) if c["want_sets"] else None
+                rc = L.kcds_poll(h, tickets[r.rid], -1, ctypes.byref(out),
+                                 ins.ctypes.data_as(kcds.U8P) if ins is not None else None)
+                if rc != 0:
+                    r.status, r.error = -2, f"poll failed ({rc})"
+                    continue
+                g = out.result
+                r.status, r.T, r.rho_num, r.rho_den = int(g.status), int(g.T), int(g.rho_num), int(g.rho_den)
+                r.code, r.batch, r.lane = int(out.code), int(out.batch), int(out.lane)
+                r.t_done = out.t_done - c0
+                r.t_submit = out.t_submit - c0
+                if ins is not None and r.status == kcds.GRAPH_SOLVED:
+                    r.in_set = ins[: r.n].astype(bool)
+            if deadline_s is not None:
+                timer.cancel()
+            wall = time.perf_counter() - t0
+        finally:
+            L.kcds_server_destroy(h)
+        solved = [r for r in reqs if r.status == kcds.GRAPH_SOLVED]
+        lat = [(r.t_done - r.t_arrival) * 1e3 for r in solved]
+        # every offered request: unanswered ones count as waiting until the end of the run
+        lat_all = [((r.t_done if r.t_done >= 0 and r.code != kcds.E_STATE else wall) - r.t_arrival) * 1e3
+                   for r in reqs]
+        # queue depth: requests submitted and not yet answered, at each submission
+        ev = sorted([(r.t_submit, 1) for r in reqs if r.t_submit >= 0] +
+                    [(r.t_done, -1) for r in reqs if r.t_submit >= 0 and r.t_done >= 0])
+        depth = max_depth = 0
+        for _, d in ev:
+            depth += d
+            max_depth = max(max_depth, depth)
+        in_batches = [r for r in reqs if r.batch > 0 and r.lane == 0]
+        nb = sum(1.0 / r.batch for r in in_batches)
         stats = ServeStats(
-            completed=len(reqs), solved=solved,
-            over_budget=sum(r.status == kcds.GRAPH_OVER_BUDGET for r in reqs),
+            offered=len(reqs), solved=len(solved),
+            over_budget=sum(r.status == kcds.GRAPH_OVER_BUDGET and r.code != kcds.E_STATE for r in reqs),
+            cancelled=sum(r.code == kcds.E_STATE for r in reqs),
             failed=sum(r.status == -2 for r in reqs), wall_s=wall,
-            solves_per_s=solved / wall if wall > 0 else 0.0,
+            solves_per_s=len(solved) / wall if wall > 0 else 0.0,
             p50_ms=float(np.percentile(lat, 50)) if lat else 0.0,
             p99_ms=float(np.percentile(lat, 99)) if lat else 0.0,
-            mean_batch=state["batched"] / max(1, state["batches"]), batches=state["batches"],
+            p50_all_ms=float(np.percentile(lat_all, 50)) if lat_all else 0.0,
+            p99_all_ms=float(np.percentile(lat_all, 99)) if lat_all else 0.0,
+            mean_batch=len(in_batches) / max(1.0, nb),
+            batches=int(round(nb)), large_lane=sum(r.lane == 1 for r in reqs), max_pending=max_depth,
             latencies_ms=lat)
         return reqs, stats
 
