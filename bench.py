@@ -15,10 +15,12 @@ Also reported on the same JSON line:
                 their hipEvent-timed duration, against 8 TB/s;
   cpu_baseline  the CPU oracle's count of the same graph (all host threads);
   extras        exact k-CDS solves/s on config C1 (ER 2000 + K40, k = 3);
-                the same counting step on C3 (Chung-Lu 1M, k = 5);
-                C4 batched vs plain solves/s (4,096 graphs, k = 4); C5 served
-                vs plain solves/s and p50/p99 latency on the first requests of
-                the Poisson 200/s stream.
+                the exact solves of C2 and C3 (dense exact search on their
+                k'-cores); the same counting step on C3 (Chung-Lu 1M, k = 5)
+                with its roofline figures; C4 batched vs plain solves/s (4,096
+                graphs, k = 4); C5 served vs plain over the first second of the
+                Poisson 200/s stream (solves/s, p50/p99, rejections, queue
+                depth); the CPU oracle on one thread and the C1 oracle solve.
 """
 import argparse
 import json
@@ -75,46 +77,69 @@ def bench_c4(np, gen, kcds, plain_sample):
             "C4_T_total": int(sum(r.T for r in res)), "C4_waves_per_root": tuned[0]}
 
 
-def bench_c5(count, graph_budget, slots):
-    """C5: the first `count` requests of the Poisson 200/s R-MAT stream (k in 3..6).
+def bench_c5(window_s, graph_budget, max_cliques, slots, large_slots, deadline_s):
+    """C5: the Poisson 200/s R-MAT stream (k in 3..6) over its first window_s seconds.
 
-    served: the continuous-batching loop over `slots` device contexts; plain:
-    the same loop with one slot and one graph per call.  Requests whose
-    cliques exceed the per-graph budget are answered over-budget (rejected
-    on the count pass) in both forms.  Latency runs from the scheduled
-    arrival to the result on the host."""
+    served: the native continuous-batching server (kcds_submit / kcds_poll)
+    with `slots` batch slots and `large_slots` large-lane slots; a graph whose
+    clique list exceeds graph_budget bytes is solved alone in the large lane
+    without a clique list; one with more than max_cliques cliques is answered
+    over budget.  plain: the same stream through one slot, one graph per call,
+    the same clique cap.  Requests not started deadline_s after the last
+    arrival are cancelled.  Latency runs from the scheduled arrival to the
+    answer; p50/p99 are given over the solved requests and over every offered
+    request (a cancelled one counts as waiting to the end of the run)."""
     import copy
     from test_amd import kcds, serve
     t0 = time.perf_counter()
-    reqs = serve.c5_requests(count=count)
+    reqs = serve.c5_requests(duration=window_s)
     t_gen = time.perf_counter() - t0
-    out = {"C5_requests": len(reqs), "C5_stream_seconds": reqs[-1].t_arrival if reqs else 0.0,
-           "C5_graph_budget_bytes": graph_budget, "C5_slots": slots, "C5_generate_s": t_gen}
-    with serve.Server(slots=slots, batch_edges=4_000_000, graph_budget=graph_budget) as srv:
-        srv.run([copy.copy(r) for r in reqs[: min(8, len(reqs))]])  # warm every slot's kernels
-        for form, plain in (("served", False), ("plain", True)):
-            done, st = srv.run([copy.copy(r) for r in reqs], plain=plain)
-            out[f"C5_{form}_solves_per_s"] = st.solves_per_s
-            out[f"C5_{form}_p50_ms"] = st.p50_ms
-            out[f"C5_{form}_p99_ms"] = st.p99_ms
-            out[f"C5_{form}_wall_s"] = st.wall_s
-            out[f"C5_{form}_solved"] = st.solved
-            out[f"C5_{form}_over_budget"] = st.over_budget
-            out[f"C5_{form}_failed"] = st.failed
-            if not plain:
-                out["C5_served_mean_batch"] = st.mean_batch
-                served = {r.rid: (r.status, r.T, r.rho_num, r.rho_den) for r in done}
-            else:
-                same = all(served[r.rid] == (r.status, r.T, r.rho_num, r.rho_den) for r in done
-                           if r.status == kcds.GRAPH_SOLVED)
-                out["C5_served_equals_plain"] = bool(same)
+    out = {"C5_window_s": window_s, "C5_requests": len(reqs), "C5_graph_budget_bytes": graph_budget,
+           "C5_max_cliques": max_cliques, "C5_slots": slots, "C5_large_slots": large_slots,
+           "C5_deadline_s": deadline_s, "C5_generate_s": t_gen}
+    warm = [copy.copy(r) for r in reqs[: min(8, len(reqs))]]
+    for r in warm:
+        r.t_arrival = 0.0
+    serve.Server(slots=slots, large_slots=large_slots, graph_budget=graph_budget,
+                 max_cliques=max_cliques).run(warm, deadline_s=deadline_s)  # warm every path's kernels
+    forms = (("served", serve.Server(slots=slots, large_slots=large_slots, graph_budget=graph_budget,
+                                     max_cliques=max_cliques), False),
+             ("plain", serve.Server(slots=1, large_slots=0, graph_budget=0, max_cliques=max_cliques), True))
+    results = {}
+    for form, srv, plain in forms:
+        done, st = srv.run([copy.copy(r) for r in reqs], plain=plain, deadline_s=deadline_s)
+        results[form] = {r.rid: (r.status, r.T, r.rho_num, r.rho_den) for r in done}
+        for key in ("solves_per_s", "p50_ms", "p99_ms", "p50_all_ms", "p99_all_ms", "wall_s", "solved",
+                    "over_budget", "cancelled", "failed", "large_lane", "max_pending", "mean_batch"):
+            out[f"C5_{form}_{key}"] = getattr(st, key)
+        out[f"C5_{form}_rejected_frac"] = (st.over_budget + st.cancelled + st.failed) / max(1, st.offered)
+    both = [rid for rid, v in results["served"].items()
+            if v[0] == kcds.GRAPH_SOLVED and results["plain"].get(rid, (None,))[0] == kcds.GRAPH_SOLVED]
+    out["C5_served_equals_plain"] = all(results["served"][rid] == results["plain"][rid] for rid in both)
+    out["C5_compared"] = len(both)
     return out
 
 
-def bench_c3(gen, kcds, reps=3):
+def bench_c2_solve(g, k, kcds):
+    """The exact solve of the headline graph (peel, dense exact search), one run."""
+    s = kcds.Solver(0)
+    s.upload(g.offset, g.nbr)
+    t0 = time.perf_counter()
+    res, _ = s.solve(k, want_set=False)
+    dt = time.perf_counter() - t0
+    s.close()
+    return {"C2_solve_ms": dt * 1e3, "C2_rho": f"{res.rho_num}/{res.rho_den}", "C2_k_prime": res.k_prime,
+            "C2_core_vertices": res.core_vertices, "C2_core_cliques": res.peel.core_cliques,
+            "C2_dense_passes": res.dense_passes,
+            "C2_stage_ms": {"orient": res.ms_orient, "list": res.ms_list, "peel": res.ms_peel,
+                            "dense_or_components": res.ms_components, "flow": res.ms_flow}}
+
+
+def bench_c3(gen, kcds, reps=3, solve=True):
     """C3 (Chung-Lu n = 1M, ~10M edges, near-K80, k = 5): the same step as the
-    headline (GPU order + DAG + per-vertex counting pass).  T = 7.3e10 is
-    beyond 2^32 clique ids, so C3 is counted, never materialised."""
+    headline (GPU order + DAG + per-vertex counting pass) with its own roofline
+    figures, and one exact solve (enumerating peel, dense exact search: T =
+    7.3e10 cliques are never listed)."""
     g, k = gen.config(3)
     s = kcds.Solver(0)
     s.upload(g.offset, g.nbr)
@@ -122,16 +147,42 @@ def bench_c3(gen, kcds, reps=3):
     s.autotune_list(k)
     s.count(k, per_vertex=False)
     s.sync()
+    kms = []
     t0 = time.perf_counter()
     for _ in range(reps):
         s.orient(None)
         T, _ = s.count(k, per_vertex=False)
+        kms.append(s.list_info().kernel_ms)
     s.sync()
     dt = (time.perf_counter() - t0) / reps
     info = s.list_info()
+    km = sum(kms) / len(kms)
+    b = 4 * (info.n + 1) + 4 * info.arcs + 4 * info.w2 + 8 * info.n
+    out = {"C3_T": int(T), "C3_step_ms": dt * 1e3, "C3_cliques_per_s": T / dt, "C3_list_pass_ms": km,
+           "C3_max_outdeg": info.max_outdeg, "C3_waves_per_root": list(info.waves_per_root),
+           "C3_roofline": {"bound": "hbm", "w2": int(info.w2), "bytes_per_pass": b,
+                           "achieved_GBps": b / (km / 1e3) / 1e9, "frac": b / (km / 1e3) / HBM_PEAK}}
+    if solve:
+        t1 = time.perf_counter()
+        res, _ = s.solve(k, want_set=False)
+        out.update({"C3_solve_ms": (time.perf_counter() - t1) * 1e3, "C3_rho": f"{res.rho_num}/{res.rho_den}",
+                    "C3_k_prime": res.k_prime, "C3_core_vertices": res.core_vertices,
+                    "C3_core_cliques": res.peel.core_cliques, "C3_dense_passes": res.dense_passes,
+                    "C3_stage_ms": {"orient": res.ms_orient, "list": res.ms_list, "peel": res.ms_peel,
+                                    "dense_or_components": res.ms_components, "flow": res.ms_flow}})
     s.close()
-    return {"C3_T": int(T), "C3_step_ms": dt * 1e3, "C3_cliques_per_s": T / dt, "C3_list_pass_ms": info.kernel_ms,
-            "C3_max_outdeg": info.max_outdeg, "C3_waves_per_root": list(info.waves_per_root)}
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -145,11 +196,17 @@ def main():
     ap.add_argument("--shape", default=None, help="waves per root of the 4 width classes, e.g. 2,2,4,8 (skips the autotuner)")
     ap.add_argument("--c4-plain-sample", type=int, default=256,
                     help="graphs of C4 solved one per call for the plain-form rate")
-    ap.add_argument("--c5-requests", type=int, default=100,
-                    help="first requests of the C5 stream served (0 skips the serving leg)")
+    ap.add_argument("--c5-window", type=float, default=1.0,
+                    help="seconds of the C5 stream served (0 skips the serving leg)")
     ap.add_argument("--c5-graph-budget", type=int, default=256 << 20,
-                    help="per-graph HBM admission budget of the serving leg, bytes")
+                    help="clique-list bytes per graph in a serving batch (larger graphs take the large lane)")
+    ap.add_argument("--c5-max-cliques", type=int, default=1 << 28,
+                    help="graphs with more cliques are answered over budget")
     ap.add_argument("--c5-slots", type=int, default=4)
+    ap.add_argument("--c5-large-slots", type=int, default=4)
+    ap.add_argument("--c5-deadline", type=float, default=60.0,
+                    help="seconds after the last arrival before unstarted requests are cancelled")
+    ap.add_argument("--no-solves", action="store_true", help="skip the exact C2 and C3 solves")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -221,6 +278,7 @@ def main():
                 "kernel_ms": kms, "bytes_per_pass": bytes_list}
 
     cpu = None
+    cpu_extra = {}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -231,7 +289,28 @@ def main():
         assert Tc == T, (Tc, T)
         cpu = {"value": Tc / tc, "unit": "cliques/s", "cores": threads, "kind": "port",
                "sample": f"full C2 graph, k={k}, per-vertex counts (oracle kClist-style listing, OpenMP)",
-               "seconds": tc}
+               "seconds": tc, "cpu_model": cpu_model()}
+        # one thread on a bounded sample: the first roots of the oracle's own
+        # degeneracy DAG, grown until the sample takes about 10 s
+        lo, hi, t_one, T_one = 0, 4096, 0.0, 0
+        while hi <= g.n:
+            t1 = time.perf_counter()
+            T_one = O.count_roots(g.offset, g.nbr, k, lo, hi, threads=1)
+            t_one = time.perf_counter() - t1
+            if t_one > 5.0 or hi == g.n:
+                break
+            hi = min(g.n, hi * 2 if t_one < 2.5 else int(hi * 10.0 / max(t_one, 1e-3)))
+        cpu_extra = {"cpu_single_thread_cliques_per_s": T_one / t_one if t_one > 0 else None,
+                     "cpu_single_thread_sample": f"C2 k={k}, roots [0, {hi}) of the oracle's degeneracy DAG: "
+                                                 f"{T_one} cliques in {t_one:.2f} s",
+                     "cpu_model": cpu_model(), "cpu_threads_all_cores": threads}
+        # the reference-compatible solve window on C1 (CSR in memory -> final density;
+        # the oracle's solve includes its BZ ordering)
+        g1, k1 = gen.config(1)
+        t1 = time.perf_counter()
+        r1 = O.solve(g1.offset, g1.nbr, k1)[0]
+        cpu_extra["cpu_C1_oracle_solve_ms"] = (time.perf_counter() - t1) * 1e3
+        cpu_extra["cpu_C1_oracle_rho"] = f"{r1.rho_num}/{r1.rho_den}"
 
     extras = {}
     if not args.no_extras:
@@ -248,10 +327,14 @@ def main():
         extras["C1_solve_ms"] = t_solve * 1e3
         extras["C1_rho"] = f"{res.rho_num}/{res.rho_den}"
         s1.close()
-        extras.update(bench_c3(gen, kcds))
+        if not args.no_solves:
+            extras.update(bench_c2_solve(g, k, kcds))
+        extras.update(bench_c3(gen, kcds, solve=not args.no_solves))
         extras.update(bench_c4(np, gen, kcds, args.c4_plain_sample))
-        if args.c5_requests > 0:
-            extras.update(bench_c5(args.c5_requests, args.c5_graph_budget, args.c5_slots))
+        if args.c5_window > 0:
+            extras.update(bench_c5(args.c5_window, args.c5_graph_budget, args.c5_max_cliques, args.c5_slots,
+                                   args.c5_large_slots, args.c5_deadline))
+    extras.update(cpu_extra)
 
     line = {
         "metric": "k-cliques listed/s", "value": value, "unit": "cliques/s", "n_gpus": 1,

@@ -90,6 +90,10 @@ void solve_batch(kcds_server *s, kcds_ctx *ctx, std::vector<std::shared_ptr<Req>
   const uint32_t n = voff.back();
   std::vector<kcds_graph_result> res(G);
   std::vector<uint8_t> ins(s->cfg.want_sets ? (size_t)n + 1 : 0);
+  // a batch graph over graph_budget is not solved here (large lane, or over
+  // budget); with no graph_budget the clique cap itself bounds the batch
+  const uint64_t cap_bytes = s->cfg.max_cliques ? s->cfg.max_cliques * (8ull * k + 4ull) : 0;
+  (void)kcds_set_graph_budget(ctx, s->cfg.graph_budget ? s->cfg.graph_budget : cap_bytes);
   int rc = kcds_upload_csr(ctx, n, off.data(), nbr.data());
   if (rc == KCDS_OK)
     rc = kcds_solve_batch(ctx, k, G, voff.data(), res.data(), s->cfg.want_sets ? ins.data() : nullptr);
@@ -252,7 +256,6 @@ int kcds_server_create(kcds_server **out, const kcds_server_config *cfg) {
       delete s;
       return rc;
     }
-    if (i < cfg->slots) (void)kcds_set_graph_budget(c, cfg->graph_budget);
     s->ctxs.push_back(c);
   }
   for (uint32_t i = 0; i < cfg->slots; ++i) s->threads.emplace_back(batch_worker, s, s->ctxs[i]);
