@@ -255,9 +255,17 @@ int kcds_sync(kcds_ctx *ctx);
  *                        up, where a clique flow network stops fitting): the
  *                        core is held as a bitmask matrix, greedy clique
  *                        splits (one listing pass each) are combined by
- *                        Frank-Wolfe, and the answer is certified by an exact
- *                        integer max-flow over vertex pairs. */
+ *                        Wolfe's minimum-norm-point method, and the answer
+ *                        is certified by an exact integer max-flow over
+ *                        vertex pairs. */
 #define KCDS_TUNE_DENSE 64u
+/*   KCDS_TUNE_NO_DENSE_PEEL  keep the enumerating peel on its step kernels to
+ *                        the end.  By default a single graph's peel goes over
+ *                        to a dense mode once at most 1,024 vertices are
+ *                        live: the live subgraph as a bitmask matrix, each
+ *                        sub-round's destroyed cliques counted in one pass
+ *                        rooted at the frontier.  Same cores either way. */
+#define KCDS_TUNE_NO_DENSE_PEEL 128u
 int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
 
 /* Listing launch shape: waves per root (workgroup size / 64, 1..8) of each

@@ -33,6 +33,7 @@
 // rho_S |A| with equality only inside S: rho* = rho_S and S is the unique
 // maximal densest set.  All quantities are exact integers (128-bit).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -507,6 +508,132 @@ struct DenseSolver {
   }
 };
 
+
+__device__ __forceinline__ ull dbelow(int q, uint32_t c) {
+  const int cq = (int)(c >> 6), cb = (int)(c & 63);
+  if (q < cq) return ~0ull;
+  if (q > cq) return 0ull;
+  return cb == 0 ? 0ull : ((1ull << cb) - 1ull);
+}
+
+// One sub-round of the dense-mode peel: the live cliques that meet the
+// frontier F, each counted once, from its smallest frontier member v: the
+// cliques v forms inside X_v = live vertices minus the frontier members below
+// v.  Task (F[fi], candidate word w0, sub-range) as in k_dn_pass.
+// dec[j] += destroyed cliques containing j (frontier members included: the
+// caller ignores them); *total += destroyed cliques.  K >= 3.
+template <int W, int K>
+__global__ __launch_bounds__(DN_THREADS) void k_dp_pass(const ull *__restrict__ M, uint32_t n,
+                                                        const uint32_t *__restrict__ F, uint32_t nf,
+                                                        const ull *__restrict__ live, const ull *__restrict__ fmask,
+                                                        uint32_t split, uint32_t *ticket, ull *dec, ull *total) {
+  constexpr uint32_t NP = 64u * W;
+  __shared__ ull A[W * NP];
+  __shared__ ull s_live[W], s_f[W];
+  for (uint32_t e = threadIdx.x; e < W * NP; e += blockDim.x) {
+    const uint32_t w = e / NP, j = e % NP;
+    A[e] = j < n ? M[(size_t)j * W + w] : 0ull;
+  }
+  if (threadIdx.x < W) {
+    s_live[threadIdx.x] = live[threadIdx.x];
+    s_f[threadIdx.x] = fmask[threadIdx.x];
+  }
+  __syncthreads();
+  const uint32_t lane = lane_id();
+  const uint32_t ntask = nf * W * split, bits = 64 / split;
+  ull tot = 0;
+  while (true) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(ticket, 1u);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+    if (t >= ntask) break;
+    const uint32_t sub = t % split, iw = t / split;
+    const uint32_t fi = iw / W, w0 = iw % W;
+    const uint32_t v = F[fi];
+    ull S[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) S[w] = drow<W>(A, v, w) & s_live[w] & ~(s_f[w] & dbelow(w, v));
+    ull cand = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      if ((uint32_t)w == w0) cand = S[w];
+    cand &= bits == 64 ? ~0ull : (((1ull << bits) - 1ull) << (sub * bits));
+    if (!cand) continue;
+    ull cr[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) cr[w] = 0;
+    ull Et = 0;
+    while (cand) {
+      const int b = __builtin_ctzll(cand);
+      cand &= cand - 1;
+      const uint32_t c = w0 * 64 + (uint32_t)b;
+      ull S2[W];
+      uint32_t pc = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        S2[w] = S[w] & drow<W>(A, c, w) & dabove(w, c);
+        pc += (uint32_t)__popcll(S2[w]);
+      }
+      if (pc + 2 < (uint32_t)K) continue;
+      const ull E = DCount<W, K - 2, true>::run(A, S2, cr);
+      dcredit<W, true>(cr, c, E);
+      Et += E;
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const uint32_t j = (uint32_t)w * 64 + lane;
+      if (cr[w] && j < n) atomicAdd(&dec[j], cr[w]);
+    }
+    tot += Et;
+  }
+  if (lane == 0 && tot) atomicAdd(total, tot);
+}
+
+// live vertices (core = -1) of the enumerating peel
+__global__ void k_dp_collect(uint32_t n, const long long *core, uint32_t *cnt, uint32_t *verts) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
+    if (core[r] == -1) {
+      const uint32_t at = atomicAdd(cnt, 1u);
+      if (at < DN_MAX) verts[at] = r;
+    }
+}
+
+__global__ void k_dp_gather(uint32_t nd, const uint32_t *verts, const ull *cnt, Scatter sc, ull *out) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < nd; u += gridDim.x * blockDim.x)
+    out[u] = cnt[slot_of(sc, verts[u])];
+}
+
+__global__ void k_dp_write_core(uint32_t nd, const uint32_t *verts, const long long *core_l, long long *core) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < nd; u += gridDim.x * blockDim.x)
+    core[verts[u]] = core_l[u];
+}
+
+typedef void (*DpFn)(const ull *, uint32_t, const uint32_t *, uint32_t, const ull *, const ull *, uint32_t,
+                     uint32_t *, ull *, ull *);
+
+template <int W> DpFn dp_fn_k(uint32_t k) {
+  switch (k) {
+    case 3: return k_dp_pass<W, 3>;
+    case 4: return k_dp_pass<W, 4>;
+    case 5: return k_dp_pass<W, 5>;
+    case 6: return k_dp_pass<W, 6>;
+    case 7: return k_dp_pass<W, 7>;
+    case 8: return k_dp_pass<W, 8>;
+  }
+  return nullptr;
+}
+
+DpFn dp_fn(uint32_t W, uint32_t k) {
+  switch (W) {
+    case 1: return dp_fn_k<1>(k);
+    case 2: return dp_fn_k<2>(k);
+    case 4: return dp_fn_k<4>(k);
+    case 8: return dp_fn_k<8>(k);
+    case 16: return dp_fn_k<16>(k);
+  }
+  return nullptr;
+}
+
 }  // namespace
 
 // Dense exact search over graph 0's k'-core (c.kdoff/c.kdnbr hold its DAG).
@@ -775,6 +902,140 @@ bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, u
   num = cS;
   den = (ull)S.size();
   return true;
+}
+
+
+// Dense mode of the enumerating peel for a single graph whose live vertices
+// (core = -1) number at most 1,024: the live subgraph as an LDS bitmask
+// matrix, every sub-round's destroyed cliques counted in one pass rooted at
+// the frontier (k_dp_pass), levels and densities kept on the host with the
+// rules of the step kernels.  cnt/sc: the working counts (scattered layout);
+// gp: graph 0's device record, brought up to date; last_level: the level
+// that just ended (-1: none yet).  Returns the levels peeled here, or -1
+// (nothing done) when more than 1,024 vertices are live.
+long long dense_peel_finish(Ctx &c, const ull *cnt, Scatter sc, GraphPeel *gp, long long last_level) {
+  const uint32_t nall = c.n, k = c.k;
+  if (k < 3) return -1;
+  buf_reserve(c, c.tmp1, sizeof(uint32_t) * (DN_MAX + 64));
+  uint32_t *dcnt = c.tmp1.as<uint32_t>(), *dverts = dcnt + 64;
+  HIP_CHECK(hipMemsetAsync(dcnt, 0, 4, c.stream));
+  k_dp_collect<<<dgrid(nall), 256, 0, c.stream>>>(nall, c.core.as<long long>(), dcnt, dverts);
+  KCDS_LAUNCH_CHECK();
+  uint32_t nd = 0;
+  HIP_CHECK(hipMemcpyAsync(&nd, dcnt, 4, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  if (nd > DN_MAX) return -1;
+  GraphPeel P;
+  HIP_CHECK(hipMemcpyAsync(&P, gp, sizeof(GraphPeel), hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  // the level that just ended has not had its density recorded yet
+  if (last_level >= 0 && P.touch && P.rem > 0 &&
+      (u128)P.live * P.max_den >= (u128)P.max_num * P.rem) {
+    P.max_num = P.live;
+    P.max_den = P.rem;
+  }
+  P.touch = 0;
+  long long levels = 0;
+  if (nd > 0) {
+    std::vector<uint32_t> verts(nd);
+    HIP_CHECK(hipMemcpyAsync(verts.data(), dverts, sizeof(uint32_t) * nd, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    std::sort(verts.begin(), verts.end());
+    const uint32_t n = nd, W = n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 512 ? 8 : 16;
+    DpFn fn = dp_fn(W, k);
+    if (!fn) fail(KCDS_E_INVALID, "dense peel: k = %u", k);
+    DevBuf M, loc, Fd, masks, dec, cl, tick, cstore;
+    struct Free {
+      Ctx &c;
+      std::vector<DevBuf *> b;
+      ~Free() {
+        for (DevBuf *x : b) buf_free(c, *x);
+      }
+    } fr{c, {&M, &loc, &Fd, &masks, &dec, &cl, &tick, &cstore}};
+    buf_reserve(c, M, sizeof(ull) * (size_t)n * W);
+    buf_reserve(c, loc, sizeof(uint32_t) * ((size_t)nall + 1));
+    buf_reserve(c, Fd, sizeof(uint32_t) * (n + 1));
+    buf_reserve(c, masks, sizeof(ull) * 2 * 16);
+    buf_reserve(c, dec, sizeof(ull) * (n + 1) + 64);
+    buf_reserve(c, cl, sizeof(long long) * (n + 1));
+    buf_reserve(c, tick, 256);
+    buf_reserve(c, cstore, sizeof(ull) * (n + 1));
+    HIP_CHECK(hipMemcpyAsync(dverts, verts.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c.stream));
+    HIP_CHECK(hipMemsetAsync(loc.p, 0xFF, sizeof(uint32_t) * (size_t)nall, c.stream));
+    k_dn_loc<<<dgrid(n), 256, 0, c.stream>>>(n, dverts, loc.as<uint32_t>());
+    k_dn_zero<<<dgrid((uint64_t)n * W), 256, 0, c.stream>>>(M.as<ull>(), (uint64_t)n * W);
+    k_dn_adj<<<dgrid((uint64_t)n * 64), 256, 0, c.stream>>>(n, W, dverts, loc.as<uint32_t>(), c.doff.as<uint32_t>(),
+                                                            c.dnbr.as<uint32_t>(), M.as<ull>());
+    k_dp_gather<<<dgrid(n), 256, 0, c.stream>>>(n, dverts, cnt, sc, cstore.as<ull>());
+    KCDS_LAUNCH_CHECK();
+    std::vector<ull> cv(n);
+    HIP_CHECK(hipMemcpyAsync(cv.data(), cstore.p, sizeof(ull) * n, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    std::vector<uint8_t> alive(n, 1);
+    std::vector<long long> core(n, -1);
+    std::vector<ull> hmask(32, 0), dech(n + 1);
+    uint32_t left = n;
+    long long L = last_level;
+    std::vector<uint32_t> F;
+    const bool log = getenv("KCDS_LOG") != nullptr;  // progress lines on stderr
+    const auto t0 = std::chrono::steady_clock::now();
+    uint64_t subrounds = 0;
+    while (left) {
+      ull m = ~0ull;
+      for (uint32_t u = 0; u < n; ++u)
+        if (alive[u]) m = std::min(m, cv[u]);
+      L = std::max<long long>(L + 1, (long long)m);  // every live count exceeds the last level
+      ++levels;
+      while (true) {  // the level's sub-rounds
+        F.clear();
+        for (uint32_t u = 0; u < n; ++u)
+          if (alive[u] && cv[u] <= (ull)L) F.push_back(u);
+        if (F.empty()) break;
+        ++subrounds;
+        std::fill(hmask.begin(), hmask.end(), 0ull);
+        for (uint32_t u = 0; u < n; ++u)
+          if (alive[u]) hmask[u >> 6] |= 1ull << (u & 63);
+        for (uint32_t u : F) hmask[16 + (u >> 6)] |= 1ull << (u & 63);
+        HIP_CHECK(hipMemcpyAsync(masks.p, hmask.data(), sizeof(ull) * 32, hipMemcpyHostToDevice, c.stream));
+        HIP_CHECK(hipMemcpyAsync(Fd.p, F.data(), sizeof(uint32_t) * F.size(), hipMemcpyHostToDevice, c.stream));
+        HIP_CHECK(hipMemsetAsync(dec.p, 0, sizeof(ull) * (n + 1), c.stream));
+        HIP_CHECK(hipMemsetAsync(tick.p, 0, 4, c.stream));
+        const uint32_t nf = (uint32_t)F.size();
+        uint32_t split = 1;
+        while (split < 64 && (uint64_t)nf * W * split < 16ull * c.num_cus * (DN_THREADS / 64)) split *= 2;
+        hipLaunchKernelGGL(fn, dim3(c.num_cus), dim3(DN_THREADS), 0, c.stream, M.as<ull>(), n, Fd.as<uint32_t>(), nf,
+                           masks.as<ull>(), masks.as<ull>() + 16, split, tick.as<uint32_t>(), dec.as<ull>(),
+                           dec.as<ull>() + n);
+        KCDS_LAUNCH_CHECK();
+        HIP_CHECK(hipMemcpyAsync(dech.data(), dec.p, sizeof(ull) * (n + 1), hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        for (uint32_t u : F) {
+          alive[u] = 0;
+          core[u] = L;
+        }
+        left -= nf;
+        P.rem -= nf;
+        P.live -= dech[n];
+        for (uint32_t u = 0; u < n; ++u)
+          if (alive[u]) cv[u] -= dech[u];
+      }
+      // density of the level just completed (ties keep the later core)
+      if (P.rem > 0 && (u128)P.live * P.max_den >= (u128)P.max_num * P.rem) {
+        P.max_num = P.live;
+        P.max_den = P.rem;
+      }
+    }
+    if (log)
+      fprintf(stderr, "kcds dense peel: %u live vertices from level %lld: %lld levels, %llu sub-rounds, %.1f ms\n", n,
+              last_level, levels, (unsigned long long)subrounds,
+              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    HIP_CHECK(hipMemcpyAsync(cl.p, core.data(), sizeof(long long) * n, hipMemcpyHostToDevice, c.stream));
+    k_dp_write_core<<<dgrid(n), 256, 0, c.stream>>>(n, dverts, cl.as<long long>(), c.core.as<long long>());
+    KCDS_LAUNCH_CHECK();
+  }
+  HIP_CHECK(hipMemcpyAsync(gp, &P, sizeof(GraphPeel), hipMemcpyHostToDevice, c.stream));
+  c.sync();
+  return levels;
 }
 
 }  // namespace kcds

@@ -53,7 +53,7 @@ constexpr uint32_t LIVE = 0xFFFFFFFFu;          // erem of a vertex still in the
 // Counters 64 words apart (same-address device atomics are serialised).
 struct EpCtl {
   ull level, target;
-  uint32_t cur, done, phase, steps, levels, pad0[64 - 9];
+  uint32_t cur, done, phase, steps, levels, handoff, pad0[64 - 10];
   uint32_t fsize[2][ESEG][64];  // vertex frontier segments
   uint32_t isize[2][ESEG][64];  // work-item segments
   ull gmin[ESEG][8];            // minimum live count above target (OPEN)
@@ -211,9 +211,13 @@ __device__ __forceinline__ ull e_count(const EFrame &f, uint32_t Wd, const ull *
 // Appends u (where want) to the next frontier: its vertex entry and its work
 // items (none when its count is 0; one for a narrow vertex; one per 16
 // neighbours for a wide one), and stamps erem[u] = the step that removes it.
+// A vertex is narrow when at most ED of its neighbours can lie in its frame
+// X_v at that step: neighbours stamped for an earlier step never come back,
+// so those not stamped below it bound the frame from above (the wave counts
+// them for a vertex of degree above ED, stopping past ED).
 // Every lane of the wave calls it.
 struct EAppend {
-  const uint32_t *foff;
+  const uint32_t *foff, *fnbr;
   uint32_t *fv, *fvsize;
   uint4 *fi;
   uint32_t *fisize;
@@ -228,9 +232,23 @@ __device__ __forceinline__ void e_append(bool want, uint32_t u, ull cu, const EA
   uint32_t ni = 0, deg = 0;
   if (want) {
     deg = a.foff[u + 1] - a.foff[u];
-    ni = cu == 0 ? 0u : deg <= ED ? 1u : (deg + EPIECE - 1) / EPIECE;
     stc_u32(&a.erem[u], a.snext);
   }
+  bool wide = false;
+  ull big = __ballot(want && cu != 0 && deg > ED);
+  while (big) {
+    const uint32_t l = (uint32_t)__builtin_ctzll(big);
+    big &= big - 1;
+    const uint32_t x = __shfl(u, l, 64), xd = __shfl(deg, l, 64), xb = a.foff[x];
+    uint32_t in_frame = 0;
+    for (uint32_t t0 = 0; t0 < xd && in_frame <= ED; t0 += KCDS_WAVE) {
+      const uint32_t t = t0 + lane;
+      const bool in = t < xd && ldc_u32(&a.erem[a.fnbr[xb + t]]) >= a.snext;
+      in_frame += (uint32_t)__popcll(__ballot(in));
+    }
+    if (lane == l) wide = in_frame > ED;
+  }
+  if (want) ni = cu == 0 ? 0u : !wide ? 1u : (deg + EPIECE - 1) / EPIECE;
   const uint32_t incl = wave_incl_scan_u32(ni);
   const uint32_t tot = __shfl(incl, 63, 64);
   uint32_t vb = 0, ib = 0;
@@ -243,8 +261,8 @@ __device__ __forceinline__ void e_append(bool want, uint32_t u, ull cu, const EA
   if (want) {
     a.fv[vb + lanes_below(bal)] = u;
     const uint32_t p = ib + incl - ni;
-    if (ni == 1 && deg <= ED) {
-      a.fi[p] = make_uint4(u, 0u, 0u, 0u);
+    if (!wide) {
+      if (ni) a.fi[p] = make_uint4(u, 0u, 0u, 0u);
     } else {
       for (uint32_t j = 0; j < ni; ++j) a.fi[p + j] = make_uint4(u, j * EPIECE, min((j + 1) * EPIECE, deg), 1u);
     }
@@ -284,7 +302,8 @@ __device__ __forceinline__ void e_flush(const EFrame &f, uint32_t Wd, uint32_t d
 
 // Stages the members of src[0, len) that lie in X_v (and v itself when
 // keep_v), with their out-list extents, frontier bits and the id -> index
-// table; clears their rows.  Every thread calls it.  len <= ED.
+// table; clears their rows.  Every thread calls it.  At most ED of them are
+// kept (len may be larger).
 __device__ void e_stage(EFrame &f, const uint32_t *__restrict__ src, uint32_t len, uint32_t v, uint32_t s,
                         const uint32_t *__restrict__ erem, const uint32_t *__restrict__ doff, bool keep_v) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
@@ -641,7 +660,7 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
     }
     if (tid == 0) s_min = ~0ull;
     lds_sync();
-    const EAppend a{foff, fv + (size_t)seg * seg_v, &ctl->fsize[cur][seg][0], fi + (size_t)seg * seg_i,
+    const EAppend a{foff, fnbr, fv + (size_t)seg * seg_v, &ctl->fsize[cur][seg][0], fi + (size_t)seg * seg_i,
                     &ctl->isize[cur][seg][0], erem, s + 1};
     ull m = ~0ull;
     const uint32_t Kc = sc.mask + 1;
@@ -687,7 +706,7 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
       }
     }
     const EArgs x{foff, fnbr, doff, dnbr, cnt, erem, gp, sc, L, s};
-    const EAppend a{foff, nv + (size_t)seg * seg_v, &ctl->fsize[cur ^ 1][seg][0], ni + (size_t)seg * seg_i,
+    const EAppend a{foff, fnbr, nv + (size_t)seg * seg_v, &ctl->fsize[cur ^ 1][seg][0], ni + (size_t)seg * seg_i,
                     &ctl->isize[cur ^ 1][seg][0], erem, s + 1};
     ull dm = ~0ull;
     while (true) {
@@ -758,6 +777,8 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
           const ull nm = min(ctl->nextmin[0], dn);
           ctl->phase = 0;
           ctl->target = nm == ~0ull ? L + 1 : max(nm, L + 1);
+          // a single graph small enough for the dense mode: hand over
+          if (ctl->handoff && ldc_u32(&gp[0].rem) <= ctl->handoff) ctl->done = 2;
         }
       } else {
         ctl->ntouch[0] = 0;
@@ -923,7 +944,13 @@ void epeel_run(Ctx &c) {
     h0->dmin[g][0] = ~0ull;
   }
   h0->nextmin[0] = ~0ull;
-  h0->done = n == 0;
+  // a single graph goes over to the dense mode (dense_peel_finish) once at
+  // most 1,024 vertices are live, at a level boundary; from the start when
+  // it is that small
+  const bool dense_mode = G == 1 && k >= 3 && !(c.tune_flags & KCDS_TUNE_NO_DENSE_PEEL);
+  h0->handoff = dense_mode ? 1024u : 0u;
+  const bool dense_now = dense_mode && n <= 1024u;
+  h0->done = n == 0 || dense_now;
   HIP_CHECK(hipMemcpyAsync(ctl, h0, sizeof(EpCtl), hipMemcpyHostToDevice, c.stream));
   c.sync();  // h0 is reused below
   if (n) {
@@ -940,7 +967,7 @@ void epeel_run(Ctx &c) {
   uint64_t rounds = 0;
   const bool log = getenv("KCDS_LOG") != nullptr;  // progress lines on stderr
   auto t_start = std::chrono::steady_clock::now(), t_log = t_start;
-  while (n) {
+  while (n && !dense_now) {
     for (uint32_t r = 0; r < batch; ++r) {
       hipLaunchKernelGGL(step, dim3(sgrid), dim3(ETHREADS), 0, c.stream, ctl, n, c.foff.as<uint32_t>(),
                          c.fnbr.as<uint32_t>(), c.doff.as<uint32_t>(), c.dnbr.as<uint32_t>(), cnt,
@@ -965,7 +992,15 @@ void epeel_run(Ctx &c) {
     }
   }
   c.peel_steps = h0->steps;
-  const uint32_t levels = h0->levels;
+  uint32_t levels = h0->levels;
+  if (log)
+    fprintf(stderr, "kcds epeel: step kernels ended after %u steps, %u levels, %.1f ms\n", h0->steps, h0->levels,
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count());
+  if (n && (dense_now || h0->done == 2)) {
+    const long long dl = dense_peel_finish(c, cnt, sc, gp, dense_now ? -1 : (long long)h0->level);
+    if (dl < 0) fail(KCDS_E_STATE, "dense peel mode: more than 1,024 live vertices");
+    levels += (uint32_t)dl;
+  }
   peel_graphs_end(c, gp, levels);
   c.enum_peel = true;
   c.core_listed = false;

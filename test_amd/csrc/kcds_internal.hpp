@@ -11,6 +11,8 @@
 
 #include "../../include/kcds.h"
 
+struct Scatter;
+
 #ifndef KCDS_ULL_DEFINED
 #define KCDS_ULL_DEFINED
 typedef unsigned long long ull;
@@ -248,6 +250,9 @@ uint64_t build_core_dag(Ctx &c);  // c.kdoff/c.kdnbr: the DAG restricted to the 
 // num/den = cliques in / size of the maximal densest set, marked in c.inset.
 bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, uint32_t &passes,
                       uint32_t max_iters);
+// Dense mode of the enumerating peel (k_dense.hip) for a single graph with at
+// most 1,024 live vertices; -1 when not applicable.
+long long dense_peel_finish(Ctx &c, const ull *cnt, Scatter sc, GraphPeel *gp, long long last_level);
 constexpr ull DENSE_MIN_T = 1ull << 24;  // k'-cores with this many cliques take the dense search
 constexpr uint32_t DENSE_MAX_ITERS = 12000;  // major cycles of the minimum-norm-point search
 

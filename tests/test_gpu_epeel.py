@@ -41,12 +41,17 @@ def graphs():
 GRAPHS = list(graphs())
 
 
+@pytest.mark.parametrize("mode", ["dense_mode", "steps_only"])
 @pytest.mark.parametrize("name,off,nbr,ks,_solve_ks", GRAPHS, ids=[g[0] for g in GRAPHS])
-def test_enum_peel_matches_oracle(solver, name, off, nbr, ks, _solve_ks):
+def test_enum_peel_matches_oracle(solver, name, off, nbr, ks, _solve_ks, mode):
+    """dense_mode: the default, where a single graph's peel goes over to the
+    bitmask-matrix mode once at most 1,024 vertices are live (from the start
+    for graphs that small); steps_only: the step kernels to the end."""
     n = len(off) - 1
     deg = np.diff(off.astype(np.int64))
+    tune = 0 if mode == "dense_mode" else kcds.TUNE_NO_DENSE_PEEL
     for k in ks:
-        solver.set_tuning(0, 0)
+        solver.set_tuning(0, tune)
         solver.upload(off, nbr)
         solver.orient(None)
         T, _ = solver.count(k, per_vertex=False)
@@ -68,6 +73,7 @@ def test_enum_peel_matches_oracle(solver, name, off, nbr, ks, _solve_ks):
         core_m, st_m = solver.peel()
         np.testing.assert_array_equal(core, core_m)
         assert (st.max_num, st.max_den, st.levels) == (st_m.max_num, st_m.max_den, st_m.levels)
+    solver.set_tuning(0, 0)
 
 
 def test_enum_peel_reference_order(solver):
