@@ -46,6 +46,8 @@ constexpr uint32_t ED = 512;                    // frame rows (the DAG out-degre
 constexpr uint32_t EHB = 11, EH = 1u << EHB;    // frame table slots (load <= 1/4)
 constexpr uint32_t ETHREADS = 256;
 constexpr uint32_t EPIECE = 16;                 // candidates per wide work item
+constexpr uint32_t ECHUNK_LOG = 16;             // a narrow frame takes a work item per 2^16 of its cliques
+constexpr uint32_t EMAXP = 64;                  //   up to 64 (k >= 4)
 constexpr uint32_t EUNROLL = 8;                 // 64-element chunks in flight per wave
 constexpr uint32_t EMPTY_ID = 0xFFFFFFFFu;
 constexpr uint32_t LIVE = 0xFFFFFFFFu;          // erem of a vertex still in the graph
@@ -77,6 +79,7 @@ struct EFrame {
   uint32_t item_idx, skip;
   uint4 item;
   uint32_t cand[EPIECE], crem[EPIECE];
+  uint32_t wcnt[ETHREADS / 64];  // members kept by each wave in the current staging chunk
 };
 
 __device__ __forceinline__ uint32_t ehslot(uint32_t key) {
@@ -209,7 +212,9 @@ __device__ __forceinline__ ull e_count(const EFrame &f, uint32_t Wd, const ull *
 }
 
 // Appends u (where want) to the next frontier: its vertex entry and its work
-// items (none when its count is 0; one for a narrow vertex; one per 16
+// items (none when its count is 0; one for a narrow vertex, or one per 2^16
+// of its cliques, at most 64 and one per 16 neighbours, each counting a
+// share of the frame's first-level members, when splits is set; one per 16
 // neighbours for a wide one), and stamps erem[u] = the step that removes it.
 // A vertex is narrow when at most ED of its neighbours can lie in its frame
 // X_v at that step: neighbours stamped for an earlier step never come back,
@@ -222,7 +227,7 @@ struct EAppend {
   uint4 *fi;
   uint32_t *fisize;
   uint32_t *erem;
-  uint32_t snext;
+  uint32_t snext, splits;
 };
 
 __device__ __forceinline__ void e_append(bool want, uint32_t u, ull cu, const EAppend &a) {
@@ -248,7 +253,10 @@ __device__ __forceinline__ void e_append(bool want, uint32_t u, ull cu, const EA
     }
     if (lane == l) wide = in_frame > ED;
   }
-  if (want) ni = cu == 0 ? 0u : !wide ? 1u : (deg + EPIECE - 1) / EPIECE;
+  uint32_t pieces = 1;
+  if (want && !wide && a.splits && (cu >> ECHUNK_LOG) > 0)
+    pieces = (uint32_t)min(min((cu >> ECHUNK_LOG) + 1, (ull)EMAXP), (ull)((deg + EPIECE - 1) / EPIECE));
+  if (want) ni = cu == 0 ? 0u : !wide ? pieces : (deg + EPIECE - 1) / EPIECE;
   const uint32_t incl = wave_incl_scan_u32(ni);
   const uint32_t tot = __shfl(incl, 63, 64);
   uint32_t vb = 0, ib = 0;
@@ -262,7 +270,9 @@ __device__ __forceinline__ void e_append(bool want, uint32_t u, ull cu, const EA
     a.fv[vb + lanes_below(bal)] = u;
     const uint32_t p = ib + incl - ni;
     if (!wide) {
-      if (ni) a.fi[p] = make_uint4(u, 0u, 0u, 0u);
+      if (ni == 1) a.fi[p] = make_uint4(u, 0u, 0u, 0u);
+      else
+        for (uint32_t j = 0; j < ni; ++j) a.fi[p + j] = make_uint4(u, j, ni, 2u);
     } else {
       for (uint32_t j = 0; j < ni; ++j) a.fi[p + j] = make_uint4(u, j * EPIECE, min((j + 1) * EPIECE, deg), 1u);
     }
@@ -303,7 +313,9 @@ __device__ __forceinline__ void e_flush(const EFrame &f, uint32_t Wd, uint32_t d
 // Stages the members of src[0, len) that lie in X_v (and v itself when
 // keep_v), with their out-list extents, frontier bits and the id -> index
 // table; clears their rows.  Every thread calls it.  At most ED of them are
-// kept (len may be larger).
+// kept (len may be larger).  Members keep the order of src, so every
+// workgroup that stages the same frame numbers its members alike (the
+// pieces of a split frame partition its cliques by first member).
 __device__ void e_stage(EFrame &f, const uint32_t *__restrict__ src, uint32_t len, uint32_t v, uint32_t s,
                         const uint32_t *__restrict__ erem, const uint32_t *__restrict__ doff, bool keep_v) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
@@ -317,6 +329,8 @@ __device__ void e_stage(EFrame &f, const uint32_t *__restrict__ src, uint32_t le
     f.iv = EMPTY_ID;
   }
   lds_sync();
+  const uint32_t wv = tid >> 6, nw = nt >> 6;
+  uint32_t run = 0;
   for (uint32_t t0 = 0; t0 < len; t0 += nt) {
     const uint32_t t = t0 + tid;
     const uint32_t u = t < len ? src[t] : EMPTY_ID;
@@ -329,17 +343,25 @@ __device__ void e_stage(EFrame &f, const uint32_t *__restrict__ src, uint32_t le
     const bool isv = keep_v && u == v;
     const bool keep = u != EMPTY_ID && (isv || r > s || (r == s && u > v));
     const ull bal = __ballot(keep);
-    uint32_t base = 0;
-    if (bal && lane == (uint32_t)__builtin_ctzll(bal)) base = atomicAdd(&f.d, (uint32_t)__popcll(bal));
-    base = __shfl(base, bal ? (uint32_t)__builtin_ctzll(bal) : 0u, 64);
+    if (lane == 0) f.wcnt[wv] = (uint32_t)__popcll(bal);
+    lds_sync();
+    uint32_t before = 0, all = 0;
+    for (uint32_t w = 0; w < nw; ++w) {
+      const uint32_t cw = f.wcnt[w];
+      before += w < wv ? cw : 0u;
+      all += cw;
+    }
     if (keep) {
-      const uint32_t pos = base + lanes_below(bal);
+      const uint32_t pos = run + before + lanes_below(bal);
       f.L[pos] = u;
       f.sinfo[pos] = make_uint2(ou, ou1 - ou);
       if (!isv && r == s) atomicOr(&f.fr[pos >> 6], 1ull << (pos & 63));
       if (isv) f.iv = pos;
     }
+    run += all;
+    lds_sync();  // wcnt is rewritten by the next chunk
   }
+  if (tid == 0) f.d = run;
   lds_sync();
   const uint32_t d = f.d;
   uint32_t *hk = reinterpret_cast<uint32_t *>(f.hent);
@@ -450,9 +472,11 @@ struct EArgs {
   uint32_t s;
 };
 
-// Narrow v: one frame of N(v) & X_v, its (K-1)-cliques.
+// Narrow v: one frame of N(v) & X_v, its (K-1)-cliques; piece j of P counts
+// those whose first member is the j-th of every P in turn (K >= 4 when P > 1).
 template <int K>
-__device__ void e_narrow(EFrame &f, uint32_t v, uint32_t g, const EArgs &x, const EAppend &a, ull &dm) {
+__device__ void e_narrow(EFrame &f, uint32_t v, uint32_t g, uint32_t j, uint32_t P, const EArgs &x,
+                         const EAppend &a, ull &dm) {
   const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = blockDim.x >> 6;
   const uint32_t b = x.foff[v], len = x.foff[v + 1] - b;
   e_stage(f, x.fnbr + b, len, v, x.s, x.erem, x.doff, false);
@@ -470,7 +494,9 @@ __device__ void e_narrow(EFrame &f, uint32_t v, uint32_t g, const EArgs &x, cons
     S[w] = d >= lo + 64 ? ~0ull : (d > lo ? ((1ull << (d - lo)) - 1ull) : 0ull);
     cr[w] = 0;
   }
-  const ull share = e_count<K - 1>(f, Wd, S, cr, wv, nw);
+  ull share;
+  if constexpr (K >= 4) share = e_count<K - 1>(f, Wd, S, cr, wv + nw * j, nw * P);
+  else share = e_count<K - 1>(f, Wd, S, cr, wv, nw);
   e_flush(f, Wd, d, cr, x.L, x.cnt, x.sc, a, dm);
   const ull E = e_task_total(f, share);
   if (threadIdx.x == 0 && E) atomicSub(&x.gp[g].live, E);
@@ -661,7 +687,7 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
     if (tid == 0) s_min = ~0ull;
     lds_sync();
     const EAppend a{foff, fnbr, fv + (size_t)seg * seg_v, &ctl->fsize[cur][seg][0], fi + (size_t)seg * seg_i,
-                    &ctl->isize[cur][seg][0], erem, s + 1};
+                    &ctl->isize[cur][seg][0], erem, s + 1, K >= 4};
     ull m = ~0ull;
     const uint32_t Kc = sc.mask + 1;
     for (uint32_t p0 = blockIdx.x * blockDim.x; p0 < Kc; p0 += gstride) {
@@ -707,7 +733,7 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
     }
     const EArgs x{foff, fnbr, doff, dnbr, cnt, erem, gp, sc, L, s};
     const EAppend a{foff, fnbr, nv + (size_t)seg * seg_v, &ctl->fsize[cur ^ 1][seg][0], ni + (size_t)seg * seg_i,
-                    &ctl->isize[cur ^ 1][seg][0], erem, s + 1};
+                    &ctl->isize[cur ^ 1][seg][0], erem, s + 1, K >= 4};
     ull dm = ~0ull;
     while (true) {
       if (tid == 0) {
@@ -733,7 +759,8 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
         const uint32_t deg = foff[v + 1] - foff[v];
         e_edges_item(f, v, g, item.w ? item.y : 0u, item.w ? item.z : deg, x, a, dm);
       } else {
-        if (item.w == 0) e_narrow<K>(f, v, g, x, a, dm);
+        if (item.w == 0) e_narrow<K>(f, v, g, 0, 1, x, a, dm);
+        else if (item.w == 2) e_narrow<K>(f, v, g, item.y, item.z, x, a, dm);
         else e_wide<K>(f, v, g, item.y, item.z, x, a, dm);
       }
     }
