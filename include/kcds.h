@@ -247,7 +247,9 @@ int kcds_sync(kcds_ctx *ctx);
  *                        when the clique list would fit; the solve then lists
  *                        only the k'-core's cliques.  Without the flag a solve
  *                        takes this path by itself when the clique list would
- *                        not fit in HBM (T * (8k + 4) bytes, or T >= 2^32). */
+ *                        not fit in HBM (T * (8k + 4) bytes, or T >= 2^32),
+ *                        and for a single graph from 2^27 cliques up, where
+ *                        enumerating is the faster peel. */
 #define KCDS_TUNE_ENUM_PEEL 32u
 /*   KCDS_TUNE_DENSE      take the dense exact search for a single graph whose
  *                        k'-core has at most 1,024 vertices, whatever its
@@ -259,13 +261,16 @@ int kcds_sync(kcds_ctx *ctx);
  *                        is certified by an exact integer max-flow over
  *                        vertex pairs. */
 #define KCDS_TUNE_DENSE 64u
-/*   KCDS_TUNE_NO_DENSE_PEEL  keep the enumerating peel on its step kernels to
- *                        the end.  By default a single graph's peel goes over
- *                        to a dense mode once at most 1,024 vertices are
- *                        live: the live subgraph as a bitmask matrix, each
- *                        sub-round's destroyed cliques counted in one pass
- *                        rooted at the frontier.  Same cores either way. */
-#define KCDS_TUNE_NO_DENSE_PEEL 128u
+/*   KCDS_TUNE_DENSE_PEEL  let a single graph's enumerating peel go over to a
+ *                        dense mode once at most 1,024 vertices are live: the
+ *                        live subgraph as a bitmask matrix, each sub-round's
+ *                        destroyed cliques counted in one pass rooted at the
+ *                        frontier, levels kept on the host.  Same cores either
+ *                        way; off by default, where the step kernels were as
+ *                        fast or faster (C2 12 ms either way for the last 59
+ *                        levels, C3 271 ms against about 160 ms for the last
+ *                        498). */
+#define KCDS_TUNE_DENSE_PEEL 128u
 int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
 
 /* Listing launch shape: waves per root (workgroup size / 64, 1..8) of each

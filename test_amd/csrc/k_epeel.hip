@@ -971,10 +971,10 @@ void epeel_run(Ctx &c) {
     h0->dmin[g][0] = ~0ull;
   }
   h0->nextmin[0] = ~0ull;
-  // a single graph goes over to the dense mode (dense_peel_finish) once at
-  // most 1,024 vertices are live, at a level boundary; from the start when
-  // it is that small
-  const bool dense_mode = G == 1 && k >= 3 && !(c.tune_flags & KCDS_TUNE_NO_DENSE_PEEL);
+  // with KCDS_TUNE_DENSE_PEEL a single graph goes over to the dense mode
+  // (dense_peel_finish) once at most 1,024 vertices are live, at a level
+  // boundary; from the start when it is that small
+  const bool dense_mode = G == 1 && k >= 3 && (c.tune_flags & KCDS_TUNE_DENSE_PEEL);
   h0->handoff = dense_mode ? 1024u : 0u;
   const bool dense_now = dense_mode && n <= 1024u;
   h0->done = n == 0 || dense_now;

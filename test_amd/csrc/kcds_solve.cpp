@@ -205,9 +205,11 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
     if (ndrop) drop_graphs(c, drop, gT);
     // the clique list, its incidence and status take T * (8k + 4) bytes and
     // 32-bit clique ids; when that does not fit, the peel enumerates instead
-    // and only the k'-cores' cliques are listed afterwards
+    // and only the k'-cores' cliques are listed afterwards.  A single graph
+    // from ENUM_MIN_T cliques up enumerates as well: listing and peeling the
+    // list costs more than enumerating each removed vertex's cliques there.
     const size_t avail = c.budget > c.used ? c.budget - c.used : 0;
-    enumerate = (c.tune_flags & KCDS_TUNE_ENUM_PEEL) ||
+    enumerate = (c.tune_flags & KCDS_TUNE_ENUM_PEEL) || (G == 1 && c.T >= ENUM_MIN_T) ||
                 (c.T && (c.T >= 0xFFFFFFFFull || (double)c.T * (8.0 * k + 4.0) > 0.85 * (double)avail));
     // the materialising peel keeps int32 clique statuses (destroy levels, at
     // most a vertex's count): larger counts take the enumerating peel

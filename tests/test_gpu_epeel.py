@@ -44,12 +44,12 @@ GRAPHS = list(graphs())
 @pytest.mark.parametrize("mode", ["dense_mode", "steps_only"])
 @pytest.mark.parametrize("name,off,nbr,ks,_solve_ks", GRAPHS, ids=[g[0] for g in GRAPHS])
 def test_enum_peel_matches_oracle(solver, name, off, nbr, ks, _solve_ks, mode):
-    """dense_mode: the default, where a single graph's peel goes over to the
-    bitmask-matrix mode once at most 1,024 vertices are live (from the start
-    for graphs that small); steps_only: the step kernels to the end."""
+    """dense_mode: a single graph's peel goes over to the bitmask-matrix mode
+    once at most 1,024 vertices are live (from the start for graphs that
+    small); steps_only: the default, the step kernels to the end."""
     n = len(off) - 1
     deg = np.diff(off.astype(np.int64))
-    tune = 0 if mode == "dense_mode" else kcds.TUNE_NO_DENSE_PEEL
+    tune = kcds.TUNE_DENSE_PEEL if mode == "dense_mode" else 0
     for k in ks:
         solver.set_tuning(0, tune)
         solver.upload(off, nbr)
