@@ -214,6 +214,27 @@ void kcds_server_destroy(kcds_server *srv);
 /* Blocks until all work queued on the context's stream has finished. */
 int kcds_sync(kcds_ctx *ctx);
 
+/* Allocation-free k-clique count.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (tight serving loops, code that
+ * must not call hipMalloc, harnesses that forbid it).  Counts what kcds_count
+ * counts, over a DAG the caller already holds in device memory in listing
+ * form: doff[n+1] and dnbr[doff[n]] with every arc u -> w having u < w < n,
+ * each out-list strictly ascending and at most 512 long (the form of the DAG
+ * kcds_orient builds, vertices numbered by listing rank).  per_vertex[n]
+ * (device memory) receives the number of k-cliques containing each vertex
+ * and *T (host memory) their total; 2 <= k <= 8, n <= KCDS_NOALLOC_MAX_N.
+ * stream: a hipStream_t of `device`, or NULL for its null stream; the call
+ * returns when the count is done.  It allocates nothing: its temporaries live
+ * in one static device array of KCDS_NOALLOC_SCRATCH_BYTES, which every device
+ * the library runs on holds until the process ends and which every call on
+ * that device shares, so calls that overlap on one device race.  Returns
+ * KCDS_E_LIMIT above n or the out-degree limit, KCDS_E_INVALID for bad
+ * arguments or a DAG not in listing form (messages on stderr with KCDS_LOG). */
+#define KCDS_NOALLOC_MAX_N (1u << 20)
+#define KCDS_NOALLOC_SCRATCH_BYTES (25u << 20)
+int kcds_count_dag_noalloc(int device, uint32_t n, const uint32_t *doff, const uint32_t *dnbr, uint32_t k,
+                           uint64_t *per_vertex, uint64_t *T, void *stream);
+
 /* Listing-kernel launch shape: waves per CU for the persistent listing
  * kernels (0 = library default), and flags:
  *   KCDS_TUNE_GRID_PEEL  peel with the grid-wide step kernels even when every

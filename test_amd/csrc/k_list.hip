@@ -32,6 +32,8 @@
 // in a fixed order and writes them row-major (k new ids per clique) at the
 // root's offset.
 #include <algorithm>
+#include <cstdlib>
+#include <mutex>
 
 #include "kcds_device.hpp"
 #include "kcds_internal.hpp"
@@ -1049,4 +1051,183 @@ void list_materialize(Ctx &c, uint32_t k) {
   c.listed = true;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free count (kcds_count_dag_noalloc): the counting kernels above
+// over a DAG the caller holds in device memory, every temporary in a static
+// device array.  Layout of the array (bytes, each part 256-aligned):
+//   [0, NA_Q)            classification counters and the class queues
+//                        (QCTR_WORDS words; word 0 = DAG-form error flag)
+//   totals               QSHARDS * 8 sharded (cliques, wedges) pairs
+//   rootT                KCDS_NOALLOC_MAX_N + 2 per-root counts
+//   roots                KCDS_NOALLOC_MAX_N + 1 root records (uint4)
+// The first three parts are zeroed by a fill kernel at every call.
+namespace {
+
+constexpr size_t na_align(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t NA_Q = na_align(sizeof(uint32_t) * QCTR_WORDS);
+constexpr size_t NA_TOT = na_align(sizeof(ull) * QSHARDS * 8);
+constexpr size_t NA_ROOTT = na_align(sizeof(ull) * ((size_t)KCDS_NOALLOC_MAX_N + 2));
+constexpr size_t NA_ROOTS = na_align(sizeof(uint4) * ((size_t)KCDS_NOALLOC_MAX_N + 1));
+constexpr size_t NA_BYTES = NA_Q + NA_TOT + NA_ROOTT + NA_ROOTS;
+static_assert(NA_BYTES <= (size_t)KCDS_NOALLOC_SCRATCH_BYTES, "scratch layout exceeds its stated size");
+
+static __device__ __align__(256) unsigned char scratch[KCDS_NOALLOC_SCRATCH_BYTES];
+
+// per device: the array's device address (hipGetSymbolAddress) and CU count
+constexpr int NA_MAX_DEV = 64;
+std::mutex na_mu;
+void *na_ptr[NA_MAX_DEV];
+int na_cus[NA_MAX_DEV];
+
+__global__ void k_na_zero(uint32_t *p, uint64_t words) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+__global__ void k_na_zero64(ull *p, uint64_t len) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
+// listing form: offsets non-decreasing, every arc r -> w with r < w < n, each
+// out-list strictly ascending; any violation sets *err
+__global__ void k_na_check(const uint32_t *doff, const uint32_t *dnbr, uint32_t n, uint32_t *err) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const uint32_t b = doff[r], e = doff[r + 1];
+    bool bad = e < b;
+    uint32_t prev = r;
+    for (uint32_t a = b; !bad && a < e; ++a) {
+      const uint32_t w = dnbr[a];
+      bad = w <= prev || w >= n;
+      prev = w;
+    }
+    if (bad) atomicOr(err, 1u);
+  }
+}
+
+uint32_t na_grid(uint64_t len) {
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((len + 255) / 256, 4096));
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int d) {
+    HIP_CHECK(hipGetDevice(&prev));
+    if (prev != d) HIP_CHECK(hipSetDevice(d));
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+void na_count(int device, uint32_t n, const uint32_t *doff, const uint32_t *dnbr, uint32_t k, ull *per_vertex,
+              ull *T_out, hipStream_t st) {
+  if (!doff || !T_out || (n && !per_vertex)) fail(KCDS_E_INVALID, "null argument");
+  if (k < 2 || k > 8) fail(KCDS_E_INVALID, "k = %u outside 2..8", k);
+  if (n > KCDS_NOALLOC_MAX_N) fail(KCDS_E_LIMIT, "n = %u above %u", n, KCDS_NOALLOC_MAX_N);
+  int ndev = 0;
+  HIP_CHECK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev || device >= NA_MAX_DEV) fail(KCDS_E_INVALID, "device %d", device);
+  DeviceGuard dg(device);
+  unsigned char *base;
+  int cus;
+  {
+    std::lock_guard<std::mutex> lk(na_mu);
+    if (!na_ptr[device]) {  // once per device, with the device current
+      void *p = nullptr;
+      HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      int nc = 0;
+      HIP_CHECK(hipDeviceGetAttribute(&nc, hipDeviceAttributeMultiprocessorCount, device));
+      na_cus[device] = nc;
+      na_ptr[device] = p;
+    }
+    base = (unsigned char *)na_ptr[device];
+    cus = na_cus[device];
+  }
+  uint32_t *q = (uint32_t *)base;
+  ull *totals = (ull *)(base + NA_Q), *rootT = (ull *)(base + NA_Q + NA_TOT);
+  uint4 *roots = (uint4 *)(base + NA_Q + NA_TOT + NA_ROOTT);
+  uint32_t *err = q, *ccount = q + 16, *cfill = q + 32;
+  const uint64_t zw = (NA_Q + NA_TOT + sizeof(ull) * ((uint64_t)n + 2)) / 4;
+  k_na_zero<<<na_grid(zw), 256, 0, st>>>(q, zw);
+  KCDS_LAUNCH_CHECK();
+  if (n) {
+    k_na_zero64<<<na_grid(n), 256, 0, st>>>(per_vertex, n);
+    KCDS_LAUNCH_CHECK();
+    k_na_check<<<na_grid(n), 256, 0, st>>>(doff, dnbr, n, err);
+    KCDS_LAUNCH_CHECK();
+    if (k > 2) {
+      const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((n + 255) / 256, 256));
+      k_class_count<<<grid, 256, 0, st>>>(doff, n, k, ccount);
+      KCDS_LAUNCH_CHECK();
+    }
+  }
+  uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0}, arcs = 0, e = 0;
+  HIP_CHECK(hipMemcpyAsync(&e, err, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipMemcpyAsync(h, ccount, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipMemcpyAsync(&arcs, doff + n, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  if (e) fail(KCDS_E_INVALID, "DAG not in listing form (arcs must go to higher ids, out-lists ascending)");
+  if (h[4]) fail(KCDS_E_LIMIT, "%u roots have out-degree above %u (the LDS row limit)", h[4], MAX_LDS_DEG);
+  if (k == 2) {
+    if (n) {
+      k_edge_out<<<na_grid(n), 256, 0, st>>>(doff, n, per_vertex, rootT);
+      KCDS_LAUNCH_CHECK();
+    }
+    if (arcs) {
+      const uint32_t grid = (uint32_t)std::max<uint64_t>(
+          1, std::min<uint64_t>((arcs + INC_E * 256 - 1) / (INC_E * 256), (uint64_t)cus * 2));
+      k_edge_in<<<grid, 256, 0, st>>>(dnbr, arcs, per_vertex);
+      KCDS_LAUNCH_CHECK();
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+    *T_out = arcs;
+    return;
+  }
+  uint32_t start[NCLASS], s = 0;
+  for (int i = 0; i < NCLASS; ++i) {
+    start[i] = s;
+    s += h[i];
+  }
+  if (s) {
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((n + 255) / 256, 256));
+    k_class_fill<<<grid, 256, 0, st>>>(doff, n, k, ccount, cfill, roots);
+    KCDS_LAUNCH_CHECK();
+  }
+  for (int cls = NCLASS - 1; cls >= 0; --cls) {
+    const uint32_t nr = h[cls];
+    if (!nr) continue;
+    uint32_t nw = kWavesPerRoot[cls];
+    if (k == 3) nw = std::min<uint32_t>(nw, 1u << cls);
+    uint32_t per_cu = std::max<uint32_t>(1, WAVE_SLOTS_PER_CU / nw);
+    per_cu = std::min<uint32_t>(per_cu, std::max<uint32_t>(1, (160u * 1024u) / lds_bytes(cls)));
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(nr, (uint32_t)cus * per_cu));
+    hipLaunchKernelGGL(class_count_fn(cls, k), dim3(grid), dim3(64 * nw), 0, st, doff, dnbr, roots + start[cls], nr,
+                       q + QBASE + cls * QUEUE_WORDS, per_vertex, copy_stride(n), 1u, rootT, totals, nullptr,
+                       nullptr, 0ull, 1ull);
+    KCDS_LAUNCH_CHECK();
+  }
+  ull ht[QSHARDS * 8];
+  HIP_CHECK(hipMemcpyAsync(ht, totals, sizeof(ht), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  ull T = 0;
+  for (uint32_t i = 0; i < QSHARDS; ++i) T += ht[i * 8];
+  *T_out = T;
+}
+
+}  // namespace
+
 }  // namespace kcds
+
+extern "C" int kcds_count_dag_noalloc(int device, uint32_t n, const uint32_t *doff, const uint32_t *dnbr, uint32_t k,
+                                      uint64_t *per_vertex, uint64_t *T, void *stream) {
+  try {
+    kcds::na_count(device, n, doff, dnbr, k, (unsigned long long *)per_vertex, (unsigned long long *)T,
+                   (hipStream_t)stream);
+    return KCDS_OK;
+  } catch (const kcds::Error &e) {
+    if (getenv("KCDS_LOG")) fprintf(stderr, "kcds_count_dag_noalloc: %s\n", e.msg.c_str());
+    return e.code;
+  }
+}

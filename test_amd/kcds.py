@@ -27,7 +27,7 @@ EXPORTS = ("kcds_create", "kcds_destroy", "kcds_last_error", "kcds_upload_csr", 
            "kcds_list", "kcds_download_cliques", "kcds_peel", "kcds_components", "kcds_maxflow", "kcds_solve",
            "kcds_sync", "kcds_set_tuning", "kcds_get_list_info", "kcds_solve_batch", "kcds_set_list_shape",
            "kcds_autotune_list", "kcds_set_graph_budget", "kcds_server_create", "kcds_submit", "kcds_poll",
-           "kcds_server_clock", "kcds_server_cancel_pending", "kcds_server_destroy")
+           "kcds_server_clock", "kcds_server_cancel_pending", "kcds_server_destroy", "kcds_count_dag_noalloc")
 
 GRAPH_SOLVED, GRAPH_OVER_BUDGET = 0, 1
 TUNE_GRID_PEEL, TUNE_GRID_FLOW, TUNE_EXACT_ORDER, TUNE_NO_GRAPHS, TUNE_BLOCK_FLOW = 1, 2, 4, 8, 16
@@ -92,6 +92,8 @@ def lib():
         L.kcds_solve.argtypes = [V, ctypes.c_uint32, ctypes.POINTER(Result), U8P]
         L.kcds_solve_batch.argtypes = [V, ctypes.c_uint32, ctypes.c_uint32, U32P, ctypes.POINTER(GraphResult), U8P]
         L.kcds_sync.argtypes = [V]
+        L.kcds_count_dag_noalloc.argtypes = [ctypes.c_int, ctypes.c_uint32, V, V, ctypes.c_uint32, V,
+                                             ctypes.POINTER(ctypes.c_uint64), V]
         L.kcds_set_tuning.argtypes = [V, ctypes.c_uint32, ctypes.c_uint32]
         L.kcds_get_list_info.argtypes = [V, ctypes.POINTER(ListInfo)]
         L.kcds_set_list_shape.argtypes = [V, U32P]
@@ -256,3 +258,16 @@ def concat_graphs(graphs):
     off = np.concatenate(offs).astype(np.uint32)
     nbr = np.concatenate(nbrs) if nbrs else np.zeros(0, np.uint32)
     return off, nbr.astype(np.uint32), voff
+
+
+def count_dag_noalloc(n, doff_ptr, dnbr_ptr, k, per_vertex_ptr, stream_ptr=0, device=0):
+    """kcds_count_dag_noalloc over device pointers (a DAG in listing form:
+    arcs to higher ids, out-lists ascending); per_vertex_ptr receives n uint64
+    counts.  Returns T; raises KcdsError on a nonzero status."""
+    T = ctypes.c_uint64()
+    rc = lib().kcds_count_dag_noalloc(device, n, ctypes.c_void_p(doff_ptr), ctypes.c_void_p(dnbr_ptr), k,
+                                      ctypes.c_void_p(per_vertex_ptr), ctypes.byref(T),
+                                      ctypes.c_void_p(stream_ptr) if stream_ptr else None)
+    if rc != 0:
+        raise KcdsError(rc, f"kcds_count_dag_noalloc failed ({rc})")
+    return T.value
