@@ -8,7 +8,7 @@
  *
  * Mapping to the reference (wajidmanzoor/test) path each entry point replaces:
  *   kcds_upload_csr        memoryAllocationGraph   src/gpuMemoryAllocation.cu:5-27
- *   kcds_orient            generateDAG             main.cu:22-91 (+ src/graph.cpp:71-81
+ *   kcds_orient            generateDAG             main.cu:22-91 (+ src/graph.cpp:41-51
  *                                                  listing order when computed on the GPU)
  *   kcds_count             listAllCliques + countCliques (T, cliqueDegree)
  *                                                  main.cu:93-244, src/helpers.cu:1074-1113
@@ -78,7 +78,7 @@ int kcds_upload_csr(kcds_ctx *ctx, uint32_t n, const uint32_t *offset, const uin
 /* Builds the DAG used for listing.  listing_order == NULL: degeneracy order
  * computed on the GPU (with slack unless KCDS_TUNE_EXACT_ORDER is set; see
  * kcds_set_tuning).  Otherwise listing_order[v] in 1..n is the reference's
- * listing order (src/graph.cpp:78-80: 1 + position in corePeelSequence); arcs
+ * listing order (src/graph.cpp:48-50: 1 + position in corePeelSequence); arcs
  * point from the vertex peeled earlier to the one peeled later.  Every
  * acyclic orientation yields the same cliques. */
 int kcds_orient(kcds_ctx *ctx, const uint32_t *listing_order);
@@ -257,7 +257,9 @@ int kcds_count_dag_noalloc(int device, uint32_t n, const uint32_t *doff, const u
 #define KCDS_TUNE_EXACT_ORDER 4u
 /*   KCDS_TUNE_NO_GRAPHS  launch the grid-wide max-flow's relabel and sweep
  *                        sequences one kernel at a time instead of replaying
- *                        them as captured HIP graphs (same results). */
+ *                        them as captured HIP graphs (same results).  The
+ *                        environment variable KCDS_NO_GRAPHS does the same for
+ *                        every context (for profiler runs). */
 #define KCDS_TUNE_NO_GRAPHS 8u
 /*   KCDS_TUNE_BLOCK_FLOW  solve every component with the workgroup max-flow
  *                        (same results; used to test one solver against the

@@ -349,7 +349,7 @@ extern "C" {
 
 void oracle_free(void *p) { free(p); }
 
-// src/graph.cpp:37-70
+// src/graph.cpp:7-40
 int oracle_load_adjlist(const char *path, uint32_t *n_out, uint32_t *m_out, uint32_t **offset_out,
                         uint32_t **nbr_out) {
   std::ifstream in(path);
@@ -368,7 +368,7 @@ int oracle_load_adjlist(const char *path, uint32_t *n_out, uint32_t *m_out, uint
     if (v != (int64_t)expect || v >= (int64_t)n) return -3;  // lines must be in order 0..n-1
     uint64_t d = 0;
     while (is >> w) {
-      if (w == v) continue;  // graph.cpp:58-59 skips self-loops
+      if (w == v) continue;  // graph.cpp:28-29 skips self-loops
       if (w < 0 || w >= (int64_t)n) return -4;
       if (off[v] + d >= nbr.size()) return -5;  // more than 2m entries
       nbr[off[v] + d] = (uint32_t)w;
@@ -430,7 +430,7 @@ int oracle_load_edgelist(const char *path, uint32_t n_hint, uint32_t *n_out, uin
   return 0;
 }
 
-// src/graph.cpp:71-143 (Batagelj-Zaversnik bin sort; identical swap order)
+// src/graph.cpp:53-113 (Batagelj-Zaversnik bin sort; identical swap order)
 void oracle_core_decompose(uint32_t n, const uint32_t *offset, const uint32_t *nbr, uint32_t *core,
                            uint32_t *peel_seq, uint32_t *listing_order, uint32_t *kmax) {
   if (n == 0) {

@@ -16,7 +16,7 @@
 extern "C" {
 #endif
 
-// Reference loader semantics (src/graph.cpp:37-70): header "n m", then one
+// Reference loader semantics (src/graph.cpp:7-40): header "n m", then one
 // line "v nbr nbr ..." per vertex, self-loops skipped, no deduplication.
 // Returns 0, or <0 on open/parse failure.  Free the arrays with oracle_free.
 int oracle_load_adjlist(const char *path, uint32_t *n, uint32_t *m, uint32_t **offset,
@@ -27,10 +27,10 @@ int oracle_load_edgelist(const char *path, uint32_t n_hint, uint32_t *n, uint32_
                          uint32_t **offset, uint32_t **nbr);
 void oracle_free(void *p);
 
-// Batagelj-Zaversnik core decomposition exactly as src/graph.cpp:71-143
+// Batagelj-Zaversnik core decomposition exactly as src/graph.cpp:53-113
 // (same bin layout and swap order, so the same peel sequence).  `degree`
 // is read-only here (the reference overwrites it).  listing_order[v] =
-// 1 + index of v in corePeelSequence (graph.cpp:78-80).
+// 1 + index of v in corePeelSequence (graph.cpp:48-50).
 void oracle_core_decompose(uint32_t n, const uint32_t *offset, const uint32_t *nbr,
                            uint32_t *core, uint32_t *peel_seq, uint32_t *listing_order,
                            uint32_t *kmax);

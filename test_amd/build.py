@@ -54,10 +54,11 @@ def build_lib(jobs=8):
 
 
 def build_cli():
-    """The drop-in CLI: Graph loader (graph.cpp) + main.cpp linked to libkcds.so."""
+    """The drop-in CLI: Graph loader (graph.cpp), its device wrappers (graph_device.cpp) and main.cpp, linked to
+    libkcds.so."""
     os.makedirs(OBJ, exist_ok=True)
     out = os.path.join(HERE, "kcds")
-    srcs = [os.path.join(SRC, "graph.cpp"), os.path.join(SRC, "main.cpp")]
+    srcs = [os.path.join(SRC, "graph.cpp"), os.path.join(SRC, "graph_device.cpp"), os.path.join(SRC, "main.cpp")]
     _run([HIPCC, "-O3", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", out + ".tmp"] + srcs +
          ["-L", HERE, "-lkcds", "-Wl,-rpath,$ORIGIN"])
     os.replace(out + ".tmp", out)

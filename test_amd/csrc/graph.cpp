@@ -1,6 +1,6 @@
 // Host graph loader, CSR build and Batagelj-Zaversnik ordering (the
-// reference's inc/graph.h / src/graph.cpp surface) plus the device lifecycle
-// wrappers of include/kcds_graph.h.
+// reference's inc/graph.h / src/graph.cpp surface).  Host code only: the
+// device lifecycle wrappers of include/kcds_graph.h are in graph_device.cpp.
 #include "../../include/kcds_graph.h"
 
 #include <algorithm>
@@ -58,10 +58,14 @@ Graph::Graph(ui n_, const std::vector<ui> &off, const std::vector<ui> &nb) : n(n
   for (ui v = 0; v < n; ++v) degree[v] = offset[v + 1] - offset[v];
 }
 
-// Format detection: the reference adjacency list (src/graph.cpp:37-70) has a
+// Format detection: the reference adjacency list (src/graph.cpp:7-40) has a
 // header "n m" followed by exactly one line per vertex whose first token is
-// the vertex id, in order 0..n-1.  Anything else is read as 'u v' pairs
-// ('#' and '%' lines are comments).
+// the vertex id, in order 0..n-1, and its lists are symmetric with at most
+// 2m entries besides self-loops (the reference sizes neighbors 2m).  A file
+// that looks like that but fails the symmetry or size check is an edge list
+// whose first lines happen to fit the pattern.  Anything else is read as
+// 'u v' pairs ('#' and '%' lines are comments).  A file that passes every
+// check reads as an adjacency list.
 Graph::Graph(std::string path) : filePath(path) {
   std::ifstream in(path, std::ios::in | std::ios::binary);
   if (!in.is_open()) throw std::runtime_error("Graph file Open Failed: " + path);
@@ -94,17 +98,30 @@ Graph::Graph(std::string path) : filePath(path) {
   }
   uint64_t nn = 0;
   if (adjacency) {
-    nn = hn;
-    for (uint64_t v = 0; v < hn; ++v) {
+    for (uint64_t v = 0; v < hn && adjacency; ++v) {
       const char *q = lines[v + 1].c_str();
       uint64_t id, w;
       parse_u64(q, id);
       while (parse_u64(q, w)) {
-        if (w >= hn) throw std::runtime_error("neighbour id out of range in " + path);
-        if (w != v) e.push_back((v << 32) | w);  // self-loops skipped (graph.cpp:58-59)
+        if (w >= hn) {
+          adjacency = false;
+          break;
+        }
+        if (w != v) e.push_back((v << 32) | w);  // self-loops skipped (graph.cpp:28-29)
       }
     }
-  } else {
+    if (adjacency && e.size() > 2 * hm) adjacency = false;
+    if (adjacency) {  // every arc v -> w has its w -> v
+      std::vector<uint64_t> a(e), t(e.size());
+      for (size_t i = 0; i < e.size(); ++i) t[i] = (e[i] << 32) | (e[i] >> 32);
+      std::sort(a.begin(), a.end());
+      std::sort(t.begin(), t.end());
+      adjacency = a == t;
+    }
+    if (adjacency) nn = hn;
+    else e.clear();
+  }
+  if (!adjacency) {
     for (const std::string &ln : lines) {
       const char *q = ln.c_str();
       uint64_t u, v;
@@ -130,7 +147,7 @@ void Graph::getListingOrder(std::vector<ui> &arr) {
 
 // Bin-sorted peel: vertices sit in degree bins; removing the lowest vertex
 // moves each higher-degree neighbour to the front of its bin and shrinks that
-// bin (the same swap order as src/graph.cpp:104-136: for neighbour lists in
+// bin (the same swap order as src/graph.cpp:80-106: for neighbour lists in
 // the same order it yields the same peel sequence).
 void Graph::coreDecompose(std::vector<ui> &arr) {
   core.assign(n, 0);
@@ -167,24 +184,4 @@ void Graph::coreDecompose(std::vector<ui> &arr) {
     }
   }
   kmax = *std::max_element(core.begin(), core.end());
-}
-
-int memoryAllocationGraph(deviceGraphPointers &G, const Graph &graph, int device, uint64_t hbm_budget_bytes) {
-  if (G.ctx) freeGraph(G);
-  int rc = kcds_create(&G.ctx, device, hbm_budget_bytes);
-  if (rc != KCDS_OK) {
-    G.ctx = nullptr;
-    return rc;
-  }
-  rc = kcds_upload_csr(G.ctx, graph.n, graph.offset.data(), graph.neighbors.data());
-  if (rc != KCDS_OK) return rc;
-  G.n = graph.n;
-  G.m = graph.m;
-  return KCDS_OK;
-}
-
-void freeGraph(deviceGraphPointers &G) {
-  if (G.ctx) kcds_destroy(G.ctx);
-  G.ctx = nullptr;
-  G.n = G.m = 0;
 }

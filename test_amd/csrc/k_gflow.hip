@@ -26,6 +26,7 @@
 // value older than the latest pushes at most what is there.  Workgroups share
 // only atomic counters inside a kernel; kernel boundaries order the steps.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "kcds_device.hpp"
@@ -834,7 +835,9 @@ void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t m
     }
     KCDS_LAUNCH_CHECK();
   };
-  const bool graphs = !(c.tune_flags & KCDS_TUNE_NO_GRAPHS);
+  // KCDS_NO_GRAPHS in the environment does what the tuning flag does (for
+  // profiling runs, where graph replay has faulted inside the runtime)
+  const bool graphs = !(c.tune_flags & KCDS_TUNE_NO_GRAPHS) && !getenv("KCDS_NO_GRAPHS");
   StreamGraph g_first(c), g_next(c), g_sweeps(c);
   if (graphs) {
     g_first.capture([&]() { relabel_batch(true); });

@@ -1,6 +1,6 @@
 // Degeneracy ordering and DAG orientation.
 //
-// Reference: src/graph.cpp:71-143 computes a Batagelj-Zaversnik peel on the
+// Reference: src/graph.cpp:53-113 computes a Batagelj-Zaversnik peel on the
 // host and main.cu:22-91 (generateDegreeDAG / generateNeighborDAG,
 // src/helpers.cu:112-173) keeps arc u->v iff listingOrder[u] <
 // listingOrder[v], i.e. from the vertex peeled LATER to the one peeled

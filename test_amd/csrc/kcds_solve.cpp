@@ -28,7 +28,12 @@
 namespace kcds {
 
 StreamGraph::~StreamGraph() {
-  if (exec) (void)hipGraphExecDestroy(exec);
+  // a replay may still run when an error unwinds past its owner: the
+  // executable graph is destroyed only once the stream has drained
+  if (exec) {
+    (void)hipStreamSynchronize(c.stream);
+    (void)hipGraphExecDestroy(exec);
+  }
 }
 
 void StreamGraph::launch() { HIP_CHECK(hipGraphLaunch(exec, c.stream)); }
@@ -132,7 +137,18 @@ void dinkelbach_components(Ctx &c, const std::vector<BlockTask> &in, std::vector
   }
   BlockRun run;
   block_launch(c, small, max_iters, bst, run);
+  // the workgroup launch on its own stream still reads the component data and
+  // writes the cut sides: when the grid solver throws, wait for it before the
+  // error leaves (the caller may reuse the context at once)
+  struct JoinOnThrow {
+    hipStream_t st;
+    bool armed;
+    ~JoinOnThrow() {
+      if (armed) (void)hipStreamSynchronize(st);
+    }
+  } join{bst, overlap};
   for (size_t i : big_at) dinkelbach_grid(c, in[i], out[i], max_iters);
+  join.armed = false;
   std::vector<BlockResult> r;
   block_finish(c, small, run, r);
   for (size_t j = 0; j < small.size(); ++j) out[small_at[j]] = r[j];

@@ -219,3 +219,21 @@ def test_cli_matches_reference_output_lines():
     out = subprocess.run([exe, os.path.join(GOLD, "ref_data_test.txt"), "4"],
                          stdout=subprocess.PIPE, text=True, check=True).stdout.splitlines()
     assert out[-2] == "Max Density 0.25 Size 4"
+
+
+def test_cli_on_c1_edge_list(tmp_path):
+    """C1 (ER(2000, 0.005) + K40, k = 3) written as a 'u v' edge list and run
+    through the drop-in CLI: the oracle's exact density 9880/40 and the K40."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "test_amd", "kcds")
+    g, k = gen.config(1)
+    off, nbr = np.asarray(g.offset, np.int64), np.asarray(g.nbr, np.int64)
+    src = np.repeat(np.arange(len(off) - 1), np.diff(off))
+    keep = src < nbr
+    path = tmp_path / "c1_edges.txt"
+    np.savetxt(path, np.stack([src[keep], nbr[keep]], 1), fmt="%d")
+    r, _, _, _, _ = O.solve(off, nbr, k)
+    assert (r.rho_num, r.rho_den) == (9880, 40)
+    out = subprocess.run([exe, str(path), str(k)], stdout=subprocess.PIPE, text=True, check=True).stdout.splitlines()
+    assert "Exact Density 9880/40" in out
+    assert out[-2] == "Max Density 247 Size 40"

@@ -333,3 +333,64 @@ def test_reference_fixture_solve():
     r, pv, core, comp, ins = O.solve(off, nbr, 4)
     assert (r.T, r.rho_num, r.rho_den) == (1, 1, 4)
     assert np.flatnonzero(ins).tolist() == [6, 7, 8, 9]
+
+
+def _core_clique_components(off, nbr, k):
+    """The prune rule (reference src/helpers.cu:874-906: an edge of the densest
+    core survives iff it lies in a clique whose status >= k') and networkx
+    connected components of the surviving edges, from the oracle's listing and
+    peel (pinned above) — not from its component code."""
+    n = len(off) - 1
+    cl = O.list_cliques(off, nbr, k)
+    core, status, st = O.peel(n, k, cl)
+    kp = int(st.k_prime)
+    G = nx.Graph()
+    G.add_nodes_from(int(v) for v in np.flatnonzero(core >= kp))
+    for c in cl[status >= kp]:
+        G.add_edges_from(itertools.combinations((int(x) for x in c), 2))
+    return core, kp, {frozenset(s) for s in nx.connected_components(G)}
+
+
+def _pruned_test_graphs():
+    yield "two_k6_and_k5_bridged", disjoint_with_bridges([6, 6, 5], [(0, 6), (5, 12)]), 3
+    yield "three_k7", disjoint_with_bridges([7, 7, 7], [(0, 7), (8, 14)]), 4
+    g = gen.er(300, 0.05, 12, seed=3)
+    yield "er300", (g.offset, g.nbr), 3
+    g = gen.rmat(9, 6, (10, 9), seed=4)
+    yield "rmat9", (g.offset, g.nbr), 4
+
+
+def disjoint_with_bridges(sizes, bridges):
+    n = sum(sizes)
+    adj = [set() for _ in range(n)]
+    at = 0
+    for s in sizes:
+        for i in range(at, at + s):
+            adj[i].update(j for j in range(at, at + s) if j != i)
+        at += s
+    for u, v in bridges:
+        adj[u].add(v)
+        adj[v].add(u)
+    off = np.zeros(n + 1, np.uint32)
+    off[1:] = np.cumsum([len(a) for a in adj])
+    nbr = np.array([w for a in adj for w in sorted(a)], np.uint32)
+    return off, nbr
+
+
+@pytest.mark.parametrize("name,g,k", list(_pruned_test_graphs()), ids=[x[0] for x in _pruned_test_graphs()])
+def test_components_vs_networkx(name, g, k):
+    off, nbr = g
+    res, _, core_o, comp, _ = O.solve(off, nbr, k)
+    core, kp, parts = _core_clique_components(off, nbr, k)
+    assert res.k_prime == kp
+    np.testing.assert_array_equal(core_o, core)
+    assert res.ncomp == len(parts)
+    # the oracle labels each densest-core vertex with its component, others with 2^32 - 1
+    in_core = core >= kp
+    assert np.all(comp[~in_core] == 0xFFFFFFFF)
+    mine = {}
+    for v in np.flatnonzero(in_core):
+        mine.setdefault(int(comp[v]), set()).add(int(v))
+    assert {frozenset(s) for s in mine.values()} == parts
+    # every label is its component's smallest vertex (the canonical label)
+    assert all(lbl == min(s) for lbl, s in mine.items())
