@@ -78,9 +78,16 @@ int kcds_upload_csr(kcds_ctx *ctx, uint32_t n, const uint32_t *offset, const uin
 /* Builds the DAG used for listing.  listing_order == NULL: degeneracy order
  * computed on the GPU (with slack unless KCDS_TUNE_EXACT_ORDER is set; see
  * kcds_set_tuning).  Otherwise listing_order[v] in 1..n is the reference's
- * listing order (src/graph.cpp:48-50: 1 + position in corePeelSequence); arcs
- * point from the vertex peeled earlier to the one peeled later.  Every
- * acyclic orientation yields the same cliques. */
+ * listing order (src/graph.cpp:48-50: 1 + position in corePeelSequence, the
+ * reverse removal order); arcs point from the smaller listing order to the
+ * larger, i.e. from the vertex peeled later (higher core) to the one peeled
+ * earlier, as the reference's generateDAG does.  Every
+ * acyclic orientation yields the same cliques, counts and solve results: an
+ * order that leaves an out-list longer than 512 (the LDS frame rows; the
+ * reference's order does on hub-heavy graphs, where hubs point to their
+ * low-core neighbours) is replaced by the degeneracy order, which
+ * kcds_get_list_info reports.  KCDS_E_LIMIT when even that order leaves an
+ * out-list longer than 512 (degeneracy above 512). */
 int kcds_orient(kcds_ctx *ctx, const uint32_t *listing_order);
 
 /* Counts k-cliques: *T and, if per_vertex != NULL, the number of k-cliques
@@ -326,6 +333,9 @@ typedef struct kcds_list_info {
   uint32_t kernel_launches;
   uint32_t class_roots[4];
   uint32_t waves_per_root[4];
+  uint32_t order_replaced;  /* 1: kcds_orient's listing_order left an out-list
+                               wider than 512, so the degeneracy order was
+                               used instead (same cliques and results) */
 } kcds_list_info;
 int kcds_get_list_info(kcds_ctx *ctx, kcds_list_info *info);
 

@@ -27,7 +27,7 @@ constexpr uint32_t SORT_WAVE = 1024;  // longest out-list a wave sorts in LDS
 
 __global__ void k_rank_from_lo(const uint32_t *lo, uint32_t n, uint32_t *rank, uint32_t *inv) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    uint32_t r = n - lo[v];  // lo in 1..n, checked on the host to be a permutation
+    uint32_t r = lo[v] - 1;  // lo in 1..n, checked on the host to be a permutation; arcs go up in lo
     rank[v] = r;
     inv[r] = v;
   }

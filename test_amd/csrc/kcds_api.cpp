@@ -206,8 +206,10 @@ int kcds_orient(kcds_ctx *ctx, const uint32_t *listing_order) {
       buf_reserve(c, c.tmp0, sizeof(uint32_t) * (size_t)n + 4);
       HIP_CHECK(hipMemcpyAsync(c.tmp0.p, listing_order, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c.stream));
       rank_from_listing_order(c, c.tmp0.as<uint32_t>());
-      orient_from_rank(c);
+      c.order_replaced = !orient_from_rank(c, true);
+      if (c.order_replaced) orient_gpu(c);  // an out-list wider than the frames: the degeneracy order
     } else {
+      c.order_replaced = false;
       orient_gpu(c);
     }
     c.oriented = true;
@@ -356,6 +358,7 @@ int kcds_get_list_info(kcds_ctx *ctx, kcds_list_info *info) {
     info->max_outdeg = c.dmax;
     info->kernel_ms = c.list_kernel_ms;
     info->kernel_launches = c.list_kernel_launches;
+    info->order_replaced = c.order_replaced ? 1u : 0u;
     for (int i = 0; i < 4; ++i) {
       info->class_roots[i] = c.class_count[i];
       info->waves_per_root[i] = list_waves_per_root(c, i, c.k ? c.k : 4);

@@ -119,6 +119,7 @@ struct Ctx {
   float tune_ms[4] = {0, 0, 0, 0};
   uint64_t order_steps = 0;  // work/control steps of the last GPU ordering
   bool order_exact = true;   // the last GPU ordering was the exact degeneracy order
+  bool order_replaced = false;  // kcds_orient's listing order was too wide; the degeneracy order is used
   uint64_t peel_steps = 0;   // steps of the last grid peel
   uint64_t graph_budget = 0; // per-graph cap on T * (8k + 4) bytes (0 = none)
 

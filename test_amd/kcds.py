@@ -60,7 +60,7 @@ class ListInfo(ctypes.Structure):
     _fields_ = [("n", ctypes.c_uint32), ("arcs", ctypes.c_uint64), ("w2", ctypes.c_uint64),
                 ("max_outdeg", ctypes.c_uint32), ("kernel_ms", ctypes.c_float),
                 ("kernel_launches", ctypes.c_uint32), ("class_roots", ctypes.c_uint32 * 4),
-                ("waves_per_root", ctypes.c_uint32 * 4)]
+                ("waves_per_root", ctypes.c_uint32 * 4), ("order_replaced", ctypes.c_uint32)]
 
 
 class KcdsError(RuntimeError):

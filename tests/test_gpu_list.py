@@ -124,6 +124,53 @@ def test_config2_exact_order_bound(solver):
     solver.set_tuning(0, 0)
 
 
+def test_reference_order_on_c2(solver):
+    """kcds_orient with the reference's listing order (src/graph.cpp:41-51) on
+    C2: hubs point to their low-core neighbours, out-degree about 25k, wider
+    than the 512-row frames, so the library lists over the degeneracy order
+    instead and says so; T and every vertex's count are those of the default
+    orientation."""
+    g, k = gen.config(2)
+    _, _, lo, _ = O.core_decompose(g.offset, g.nbr)
+    off, nbr = np.asarray(g.offset, np.int64), np.asarray(g.nbr, np.int64)
+    src = np.repeat(np.arange(len(off) - 1), np.diff(off))
+    lo64 = lo.astype(np.int64)
+    widest = np.bincount(src[lo64[src] < lo64[nbr]], minlength=len(off) - 1).max()  # earlier -> later in the order
+    print(f"  reference order's widest out-list: {widest}", flush=True)
+    assert widest > 512
+    solver.upload(g.offset, g.nbr)
+    solver.orient(lo)
+    T, pv = solver.count(k)
+    info = solver.list_info()
+    assert info.order_replaced == 1 and info.max_outdeg <= 512
+    assert T == 2281884915
+    solver.upload(g.offset, g.nbr)
+    solver.orient(None)
+    T0, pv0 = solver.count(k)
+    assert solver.list_info().order_replaced == 0
+    assert T0 == T
+    np.testing.assert_array_equal(pv, pv0)
+
+
+def test_reference_order_kept_when_it_fits(solver):
+    """Where the reference's order leaves every out-list within 512 it is the
+    DAG: the widest out-list is the one that order gives."""
+    g = gen.er(400, 0.04, 18, seed=21)
+    _, _, lo, _ = O.core_decompose(g.offset, g.nbr)
+    off, nbr = np.asarray(g.offset, np.int64), np.asarray(g.nbr, np.int64)
+    src = np.repeat(np.arange(len(off) - 1), np.diff(off))
+    lo64 = lo.astype(np.int64)
+    widest = np.bincount(src[lo64[src] < lo64[nbr]], minlength=len(off) - 1).max()
+    solver.upload(g.offset, g.nbr)
+    solver.orient(lo)
+    T, pv = solver.count(4)
+    info = solver.list_info()
+    assert info.order_replaced == 0 and info.max_outdeg == widest
+    Tr, pvr = O.count(g.offset, g.nbr, 4)
+    assert T == Tr
+    np.testing.assert_array_equal(pv, pvr)
+
+
 @pytest.mark.slow
 def test_config3_full_counts_match_oracle_golden(solver):
     """C3 at full size (n = 1,000,000, ~10M edges, k = 5): T = 7.3e10 5-cliques
