@@ -207,6 +207,8 @@ def main():
     ap.add_argument("--c5-deadline", type=float, default=60.0,
                     help="seconds after the last arrival before unstarted requests are cancelled")
     ap.add_argument("--no-solves", action="store_true", help="skip the exact C2 and C3 solves")
+    ap.add_argument("--only", choices=["c2solve", "c3", "c4", "c5"], default=None,
+                    help="run one extras leg alone and print its figures (for profiler runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -228,6 +230,20 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "copies": 5}))
+        return
+
+    if args.only:
+        if args.only == "c2solve":
+            g, k = gen.config(2)
+            out = bench_c2_solve(g, k, kcds)
+        elif args.only == "c3":
+            out = bench_c3(gen, kcds, solve=not args.no_solves)
+        elif args.only == "c4":
+            out = bench_c4(np, gen, kcds, args.c4_plain_sample)
+        else:
+            out = bench_c5(args.c5_window, args.c5_graph_budget, args.c5_max_cliques, args.c5_slots,
+                           args.c5_large_slots, args.c5_deadline)
+        print(json.dumps({"only": args.only, **out}))
         return
 
     g, k = gen.config(2)

@@ -266,7 +266,9 @@ int kcds_count_dag_noalloc(int device, uint32_t n, const uint32_t *doff, const u
  *                        sequences one kernel at a time instead of replaying
  *                        them as captured HIP graphs (same results).  The
  *                        environment variable KCDS_NO_GRAPHS does the same for
- *                        every context (for profiler runs). */
+ *                        every context, and so does running under rocprofv3
+ *                        (graph replay under its kernel tracing has crashed
+ *                        the host process inside hipGraphLaunch). */
 #define KCDS_TUNE_NO_GRAPHS 8u
 /*   KCDS_TUNE_BLOCK_FLOW  solve every component with the workgroup max-flow
  *                        (same results; used to test one solver against the

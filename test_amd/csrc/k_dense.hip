@@ -283,6 +283,12 @@ __global__ void k_dn_zero(ull *p, uint64_t len) {
     p[i] = 0;
 }
 
+// 64-bit words per matrix row for n vertices (the instantiated widths; the
+// count's work per row operation grows with the width)
+uint32_t dense_words(uint32_t n) {
+  return n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 512 ? 8 : n <= 640 ? 10 : n <= 768 ? 12 : 16;
+}
+
 uint32_t dgrid(uint64_t len) {
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((len + 255) / 256, 65535ull * 16));
 }
@@ -307,6 +313,8 @@ template <bool WP> PassFn pass_fn(uint32_t W, uint32_t k) {
     case 2: return pass_fn_k<2, WP>(k);
     case 4: return pass_fn_k<4, WP>(k);
     case 8: return pass_fn_k<8, WP>(k);
+    case 10: return pass_fn_k<10, WP>(k);
+    case 12: return pass_fn_k<12, WP>(k);
     case 16: return pass_fn_k<16, WP>(k);
   }
   return nullptr;
@@ -469,7 +477,7 @@ struct DenseSolver {
   uint32_t passes = 0;
 
   DenseSolver(Ctx &cc, uint32_t kk, uint32_t nn) : c(cc), k(kk), n(nn) {
-    W = n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 512 ? 8 : 16;
+    W = dense_words(n);
   }
   ~DenseSolver() {
     for (DevBuf *b : {&M0, &Mr, &rankd, &qd, &Pd, &ql, &Phist, &mu, &slotd, &Hlo, &Hhi, &tick}) buf_free(c, *b);
@@ -629,6 +637,8 @@ DpFn dp_fn(uint32_t W, uint32_t k) {
     case 2: return dp_fn_k<2>(k);
     case 4: return dp_fn_k<4>(k);
     case 8: return dp_fn_k<8>(k);
+    case 10: return dp_fn_k<10>(k);
+    case 12: return dp_fn_k<12>(k);
     case 16: return dp_fn_k<16>(k);
   }
   return nullptr;
@@ -941,7 +951,7 @@ long long dense_peel_finish(Ctx &c, const ull *cnt, Scatter sc, GraphPeel *gp, l
     HIP_CHECK(hipMemcpyAsync(verts.data(), dverts, sizeof(uint32_t) * nd, hipMemcpyDeviceToHost, c.stream));
     c.sync();
     std::sort(verts.begin(), verts.end());
-    const uint32_t n = nd, W = n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 512 ? 8 : 16;
+    const uint32_t n = nd, W = dense_words(n);
     DpFn fn = dp_fn(W, k);
     if (!fn) fail(KCDS_E_INVALID, "dense peel: k = %u", k);
     DevBuf M, loc, Fd, masks, dec, cl, tick, cstore;

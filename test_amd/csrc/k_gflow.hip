@@ -835,9 +835,14 @@ void dinkelbach_grid(Ctx &c, const BlockTask &task, BlockResult &out, uint32_t m
     }
     KCDS_LAUNCH_CHECK();
   };
-  // KCDS_NO_GRAPHS in the environment does what the tuning flag does (for
-  // profiling runs, where graph replay has faulted inside the runtime)
-  const bool graphs = !(c.tune_flags & KCDS_TUNE_NO_GRAPHS) && !getenv("KCDS_NO_GRAPHS");
+  // Replay is off with the tuning flag, with KCDS_NO_GRAPHS in the
+  // environment, and under rocprofv3 (its tool sets ROCPROFILER_LIBRARY_CTOR
+  // / ROCPROF_KERNEL_TRACE): with kernel tracing on, hipGraphLaunch of these
+  // graphs ended in a host SIGSEGV inside the runtime's launch path (in a
+  // single-threaded batch solve as well as under four server slots), while
+  // the same graphs replay correctly without the profiler.
+  const bool graphs = !(c.tune_flags & KCDS_TUNE_NO_GRAPHS) && !getenv("KCDS_NO_GRAPHS") &&
+                      !getenv("ROCPROFILER_LIBRARY_CTOR") && !getenv("ROCPROF_KERNEL_TRACE");
   StreamGraph g_first(c), g_next(c), g_sweeps(c);
   if (graphs) {
     g_first.capture([&]() { relabel_batch(true); });

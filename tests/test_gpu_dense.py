@@ -1,6 +1,8 @@
 """GPU parity of the dense exact search (k_dense.hip): the k'-core held as a
-bitmask matrix, greedy clique splits combined by Frank-Wolfe, and the answer
-certified by an exact integer max-flow over vertex pairs.
+bitmask matrix, greedy clique splits combined by Wolfe's minimum-norm-point
+method, and the answer certified by an exact integer max-flow over vertex
+pairs.  Core sizes cover every matrix width (1, 2, 4, 8, 10, 12 and 16 words
+per row).
 
 The search is forced (KCDS_TUNE_DENSE) on graphs whose k'-core has at most
 1,024 vertices, after either peel, and must return the oracle's density
@@ -52,6 +54,12 @@ def graphs():
     yield "er1500", g.offset, g.nbr, (3, 4)
     g = gen.chunglu(20000, 2.1, 2.0, 900.0, 120000, 25, 0.95, seed=17)
     yield "chunglu20k", g.offset, g.nbr, (4,)
+    g = gen.er(600, 0.07, 0, seed=11)  # k'-core of about 600 vertices: 10-word rows
+    yield "er600", g.offset, g.nbr, (3, 4)
+    g = gen.er(700, 0.06, 0, seed=11)  # 696 vertices: 12-word rows
+    yield "er700", g.offset, g.nbr, (3,)
+    off, nbr = disjoint_cliques([6] * 100)  # 600 vertices in 100 tied components
+    yield "k6_x100", off, nbr, (3,)
 
 
 GRAPHS = list(graphs())
