@@ -135,7 +135,7 @@ def bench_c2_solve(g, k, kcds):
                             "dense_or_components": res.ms_components, "flow": res.ms_flow}}
 
 
-def bench_c3(gen, kcds, reps=3, solve=True):
+def bench_c3(gen, kcds, reps=3, solve=True, shape=None):
     """C3 (Chung-Lu n = 1M, ~10M edges, near-K80, k = 5): the same step as the
     headline (GPU order + DAG + per-vertex counting pass) with its own roofline
     figures, and one exact solve (enumerating peel, dense exact search: T =
@@ -144,7 +144,10 @@ def bench_c3(gen, kcds, reps=3, solve=True):
     s = kcds.Solver(0)
     s.upload(g.offset, g.nbr)
     s.orient(None)
-    s.autotune_list(k)
+    if shape:
+        s.set_list_shape(shape)
+    else:
+        s.autotune_list(k)
     s.count(k, per_vertex=False)
     s.sync()
     kms = []
@@ -158,10 +161,13 @@ def bench_c3(gen, kcds, reps=3, solve=True):
     info = s.list_info()
     km = sum(kms) / len(kms)
     b = 4 * (info.n + 1) + 4 * info.arcs + 4 * info.w2 + 8 * info.n
+    traffic, traffic_src = committed_traffic("list_count_c3_traffic.json")
     out = {"C3_T": int(T), "C3_step_ms": dt * 1e3, "C3_cliques_per_s": T / dt, "C3_list_pass_ms": km,
            "C3_max_outdeg": info.max_outdeg, "C3_waves_per_root": list(info.waves_per_root),
-           "C3_roofline": {"bound": "hbm", "w2": int(info.w2), "bytes_per_pass": b,
-                           "achieved_GBps": b / (km / 1e3) / 1e9, "frac": b / (km / 1e3) / HBM_PEAK}}
+           "C3_roofline": {"bound": "hbm", "achieved": b / (km / 1e3) / 1e9, "peak": HBM_PEAK / 1e9,
+                           "unit": "GB/s", "frac": b / (km / 1e3) / HBM_PEAK, "traffic": traffic,
+                           "traffic_source": traffic_src, "kernel": "k_list_count<W,5>", "kernel_ms": km,
+                           "w2": int(info.w2), "bytes_per_pass": b}}
     if solve:
         t1 = time.perf_counter()
         res, _ = s.solve(k, want_set=False)
@@ -172,6 +178,17 @@ def bench_c3(gen, kcds, reps=3, solve=True):
                                     "dense_or_components": res.ms_components, "flow": res.ms_flow}})
     s.close()
     return out
+
+
+def committed_traffic(name):
+    """HBM bytes per pass from a committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, each in its own
+    run, scaled by a plain-copy calibration); counters cannot be read from inside this process."""
+    tpath = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(tpath):
+        return None, None
+    with open(tpath) as fh:
+        tj = json.load(fh)
+    return tj.get("traffic_bytes"), tj.get("source")
 
 
 def cpu_model():
@@ -237,7 +254,8 @@ def main():
             g, k = gen.config(2)
             out = bench_c2_solve(g, k, kcds)
         elif args.only == "c3":
-            out = bench_c3(gen, kcds, solve=not args.no_solves)
+            out = bench_c3(gen, kcds, solve=not args.no_solves,
+                           shape=[int(x) for x in args.shape.split(",")] if args.shape else None)
         elif args.only == "c4":
             out = bench_c4(np, gen, kcds, args.c4_plain_sample)
         else:
@@ -282,12 +300,7 @@ def main():
     achieved = bytes_list / (kms / 1e3)
     # HBM bytes per pass from the committed rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE scaled by a
     # plain-copy calibration); counters cannot be read from inside this process
-    traffic, traffic_src = None, None
-    tpath = os.path.join(ROOT, "profiles", "list_count_c2_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            tj = json.load(fh)
-        traffic, traffic_src = tj.get("traffic_bytes"), tj.get("source")
+    traffic, traffic_src = committed_traffic("list_count_c2_traffic.json")
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": "k_list_count<W,4> (one launch per width class W=1,2,4,8, concurrently on four streams)",

@@ -16,8 +16,9 @@
 // order -- and a greedy split is one listing pass on the GPU: orient by the
 // order and count the cliques rooted at each vertex (the k'-core, at most
 // 1,024 vertices, is held as a bitmask matrix in LDS; a wave per (root,
-// 64-candidate word) task).  Frank-Wolfe with line search on the squared
-// norm drives a convex combination of greedy splits toward the balanced
+// 64-candidate word) task).  Wolfe's minimum-norm-point algorithm (a corral
+// of greedy splits, each new one the greedy split of the current point's
+// order) drives a convex combination of greedy splits toward the balanced
 // split; its level sets give the candidate S (the best prefix of the vertices
 // ordered by load, counted exactly in one more pass).
 //
@@ -648,7 +649,8 @@ DpFn dp_fn(uint32_t W, uint32_t k) {
 
 // Dense exact search over graph 0's k'-core (c.kdoff/c.kdnbr hold its DAG).
 // Returns false when the core has more than 1,024 vertices or the
-// certificate was not reached within max_iters Frank-Wolfe steps.  On
+// certificate was not reached within max_iters major cycles of the
+// minimum-norm-point search.  On
 // success: num/den = cliques in / size of the maximal densest set, which
 // c.inset (new ids) marks; ncomp = components of the core's clique graph;
 // passes = greedy passes run.
