@@ -186,63 +186,100 @@ template <int W> __device__ __forceinline__ void credit(ull *cnt, uint32_t c, ul
     if ((uint32_t)q == (c >> 6) && lane == (c & 63)) cnt[q] += E;
 }
 
+// Bits above position b of one word (none for b = 63).
+__device__ __forceinline__ ull above_bit(uint32_t b) { return b == 63 ? 0ull : (~0ull << (b + 1)); }
+
 // Counting recursion: REM vertices still to choose from candidate set S.
 // Returns the number of REM-cliques inside S (wave-uniform); every member
-// vertex's counter is credited with the cliques it belongs to.
-template <int W, int REM> struct Count {
-  static __device__ ull run(const Frame<W> &f, const ull *S, ull *cnt) {
-    ull total = 0;
-#pragma unroll
-    for (int q = 0; q < W; ++q) {
-      ull word = S[q];
+// vertex's counter is credited with the cliques it belongs to.  Words of S
+// below LO are zero (LO is the word of the last chosen vertex: candidates
+// only go up), so they are skipped at compile time; words from hi (the
+// root's row count in words) up are zero too.  Every row word a step needs
+// is requested before any is used, so a step waits for the LDS once, not
+// once per word.
+template <int W, int REM, int LO> struct Count;
+
+// The candidates of S in word Q and above (Q is a compile-time word index,
+// so the words below it are known to be zero).
+template <int W, int REM, int Q> struct CountFrom {
+  static __device__ ull run(const Frame<W> &f, const ull *S, ull *cnt, uint32_t hi) {
+    if constexpr (Q >= W) {
+      return 0;
+    } else {
+      if ((uint32_t)Q >= hi) return 0;
+      ull total = 0;
+      ull word = S[Q];
       while (word) {
-        const int b = __builtin_ctzll(word);
+        const uint32_t b = (uint32_t)__builtin_ctzll(word);
         word &= word - 1;
-        const uint32_t c = (uint32_t)(q * 64 + b);
+        const uint32_t c = (uint32_t)Q * 64 + b;
         ull S2[W];
         uint32_t pc = 0;
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-          S2[w] = S[w] & row(f, c, w) & above_mask(w, c);
+        for (int w = 0; w < W; ++w) S2[w] = w < Q ? 0ull : row(f, c, w);
+#pragma unroll
+        for (int w = Q; w < W; ++w) {
+          S2[w] = S[w] & uniform_u64(S2[w]) & (w == Q ? above_bit(b) : ~0ull);
           pc += (uint32_t)__popcll(S2[w]);
         }
         if (pc < (uint32_t)(REM - 1)) continue;
-        const ull E = Count<W, REM - 1>::run(f, S2, cnt);
-        credit<W>(cnt, c, E);
+        const ull E = Count<W, REM - 1, Q>::run(f, S2, cnt, hi);
+        if (lane_id() == b) cnt[Q] += E;
         total += E;
       }
+      return total + CountFrom<W, REM, Q + 1>::run(f, S, cnt, hi);
     }
-    return total;
   }
 };
 
-// Edges inside S, credited to both endpoints; rows q of S handled by this wave
-// are those with q % nw == wv (all rows when nw == 1).  Returns the wave's
-// share of 2 * edges.
-template <int W>
-__device__ __forceinline__ uint32_t edges_in(const Frame<W> &f, const ull *S, ull *cnt, uint32_t wv, uint32_t nw) {
+template <int W, int REM, int LO> struct Count {
+  static __device__ ull run(const Frame<W> &f, const ull *S, ull *cnt, uint32_t hi) {
+    return CountFrom<W, REM, LO>::run(f, S, cnt, hi);
+  }
+};
+
+// Edges inside S (words below LO and from hi up zero), credited to both
+// endpoints; rows q of S handled by this wave are those with q % nw == wv
+// (all rows when nw == 1).  Returns the wave's share of 2 * edges.  The
+// whole wave is active here (control flow above is wave-uniform).
+template <int W, int LO>
+__device__ __forceinline__ uint32_t edges_in(const Frame<W> &f, const ull *S, ull *cnt, uint32_t hi, uint32_t wv,
+                                             uint32_t nw) {
   const uint32_t lane = lane_id();
   uint32_t xs = 0;
 #pragma unroll
-  for (int q = 0; q < W; ++q) {
-    if ((uint32_t)q % nw != wv) continue;
+  for (int q = LO; q < W; ++q) {
+    if ((uint32_t)q >= hi) break;
+    if (!S[q] || (uint32_t)q % nw != wv) continue;
     if ((S[q] >> lane) & 1ull) {
       const uint32_t j = (uint32_t)q * 64 + lane;
+      ull r[W];
+#pragma unroll
+      for (int w = LO; w < W; ++w) r[w] = row(f, j, w);
       uint32_t x = 0;
 #pragma unroll
-      for (int w = 0; w < W; ++w) x += (uint32_t)__popcll(row(f, j, w) & S[w]);
+      for (int w = LO; w < W; ++w) x += (uint32_t)__popcll(r[w] & S[w]);
       cnt[q] += x;
       xs += x;
     }
   }
-  return wave_sum_u32(xs);
+  return wave_sum_u32_full(xs);
 }
 
-template <int W> struct Count<W, 2> {
-  static __device__ ull run(const Frame<W> &f, const ull *S, ull *cnt) {
-    return (ull)(edges_in<W>(f, S, cnt, 0, 1) >> 1);
+template <int W, int LO> struct Count<W, 2, LO> {
+  static __device__ ull run(const Frame<W> &f, const ull *S, ull *cnt, uint32_t hi) {
+    return (ull)(edges_in<W, LO>(f, S, cnt, hi, 0, 1) >> 1);
   }
 };
+
+// Count<W, REM, cq> for a first-level vertex in word cq (wave-uniform).
+template <int W, int REM, int Q = 0>
+__device__ __forceinline__ ull count_from(uint32_t cq, const Frame<W> &f, const ull *S, ull *cnt, uint32_t hi) {
+  if constexpr (Q + 1 < W) {
+    if (cq != (uint32_t)Q) return count_from<W, REM, Q + 1>(cq, f, S, cnt, hi);
+  }
+  return Count<W, REM, Q>::run(f, S, cnt, hi);
+}
 
 // Materialising recursion: same traversal, writes each clique as K new ids.
 template <int W, int K, int REM> struct Emit {
@@ -393,21 +430,25 @@ __global__ __launch_bounds__(64 * MAX_WAVES) void k_list_count(const uint32_t *_
     for (int q = 0; q < W; ++q) cnt[q] = 0;
     ull S[W];
     all_bits<W>(S, d);
+    const uint32_t hi = (d + 63) >> 6;  // row words in use
     ull rt = 0;
     if constexpr (K == 3) {
-      rt = edges_in<W>(f, S, cnt, wv, nw);  // twice the wave's share of edges
+      rt = edges_in<W, 0>(f, S, cnt, hi, wv, nw);  // twice the wave's share of edges
       if (cap) admit(gsum, g, rt >> 1, pend, cap, theta);
     } else {
       for (uint32_t c = wv; c < d; c += nw) {
+        const uint32_t cq = c >> 6;
         ull S2[W];
         uint32_t pc = 0;
 #pragma unroll
+        for (int w = 0; w < W; ++w) S2[w] = row(f, c, w);
+#pragma unroll
         for (int w = 0; w < W; ++w) {
-          S2[w] = row(f, c, w) & above_mask(w, c);
+          S2[w] = uniform_u64(S2[w]) & above_mask(w, c);
           pc += (uint32_t)__popcll(S2[w]);
         }
         if (pc < (uint32_t)(K - 2)) continue;
-        const ull E = Count<W, K - 2>::run(f, S2, cnt);
+        const ull E = count_from<W, K - 2>(cq, f, S2, cnt, hi);
         credit<W>(cnt, c, E);
         rt += E;
         if (cap && admit(gsum, g, E, pend, cap, theta)) break;

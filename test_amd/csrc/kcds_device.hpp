@@ -82,6 +82,30 @@ __device__ __forceinline__ ull wave_sum_u64(ull v) {
   return v;
 }
 
+// Sum over the 64 lanes of a wave in which every lane is active, as a scalar.
+// Data-parallel-primitive (DPP) row shifts build each 16-lane row's inclusive
+// prefix sum (lane 15 holds the row's total), two row broadcasts carry the
+// totals of rows 0..2 upward, and lane 63 ends with the sum: six VALU adds and
+// no LDS traffic (a shuffle round trips through the LDS crossbar six times).
+// Lanes switched off by divergence would contribute stale values, so callers
+// use it only where control flow has been wave-uniform.
+__device__ __forceinline__ uint32_t wave_sum_u32_full(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);   // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);   // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// A wave-uniform 64-bit value moved to scalar registers.
+__device__ __forceinline__ ull uniform_u64(ull v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((ull)hi << 32) | lo;
+}
+
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o, 64));
