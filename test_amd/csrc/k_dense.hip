@@ -76,7 +76,9 @@ template <int W, bool WP> __device__ __forceinline__ void dcredit(ull *cr, uint3
   }
 }
 
-// REM-cliques inside S (one wave); with WP every member is credited.
+// REM-cliques inside S (one wave); with WP every member is credited.  Every
+// row word of a step is requested before the first is used (one LDS wait per
+// step), and candidate sets live in scalar registers.
 template <int W, int REM, bool WP> struct DCount {
   static __device__ ull run(const ull *A, const ull *S, ull *cr) {
     ull total = 0;
@@ -90,8 +92,10 @@ template <int W, int REM, bool WP> struct DCount {
         ull S2[W];
         uint32_t pc = 0;
 #pragma unroll
+        for (int w = 0; w < W; ++w) S2[w] = drow<W>(A, c, w);
+#pragma unroll
         for (int w = 0; w < W; ++w) {
-          S2[w] = S[w] & drow<W>(A, c, w) & dabove(w, c);
+          S2[w] = S[w] & uniform_u64(S2[w]) & dabove(w, c);
           pc += (uint32_t)__popcll(S2[w]);
         }
         if (pc < (uint32_t)(REM - 1)) continue;
@@ -104,22 +108,40 @@ template <int W, int REM, bool WP> struct DCount {
   }
 };
 
+// Row words [LO, W) of lane j against S, all requested before any is used.
+template <int W, int LO>
+__device__ __forceinline__ uint32_t drow_popc(const ull *A, uint32_t j, const ull *S) {
+  ull r[W];
+#pragma unroll
+  for (int w = LO; w < W; ++w) r[w] = drow<W>(A, j, w);
+  uint32_t x = 0;
+#pragma unroll
+  for (int w = LO; w < W; ++w) x += (uint32_t)__popcll(r[w] & S[w]);
+  return x;
+}
+
+// Edges inside S.  When the lower half of S is empty (candidates only go up,
+// so that is common) only the upper half of each row is read.  The whole
+// wave is active here (control flow above is wave-uniform).
 template <int W, bool WP> struct DCount<W, 2, WP> {
   static __device__ ull run(const ull *A, const ull *S, ull *cr) {
     const uint32_t lane = lane_id();
+    constexpr int H = W >= 8 ? W / 2 : 0;
+    bool low = false;
+#pragma unroll
+    for (int w = 0; w < H; ++w) low |= S[w] != 0ull;
     uint32_t xs = 0;
 #pragma unroll
     for (int q = 0; q < W; ++q) {
+      if (!S[q]) continue;
       if ((S[q] >> lane) & 1ull) {
         const uint32_t j = (uint32_t)q * 64 + lane;
-        uint32_t x = 0;
-#pragma unroll
-        for (int w = 0; w < W; ++w) x += (uint32_t)__popcll(drow<W>(A, j, w) & S[w]);
+        const uint32_t x = low ? drow_popc<W, 0>(A, j, S) : drow_popc<W, H>(A, j, S);
         if constexpr (WP) cr[q] += x;
         xs += x;
       }
     }
-    return (ull)(wave_sum_u32(xs) >> 1);
+    return (ull)(wave_sum_u32_full(xs) >> 1);
   }
 };
 
@@ -166,7 +188,7 @@ __global__ __launch_bounds__(DN_THREADS) void k_dn_pass(const ull *__restrict__ 
     if (w0 < (i >> 6)) continue;
     ull Sr[W];
 #pragma unroll
-    for (int w = 0; w < W; ++w) Sr[w] = drow<W>(A, i, w) & dabove(w, i);
+    for (int w = 0; w < W; ++w) Sr[w] = uniform_u64(drow<W>(A, i, w)) & dabove(w, i);
     ull cand = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w)
@@ -184,8 +206,10 @@ __global__ __launch_bounds__(DN_THREADS) void k_dn_pass(const ull *__restrict__ 
       ull S2[W];
       uint32_t pc = 0;
 #pragma unroll
+      for (int w = 0; w < W; ++w) S2[w] = drow<W>(A, c, w);
+#pragma unroll
       for (int w = 0; w < W; ++w) {
-        S2[w] = Sr[w] & drow<W>(A, c, w) & dabove(w, c);
+        S2[w] = Sr[w] & uniform_u64(S2[w]) & dabove(w, c);
         pc += (uint32_t)__popcll(S2[w]);
       }
       if (pc + 2 < (uint32_t)K) continue;
@@ -561,7 +585,7 @@ __global__ __launch_bounds__(DN_THREADS) void k_dp_pass(const ull *__restrict__ 
     const uint32_t v = F[fi];
     ull S[W];
 #pragma unroll
-    for (int w = 0; w < W; ++w) S[w] = drow<W>(A, v, w) & s_live[w] & ~(s_f[w] & dbelow(w, v));
+    for (int w = 0; w < W; ++w) S[w] = uniform_u64(drow<W>(A, v, w) & s_live[w] & ~(s_f[w] & dbelow(w, v)));
     ull cand = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w)
@@ -579,8 +603,10 @@ __global__ __launch_bounds__(DN_THREADS) void k_dp_pass(const ull *__restrict__ 
       ull S2[W];
       uint32_t pc = 0;
 #pragma unroll
+      for (int w = 0; w < W; ++w) S2[w] = drow<W>(A, c, w);
+#pragma unroll
       for (int w = 0; w < W; ++w) {
-        S2[w] = S[w] & drow<W>(A, c, w) & dabove(w, c);
+        S2[w] = S[w] & uniform_u64(S2[w]) & dabove(w, c);
         pc += (uint32_t)__popcll(S2[w]);
       }
       if (pc + 2 < (uint32_t)K) continue;

@@ -104,58 +104,91 @@ template <int W> __device__ __forceinline__ void ecredit(ull *cr, uint32_t c, ul
     if ((uint32_t)q == (c >> 6) && lane == (c & 63)) cr[q] += E;
 }
 
-// Edges inside S, both endpoints credited; rows q with q % nw == wv belong to
-// this wave.  Returns the wave's share of 2 x edges.
-template <int W>
+// Bits above position b of one word (none for b = 63).
+__device__ __forceinline__ ull above_bit1(uint32_t b) { return b == 63 ? 0ull : (~0ull << (b + 1)); }
+
+// Edges inside S (words below LO zero), both endpoints credited; rows q with
+// q % nw == wv belong to this wave.  Returns the wave's share of 2 x edges.
+// Each lane requests all its row words before using any; the whole wave is
+// active here (the callers' control flow is workgroup-uniform).
+template <int W, int LO>
 __device__ __forceinline__ uint32_t e_edges(const EFrame &f, const ull *S, ull *cr, uint32_t wv, uint32_t nw) {
   const uint32_t lane = lane_id();
   uint32_t xs = 0;
 #pragma unroll
-  for (int q = 0; q < W; ++q) {
-    if ((uint32_t)q % nw != wv) continue;
+  for (int q = LO; q < W; ++q) {
+    if (!S[q] || (uint32_t)q % nw != wv) continue;
     if ((S[q] >> lane) & 1ull) {
       const uint32_t j = (uint32_t)q * 64 + lane;
+      ull r[W];
+#pragma unroll
+      for (int w = LO; w < W; ++w) r[w] = erow(f, j, w);
       uint32_t x = 0;
 #pragma unroll
-      for (int w = 0; w < W; ++w) x += (uint32_t)__popcll(erow(f, j, w) & S[w]);
+      for (int w = LO; w < W; ++w) x += (uint32_t)__popcll(r[w] & S[w]);
       cr[q] += x;
       xs += x;
     }
   }
-  return wave_sum_u32(xs);
+  return wave_sum_u32_full(xs);
 }
 
-// REM-cliques inside S (one wave), every member credited; returns the count.
-template <int W, int REM> struct ECount {
+template <int W, int REM, int LO> struct ECount;
+
+// REM-cliques inside S whose first member lies in word Q or above (Q is a
+// compile-time word index; the words of S below LO <= Q are zero).  The row
+// words of a step are all requested before the first is used, and S stays in
+// scalar registers.
+template <int W, int REM, int Q> struct ECountFrom {
   static __device__ ull run(const EFrame &f, const ull *S, ull *cr) {
-    ull total = 0;
-#pragma unroll
-    for (int q = 0; q < W; ++q) {
-      ull word = S[q];
+    if constexpr (Q >= W) {
+      return 0;
+    } else {
+      ull total = 0;
+      ull word = S[Q];
       while (word) {
-        const int b = __builtin_ctzll(word);
+        const uint32_t b = (uint32_t)__builtin_ctzll(word);
         word &= word - 1;
-        const uint32_t c = (uint32_t)(q * 64 + b);
+        const uint32_t c = (uint32_t)Q * 64 + b;
         ull S2[W];
         uint32_t pc = 0;
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-          S2[w] = S[w] & erow(f, c, w) & above_bits(w, c);
+        for (int w = 0; w < W; ++w) S2[w] = w < Q ? 0ull : erow(f, c, w);
+#pragma unroll
+        for (int w = Q; w < W; ++w) {
+          S2[w] = S[w] & uniform_u64(S2[w]) & (w == Q ? above_bit1(b) : ~0ull);
           pc += (uint32_t)__popcll(S2[w]);
         }
         if (pc < (uint32_t)(REM - 1)) continue;
-        const ull E = ECount<W, REM - 1>::run(f, S2, cr);
-        ecredit<W>(cr, c, E);
+        const ull E = ECount<W, REM - 1, Q>::run(f, S2, cr);
+        if (lane_id() == b) cr[Q] += E;
         total += E;
       }
+      return total + ECountFrom<W, REM, Q + 1>::run(f, S, cr);
     }
-    return total;
   }
 };
 
-template <int W> struct ECount<W, 2> {
-  static __device__ ull run(const EFrame &f, const ull *S, ull *cr) { return (ull)(e_edges<W>(f, S, cr, 0, 1) >> 1); }
+// REM-cliques inside S (one wave; words of S below LO are zero), every member
+// credited; returns the count.
+template <int W, int REM, int LO> struct ECount {
+  static __device__ ull run(const EFrame &f, const ull *S, ull *cr) { return ECountFrom<W, REM, LO>::run(f, S, cr); }
 };
+
+template <int W, int LO> struct ECount<W, 2, LO> {
+  static __device__ ull run(const EFrame &f, const ull *S, ull *cr) {
+    return (ull)(e_edges<W, LO>(f, S, cr, 0, 1) >> 1);
+  }
+};
+
+// ECount<W, REM, cq> for a first-level vertex in word cq (wave-uniform).
+template <int W, int REM, int Q = 0>
+__device__ __forceinline__ ull ecount_from(uint32_t cq, const EFrame &f, const ull *S, ull *cr) {
+  if constexpr (Q + 1 < W) {
+    if (cq != (uint32_t)Q) return ecount_from<W, REM, Q + 1>(cq, f, S, cr);
+  }
+  return ECount<W, REM, Q>::run(f, S, cr);
+}
 
 // REM-cliques inside S with per-member credits, the first level split over
 // the workgroup's waves.  Returns this wave's share of 2 x the count.
@@ -172,7 +205,7 @@ __device__ ull e_count_set(const EFrame &f, const ull *S, ull *cr, uint32_t wv, 
     }
     return 2 * tot;
   } else if constexpr (REM == 2) {
-    return e_edges<W>(f, S, cr, wv, nw);
+    return e_edges<W, 0>(f, S, cr, wv, nw);
   } else {
     ull tot = 0;
     uint32_t idx = 0;
@@ -187,12 +220,14 @@ __device__ ull e_count_set(const EFrame &f, const ull *S, ull *cr, uint32_t wv, 
         ull S2[W];
         uint32_t pc = 0;
 #pragma unroll
+        for (int w = 0; w < W; ++w) S2[w] = erow(f, c, w);
+#pragma unroll
         for (int w = 0; w < W; ++w) {
-          S2[w] = S[w] & erow(f, c, w) & above_bits(w, c);
+          S2[w] = S[w] & uniform_u64(S2[w]) & above_bits(w, c);
           pc += (uint32_t)__popcll(S2[w]);
         }
         if (pc < (uint32_t)(REM - 1)) continue;
-        const ull E = ECount<W, REM - 1>::run(f, S2, cr);
+        const ull E = ecount_from<W, REM - 1>(c >> 6, f, S2, cr);
         ecredit<W>(cr, c, E);
         tot += E;
       }
