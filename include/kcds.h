@@ -303,6 +303,14 @@ int kcds_count_dag_noalloc(int device, uint32_t n, const uint32_t *doff, const u
  *                        levels, C3 271 ms against about 160 ms for the last
  *                        498). */
 #define KCDS_TUNE_DENSE_PEEL 128u
+/*   KCDS_TUNE_FLOW_ALL   run the max-flow search on every component.  By
+ *                        default a component whose clique count is C(n_c, k)
+ *                        (a complete graph on its n_c vertices) is answered
+ *                        without one: its k-clique density C(t, k)/t grows
+ *                        with the size t of a subset, so the whole component
+ *                        is its unique densest subset.  Same results either
+ *                        way. */
+#define KCDS_TUNE_FLOW_ALL 256u
 int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
 
 /* Listing launch shape: waves per root (workgroup size / 64, 1..8) of each

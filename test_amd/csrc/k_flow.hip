@@ -456,6 +456,13 @@ __global__ void k_mark_csides(uint32_t nv, const uint32_t *verts, const uint32_t
     if (cside[i] && sel[cid[verts[i]]]) inset[verts[i]] = 1;
 }
 
+// cside = 1 for every core vertex of the components flagged in all
+__global__ void k_set_csides(uint32_t nv, const uint32_t *verts, const uint32_t *cid, const uint8_t *all,
+                             uint8_t *cside) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x)
+    if (all[cid[verts[i]]]) cside[i] = 1;
+}
+
 }  // namespace
 
 void mark_component_sides(Ctx &c, const std::vector<uint8_t> &sel) {
@@ -469,6 +476,18 @@ void mark_component_sides(Ctx &c, const std::vector<uint8_t> &sel) {
                                                   c.inset.as<uint8_t>());
   KCDS_LAUNCH_CHECK();
   c.sync();  // sel lives on the host
+}
+
+void mark_component_sides_all(Ctx &c, const std::vector<uint8_t> &all) {
+  CompData &g = c.cd;
+  const uint32_t nv = g.h_voff.back();
+  if (!nv) return;
+  buf_reserve(c, c.tmp3, all.size() + 8);
+  HIP_CHECK(hipMemcpyAsync(c.tmp3.p, all.data(), all.size(), hipMemcpyHostToDevice, c.stream));
+  k_set_csides<<<grid1d(nv), 256, 0, c.stream>>>(nv, g.verts.as<uint32_t>(), g.cid.as<uint32_t>(),
+                                                 c.tmp3.as<uint8_t>(), c.fb.cside.as<uint8_t>());
+  KCDS_LAUNCH_CHECK();
+  c.sync();  // all lives on the host
 }
 
 void mark_component_side(Ctx &c, uint32_t d) {

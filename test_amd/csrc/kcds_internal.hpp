@@ -269,6 +269,9 @@ void components_run(Ctx &c);
 void mark_component_side(Ctx &c, uint32_t d);
 // The same for every component d with sel[d] (one launch).
 void mark_component_sides(Ctx &c, const std::vector<uint8_t> &sel);
+// Sets the min-cut side flag of every vertex of the components flagged in all
+// (components answered without a max-flow: the whole component).
+void mark_component_sides_all(Ctx &c, const std::vector<uint8_t> &all);
 
 // Exact search (or one max-flow when max_iters == 1) of many components at
 // once, one workgroup each.

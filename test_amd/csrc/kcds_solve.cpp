@@ -87,6 +87,20 @@ bool may_reach(uint32_t k, ull max_core, ull Tc, ull num, ull den) {
   return ub * (1 + 1e-9) + 1e-9 >= (double)num / (double)den;
 }
 
+// True when Tc == C(nc, k): every k-subset of the component's nc vertices is
+// a clique, so the component is a complete graph.  Its k-clique density
+// C(t, k)/t = C(t-1, k-1)/k grows with the size t of a subset, so the whole
+// component is its unique densest subset and no max-flow is needed.
+bool complete_component(ull Tc, ull nc, uint32_t k) {
+  if (nc < k) return false;
+  u128 b = 1;  // C(nc, i) for i = 1..k, stopped once it passes Tc
+  for (uint32_t i = 1; i <= k; ++i) {
+    b = b * (u128)(nc - k + i) / i;
+    if (b > (u128)Tc) return false;
+  }
+  return b == (u128)Tc;
+}
+
 }  // namespace
 
 void dinkelbach_components(Ctx &c, const std::vector<BlockTask> &in, std::vector<BlockResult> &out,
@@ -355,15 +369,27 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
   buf_reserve(c, c.fb.cside, core_vertices + 8);
   HIP_CHECK(hipMemsetAsync(c.fb.cside.p, 0, core_vertices + 8, c.stream));
   std::vector<BlockTask> tasks;
-  for (uint32_t d = 0; d < C; ++d) {
-    const ull tc = g.h_coff[d + 1] - g.h_coff[d];
-    const uint32_t gi = cg[d];
-    if (tc == 0 || !may_reach(k, c.h_gpeel[gi].max_core, tc, an[gi], ad[gi])) continue;
-    tasks.push_back(BlockTask{d, an[gi], ad[gi]});
-  }
   // final alpha and side size of every solved component
   std::vector<BlockResult> res(C);
-  std::vector<uint8_t> solved(C, 0);
+  std::vector<uint8_t> solved(C, 0), complete(C, 0);
+  bool any_complete = false;
+  for (uint32_t d = 0; d < C; ++d) {
+    const ull tc = g.h_coff[d + 1] - g.h_coff[d], nc = g.h_voff[d + 1] - g.h_voff[d];
+    const uint32_t gi = cg[d];
+    if (tc == 0 || !may_reach(k, c.h_gpeel[gi].max_core, tc, an[gi], ad[gi])) continue;
+    if (!(c.tune_flags & KCDS_TUNE_FLOW_ALL) && complete_component(tc, nc, k)) {  // its answer is itself: alpha = Tc/nc, side = all of it
+      const ull gg = gcd(tc, nc);
+      res[d].p = tc / gg;
+      res[d].q = nc / gg;
+      res[d].side_n = nc;
+      res[d].side_T = tc;
+      solved[d] = complete[d] = 1;
+      any_complete = true;
+      continue;
+    }
+    tasks.push_back(BlockTask{d, an[gi], ad[gi]});
+  }
+  if (any_complete) mark_component_sides_all(c, complete);
   {
     std::vector<BlockResult> r;
     dinkelbach_components(c, tasks, r, ALL_ITERATIONS);

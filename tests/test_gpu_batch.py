@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from test_amd import gen
+from test_amd import gen, kcds
 
 pytestmark = pytest.mark.gpu
 
@@ -117,10 +117,54 @@ def test_batch_grid_and_workgroup_flows_together(solver, count):
     (which the oracle tests pin)."""
     specs = gen.batch_specs()
     gs = [gen.batch_graph(s) for s in specs[:count]] + [gen.batch_graph(specs[546])]
-    res, sets = solver.solve_batch([(g.offset, g.nbr) for g in gs], 4)
+    pairs = [(g.offset, g.nbr) for g in gs]
+    # every component through a max-flow (the planted cliques are complete
+    # components, which the default answers without one)
+    solver.set_tuning(0, kcds.TUNE_FLOW_ALL)
+    try:
+        res, sets = solver.solve_batch(pairs, 4)
+        assert sum(r.flow_calls for r in res) >= len(gs)
+    finally:
+        solver.set_tuning(0, 0)
     for i, g in enumerate(gs):
         solver.upload(g.offset, g.nbr)
         solo, ins = solver.solve(4)
         assert (solo.T, solo.rho_num, solo.rho_den) == (res[i].T, res[i].rho_num, res[i].rho_den), i
         np.testing.assert_array_equal(ins, sets[i])
     assert res[-1].T == 100529
+    # the default answers the same
+    res_d, sets_d = solver.solve_batch(pairs, 4)
+    for i in range(len(gs)):
+        assert (res_d[i].T, res_d[i].rho_num, res_d[i].rho_den) == (res[i].T, res[i].rho_num, res[i].rho_den), i
+        np.testing.assert_array_equal(sets_d[i], sets[i])
+
+
+def test_complete_components_answered_without_flow(solver):
+    """Components that are complete graphs (clique count C(n_c, k)) take no
+    max-flow, and their answers equal the oracle's: three disjoint cliques
+    K9, K9, K7 (the two K9 tie, so both are returned), and planted K20 / K26 in
+    ER graphs, whose k'-cores are the cliques alone."""
+    sizes = (9, 9, 7)
+    adj, base = [], 0
+    for s_ in sizes:
+        adj += [[base + b for b in range(s_) if b != a] for a in range(s_)]
+        base += s_
+    off = np.zeros(base + 1, np.uint32)
+    off[1:] = np.cumsum([len(x) for x in adj])
+    cases = [("K9+K9+K7", off, np.array([v for x in adj for v in x], np.uint32))]
+    for n_, s_ in ((300, 20), (800, 26)):
+        g = gen.er(n_, 0.02, s_, seed=n_)
+        cases.append((f"er{n_}+K{s_}", g.offset, g.nbr))
+    solver.set_tuning(0, 0)
+    for k in (3, 4):
+        for name, off, nbr in cases:
+            solver.upload(off, nbr)
+            res, ins = solver.solve(k)
+            r, _, _, _, ins_r = O.solve(off, nbr, k)
+            assert (res.T, res.rho_num, res.rho_den) == (r.T, r.rho_num, r.rho_den), (name, k)
+            np.testing.assert_array_equal(ins, ins_r)
+            assert res.flow_calls == 0, (name, k, res.flow_calls)
+    # the tie: both K9 are in the answer
+    solver.upload(cases[0][1], cases[0][2])
+    res, ins = solver.solve(3)
+    assert int(ins.sum()) == 18 and (res.rho_num, res.rho_den) == (2 * math.comb(9, 3), 18)

@@ -90,11 +90,13 @@ def _peel_and_components(solver, name, off, nbr, ks):
 
 
 # block_flow: the workgroup max-flow for every component; default: each
-# component to the solver the library picks; grid_flow: the grid-wide one,
-# its relabel and sweep sequences replayed as captured HIP graphs (many
-# replays per flow, with host reads and syncs between them);
-# grid_flow_eager: the same launches one at a time
-FLOWS = {"block_flow": 16, "default_flow": 0, "grid_flow": 2, "grid_flow_eager": 2 | 8}
+# component to the solver the library picks (complete components answered
+# without a max-flow); grid_flow: the grid-wide one, its relabel and sweep
+# sequences replayed as captured HIP graphs (many replays per flow, with host
+# reads and syncs between them); grid_flow_eager: the same launches one at a
+# time.  The forced forms run a max-flow on every component (256 =
+# KCDS_TUNE_FLOW_ALL), complete ones included.
+FLOWS = {"block_flow": 16 | 256, "default_flow": 0, "grid_flow": 2 | 256, "grid_flow_eager": 2 | 8 | 256}
 
 
 @pytest.mark.parametrize("flow", list(FLOWS))
