@@ -688,7 +688,8 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
                                                      const uint32_t *__restrict__ dnbr, ull *cnt, long long *core,
                                                      uint32_t *erem, uint32_t *fv0, uint32_t *fv1, uint4 *fi0,
                                                      uint4 *fi1, uint32_t seg_v, uint32_t seg_i, GraphPeel *gp,
-                                                     const uint32_t *__restrict__ gid, uint32_t *tlist, Scatter sc) {
+                                                     const uint32_t *__restrict__ gid, uint32_t *tlist, Scatter sc,
+                                                     uint8_t *alive) {
   __shared__ EFrame f;
   __shared__ ull s_hdr[2];
   __shared__ uint32_t s_hdr2[4], s_f[ESEG + 1], s_i[ESEG + 1], s_last, s_sz[2][2][ESEG];
@@ -727,7 +728,7 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
     const uint32_t Kc = sc.mask + 1;
     for (uint32_t p0 = blockIdx.x * blockDim.x; p0 < Kc; p0 += gstride) {
       const uint32_t p = p0 + tid, v = vertex_of(sc, p);
-      const bool live = p < Kc && v < n && core[v] == -1;
+      const bool live = p < Kc && alive[p];  // read in slot order: no gather of core[v]
       const ull cv = live ? cnt[p] : ~0ull;
       e_append(live && cv <= target, v, cv, a);
       if (live && cv > target) m = min(m, cv);
@@ -758,6 +759,7 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
         while (i >= s_f[sg + 1]) ++sg;
         v = fv[(size_t)sg * seg_v + (i - s_f[sg])];
         __hip_atomic_store(&core[v], (long long)L, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        alive[slot_of(sc, v)] = 0;
         g = gid[v];
       }
       const uint32_t gsize = e_group(valid, g);
@@ -859,10 +861,14 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
   }
 }
 
-// working counts (scattered layout), core = -1, every vertex live
-__global__ void k_ep_init(uint32_t n, const ull *cnt0, ull *cnt, long long *core, uint32_t *erem, Scatter sc) {
+// working counts (scattered layout), core = -1, every vertex live (alive is
+// the slot-ordered live flag that OPEN scans; slots past n stay 0)
+__global__ void k_ep_init(uint32_t n, const ull *cnt0, ull *cnt, long long *core, uint32_t *erem, Scatter sc,
+                          uint8_t *alive) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    cnt[slot_of(sc, i)] = cnt0[i];
+    const uint32_t p = slot_of(sc, i);
+    cnt[p] = cnt0[i];
+    alive[p] = 1;
     core[i] = -1;
     erem[i] = LIVE;
   }
@@ -927,7 +933,7 @@ __global__ void k_fill_i32(int32_t *p, int32_t v, uint64_t len) {
 
 typedef void (*EpStepFn)(EpCtl *, uint32_t, const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *,
                          ull *, long long *, uint32_t *, uint32_t *, uint32_t *, uint4 *, uint4 *, uint32_t, uint32_t,
-                         GraphPeel *, const uint32_t *, uint32_t *, Scatter);
+                         GraphPeel *, const uint32_t *, uint32_t *, Scatter, uint8_t *);
 
 EpStepFn ep_step_fn(uint32_t k) {
   switch (k) {
@@ -992,13 +998,14 @@ void epeel_run(Ctx &c) {
   const Scatter sc = make_scatter(n);
   const size_t kslots = ((size_t)sc.mask + 1 + 63) / 64 * 64;
   const size_t bytes = sizeof(ull) * kslots + sizeof(uint32_t) * 2 * ESEG * (size_t)seg_v +
-                       sizeof(uint4) * 2 * ESEG * (size_t)seg_i + sizeof(EpCtl) + 1024;
+                       sizeof(uint4) * 2 * ESEG * (size_t)seg_i + sizeof(EpCtl) + kslots + 1024;
   buf_reserve(c, c.tmp0, bytes);
   ull *cnt = c.tmp0.as<ull>();
   uint32_t *fv0 = (uint32_t *)(cnt + kslots), *fv1 = fv0 + (size_t)ESEG * seg_v;
   uint4 *fi0 = (uint4 *)(((uintptr_t)(fv1 + (size_t)ESEG * seg_v) + 255) & ~(uintptr_t)255);
   uint4 *fi1 = fi0 + (size_t)ESEG * seg_i;
   EpCtl *ctl = (EpCtl *)(((uintptr_t)(fi1 + (size_t)ESEG * seg_i) + 255) & ~(uintptr_t)255);
+  uint8_t *alive = (uint8_t *)(ctl + 1);  // kslots bytes, live flag per counter slot
   EpCtl *h0 = (EpCtl *)c.host_scratch(sizeof(EpCtl));
   memset(h0, 0, sizeof(EpCtl));
   for (uint32_t g = 0; g < ESEG; ++g) {
@@ -1016,8 +1023,9 @@ void epeel_run(Ctx &c) {
   HIP_CHECK(hipMemcpyAsync(ctl, h0, sizeof(EpCtl), hipMemcpyHostToDevice, c.stream));
   c.sync();  // h0 is reused below
   if (n) {
+    HIP_CHECK(hipMemsetAsync(alive, 0, kslots, c.stream));
     k_ep_init<<<egrid(n), 256, 0, c.stream>>>(n, c.cnt.as<ull>(), cnt, c.core.as<long long>(), c.erem.as<uint32_t>(),
-                                               sc);
+                                               sc, alive);
     KCDS_LAUNCH_CHECK();
   }
   peel_graphs_begin(c, gp);
@@ -1034,7 +1042,7 @@ void epeel_run(Ctx &c) {
       hipLaunchKernelGGL(step, dim3(sgrid), dim3(ETHREADS), 0, c.stream, ctl, n, c.foff.as<uint32_t>(),
                          c.fnbr.as<uint32_t>(), c.doff.as<uint32_t>(), c.dnbr.as<uint32_t>(), cnt,
                          c.core.as<long long>(), c.erem.as<uint32_t>(), fv0, fv1, fi0, fi1, seg_v, seg_i, gp,
-                         c.gid.as<uint32_t>(), c.tlist.as<uint32_t>(), sc);
+                         c.gid.as<uint32_t>(), c.tlist.as<uint32_t>(), sc, alive);
       KCDS_LAUNCH_CHECK();
     }
     rounds += batch;
