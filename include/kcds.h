@@ -75,9 +75,12 @@ const char *kcds_last_error(const kcds_ctx *ctx);
  * later stages. */
 int kcds_upload_csr(kcds_ctx *ctx, uint32_t n, const uint32_t *offset, const uint32_t *nbr);
 
-/* Builds the DAG used for listing.  listing_order == NULL: degeneracy order
- * computed on the GPU (with slack unless KCDS_TUNE_EXACT_ORDER is set; see
- * kcds_set_tuning).  Otherwise listing_order[v] in 1..n is the reference's
+/* Builds the DAG used for listing.  listing_order == NULL: an order computed
+ * on the GPU -- by default the degree order (vertices sorted by degree, ties
+ * by id), or the degeneracy order (KCDS_TUNE_DEGENERACY_ORDER, with slack
+ * unless KCDS_TUNE_EXACT_ORDER is set; see kcds_set_tuning); the degree order
+ * falls back to the degeneracy order when it leaves an out-list longer than
+ * 512.  Otherwise listing_order[v] in 1..n is the reference's
  * listing order (src/graph.cpp:48-50: 1 + position in corePeelSequence, the
  * reverse removal order); arcs point from the smaller listing order to the
  * larger, i.e. from the vertex peeled later (higher core) to the one peeled
@@ -85,7 +88,7 @@ int kcds_upload_csr(kcds_ctx *ctx, uint32_t n, const uint32_t *offset, const uin
  * acyclic orientation yields the same cliques, counts and solve results: an
  * order that leaves an out-list longer than 512 (the LDS frame rows; the
  * reference's order does on hub-heavy graphs, where hubs point to their
- * low-core neighbours) is replaced by the degeneracy order, which
+ * low-core neighbours) is replaced by the GPU order, which
  * kcds_get_list_info reports.  KCDS_E_LIMIT when even that order leaves an
  * out-list longer than 512 (degeneracy above 512). */
 int kcds_orient(kcds_ctx *ctx, const uint32_t *listing_order);
@@ -311,6 +314,14 @@ int kcds_count_dag_noalloc(int device, uint32_t n, const uint32_t *doff, const u
  *                        is its unique densest subset.  Same results either
  *                        way. */
 #define KCDS_TUNE_FLOW_ALL 256u
+/*   KCDS_TUNE_DEGENERACY_ORDER  orient by the degeneracy order (with slack
+ *                        unless KCDS_TUNE_EXACT_ORDER is set) instead of the
+ *                        default degree order.  By default kcds_orient(ctx,
+ *                        NULL) sorts the vertices by (degree, id) and points
+ *                        every arc to the later one; if that leaves an
+ *                        out-list wider than 512 it falls back to the slack
+ *                        degeneracy order, then to the exact one. */
+#define KCDS_TUNE_DEGENERACY_ORDER 512u
 int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
 
 /* Listing launch shape: waves per root (workgroup size / 64, 1..8) of each
@@ -344,8 +355,11 @@ typedef struct kcds_list_info {
   uint32_t class_roots[4];
   uint32_t waves_per_root[4];
   uint32_t order_replaced;  /* 1: kcds_orient's listing_order left an out-list
-                               wider than 512, so the degeneracy order was
-                               used instead (same cliques and results) */
+                               wider than 512, so the GPU order was used
+                               instead (same cliques and results) */
+  uint32_t order_kind;      /* the DAG's orientation: 0 degree order, 1 slack
+                               degeneracy order, 2 exact degeneracy order,
+                               3 the caller's listing_order */
 } kcds_list_info;
 int kcds_get_list_info(kcds_ctx *ctx, kcds_list_info *info);
 

@@ -680,7 +680,113 @@ __global__ void k_init_deg(const uint32_t *off, uint32_t n, Scatter sc, uint32_t
   for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < (n + 31) / 32; w += gridDim.x * blockDim.x) gone[w] = 0;
 }
 
+// Degree order: rank[v] = position of v among the vertices sorted by (degree,
+// id).  Arcs then point from the lower-degree end to the higher (ties by id);
+// the widest out-list is usually near the degeneracy (C2: 423 against a
+// degeneracy of 373; C3: 294 against 273) and the out-wedge sum W2 within
+// 1 % of the degeneracy order's, but the order is one sort instead of a peel
+// of some fifty level-synchronous steps.  The sort is a stable LSD radix
+// sort of the ids by degree, 8 bits per pass: ids start in increasing order,
+// so equal degrees keep id order.  Each pass is a per-tile digit histogram,
+// a scan over (digit, tile), and a scatter in which one wave walks its tile
+// in order and places each element after the earlier ones of its digit.
+constexpr uint32_t RS_TILE = 1024;  // elements per tile (a scatter wave walks 16 chunks of 64)
+constexpr uint32_t RS_CHUNKS = RS_TILE / 64;
+
+__device__ __forceinline__ uint32_t rs_key(const uint32_t *off, const uint32_t *key_in, uint32_t i) {
+  return key_in ? key_in[i] : off[i + 1] - off[i];  // first pass: the degree, read from the CSR
+}
+
+__global__ __launch_bounds__(256) void k_rs_hist(const uint32_t *off, const uint32_t *key_in, uint32_t n,
+                                                 uint32_t shift, uint32_t ntile, uint32_t *hist) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * RS_TILE, end = min(n, base + RS_TILE);
+  for (uint32_t i = base + threadIdx.x; i < end; i += blockDim.x)
+    atomicAdd(&h[(rs_key(off, key_in, i) >> shift) & 255u], 1u);
+  __syncthreads();
+  hist[threadIdx.x * ntile + blockIdx.x] = h[threadIdx.x];  // digit-major: the scan orders (digit, tile)
+}
+
+// One wave per tile.  hoff = exclusive scan of hist.  The last pass writes
+// rank/inv instead of the next pass's keys and ids.
+__global__ __launch_bounds__(64) void k_rs_scatter(const uint32_t *off, const uint32_t *key_in, const uint32_t *id_in,
+                                                   uint32_t n, uint32_t shift, uint32_t ntile, const uint32_t *hoff,
+                                                   uint32_t *key_out, uint32_t *id_out, uint32_t *rank,
+                                                   uint32_t *inv) {
+  __shared__ uint32_t run[256];
+  const uint32_t lane = threadIdx.x, base = blockIdx.x * RS_TILE;
+#pragma unroll
+  for (uint32_t t = 0; t < 4; ++t) run[lane + 64 * t] = hoff[(lane + 64 * t) * ntile + blockIdx.x];
+  uint32_t key[RS_CHUNKS], id[RS_CHUNKS];
+#pragma unroll
+  for (uint32_t ch = 0; ch < RS_CHUNKS; ++ch) {  // every load of the tile in flight at once
+    const uint32_t i = base + ch * 64 + lane;
+    key[ch] = i < n ? rs_key(off, key_in, i) : 0u;
+    id[ch] = i < n ? (id_in ? id_in[i] : i) : 0u;
+  }
+  __syncthreads();
+  const ull below_mask = (1ull << lane) - 1ull;
+#pragma unroll
+  for (uint32_t ch = 0; ch < RS_CHUNKS; ++ch) {
+    const uint32_t i = base + ch * 64 + lane;
+    const bool valid = i < n;
+    const uint32_t d = (key[ch] >> shift) & 255u;
+    ull peers = __ballot(valid);  // lanes of this chunk with the same digit
+#pragma unroll
+    for (uint32_t b = 0; b < 8; ++b) {
+      const ull m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    const uint32_t before = (uint32_t)__popcll(peers & below_mask), cnt = (uint32_t)__popcll(peers);
+    const uint32_t pos = valid ? run[d] + before : 0u;
+    __syncthreads();  // every lane has read run[] before the groups advance it
+    if (valid && before + 1 == cnt) run[d] += cnt;
+    __syncthreads();
+    if (valid) {
+      if (rank) {
+        rank[id[ch]] = pos;
+        inv[pos] = id[ch];
+      } else {
+        key_out[pos] = key[ch];
+        id_out[pos] = id[ch];
+      }
+    }
+  }
+}
+
 }  // namespace
+
+void degree_rank_gpu(Ctx &c) {
+  const uint32_t n = c.n;
+  buf_reserve(c, c.rank, sizeof(uint32_t) * (size_t)n + 4);
+  buf_reserve(c, c.inv, sizeof(uint32_t) * (size_t)n + 4);
+  if (n == 0) return;
+  uint32_t bits = 0;
+  while (bits < 32 && (c.maxdeg >> bits)) ++bits;
+  const uint32_t passes = std::max(1u, (bits + 7) / 8);
+  const uint32_t ntile = (n + RS_TILE - 1) / RS_TILE;
+  const size_t nh = (size_t)256 * ntile;
+  buf_reserve(c, c.tmp0, sizeof(uint32_t) * (4 * ((size_t)n + 64) + 2 * (nh + 64)));
+  uint32_t *k0 = c.tmp0.as<uint32_t>(), *i0 = k0 + n + 64, *k1 = i0 + n + 64, *i1 = k1 + n + 64;
+  uint32_t *hist = i1 + n + 64, *hoff = hist + nh + 64;
+  const uint32_t *kin = nullptr, *iin = nullptr;
+  for (uint32_t p = 0; p < passes; ++p) {
+    const uint32_t shift = 8 * p;
+    const bool last = p + 1 == passes;
+    k_rs_hist<<<ntile, 256, 0, c.stream>>>(c.off.as<uint32_t>(), kin, n, shift, ntile, hist);
+    KCDS_LAUNCH_CHECK();
+    scan_u32_to_u32(c, hist, hoff, nh);
+    uint32_t *kout = (p & 1) ? k1 : k0, *iout = (p & 1) ? i1 : i0;
+    k_rs_scatter<<<ntile, 64, 0, c.stream>>>(c.off.as<uint32_t>(), kin, iin, n, shift, ntile, hoff, kout, iout,
+                                             last ? c.rank.as<uint32_t>() : nullptr,
+                                             last ? c.inv.as<uint32_t>() : nullptr);
+    KCDS_LAUNCH_CHECK();
+    kin = kout;
+    iin = iout;
+  }
+}
 
 void degeneracy_rank_gpu(Ctx &c, uint32_t slack_shift) {
   const uint32_t n = c.n;
@@ -792,12 +898,20 @@ bool orient_from_rank(Ctx &c, bool approx) {
 // exact degeneracy order when KCDS_TUNE_EXACT_ORDER is set or the slack order
 // leaves an out-list wider than the listing's LDS rows.
 void orient_gpu(Ctx &c) {
-  const bool approx = !(c.tune_flags & KCDS_TUNE_EXACT_ORDER);
-  degeneracy_rank_gpu(c, approx ? ORDER_SLACK_SHIFT : NO_SLACK);
-  c.order_exact = !approx;
-  if (orient_from_rank(c, approx)) return;
+  const bool exact = c.tune_flags & KCDS_TUNE_EXACT_ORDER;
+  if (!exact && !(c.tune_flags & KCDS_TUNE_DEGENERACY_ORDER)) {
+    degree_rank_gpu(c);
+    c.order_exact = false;
+    c.order_kind = 0;
+    if (orient_from_rank(c, true)) return;
+  }
+  degeneracy_rank_gpu(c, exact ? NO_SLACK : ORDER_SLACK_SHIFT);
+  c.order_exact = exact;
+  c.order_kind = exact ? 2 : 1;
+  if (orient_from_rank(c, !exact)) return;
   degeneracy_rank_gpu(c, NO_SLACK);
   c.order_exact = true;
+  c.order_kind = 2;
   orient_from_rank(c, false);
 }
 

@@ -163,8 +163,10 @@ int kcds_upload_csr(kcds_ctx *ctx, uint32_t n, const uint32_t *offset, const uin
     // every list must hold ids < n, no self-loop, strictly ascending (the
     // loaders and generators produce this form; duplicates would count
     // cliques twice)
+    uint32_t maxdeg = 0;
     for (uint32_t v = 0; v < n; ++v) {
       if (offset[v + 1] < offset[v]) fail(KCDS_E_INVALID, "offset not monotone at %u", v);
+      maxdeg = std::max(maxdeg, offset[v + 1] - offset[v]);
       for (uint32_t i = offset[v]; i < offset[v + 1]; ++i) {
         uint32_t u = nbr[i];
         if (u >= n) fail(KCDS_E_INVALID, "neighbour id %u of vertex %u out of range", u, v);
@@ -183,6 +185,7 @@ int kcds_upload_csr(kcds_ctx *ctx, uint32_t n, const uint32_t *offset, const uin
     c.sync();
     c.n = n;
     c.nnz = nnz;
+    c.maxdeg = maxdeg;
     c.ngraphs = 1;
     c.h_gvoff.assign({0u, n});
     c.have_graph = true;
@@ -207,7 +210,8 @@ int kcds_orient(kcds_ctx *ctx, const uint32_t *listing_order) {
       HIP_CHECK(hipMemcpyAsync(c.tmp0.p, listing_order, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c.stream));
       rank_from_listing_order(c, c.tmp0.as<uint32_t>());
       c.order_replaced = !orient_from_rank(c, true);
-      if (c.order_replaced) orient_gpu(c);  // an out-list wider than the frames: the degeneracy order
+      c.order_kind = 3;
+      if (c.order_replaced) orient_gpu(c);  // an out-list wider than the frames: the GPU order
     } else {
       c.order_replaced = false;
       orient_gpu(c);
@@ -359,6 +363,7 @@ int kcds_get_list_info(kcds_ctx *ctx, kcds_list_info *info) {
     info->kernel_ms = c.list_kernel_ms;
     info->kernel_launches = c.list_kernel_launches;
     info->order_replaced = c.order_replaced ? 1u : 0u;
+    info->order_kind = c.order_kind;
     for (int i = 0; i < 4; ++i) {
       info->class_roots[i] = c.class_count[i];
       info->waves_per_root[i] = list_waves_per_root(c, i, c.k ? c.k : 4);

@@ -119,6 +119,7 @@ struct Ctx {
   float tune_ms[4] = {0, 0, 0, 0};
   uint64_t order_steps = 0;  // work/control steps of the last GPU ordering
   bool order_exact = true;   // the last GPU ordering was the exact degeneracy order
+  uint32_t order_kind = 0;   // orientation of the DAG: 0 degree, 1 slack degeneracy, 2 exact degeneracy, 3 caller's
   bool order_replaced = false;  // kcds_orient's listing order was too wide; the degeneracy order is used
   uint64_t peel_steps = 0;   // steps of the last grid peel
   uint64_t graph_budget = 0; // per-graph cap on T * (8k + 4) bytes (0 = none)
@@ -130,6 +131,7 @@ struct Ctx {
   // input graph (original ids)
   uint32_t n = 0;
   uint64_t nnz = 0;
+  uint32_t maxdeg = 0;       // largest degree of the uploaded graph
   DevBuf off, nbr;
   bool have_graph = false;
 

@@ -31,7 +31,7 @@ EXPORTS = ("kcds_create", "kcds_destroy", "kcds_last_error", "kcds_upload_csr", 
 
 GRAPH_SOLVED, GRAPH_OVER_BUDGET = 0, 1
 TUNE_GRID_PEEL, TUNE_GRID_FLOW, TUNE_EXACT_ORDER, TUNE_NO_GRAPHS, TUNE_BLOCK_FLOW = 1, 2, 4, 8, 16
-TUNE_ENUM_PEEL, TUNE_DENSE, TUNE_DENSE_PEEL, TUNE_FLOW_ALL = 32, 64, 128, 256
+TUNE_ENUM_PEEL, TUNE_DENSE, TUNE_DENSE_PEEL, TUNE_FLOW_ALL, TUNE_DEGENERACY_ORDER = 32, 64, 128, 256, 512
 E_INVALID, E_HIP, E_NOMEM, E_STATE, E_LIMIT, E_PENDING = -1, -2, -3, -4, -5, -6
 
 
@@ -60,7 +60,8 @@ class ListInfo(ctypes.Structure):
     _fields_ = [("n", ctypes.c_uint32), ("arcs", ctypes.c_uint64), ("w2", ctypes.c_uint64),
                 ("max_outdeg", ctypes.c_uint32), ("kernel_ms", ctypes.c_float),
                 ("kernel_launches", ctypes.c_uint32), ("class_roots", ctypes.c_uint32 * 4),
-                ("waves_per_root", ctypes.c_uint32 * 4), ("order_replaced", ctypes.c_uint32)]
+                ("waves_per_root", ctypes.c_uint32 * 4), ("order_replaced", ctypes.c_uint32),
+                ("order_kind", ctypes.c_uint32)]
 
 
 class KcdsError(RuntimeError):

@@ -99,35 +99,58 @@ def test_config2_full(solver):
     assert np.all(pv[g.planted[:60]] >= math.comb(59, 3))
 
 
+def degree_order_stats(off, nbr, k):
+    """Widest out-list and staged out-wedge sum W2 (over roots of out-degree
+    >= k-1) when the vertices are ranked by (degree, id) and every arc points
+    to the later one."""
+    off, nbr = np.asarray(off, np.int64), np.asarray(nbr, np.int64)
+    n = len(off) - 1
+    deg = np.diff(off)
+    rank = np.empty(n, np.int64)
+    rank[np.lexsort((np.arange(n), deg))] = np.arange(n)
+    src = np.repeat(np.arange(n), deg)
+    keep = rank[src] < rank[nbr]
+    a, b = rank[src[keep]], rank[nbr[keep]]
+    dout = np.bincount(a, minlength=n)
+    w2 = np.bincount(a, weights=dout[b].astype(np.float64), minlength=n)
+    return int(dout.max()) if n else 0, int(w2[dout >= k - 1].sum())
+
+
 def test_config2_exact_order_bound(solver):
     """The exact GPU degeneracy order of the C2 graph: its widest out-list is
     the oracle's degeneracy, 373.  Its aggregated ordering steps reach 8,513
     adjacency pieces, up to 3 rounds per workgroup on a 256-CU device (512
     workgroups of 8 pieces per round), so decrement sums that stay in LDS
     across rounds are held to that exact bound.  The slack order stays within
-    kmax + kmax/2; both give C2's clique count."""
+    kmax + kmax/2.  The default degree order is exactly the (degree, id) order:
+    its widest out-list and staged out-wedge sum equal a host computation.
+    All give C2's clique count."""
     g, k = gen.config(2)
     _, _, _, kmax = O.core_decompose(g.offset, g.nbr)
     assert kmax == 373
-    for flags in (kcds.TUNE_EXACT_ORDER, 0):
+    for flags in (kcds.TUNE_EXACT_ORDER, kcds.TUNE_DEGENERACY_ORDER, 0):
         solver.set_tuning(0, flags)
         solver.upload(g.offset, g.nbr)
         solver.orient(None)
         T, _ = solver.count(k, per_vertex=False)
-        mo = solver.list_info().max_outdeg
-        print(f"  C2 flags={flags} max_outdeg={mo} T={T}", flush=True)
+        info = solver.list_info()
+        mo = info.max_outdeg
+        print(f"  C2 flags={flags} order={info.order_kind} max_outdeg={mo} w2={info.w2} T={T}", flush=True)
         assert T == 2281884915
-        if flags:
-            assert mo == kmax
-        else:
-            assert mo <= kmax + kmax // 2
+        if flags == kcds.TUNE_EXACT_ORDER:
+            assert info.order_kind == 2 and mo == kmax
+        elif flags:
+            assert info.order_kind == 1 and mo <= kmax + kmax // 2
+        else:  # the default: (degree, id) order, exactly
+            assert info.order_kind == 0
+            assert (mo, info.w2) == degree_order_stats(g.offset, g.nbr, k)
     solver.set_tuning(0, 0)
 
 
 def test_reference_order_on_c2(solver):
     """kcds_orient with the reference's listing order (src/graph.cpp:41-51) on
     C2: hubs point to their low-core neighbours, out-degree about 25k, wider
-    than the 512-row frames, so the library lists over the degeneracy order
+    than the 512-row frames, so the library lists over its own GPU order
     instead and says so; T and every vertex's count are those of the default
     orientation."""
     g, k = gen.config(2)
@@ -201,8 +224,9 @@ def test_config3_full_counts_match_oracle_golden(solver):
 def test_exact_and_slack_orders(solver):
     """The exact GPU degeneracy order's widest out-list is the degeneracy (the
     first vertex the peel takes from the kmax-core has exactly kmax live
-    neighbours); the slack order stays within kmax + kmax/2.  Both give
-    the oracle's counts.  The 65,536-vertex graph has ordering steps of
+    neighbours); the slack order stays within kmax + kmax/2; the default
+    degree order is the (degree, id) order exactly.  All give the oracle's
+    counts.  The 65,536-vertex graph has ordering steps of
     2,321 (exact) and 4,458 (slack) adjacency pieces (256 arcs each, for
     vertices of degree > 4), enough for the workgroup-aggregated decrements
     (2,048 pieces and up), so the exact bound checks that path."""
@@ -212,19 +236,26 @@ def test_exact_and_slack_orders(solver):
         g = gen.rmat(int(np.ceil(np.log2(n))), ef, cl, seed=seed, n=n)
         _, _, _, kmax = O.core_decompose(g.offset, g.nbr)
         Tr, pvr = O.count(g.offset, g.nbr, 4)
-        for flags in (kcds.TUNE_EXACT_ORDER, 0):
+        for flags in (kcds.TUNE_EXACT_ORDER, kcds.TUNE_DEGENERACY_ORDER, 0):
             solver.set_tuning(0, flags)
             solver.upload(g.offset, g.nbr)
             solver.orient(None)
             T, pv = solver.count(4)
-            mo = solver.list_info().max_outdeg
-            print(f"  n={n} kmax={kmax} flags={flags} max_outdeg={mo} T={T}", flush=True)
+            info = solver.list_info()
+            mo = info.max_outdeg
+            print(f"  n={n} kmax={kmax} flags={flags} order={info.order_kind} max_outdeg={mo} T={T}", flush=True)
             assert T == Tr
             np.testing.assert_array_equal(pv, pvr)
-            if flags:
+            if flags == kcds.TUNE_EXACT_ORDER:
                 assert mo == kmax
-            else:
+            elif flags:
                 assert mo <= kmax + kmax // 2
+            else:
+                dmo, dw2 = degree_order_stats(g.offset, g.nbr, 4)
+                if dmo <= 512:
+                    assert info.order_kind == 0 and (mo, info.w2) == (dmo, dw2)
+                else:
+                    assert info.order_kind in (1, 2) and mo <= 512
     solver.set_tuning(0, 0)
 
 
