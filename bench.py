@@ -3,10 +3,10 @@
 Workload (one step = one pass of the hot path over the graph already resident
 in HBM): config C2 -- R-MAT scale 18 (n = 262,144), edge factor 16,
 (a,b,c,d) = (.57,.19,.19,.05), symmetrised and deduplicated, plus a planted
-60-clique and five 30-cliques; k = 4.  A step is GPU degeneracy ordering +
-DAG build (generateDAG) followed by the wave64 listing pass that counts every
-4-clique and every per-vertex 4-clique count (listAllCliques' T and
-cliqueDegree).  value = T / seconds per step.
+60-clique and five 30-cliques; k = 4.  A step is the GPU listing order (the
+vertices sorted by (degree, id)) + DAG build (generateDAG) followed by the
+wave64 listing pass that counts every 4-clique and every per-vertex 4-clique
+count (listAllCliques' T and cliqueDegree).  value = T / seconds per step.
 
 Also reported on the same JSON line:
   roofline      the listing kernels' algorithmic HBM bytes per pass
@@ -137,7 +137,7 @@ def bench_c2_solve(g, k, kcds):
 
 def bench_c3(gen, kcds, reps=3, solve=True, shape=None):
     """C3 (Chung-Lu n = 1M, ~10M edges, near-K80, k = 5): the same step as the
-    headline (GPU order + DAG + per-vertex counting pass) with its own roofline
+    headline (GPU (degree, id) order + DAG + per-vertex counting pass) with its own roofline
     figures, and one exact solve (enumerating peel, dense exact search: T =
     7.3e10 cliques are never listed)."""
     g, k = gen.config(3)
@@ -369,8 +369,9 @@ def main():
         "metric": "k-cliques listed/s", "value": value, "unit": "cliques/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
-        "config": {"workload": "C2 R-MAT scale 18 ef16 +K60+5xK30, k=4: GPU degeneracy order + DAG + "
+        "config": {"workload": "C2 R-MAT scale 18 ef16 +K60+5xK30, k=4: GPU (degree, id) order + DAG + "
                                "per-vertex 4-clique counts", "n": g.n, "m": g.m, "k": k, "T": T,
+                   "order": ["degree", "slack degeneracy", "exact degeneracy", "caller"][info.order_kind],
                    "dag_arcs": arcs, "w2": w2, "max_outdeg": info.max_outdeg,
                    "class_roots": list(info.class_roots), "waves_per_root": list(info.waves_per_root),
                    "autotune_class_ms": tuned[1] if tuned else None},
