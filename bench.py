@@ -215,7 +215,7 @@ def main():
                     help="graphs of C4 solved one per call for the plain-form rate")
     ap.add_argument("--c5-window", type=float, default=1.0,
                     help="seconds of the C5 stream served (0 skips the serving leg)")
-    ap.add_argument("--c5-graph-budget", type=int, default=256 << 20,
+    ap.add_argument("--c5-graph-budget", type=int, default=2 << 30,
                     help="clique-list bytes per graph in a serving batch (larger graphs take the large lane)")
     ap.add_argument("--c5-max-cliques", type=int, default=1 << 28,
                     help="graphs with more cliques are answered over budget")
