@@ -44,7 +44,7 @@ namespace {
 constexpr uint32_t ESEG = 4;                    // frontier segments / counter shards
 constexpr uint32_t ED = 512;                    // frame rows (the DAG out-degree bound)
 constexpr uint32_t EHB = 11, EH = 1u << EHB;    // frame table slots (load <= 1/4)
-constexpr uint32_t ETHREADS = 256;
+constexpr uint32_t ETHREADS = 512;
 constexpr uint32_t EPIECE = 16;                 // candidates per wide work item
 constexpr uint32_t ECHUNK_LOG = 16;             // a narrow frame takes a work item per 2^16 of its cliques
 constexpr uint32_t EMAXP = 64;                  //   up to 64 (k >= 4)
@@ -1031,8 +1031,9 @@ void epeel_run(Ctx &c) {
   peel_graphs_begin(c, gp);
   EpStepFn step = ep_step_fn(k);
   if (!step) fail(KCDS_E_INVALID, "k = %u outside 2..8", k);
-  // two workgroups per CU (a 55 KB frame each)
-  const uint32_t sgrid = (uint32_t)c.num_cus * 2;
+  // one workgroup of ETHREADS threads per CU (a 55 KB frame; its 150-250
+  // VGPRs per thread leave room for one 512-thread workgroup per CU)
+  const uint32_t sgrid = (uint32_t)c.num_cus * (ETHREADS >= 512 ? 1u : 2u);
   uint32_t batch = c.peel_steps ? (uint32_t)std::min<uint64_t>(c.peel_steps, 1u << 20) : 32;
   uint64_t rounds = 0;
   const bool log = getenv("KCDS_LOG") != nullptr;  // progress lines on stderr
