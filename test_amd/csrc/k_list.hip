@@ -525,8 +525,289 @@ __global__ __launch_bounds__(64) void k_list_emit(const uint32_t *__restrict__ d
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wide roots: out-degree in (512, LIST_WIDE_MAX].  The reference sizes its
+// neighbour masks from the widest out-list (src/gpuMemoryAllocation.cu:58),
+// so any out-degree lists there.  A root too wide for an LDS frame keeps its
+// bitmask rows in HBM instead; one workgroup of WIDE_THREADS takes one root
+// at a time:
+//   - L = N+(r) and an id -> index table (load <= 1/2) in LDS;
+//   - rows G[j*Wd + w] (Wd = ceil(d/64) words, row-major) in the
+//     workgroup's own slice of HBM, built with global atomics from the
+//     members' out-lists (bit j of row i and bit i of row j per hit);
+//   - candidate sets are spread over the lanes of a wave (lane w holds word
+//     w; Wd <= 64), so S & row(c) is one coalesced row read and a set's size
+//     one DPP sum;
+//   - per-member counts in LDS, flushed with one atomic per member.
+// Only graphs whose orientation leaves out-lists wider than 512 reach this
+// path (a degeneracy above 512), so it favours simplicity over speed.
+constexpr uint32_t WIDE_THREADS = 1024;
+constexpr uint32_t WIDE_HB = 13, WIDE_H = 1u << WIDE_HB;  // 8,192 table slots for up to 4,096 members
+
+struct WideFrame {
+  ull hent[WIDE_H];          // low word member id (EMPTY_KEY when free), high word its index
+  ull cnt[LIST_WIDE_MAX];    // cliques of the current root containing member j
+  uint32_t L[LIST_WIDE_MAX];  // N+(r), ascending
+  ull rt;
+  uint32_t root, skip;
+};
+
+__device__ __forceinline__ uint32_t whslot(uint32_t key) {
+  return (uint32_t)__umul24(key ^ (key >> 24), 0x9E3779u) >> (32 - WIDE_HB);
+}
+
+// Workgroup barrier that also orders this workgroup's global stores and
+// atomics (the rows) with its later global loads: release, barrier, acquire
+// at agent scope (the acquire drops stale L1 lines).
+__device__ __forceinline__ void wide_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Builds the rows of the root whose out-list is dnbr[base, base + d) in G.
+// Returns this thread's share of the staged out-wedge sum.
+__device__ uint32_t wide_build(WideFrame &f, ull *G, const uint32_t *__restrict__ doff,
+                               const uint32_t *__restrict__ dnbr, uint32_t base, uint32_t d) {
+  const uint32_t tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, nw = nt >> 6;
+  const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint32_t Wd = (d + 63) >> 6;
+  for (uint32_t t = tid; t < WIDE_H; t += nt) f.hent[t] = ~0ull;
+  for (uint32_t t = tid; t < d; t += nt) f.cnt[t] = 0;
+  for (uint64_t e = tid; e < (uint64_t)d * Wd; e += nt) G[e] = 0;
+  wide_sync();
+  uint32_t *hk = reinterpret_cast<uint32_t *>(f.hent);
+  for (uint32_t t = tid; t < d; t += nt) {
+    const uint32_t u = dnbr[base + t];
+    f.L[t] = u;
+    uint32_t h = whslot(u);
+    while (atomicCAS(&hk[2 * h], EMPTY_KEY, u) != EMPTY_KEY) h = (h + 1) & (WIDE_H - 1);
+    hk[2 * h + 1] = t;
+  }
+  wide_sync();
+  uint32_t raw = 0;
+  const uint32_t last = f.L[d - 1];
+  for (uint32_t i = wv; i < d; i += nw) {
+    const uint32_t u = f.L[i];
+    const uint32_t ou = doff[u], du = doff[u + 1] - ou;
+    if (lane == 0) raw += du;
+    if (i + 1 == d) continue;  // the last member has no later member
+    for (uint32_t c0 = 0; c0 < du; c0 += 64) {
+      const uint32_t t = c0 + lane;
+      const uint32_t w = t < du ? dnbr[ou + t] : 0xFFFFFFFFu;
+      if (w > last) continue;
+      uint32_t h = whslot(w);
+      ull e = f.hent[h];
+      while ((uint32_t)e != w && (uint32_t)e != EMPTY_KEY) {
+        h = (h + 1) & (WIDE_H - 1);
+        e = f.hent[h];
+      }
+      if ((uint32_t)e == w) {
+        const uint32_t j = (uint32_t)(e >> 32);  // j > i: ids ascend along arcs and L is ascending
+        atomicOr(&G[(uint64_t)i * Wd + (j >> 6)], 1ull << (j & 63));
+        atomicOr(&G[(uint64_t)j * Wd + (i >> 6)], 1ull << (i & 63));
+      }
+    }
+  }
+  wide_sync();
+  return raw;
+}
+
+// Word `lane` of row j (0 past the row's Wd words).
+__device__ __forceinline__ ull wide_word(const ull *G, uint32_t Wd, uint32_t j, uint32_t lane) {
+  return lane < Wd ? G[(uint64_t)j * Wd + lane] : 0ull;
+}
+
+// Lane-distributed bits above vertex c = 64q + b.
+__device__ __forceinline__ ull wide_above(uint32_t lane, uint32_t q, uint32_t b) {
+  return lane < q ? 0ull : lane > q ? ~0ull : (b == 63 ? 0ull : (~0ull << (b + 1)));
+}
+
+__device__ __forceinline__ ull wave_word(ull S, uint32_t q) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)S, q), hi = __builtin_amdgcn_readlane((uint32_t)(S >> 32), q);
+  return ((ull)hi << 32) | lo;
+}
+
+// REM-cliques inside the lane-distributed set S, every member credited in
+// f.cnt; returns the count (wave-uniform).  The whole wave is active.
+template <int REM> struct WCount {
+  static __device__ ull run(WideFrame &f, const ull *G, uint32_t Wd, ull S) {
+    const uint32_t lane = lane_id();
+    ull total = 0;
+    ull nz = __ballot(S != 0ull);
+    while (nz) {
+      const uint32_t q = (uint32_t)__builtin_ctzll(nz);
+      nz &= nz - 1;
+      ull word = wave_word(S, q);
+      while (word) {
+        const uint32_t b = (uint32_t)__builtin_ctzll(word);
+        word &= word - 1;
+        const uint32_t c = q * 64 + b;
+        const ull row = wide_word(G, Wd, c, lane);
+        if constexpr (REM == 2) {
+          const uint32_t x = wave_sum_u32_full((uint32_t)__popcll(row & S));
+          if (lane == 0 && x) atomicAdd(&f.cnt[c], (ull)x);
+          total += x;
+        } else {
+          const ull S2 = S & row & wide_above(lane, q, b);
+          const uint32_t pc = wave_sum_u32_full((uint32_t)__popcll(S2));
+          if (pc < (uint32_t)(REM - 1)) continue;
+          const ull E = WCount<REM - 1>::run(f, G, Wd, S2);
+          if (lane == 0 && E) atomicAdd(&f.cnt[c], E);
+          total += E;
+        }
+      }
+    }
+    return REM == 2 ? total / 2 : total;
+  }
+};
+
+template <int K>
+__global__ __launch_bounds__(WIDE_THREADS) void k_list_count_wide(
+    const uint32_t *__restrict__ doff, const uint32_t *__restrict__ dnbr, const uint4 *__restrict__ roots,
+    uint32_t nroots, uint32_t *queue, ull *gcnt, uint64_t cstride, uint32_t ncopy, ull *rootT, ull *gtotal,
+    const uint32_t *gid, ull *gsum, ull cap, ull *rowbuf, uint64_t rowstride) {
+  __shared__ WideFrame f;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, nw = blockDim.x >> 6;
+  const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  ull *G = rowbuf + (uint64_t)blockIdx.x * rowstride;
+  gcnt += (uint64_t)(blockIdx.x % ncopy) * cstride;
+  ull tot = 0, w2 = 0;
+  QueueCursor qc = queue_start(blockIdx.x);
+  uint32_t pre = tid == 0 ? queue_next(queue, nroots, qc) : 0u;
+  while (true) {
+    if (tid == 0) {
+      f.root = pre;
+      f.rt = 0;
+      f.skip = cap && pre < nroots && ldc_u64(&gsum[gid ? gid[roots[pre].x] : 0u]) > cap;
+    }
+    lds_sync();
+    const uint32_t idx = f.root, skip = f.skip;
+    lds_sync();
+    if (idx >= nroots) break;
+    if (tid == 0) pre = queue_next(queue, nroots, qc);
+    if (skip) continue;
+    const uint4 rec = roots[idx];
+    const uint32_t r = rec.x, base = rec.y, d = rec.z, Wd = (d + 63) >> 6;
+    w2 += wide_build(f, G, doff, dnbr, base, d);
+    ull rt = 0;
+    if constexpr (K == 3) {
+      const ull Sall = lane < Wd ? (lane + 1 < Wd || (d & 63) == 0 ? ~0ull : ((1ull << (d & 63)) - 1ull)) : 0ull;
+      for (uint32_t j = wv; j < d; j += nw) {
+        const uint32_t x = wave_sum_u32_full((uint32_t)__popcll(wide_word(G, Wd, j, lane) & Sall));
+        if (lane == 0 && x) atomicAdd(&f.cnt[j], (ull)x);
+        rt += x;  // twice the wave's share of edges
+      }
+    } else {
+      for (uint32_t c = wv; c < d; c += nw) {
+        const ull S = wide_word(G, Wd, c, lane) & wide_above(lane, c >> 6, c & 63);
+        const uint32_t pc = wave_sum_u32_full((uint32_t)__popcll(S));
+        if (pc < (uint32_t)(K - 2)) continue;
+        const ull E = WCount<K - 2>::run(f, G, Wd, S);
+        if (lane == 0 && E) atomicAdd(&f.cnt[c], E);
+        rt += E;
+      }
+    }
+    if (lane == 0 && rt) atomicAdd(&f.rt, rt);
+    lds_sync();
+    const ull all = K == 3 ? (f.rt >> 1) : f.rt;
+    for (uint32_t t = tid; t < d; t += blockDim.x)
+      if (f.cnt[t]) atomicAdd(&gcnt[f.L[t]], f.cnt[t]);
+    if (tid == 0) {
+      if (all) atomicAdd(&gcnt[r], all);
+      rootT[r] = all;
+      if (cap && all) atomicAdd(&gsum[gid ? gid[r] : 0u], all);
+      tot += all;
+    }
+    lds_sync();  // the counts are read before the next root clears them
+  }
+  ull *mine = gtotal + (blockIdx.x % QSHARDS) * 8;
+  w2 = wave_sum_u64(w2);
+  if (lane == 0 && w2) atomicAdd(mine + 1, w2);
+  if (tid == 0 && tot) atomicAdd(mine, tot);
+}
+
+// Materialises the cliques of wide roots: the workgroup builds the rows and
+// wave 0 writes the root's cliques in a fixed order (REM-cliques of the
+// lane-distributed set S below the prefix pfx) from rootOff[r] on.
+template <int K, int REM> struct WEmit {
+  static __device__ void run(const WideFrame &f, const ull *G, uint32_t Wd, ull S, uint32_t *pfx, uint32_t *out,
+                             ull &pos, ull lim) {
+    const uint32_t lane = lane_id();
+    ull nz = __ballot(S != 0ull);
+    while (nz) {
+      const uint32_t q = (uint32_t)__builtin_ctzll(nz);
+      nz &= nz - 1;
+      ull word = wave_word(S, q);
+      while (word) {
+        const uint32_t b = (uint32_t)__builtin_ctzll(word);
+        word &= word - 1;
+        const uint32_t c = q * 64 + b;
+        const ull H = S & wide_word(G, Wd, c, lane) & wide_above(lane, q, b);
+        if constexpr (REM == 2) {
+          const uint32_t x = (uint32_t)__popcll(H);
+          const uint32_t incl = wave_incl_scan_u32(x);
+          const uint32_t tot = __shfl(incl, 63, 64);
+          ull at = pos + (incl - x);
+          ull h = H;
+          while (h) {
+            const uint32_t bb = (uint32_t)__builtin_ctzll(h);
+            h &= h - 1;
+            if (at >= lim) break;  // never taken when count and emit agree
+            uint32_t *rw = out + at * K;
+#pragma unroll
+            for (int t = 0; t < K - 2; ++t) rw[t] = pfx[t];
+            rw[K - 2] = f.L[c];
+            rw[K - 1] = f.L[lane * 64 + bb];
+            ++at;
+          }
+          pos += tot;
+        } else {
+          const uint32_t pc = wave_sum_u32_full((uint32_t)__popcll(H));
+          if (pc < (uint32_t)(REM - 1)) continue;
+          pfx[K - REM] = f.L[c];
+          WEmit<K, REM - 1>::run(f, G, Wd, H, pfx, out, pos, lim);
+        }
+      }
+    }
+  }
+};
+
+template <int K>
+__global__ __launch_bounds__(WIDE_THREADS) void k_list_emit_wide(
+    const uint32_t *__restrict__ doff, const uint32_t *__restrict__ dnbr, const uint4 *__restrict__ roots,
+    uint32_t nroots, uint32_t *queue, const ull *rootOff, uint32_t *out, ull *rowbuf, uint64_t rowstride) {
+  __shared__ WideFrame f;
+  const uint32_t tid = threadIdx.x, lane = tid & 63;
+  ull *G = rowbuf + (uint64_t)blockIdx.x * rowstride;
+  QueueCursor qc = queue_start(blockIdx.x);
+  uint32_t pre = tid == 0 ? queue_next(queue, nroots, qc) : 0u;
+  while (true) {
+    if (tid == 0) f.root = pre;
+    lds_sync();
+    const uint32_t idx = f.root;
+    lds_sync();
+    if (idx >= nroots) break;
+    if (tid == 0) pre = queue_next(queue, nroots, qc);
+    const uint4 rec = roots[idx];
+    const uint32_t r = rec.x, base = rec.y, d = rec.z, Wd = (d + 63) >> 6;
+    ull pos = rootOff[r];
+    const ull lim = rootOff[r + 1];
+    if (lim == pos) continue;
+    wide_build(f, G, doff, dnbr, base, d);
+    if (tid < 64) {  // wave 0: a fixed order
+      const ull Sall = lane < Wd ? (lane + 1 < Wd || (d & 63) == 0 ? ~0ull : ((1ull << (d & 63)) - 1ull)) : 0ull;
+      uint32_t pfx[K];
+      pfx[0] = r;
+      WEmit<K, K - 1>::run(f, G, Wd, Sall, pfx, out, pos, lim);
+    }
+    lds_sync();  // wave 0 is done with the rows before the next root rebuilds them
+  }
+}
+
 // Root classification: class c holds roots with 64*2^(c-1) < outdeg <= 64*2^c
-// and outdeg >= k-1 (smaller roots have no k-clique); class 4 = over the limit.
+// and outdeg >= k-1 (smaller roots have no k-clique); class 4 = the wide
+// roots (rows in HBM).
 __device__ __forceinline__ int width_class(uint32_t d) {
   return d <= 64 ? 0 : d <= 128 ? 1 : d <= 256 ? 2 : d <= 512 ? 3 : 4;
 }
@@ -556,9 +837,11 @@ __global__ __launch_bounds__(256) void k_class_count(const uint32_t *doff, uint3
 // Writes each root into its class's range (classes in order, each starting
 // where the counts of the earlier ones end): one global atomic per class per
 // 256 roots.
+constexpr int NCLASS_ALL = NCLASS + 1;  // the LDS classes, then the wide roots
+
 __global__ __launch_bounds__(256) void k_class_fill(const uint32_t *doff, uint32_t n, uint32_t k,
                                                     const uint32_t *ccount, uint32_t *cfill, uint4 *roots) {
-  __shared__ uint32_t wc[4][NCLASS], bbase[NCLASS];
+  __shared__ uint32_t wc[4][NCLASS_ALL], bbase[NCLASS_ALL];
   const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
   for (uint32_t r0 = blockIdx.x * 256; r0 < n; r0 += gridDim.x * 256) {
     const uint32_t r = r0 + threadIdx.x;
@@ -569,14 +852,14 @@ __global__ __launch_bounds__(256) void k_class_fill(const uint32_t *doff, uint32
       d = doff[r + 1] - b;
       if (d + 1 >= k) c = width_class(d);
     }
-    ull bal[NCLASS];
+    ull bal[NCLASS_ALL];
 #pragma unroll
-    for (int i = 0; i < NCLASS; ++i) {
+    for (int i = 0; i < NCLASS_ALL; ++i) {
       bal[i] = __ballot(c == i);
       if (lane == 0) wc[wv][i] = (uint32_t)__popcll(bal[i]);
     }
     __syncthreads();
-    if (threadIdx.x < NCLASS) {
+    if (threadIdx.x < NCLASS_ALL) {
       const uint32_t i = threadIdx.x;
       const uint32_t tot = wc[0][i] + wc[1][i] + wc[2][i] + wc[3][i];
       uint32_t start = 0;
@@ -584,7 +867,7 @@ __global__ __launch_bounds__(256) void k_class_fill(const uint32_t *doff, uint32
       bbase[i] = tot ? start + atomicAdd(&cfill[i], tot) : 0u;
     }
     __syncthreads();
-    if (c >= 0 && c < NCLASS) {
+    if (c >= 0 && c < NCLASS_ALL) {
       uint32_t at = bbase[c] + lanes_below(bal[c]);
       for (uint32_t v = 0; v < wv; ++v) at += wc[v][c];
       roots[at] = make_uint4(r, b, d, 0u);
@@ -719,10 +1002,51 @@ uint32_t wg_grid(const Ctx &c, int cls, uint32_t nroots, uint32_t nw) {
 // sharded queue per class for counting, one per class for materialising and
 // one for the autotuner.
 constexpr uint32_t QBASE = 256;
-constexpr uint32_t QCTR_WORDS = QBASE + (2 * NCLASS + 1) * QUEUE_WORDS;
+constexpr uint32_t QCTR_WORDS = QBASE + (2 * NCLASS + 3) * QUEUE_WORDS;
 uint32_t *count_queue(Ctx &c, int cls) { return c.qctr.as<uint32_t>() + QBASE + cls * QUEUE_WORDS; }
 uint32_t *emit_queue(Ctx &c, int cls) { return c.qctr.as<uint32_t>() + QBASE + (NCLASS + cls) * QUEUE_WORDS; }
 uint32_t *tune_queue(Ctx &c) { return c.qctr.as<uint32_t>() + QBASE + 2 * NCLASS * QUEUE_WORDS; }
+uint32_t *wide_count_queue(Ctx &c) { return c.qctr.as<uint32_t>() + QBASE + (2 * NCLASS + 1) * QUEUE_WORDS; }
+uint32_t *wide_emit_queue(Ctx &c) { return c.qctr.as<uint32_t>() + QBASE + (2 * NCLASS + 2) * QUEUE_WORDS; }
+
+typedef void (*WideCountFn)(const uint32_t *, const uint32_t *, const uint4 *, uint32_t, uint32_t *, ull *, uint64_t,
+                            uint32_t, ull *, ull *, const uint32_t *, ull *, ull, ull *, uint64_t);
+typedef void (*WideEmitFn)(const uint32_t *, const uint32_t *, const uint4 *, uint32_t, uint32_t *, const ull *,
+                           uint32_t *, ull *, uint64_t);
+
+WideCountFn wide_count_fn(uint32_t k) {
+  switch (k) {
+    case 3: return k_list_count_wide<3>;
+    case 4: return k_list_count_wide<4>;
+    case 5: return k_list_count_wide<5>;
+    case 6: return k_list_count_wide<6>;
+    case 7: return k_list_count_wide<7>;
+    case 8: return k_list_count_wide<8>;
+  }
+  return nullptr;
+}
+
+WideEmitFn wide_emit_fn(uint32_t k) {
+  switch (k) {
+    case 3: return k_list_emit_wide<3>;
+    case 4: return k_list_emit_wide<4>;
+    case 5: return k_list_emit_wide<5>;
+    case 6: return k_list_emit_wide<6>;
+    case 7: return k_list_emit_wide<7>;
+    case 8: return k_list_emit_wide<8>;
+  }
+  return nullptr;
+}
+
+// Workgroups for the wide roots and the HBM rows they need: one slice of
+// dmax x ceil(dmax/64) words per workgroup.
+uint32_t wide_grid(Ctx &c, uint64_t &rowstride) {
+  const uint32_t nr = c.class_count[NCLASS];
+  rowstride = (uint64_t)c.dmax * ((c.dmax + 63) / 64);
+  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(nr, (uint32_t)c.num_cus));
+  buf_reserve(c, c.wide, sizeof(ull) * (rowstride * grid + 64));
+  return grid;
+}
 
 // Copies of the per-vertex counters (8 while they fit comfortably).
 uint32_t count_copies(uint32_t n) { return n <= (1u << 26) ? 8u : 1u; }
@@ -743,9 +1067,10 @@ void classify_roots(Ctx &c, uint32_t k) {
   uint32_t *h = (uint32_t *)c.host_scratch(64);
   HIP_CHECK(hipMemcpyAsync(h, ccount, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
   c.sync();
-  if (h[4]) fail(KCDS_E_LIMIT, "%u roots have out-degree above %u (the LDS row limit)", h[4], MAX_LDS_DEG);
+  if (h[4] && c.dmax > LIST_WIDE_MAX)
+    fail(KCDS_E_LIMIT, "out-degree %u above %u (the widest out-list listed)", c.dmax, LIST_WIDE_MAX);
   uint32_t s = 0;
-  for (int i = 0; i < NCLASS; ++i) {
+  for (int i = 0; i < NCLASS_ALL; ++i) {
     c.class_count[i] = h[i];
     c.class_start[i] = s;
     s += h[i];
@@ -796,8 +1121,19 @@ uint32_t run_classes(Ctx &c, uint32_t k, const uint32_t nw[NCLASS], ull *gcnt, u
       HIP_CHECK(hipEventCreateWithFlags(&c.ljoin[i], hipEventDisableTiming));
     }
   }
-  HIP_CHECK(hipEventRecord(c.lfork, c.stream));
   uint32_t launches = 0;
+  if (c.class_count[NCLASS]) {  // the wide roots first, on the context stream (their rows buffer is reserved here)
+    uint64_t rowstride = 0;
+    const uint32_t grid = wide_grid(c, rowstride);
+    const uint32_t *gid = c.ngraphs > 1 ? c.gid.as<uint32_t>() : nullptr;
+    hipLaunchKernelGGL(wide_count_fn(k), dim3(grid), dim3(WIDE_THREADS), 0, c.stream, c.doff.as<uint32_t>(),
+                       c.dnbr.as<uint32_t>(), c.roots.as<uint4>() + c.class_start[NCLASS], c.class_count[NCLASS],
+                       wide_count_queue(c), gcnt, copy_stride(c.n), count_copies(c.n), rootT, totals, gid,
+                       cap ? c.gsum.as<ull>() : nullptr, cap, c.wide.as<ull>(), rowstride);
+    KCDS_LAUNCH_CHECK();
+    ++launches;
+  }
+  HIP_CHECK(hipEventRecord(c.lfork, c.stream));
   for (int cls = NCLASS - 1; cls >= 0; --cls) {
     if (!c.class_count[cls]) continue;
     hipStream_t st = cls == NCLASS - 1 ? c.stream : c.lstream[cls];
@@ -1070,6 +1406,15 @@ void list_materialize(Ctx &c, uint32_t k) {
     KCDS_LAUNCH_CHECK();
   }
   if (k > 2) HIP_CHECK(hipMemsetAsync(emit_queue(c, 0), 0, NCLASS * QUEUE_WORDS * sizeof(uint32_t), c.stream));
+  if (k > 2 && c.class_count[NCLASS]) {
+    HIP_CHECK(hipMemsetAsync(wide_emit_queue(c), 0, QUEUE_WORDS * sizeof(uint32_t), c.stream));
+    uint64_t rowstride = 0;
+    const uint32_t grid = wide_grid(c, rowstride);
+    hipLaunchKernelGGL(wide_emit_fn(k), dim3(grid), dim3(WIDE_THREADS), 0, c.stream, c.doff.as<uint32_t>(),
+                       c.dnbr.as<uint32_t>(), c.roots.as<uint4>() + c.class_start[NCLASS], c.class_count[NCLASS],
+                       wide_emit_queue(c), rootOff, c.cliques.as<uint32_t>(), c.wide.as<ull>(), rowstride);
+    KCDS_LAUNCH_CHECK();
+  }
   for (int cls = NCLASS - 1; cls >= 0 && k > 2; --cls) {
     const uint32_t nr = c.class_count[cls];
     if (!nr) continue;

@@ -52,7 +52,8 @@ __device__ __forceinline__ uint32_t rank_in(const uint32_t *a, uint32_t cnt, uin
   return p0 + p1;
 }
 
-constexpr uint32_t HEAVY_DEG = 2048;   // vertices of higher degree are gathered by a whole workgroup
+constexpr uint32_t HEAVY_DEG = 1024;   // vertices of higher degree are gathered by a whole workgroup
+constexpr uint32_t HEAVY_SORT = LIST_WIDE_MAX;  // longest out-list the workgroup gather sorts
 constexpr uint32_t HEAVY_THREADS = 1024;
 constexpr uint32_t SMALL_DEG = 32;     // vertices of degree <= SMALL_DEG are gathered 64 to a wave
 constexpr uint32_t TILE_CAP = 1024;    // arcs a tile pass holds in LDS (>= SMALL_DEG)
@@ -208,7 +209,8 @@ constexpr uint32_t MUNROLL = 8;  // 64-arc chunks in flight per mid vertex
 // block, 4 KB of LDS each): gathers the new ids of its later neighbours with
 // MUNROLL chunks of loads in flight, sorts them (wave_rank_sort) and writes
 // them to tmp[off[v] ..); dout[rank[v]] = count.  A list longer than
-// SORT_WAVE is counted but not written (the caller fails).
+// SORT_WAVE is counted but not written (it cannot happen: a mid vertex's
+// degree is at most HEAVY_DEG = SORT_WAVE).
 __global__ __launch_bounds__(256) void k_dag_gather(const uint32_t *__restrict__ off,
                                                     const uint32_t *__restrict__ nbr,
                                                     const uint32_t *__restrict__ rank,
@@ -253,12 +255,14 @@ __global__ __launch_bounds__(256) void k_dag_gather(const uint32_t *__restrict__
 }
 
 // Workgroup per heavy vertex: the same gather with 16 waves sharing one LDS
-// list (slots taken with one LDS atomic per wave and chunk).
+// list (slots taken with one LDS atomic per wave and chunk).  Out-lists of up
+// to HEAVY_SORT ids are sorted and written; a longer one is counted but not
+// written (the caller fails).
 __global__ __launch_bounds__(HEAVY_THREADS) void k_dag_gather_heavy(
     const uint32_t *__restrict__ off, const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ rank,
     const uint32_t *__restrict__ heavy, const uint32_t *nheavy, uint32_t *__restrict__ dout,
     uint32_t *__restrict__ tmp) {
-  __shared__ __align__(16) uint32_t buf[SORT_WAVE + 4];
+  __shared__ __align__(16) uint32_t buf[HEAVY_SORT + 4];
   __shared__ uint32_t cnt;
   const uint32_t tid = threadIdx.x, lane = lane_id();
   const uint32_t nh = *nheavy;
@@ -277,15 +281,15 @@ __global__ __launch_bounds__(HEAVY_THREADS) void k_dag_gather_heavy(
         uint32_t at = 0;
         if (lane == lead) at = atomicAdd(&cnt, (uint32_t)__popcll(bal));
         at = __shfl(at, lead, 64) + lanes_below(bal);
-        if (keep && at < SORT_WAVE) buf[at] = x;
+        if (keep && at < HEAVY_SORT) buf[at] = x;
       }
     }
     block_sync();
     const uint32_t c = cnt;
     if (tid == 0) dout[r] = c;
-    if (tid < 4 && c + tid < SORT_WAVE) buf[c + tid] = 0xFFFFFFFFu;  // pad to whole 4-entry reads
+    if (tid < 4 && c + tid < HEAVY_SORT) buf[c + tid] = 0xFFFFFFFFu;  // pad to whole 4-entry reads
     block_sync();
-    if (c <= SORT_WAVE) {
+    if (c <= HEAVY_SORT) {
       for (uint32_t i = tid; i < c; i += HEAVY_THREADS) tmp[b + rank_in(buf, c, buf[i])] = buf[i];
     }
     block_sync();
@@ -879,8 +883,8 @@ bool orient_from_rank(Ctx &c, bool approx) {
   c.darcs = h[0];
   c.dmax = h[1];
   if (approx && c.dmax > LIST_MAX_OUTDEG) return false;  // the caller orders exactly instead
-  if (c.dmax > SORT_WAVE)
-    fail(KCDS_E_LIMIT, "DAG out-degree %u exceeds %u (orientation not degeneracy-bounded)", c.dmax, SORT_WAVE);
+  if (c.dmax > LIST_WIDE_MAX)
+    fail(KCDS_E_LIMIT, "DAG out-degree %u exceeds %u (orientation not degeneracy-bounded)", c.dmax, LIST_WIDE_MAX);
   buf_reserve(c, c.dnbr, sizeof(uint32_t) * (c.darcs + 1));
   if (n) {
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 15) / 16, (uint64_t)c.num_cus * 40);

@@ -192,6 +192,7 @@ struct Ctx {
 
   // generic temporaries
   DevBuf tmp0, tmp1, tmp2, tmp3;
+  DevBuf wide;              // HBM bitmask rows of the wide roots (one slice per workgroup)
   DevBuf scan_ws;  // block sums of the multi-level scans
 
   // last listing kernel timing
@@ -307,6 +308,7 @@ void grid_reserve(Ctx &c, uint32_t nc, uint64_t Tc);
 constexpr uint64_t GRID_FLOW_MIN_T = 3u << 14, GRID_FLOW_ALONE_T = 1u << 12;
 constexpr uint32_t NO_SLACK = 32, ORDER_SLACK_SHIFT = 1;  // slack order: thresholds x + x/2
 constexpr uint32_t LIST_MAX_OUTDEG = 512;                  // widest out-list the listing stages in LDS
+constexpr uint32_t LIST_WIDE_MAX = 4096;                   // widest out-list at all (wider roots: rows in HBM)
 // All tasks: large components one after another on the whole GPU, the rest in
 // one workgroup-per-component launch.
 void dinkelbach_components(Ctx &c, const std::vector<BlockTask> &in, std::vector<BlockResult> &out,

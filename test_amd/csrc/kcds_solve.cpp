@@ -244,6 +244,14 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
     // the materialising peel keeps int32 clique statuses (destroy levels, at
     // most a vertex's count): larger counts take the enumerating peel
     if (!enumerate && c.T > 0x7FFFFFFFull && max_u64_host(c, c.cnt.as<ull>(), c.n) > 0x7FFFFFFFull) enumerate = true;
+    // the enumerating peel stages a vertex's live neighbours in 512-row
+    // frames: a DAG with wider out-lists peels over the clique list
+    if (enumerate && c.dmax > LIST_MAX_OUTDEG) {
+      if (c.T >= 0xFFFFFFFFull || (double)c.T * (8.0 * k + 4.0) > 0.85 * (double)avail)
+        fail(KCDS_E_LIMIT, "out-lists up to %u (above %u) and %llu %u-cliques: too many to list for the peel",
+             c.dmax, LIST_MAX_OUTDEG, (unsigned long long)c.T, k);
+      enumerate = false;
+    }
     if (c.T && !enumerate) list_materialize(c, k);
     R.ms_list = t.stop();
   }
