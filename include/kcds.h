@@ -89,8 +89,10 @@ int kcds_upload_csr(kcds_ctx *ctx, uint32_t n, const uint32_t *offset, const uin
  * order that leaves an out-list longer than 512 (the LDS frame rows; the
  * reference's order does on hub-heavy graphs, where hubs point to their
  * low-core neighbours) is replaced by the GPU order, which
- * kcds_get_list_info reports.  KCDS_E_LIMIT when even that order leaves an
- * out-list longer than 512 (degeneracy above 512). */
+ * kcds_get_list_info reports.  When no GPU order fits 512 (degeneracy above
+ * 512) the exact degeneracy order is used and its wider roots list with their
+ * bitmask rows in HBM; KCDS_E_LIMIT when an out-list exceeds 4,096
+ * (degeneracy above 4,096). */
 int kcds_orient(kcds_ctx *ctx, const uint32_t *listing_order);
 
 /* Counts k-cliques: *T and, if per_vertex != NULL, the number of k-cliques
