@@ -61,8 +61,9 @@ typedef struct kcds_result {
   uint32_t flow_calls;   /* max-flow solves */
   kcds_peel_stats peel;
   float ms_orient, ms_list, ms_peel, ms_components, ms_flow, ms_total; /* stage times (hipEvents) */
-  uint32_t dense_passes; /* greedy passes of the dense exact search (0: clique flow networks were used) */
-  uint32_t pad;
+  uint32_t dense_passes;   /* greedy passes of the dense exact search (0: clique flow networks were used) */
+  uint32_t dense_critical; /* critical vertices its certificate ended with (members of S given first claim
+                              on the shares of their cliques) */
 } kcds_result;
 
 /* Creates a context on `device` with an HBM budget (0 = 90% of free memory). */

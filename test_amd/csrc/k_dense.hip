@@ -62,6 +62,13 @@ __device__ __forceinline__ ull dabove(int q, uint32_t c) {
   return cb == 63 ? 0ull : (~0ull << (cb + 1));
 }
 
+__device__ __forceinline__ ull dbelow(int q, uint32_t c) {
+  const int cq = (int)(c >> 6), cb = (int)(c & 63);
+  if (q < cq) return ~0ull;
+  if (q > cq) return 0ull;
+  return cb == 0 ? 0ull : ((1ull << cb) - 1ull);
+}
+
 // Rows in LDS, word-major: A[w * 64W + j] holds bits 64w..64w+63 of row j.
 template <int W> __device__ __forceinline__ ull drow(const ull *A, uint32_t j, int w) {
   return A[(uint32_t)w * (64u * W) + j];
@@ -76,10 +83,10 @@ template <int W, bool WP> __device__ __forceinline__ void dcredit(ull *cr, uint3
   }
 }
 
-// REM-cliques inside S (one wave); with WP every member is credited.  Every
-// row word of a step is requested before the first is used (one LDS wait per
-// step), and candidate sets live in scalar registers.
-template <int W, int REM, bool WP> struct DCount {
+// REM-cliques inside S (one wave); with WP every member is credited U per
+// clique.  Every row word of a step is requested before the first is used
+// (one LDS wait per step), and candidate sets live in scalar registers.
+template <int W, int REM, bool WP, ull U = 1> struct DCount {
   static __device__ ull run(const ull *A, const ull *S, ull *cr) {
     ull total = 0;
 #pragma unroll
@@ -99,8 +106,8 @@ template <int W, int REM, bool WP> struct DCount {
           pc += (uint32_t)__popcll(S2[w]);
         }
         if (pc < (uint32_t)(REM - 1)) continue;
-        const ull E = DCount<W, REM - 1, WP>::run(A, S2, cr);
-        dcredit<W, WP>(cr, c, E);
+        const ull E = DCount<W, REM - 1, WP, U>::run(A, S2, cr);
+        dcredit<W, WP>(cr, c, E * U);
         total += E;
       }
     }
@@ -123,7 +130,7 @@ __device__ __forceinline__ uint32_t drow_popc(const ull *A, uint32_t j, const ul
 // Edges inside S.  When the lower half of S is empty (candidates only go up,
 // so that is common) only the upper half of each row is read.  The whole
 // wave is active here (control flow above is wave-uniform).
-template <int W, bool WP> struct DCount<W, 2, WP> {
+template <int W, bool WP, ull U> struct DCount<W, 2, WP, U> {
   static __device__ ull run(const ull *A, const ull *S, ull *cr) {
     const uint32_t lane = lane_id();
     constexpr int H = W >= 8 ? W / 2 : 0;
@@ -137,7 +144,7 @@ template <int W, bool WP> struct DCount<W, 2, WP> {
       if ((S[q] >> lane) & 1ull) {
         const uint32_t j = (uint32_t)q * 64 + lane;
         const uint32_t x = low ? drow_popc<W, 0>(A, j, S) : drow_popc<W, H>(A, j, S);
-        if constexpr (WP) cr[q] += x;
+        if constexpr (WP) cr[q] += (ull)x * U;  // modulo 2^64: U may stand for -U
         xs += x;
       }
     }
@@ -145,28 +152,142 @@ template <int W, bool WP> struct DCount<W, 2, WP> {
   }
 };
 
-template <int W, bool WP> struct DCount<W, 1, WP> {
+template <int W, bool WP, ull U> struct DCount<W, 1, WP, U> {
   static __device__ ull run(const ull *, const ull *S, ull *cr) {
     const uint32_t lane = lane_id();
     ull tot = 0;
 #pragma unroll
     for (int q = 0; q < W; ++q) {
       if constexpr (WP)
-        if ((S[q] >> lane) & 1ull) cr[q] += 1;
+        if ((S[q] >> lane) & 1ull) cr[q] += U;
       tot += (ull)__popcll(S[q]);
     }
     return tot;
   }
 };
 
+// Pair holdings with a critical set C.  A clique's share may move from its
+// holder to its other members in any proportions that sum to at most 1; the
+// rule used here: a clique with nc >= 1 critical members besides its holder
+// offers 1/nc to each of them and nothing to the rest, one with none offers
+// 1/(k-1) to each.  Holdings are kept in units of 1/L, L = lcm(1..k-1), so
+// the plain pass credits U = L/(k-1) per clique and k_dn_crit adds, for the
+// cliques that meet C, the difference between the two rules.
+template <int K> struct Units {
+  static constexpr uint32_t L = K <= 3 ? 2u : K == 4 ? 6u : K == 5 ? 12u : K <= 7 ? 60u : 420u;
+  static constexpr uint32_t U = L / (K - 1);
+  static constexpr ull NEG_U = 0ull - (ull)U;  // -U modulo 2^64
+};
+
+// REM-cliques inside S, each completing a clique that already has pc >= 1
+// critical members besides its holder: members are credited the rule's
+// share minus U (modulo 2^64: a negative difference wraps, and every total
+// it joins is non-negative); h[t] += REM-cliques with t critical members.
+template <int W, int K, int REM> struct DCountD {
+  static __device__ void run(const ull *A, const ull *S, const ull *Cm, uint32_t pc, ull *cr, ull *h) {
+    constexpr uint32_t L = Units<K>::L, U = Units<K>::U;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      ull word = S[q];
+      while (word) {
+        const int b = __builtin_ctzll(word);
+        word &= word - 1;
+        const uint32_t c = (uint32_t)(q * 64 + b);
+        ull S2[W];
+        uint32_t pcnt = 0;
+        bool anyc = false;
+#pragma unroll
+        for (int w = 0; w < W; ++w) S2[w] = drow<W>(A, c, w);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          S2[w] = S[w] & uniform_u64(S2[w]) & dabove(w, c);
+          pcnt += (uint32_t)__popcll(S2[w]);
+          anyc |= (S2[w] & Cm[w]) != 0ull;
+        }
+        if (pcnt < (uint32_t)(REM - 1)) continue;
+        const uint32_t cc = (uint32_t)((Cm[q] >> b) & 1ull), pc2 = pc + cc;
+        ull hz[REM];
+#pragma unroll
+        for (int t = 0; t < REM; ++t) hz[t] = 0;
+        if (anyc)
+          DCountD<W, K, REM - 1>::run(A, S2, Cm, pc2, cr, hz);
+        else
+          hz[0] = DCount<W, REM - 1, true, Units<K>::NEG_U>::run(A, S2, cr);
+        ull credit = 0;
+#pragma unroll
+        for (int t = 0; t < REM; ++t) credit += hz[t] * (cc ? (ull)(L / (pc2 + (uint32_t)t)) - U : Units<K>::NEG_U);
+        dcredit<W, true>(cr, c, credit);
+#pragma unroll
+        for (int t = 0; t < REM; ++t) h[t + (int)cc] += hz[t];
+      }
+    }
+  }
+};
+
+template <int W, int K> struct DCountD<W, K, 2> {
+  static __device__ void run(const ull *A, const ull *S, const ull *Cm, uint32_t pc, ull *cr, ull *h) {
+    constexpr uint32_t L = Units<K>::L, U = Units<K>::U;
+    const uint32_t lane = lane_id();
+    // a critical lane's cliques have pc + 1 or pc + 2 critical members (pc +
+    // 2 <= k - 1 at this level)
+    const ull da = (ull)(L / (pc + 1)) - U, db = (ull)(L / (pc + 2)) - U;
+    uint32_t xs = 0, x0s = 0, x2s = 0;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      if (!S[q]) continue;
+      if ((S[q] >> lane) & 1ull) {
+        const uint32_t j = (uint32_t)q * 64 + lane;
+        ull r[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) r[w] = drow<W>(A, j, w);
+        uint32_t x = 0, x1 = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          x += (uint32_t)__popcll(r[w] & S[w]);
+          x1 += (uint32_t)__popcll(r[w] & S[w] & Cm[w]);
+        }
+        if ((Cm[q] >> lane) & 1ull) {
+          cr[q] += (ull)(x - x1) * da + (ull)x1 * db;
+          x2s += x1;
+        } else {
+          cr[q] += (ull)x * Units<K>::NEG_U;
+          x0s += x - x1;
+        }
+        xs += x;
+      }
+    }
+    // every edge is seen from both ends
+    const ull e = wave_sum_u32_full(xs) >> 1, e0 = wave_sum_u32_full(x0s) >> 1, e2 = wave_sum_u32_full(x2s) >> 1;
+    h[0] += e0;
+    h[1] += e - e0 - e2;
+    h[2] += e2;
+  }
+};
+
+template <int W, int K> struct DCountD<W, K, 1> {
+  static __device__ void run(const ull *, const ull *S, const ull *Cm, uint32_t pc, ull *cr, ull *h) {
+    constexpr uint32_t L = Units<K>::L, U = Units<K>::U;
+    const uint32_t lane = lane_id();
+    const ull da = (ull)(L / (pc + 1)) - U;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      if ((S[q] >> lane) & 1ull) cr[q] += ((Cm[q] >> lane) & 1ull) ? da : Units<K>::NEG_U;
+      h[0] += (ull)__popcll(S[q] & ~Cm[q]);
+      h[1] += (ull)__popcll(S[q] & Cm[q]);
+    }
+  }
+};
+
 // One greedy pass over the rank-ordered core matrix Mr (n rows of W words,
 // row-major; vertex i = rank i, arcs point up in rank).  q[i] = cliques whose
-// lowest-rank member is i; with WP, P[i*n + j] = those of them containing j.
-// A wave takes (root i, candidate word w0) tasks from a ticket counter.
+// lowest-rank member is i; with WP, P[i*n + j] = U times those of them
+// containing j (the 1/(k-1) rule in units of 1/L).  A wave takes (root i,
+// candidate word w0) tasks from a ticket counter.
 template <int W, int K, bool WP>
 __global__ __launch_bounds__(DN_THREADS) void k_dn_pass(const ull *__restrict__ Mr, uint32_t n, uint32_t split,
                                                         uint32_t *ticket, ull *q, ull *P) {
   constexpr uint32_t NP = 64u * W;
+  constexpr ull U = Units<K>::U;
   __shared__ ull A[W * NP];
   for (uint32_t e = threadIdx.x; e < W * NP; e += blockDim.x) {
     const uint32_t w = e / NP, j = e % NP;
@@ -213,8 +334,8 @@ __global__ __launch_bounds__(DN_THREADS) void k_dn_pass(const ull *__restrict__ 
         pc += (uint32_t)__popcll(S2[w]);
       }
       if (pc + 2 < (uint32_t)K) continue;
-      const ull E = DCount<W, K - 2, WP>::run(A, S2, cr);
-      dcredit<W, WP>(cr, c, E);
+      const ull E = DCount<W, K - 2, WP, U>::run(A, S2, cr);
+      dcredit<W, WP>(cr, c, E * U);
       Et += E;
     }
     if (lane == 0 && Et) atomicAdd(&q[i], Et);
@@ -224,6 +345,65 @@ __global__ __launch_bounds__(DN_THREADS) void k_dn_pass(const ull *__restrict__ 
         const uint32_t j = (uint32_t)w * 64 + lane;
         if (cr[w] && j < n) atomicAdd(&P[(size_t)i * n + j], cr[w]);
       }
+    }
+  }
+}
+
+// After k_dn_pass with WP: the holdings of the cliques that meet the
+// critical set C besides their holder, moved from the 1/(k-1) rule to the
+// critical rule.  Each such clique is visited once, from its holder h and
+// its lowest-rank critical member v other than h: task (v = crit[ci], h)
+// with h < v adjacent to v; the other members lie above h, adjacent to h and
+// v, and are not critical below v.  Cm: C as a bitmask in rank order.
+template <int W, int K>
+__global__ __launch_bounds__(DN_THREADS) void k_dn_crit(const ull *__restrict__ Mr, uint32_t n,
+                                                        const uint32_t *__restrict__ crit, uint32_t ncrit,
+                                                        const ull *__restrict__ cmask, uint32_t *ticket, ull *P) {
+  constexpr uint32_t NP = 64u * W;
+  constexpr uint32_t L = Units<K>::L, U = Units<K>::U;
+  __shared__ ull A[W * NP];
+  for (uint32_t e = threadIdx.x; e < W * NP; e += blockDim.x) {
+    const uint32_t w = e / NP, j = e % NP;
+    A[e] = j < n ? Mr[(size_t)j * W + w] : 0ull;
+  }
+  __syncthreads();
+  ull Cm[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) Cm[w] = uniform_u64(cmask[w]);
+  const uint32_t lane = lane_id();
+  const uint32_t ntask = ncrit * n;
+  while (true) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(ticket, 1u);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+    if (t >= ntask) break;
+    const uint32_t v = __builtin_amdgcn_readfirstlane(crit[t / n]), h = t % n;
+    if (h >= v) continue;
+    ull S[W];
+    bool adj = false, any = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const ull rv = uniform_u64(drow<W>(A, v, w));
+      S[w] = uniform_u64(drow<W>(A, h, w)) & dabove(w, h) & rv & ~(Cm[w] & dbelow(w, v));
+      if ((uint32_t)w == (h >> 6)) adj = (rv >> (h & 63)) & 1ull;
+      any |= S[w] != 0ull;
+    }
+    if (!adj || (K > 2 && !any)) continue;
+    ull cr[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) cr[w] = 0;
+    ull hz[K - 1];
+#pragma unroll
+    for (int j = 0; j < K - 1; ++j) hz[j] = 0;
+    DCountD<W, K, K - 2>::run(A, S, Cm, 1, cr, hz);
+    ull credit = 0;
+#pragma unroll
+    for (int j = 0; j < K - 1; ++j) credit += hz[j] * ((ull)(L / (1u + (uint32_t)j)) - U);
+    dcredit<W, true>(cr, v, credit);
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const uint32_t j = (uint32_t)w * 64 + lane;
+      if (cr[w] && j < n) atomicAdd(&P[(size_t)h * n + j], cr[w]);
     }
   }
 }
@@ -341,6 +521,33 @@ template <bool WP> PassFn pass_fn(uint32_t W, uint32_t k) {
     case 10: return pass_fn_k<10, WP>(k);
     case 12: return pass_fn_k<12, WP>(k);
     case 16: return pass_fn_k<16, WP>(k);
+  }
+  return nullptr;
+}
+
+typedef void (*CritFn)(const ull *, uint32_t, const uint32_t *, uint32_t, const ull *, uint32_t *, ull *);
+
+template <int W> CritFn crit_fn_k(uint32_t k) {
+  switch (k) {
+    case 3: return k_dn_crit<W, 3>;
+    case 4: return k_dn_crit<W, 4>;
+    case 5: return k_dn_crit<W, 5>;
+    case 6: return k_dn_crit<W, 6>;
+    case 7: return k_dn_crit<W, 7>;
+    case 8: return k_dn_crit<W, 8>;
+  }
+  return nullptr;
+}
+
+CritFn crit_fn(uint32_t W, uint32_t k) {
+  switch (W) {
+    case 1: return crit_fn_k<1>(k);
+    case 2: return crit_fn_k<2>(k);
+    case 4: return crit_fn_k<4>(k);
+    case 8: return crit_fn_k<8>(k);
+    case 10: return crit_fn_k<10>(k);
+    case 12: return crit_fn_k<12>(k);
+    case 16: return crit_fn_k<16>(k);
   }
   return nullptr;
 }
@@ -500,10 +707,23 @@ struct DenseSolver {
   DevBuf M0, Mr, rankd, qd, Pd, ql, Phist, mu, slotd, Hlo, Hhi, tick;
   uint32_t hist_cap = 0;
   uint32_t passes = 0;
+  double pass_ms = 0;  // wall time inside pass()
+  // the critical set (local ids) and, per history slot, the order and the
+  // critical-set size its holdings were computed with
+  std::vector<uint8_t> crit;
+  uint32_t ncrit = 0;
+  std::vector<std::vector<uint32_t>> slot_order;
+  std::vector<uint32_t> slot_ncrit;
 
-  DenseSolver(Ctx &cc, uint32_t kk, uint32_t nn) : c(cc), k(kk), n(nn) {
+  DenseSolver(Ctx &cc, uint32_t kk, uint32_t nn) : c(cc), k(kk), n(nn), crit(nn, 0) {
     W = dense_words(n);
   }
+  // upload layout: ranks (n), the critical set as a rank-order bitmask (W
+  // words), the critical vertices' ranks (at most CRIT_MAX)
+  static constexpr uint32_t CRIT_MAX = 64;
+  size_t cm_at() const { return (sizeof(uint32_t) * n + 7) & ~(size_t)7; }
+  size_t cl_at() const { return cm_at() + sizeof(ull) * W; }
+  size_t rank_bytes() const { return cl_at() + sizeof(uint32_t) * CRIT_MAX; }
   ~DenseSolver() {
     for (DevBuf *b : {&M0, &Mr, &rankd, &qd, &Pd, &ql, &Phist, &mu, &slotd, &Hlo, &Hhi, &tick}) buf_free(c, *b);
   }
@@ -511,14 +731,30 @@ struct DenseSolver {
   // q (local ids) of the greedy split by order (order[0] first); with want_P
   // the pair holdings go to history slot `slot`.
   std::vector<ull> pass(const std::vector<uint32_t> &order, bool want_P, uint32_t slot) {
+    const auto t0 = std::chrono::steady_clock::now();
     const uint64_t nn = (uint64_t)n * n;
-    std::vector<uint32_t> rank(n);
-    for (uint32_t i = 0; i < n; ++i) rank[order[i]] = i;
-    HIP_CHECK(hipMemcpyAsync(rankd.p, rank.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c.stream));
+    // ranks and the critical set, in one upload
+    std::vector<uint8_t> up(rank_bytes(), 0);
+    uint32_t *rank = (uint32_t *)up.data(), *cl = (uint32_t *)(up.data() + cl_at());
+    ull *cm = (ull *)(up.data() + cm_at());
+    uint32_t nc = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      rank[order[i]] = i;
+      if (crit[order[i]]) {
+        cm[i >> 6] |= 1ull << (i & 63);
+        cl[nc++] = i;
+      }
+    }
+    if (nc != ncrit || nc > CRIT_MAX) fail(KCDS_E_INVALID, "dense pass: critical set of %u", nc);
+    HIP_CHECK(hipMemcpyAsync(rankd.p, up.data(), up.size(), hipMemcpyHostToDevice, c.stream));
+    if (want_P) {
+      slot_order[slot] = order;
+      slot_ncrit[slot] = ncrit;
+    }
     k_dn_zero<<<dgrid((uint64_t)n * W), 256, 0, c.stream>>>(Mr.as<ull>(), (uint64_t)n * W);
     k_dn_zero<<<dgrid(n), 256, 0, c.stream>>>(qd.as<ull>(), n);
     if (want_P) k_dn_zero<<<dgrid(nn), 256, 0, c.stream>>>(Pd.as<ull>(), nn);
-    HIP_CHECK(hipMemsetAsync(tick.p, 0, 4, c.stream));
+    HIP_CHECK(hipMemsetAsync(tick.p, 0, 8, c.stream));
     k_dn_perm<<<dgrid(nn), 256, 0, c.stream>>>(M0.as<ull>(), n, W, rankd.as<uint32_t>(), Mr.as<ull>());
     KCDS_LAUNCH_CHECK();
     PassFn fn = want_P ? pass_fn<true>(W, k) : pass_fn<false>(W, k);
@@ -529,6 +765,14 @@ struct DenseSolver {
     hipLaunchKernelGGL(fn, dim3(c.num_cus), dim3(DN_THREADS), 0, c.stream, Mr.as<ull>(), n, split,
                        tick.as<uint32_t>(), qd.as<ull>(), want_P ? Pd.as<ull>() : nullptr);
     KCDS_LAUNCH_CHECK();
+    if (want_P && nc) {
+      CritFn cf = crit_fn(W, k);
+      if (!cf) fail(KCDS_E_INVALID, "dense critical pass: k = %u, %u words", k, W);
+      hipLaunchKernelGGL(cf, dim3(c.num_cus), dim3(DN_THREADS), 0, c.stream, Mr.as<ull>(), n,
+                         (const uint32_t *)(rankd.as<uint8_t>() + cl_at()), nc,
+                         (const ull *)(rankd.as<uint8_t>() + cm_at()), tick.as<uint32_t>() + 1, Pd.as<ull>());
+      KCDS_LAUNCH_CHECK();
+    }
     ull *Pl = want_P ? Phist.as<ull>() + (size_t)slot * nn : nullptr;
     k_dn_unrank<<<dgrid(nn), 256, 0, c.stream>>>(n, rankd.as<uint32_t>(), qd.as<ull>(),
                                                  want_P ? Pd.as<ull>() : nullptr, ql.as<ull>(), Pl);
@@ -537,17 +781,12 @@ struct DenseSolver {
     HIP_CHECK(hipMemcpyAsync(out.data(), ql.p, sizeof(ull) * n, hipMemcpyDeviceToHost, c.stream));
     c.sync();
     ++passes;
+    pass_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return out;
   }
 };
 
 
-__device__ __forceinline__ ull dbelow(int q, uint32_t c) {
-  const int cq = (int)(c >> 6), cb = (int)(c & 63);
-  if (q < cq) return ~0ull;
-  if (q > cq) return 0ull;
-  return cb == 0 ? 0ull : ((1ull << cb) - 1ull);
-}
 
 // One sub-round of the dense-mode peel: the live cliques that meet the
 // frontier F, each counted once, from its smallest frontier member v: the
@@ -680,7 +919,7 @@ DpFn dp_fn(uint32_t W, uint32_t k) {
 // success: num/den = cliques in / size of the maximal densest set, which
 // c.inset (new ids) marks; ncomp = components of the core's clique graph;
 // passes = greedy passes run.
-bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, uint32_t &passes,
+bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, uint32_t &passes, uint32_t &critical,
                       uint32_t max_iters) {
   const uint32_t nall = c.n;
   const long long kp = (long long)c.h_gpeel[0].k_prime;
@@ -699,10 +938,12 @@ bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, u
   std::sort(verts.begin(), verts.end());
   DenseSolver D(c, k, nc);
   const uint32_t n = nc, W = D.W;
+  // holdings are in units of 1/L, L = lcm(1..k-1); unit = L/(k-1)
+  const uint32_t L = k <= 3 ? 2u : k == 4 ? 6u : k == 5 ? 12u : k <= 7 ? 60u : 420u, unit = L / (k - 1);
   const uint64_t nn = (uint64_t)n * n;
   buf_reserve(c, D.M0, sizeof(ull) * (size_t)n * W);
   buf_reserve(c, D.Mr, sizeof(ull) * (size_t)n * W);
-  buf_reserve(c, D.rankd, sizeof(uint32_t) * (n + 1));
+  buf_reserve(c, D.rankd, D.rank_bytes());
   buf_reserve(c, D.qd, sizeof(ull) * (n + 1));
   buf_reserve(c, D.ql, sizeof(ull) * (n + 1));
   buf_reserve(c, D.Pd, sizeof(ull) * nn);
@@ -711,6 +952,8 @@ bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, u
   // newest split
   D.hist_cap = n + 3;
   buf_reserve(c, D.Phist, sizeof(ull) * nn * D.hist_cap);
+  D.slot_order.assign(D.hist_cap, {});
+  D.slot_ncrit.assign(D.hist_cap, 0);
   // local ids and the core matrix
   buf_reserve(c, c.tmp2, sizeof(uint32_t) * ((size_t)nall + 1));
   uint32_t *loc = c.tmp2.as<uint32_t>();
@@ -739,7 +982,7 @@ bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, u
       deg[u] += q0[u];
       for (uint32_t w = 0; w < n; ++w)
         if (P0[(size_t)u * n + w]) {
-          deg[w] += P0[(size_t)u * n + w];
+          deg[w] += P0[(size_t)u * n + w] / unit;  // no critical set yet: unit per clique
           any[u] = any[w] = 1;
           par[find(u)] = find(w);
         }
@@ -793,8 +1036,12 @@ bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, u
     S.assign(desc.begin(), desc.begin() + bj);
     cS = bn;
   };
-  // the certificate for S (see the file comment); true when it holds
-  auto certify = [&](const std::vector<uint32_t> &S, ull cS) -> bool {
+  const bool log = getenv("KCDS_LOG") != nullptr;  // progress lines on stderr
+  // the certificate for S (see the file comment); true when it holds.  When
+  // it fails, unsat = the vertices whose shortfall or room the flow could not
+  // fill, the largest first.
+  auto certify = [&](const std::vector<uint32_t> &S, ull cS, std::vector<uint32_t> &unsat) -> bool {
+    unsat.clear();
     const uint32_t T = Q.m;
     double ls = 0;
     for (double l : Q.lam) ls += std::max(0.0, l);
@@ -846,25 +1093,71 @@ bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, u
     }
     if (sup < mand) return false;
     F.add(OPT, SNK, sup - mand);
+    // u may hand w the part of its holdings that w may take: H(u, w) / L
     for (uint32_t u = 0; u < n; ++u)
       for (uint32_t w = 0; w < n; ++w) {
         if (u == w) continue;
         const u128 h = ((u128)hhi[(size_t)u * n + w] << 64) | hlo[(size_t)u * n + w];
         if (!h) continue;
-        F.add(u, w, (i128)((h * (u128)s) / (u128)(k - 1)));
+        F.add(u, w, (i128)((h * (u128)s) / (u128)L));
       }
-    return F.run(SRC, SNK) == sup;
+    const i128 fl = F.run(SRC, SNK);
+    if (fl == sup) return true;
+    // the vertices the flow could not fill: members of S short of tau and
+    // outsiders with room left (the excess of S is mostly the shares S holds
+    // in cliques with outsiders, which have to pass to those outsiders)
+    std::vector<std::pair<i128, uint32_t>> un;
+    for (uint32_t u = 0; u < n; ++u)
+      for (const Dinic::Arc &a : F.g[u])
+        if ((a.to == SNK || a.to == OPT) && a.cap > 0) un.push_back({a.cap, u});
+    std::sort(un.begin(), un.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+    for (const auto &e : un) unsat.push_back(e.second);
+    if (log)
+      fprintf(stderr, "kcds dense: certificate short: flow/supply %.6f, %zu vertices unfilled, %u critical\n",
+              (double)fl / (double)sup, un.size(), D.ncrit);
+    return false;
   };
   std::vector<uint32_t> S;
   ull cS = 0;
   bool ok = false;
+  uint32_t attempts = 0;
+  double cert_ms = 0;
+  // A member of S whose clique-degree in S barely exceeds rho_S must end
+  // with almost all of its cliques, and an outsider must end with almost all
+  // the shares of its cliques with S; under the 1/(k-1) rule a holder can
+  // hand such a vertex only a (k-1)-th of its share, so the certificate
+  // would wait for x to converge far.  The vertices the flow left unfilled
+  // join the critical set, the corral's holdings are recomputed under it,
+  // and the certificate is tried again at once.
+  constexpr uint32_t CRIT_STEP = 8, CRIT_MAX = DenseSolver::CRIT_MAX;
   auto attempt = [&]() {
+    ++attempts;
     candidate(S, cS);
-    return !S.empty() && certify(S, cS);
+    if (S.empty()) return false;
+    const auto t0 = std::chrono::steady_clock::now();
+    bool r = false;
+    std::vector<uint32_t> unsat;
+    while (!(r = certify(S, cS, unsat))) {
+      uint32_t added = 0;
+      for (uint32_t u : unsat)
+        if (!D.crit[u] && added < CRIT_STEP && D.ncrit < CRIT_MAX) {
+          D.crit[u] = 1;
+          ++D.ncrit;
+          ++added;
+        }
+      if (!added) break;
+      for (uint32_t t = 0; t < Q.m; ++t)
+        if (D.slot_ncrit[Q.slot[t]] != D.ncrit) (void)D.pass(D.slot_order[Q.slot[t]], true, Q.slot[t]);
+    }
+    cert_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return r;
   };
-  const bool log = getenv("KCDS_LOG") != nullptr;  // progress lines on stderr
-  double next_try_gap = 1e-6;
-  uint32_t next_try_major = 0;
+  const auto t_start = std::chrono::steady_clock::now();
+  // the certificate has slack, so it often holds well before x has
+  // converged: it is tried from cycle 32 or a gap of 1e-3 on, then each time
+  // the gap has fallen tenfold or the cycle count has grown by half
+  double next_try_gap = 1e-3;
+  uint32_t next_try_major = 32;
   for (uint32_t major = 1; major <= max_iters; ++major) {
     // linear oracle: the greedy split minimising <x, q> (each clique to its
     // member of smallest load)
@@ -887,13 +1180,10 @@ bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, u
       ok = attempt();  // x is the minimum-norm point to working precision
       break;
     }
-    // the certificate has slack (vertices whose load is far below their
-    // clique-degree absorb imbalance), so it often holds well before x has
-    // converged: try it from a gap of 1e-6 on, each time the gap has fallen
-    // tenfold and at least every 256 cycles
-    if (gap <= next_try_gap || (gap <= 1e-6 && major >= next_try_major)) {
+    if (gap <= next_try_gap || major >= next_try_major) {
       next_try_gap = gap * 0.1;
-      next_try_major = major + 256;
+      next_try_major = major + std::max(32u, major / 2);
+      if (log) fprintf(stderr, "kcds dense: attempt at major=%u gap=%.3e corral=%u\n", major, gap, Q.m);
       if ((ok = attempt())) break;
     }
     free_slots.pop_back();
@@ -927,6 +1217,12 @@ bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, u
     Q.point(Q.lam, x);
   }
   passes = D.passes;
+  critical = D.ncrit;
+  if (log)
+    fprintf(stderr, "kcds dense: n=%u k=%u passes=%u attempts=%u critical=%u ok=%d total %.1f ms, passes %.1f ms, "
+            "certify %.1f ms\n", n, k, D.passes, attempts, D.ncrit, (int)ok,
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count(), D.pass_ms,
+            cert_ms);
   if (!ok) return false;
   // mark S (new ids) in c.inset
   buf_reserve(c, c.inset, (size_t)nall + 8);

@@ -250,9 +250,10 @@ void core_materialize(Ctx &c, bool materialize = true);
 uint64_t build_core_dag(Ctx &c);  // c.kdoff/c.kdnbr: the DAG restricted to the k'-cores
 // Exact search on graph 0's k'-core when it has at most 1,024 vertices (false
 // otherwise, or when the certificate is not reached within max_iters greedy
-// passes): the Frank-Wolfe / integer-certificate search of k_dense.hip.
-// num/den = cliques in / size of the maximal densest set, marked in c.inset.
-bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, uint32_t &passes,
+// passes): the minimum-norm-point / integer-certificate search of k_dense.hip.
+// num/den = cliques in / size of the maximal densest set, marked in c.inset;
+// critical = the critical set its certificate ended with.
+bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, uint32_t &passes, uint32_t &critical,
                       uint32_t max_iters);
 // Dense mode of the enumerating peel (k_dense.hip) for a single graph with at
 // most 1,024 live vertices; -1 when not applicable.

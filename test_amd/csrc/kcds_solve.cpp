@@ -300,12 +300,13 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
     }
     if ((c.tune_flags & KCDS_TUNE_DENSE) || c.peel.core_cliques >= DENSE_MIN_T) {
       ull num = 0, den = 0;
-      uint32_t ncomp = 0, passes = 0;
-      if (dense_core_solve(c, k, num, den, ncomp, passes, DENSE_MAX_ITERS)) {
+      uint32_t ncomp = 0, passes = 0, critical = 0;
+      if (dense_core_solve(c, k, num, den, ncomp, passes, critical, DENSE_MAX_ITERS)) {
         R.ms_components = t.stop();
         Timer tf(c.stream);
         R.ncomp = ncomp;
         R.dense_passes = passes;
+        R.dense_critical = critical;
         R.rho_num = num;
         R.rho_den = den;
         out[0].T = c.h_gpeel[0].T;

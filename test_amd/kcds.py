@@ -47,7 +47,7 @@ class Result(ctypes.Structure):
                 ("flow_calls", ctypes.c_uint32), ("peel", PeelStats),
                 ("ms_orient", ctypes.c_float), ("ms_list", ctypes.c_float), ("ms_peel", ctypes.c_float),
                 ("ms_components", ctypes.c_float), ("ms_flow", ctypes.c_float), ("ms_total", ctypes.c_float),
-                ("dense_passes", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+                ("dense_passes", ctypes.c_uint32), ("dense_critical", ctypes.c_uint32)]
 
 
 class GraphResult(ctypes.Structure):

@@ -1,7 +1,7 @@
 """GPU parity of the dense exact search (k_dense.hip): the k'-core held as a
 bitmask matrix, greedy clique splits combined by Wolfe's minimum-norm-point
 method, and the answer certified by an exact integer max-flow over vertex
-pairs.  Core sizes cover every matrix width (1, 2, 4, 8, 10, 12 and 16 words
+pairs (with the critical-set rule for vertices the flow leaves unfilled).  Core sizes cover every matrix width (1, 2, 4, 8, 10, 12 and 16 words
 per row).
 
 The search is forced (KCDS_TUNE_DENSE) on graphs whose k'-core has at most
@@ -136,3 +136,36 @@ def test_dense_brute_force_small(solver):
                 assert int(sum(1 << int(v) for v in np.flatnonzero(ins))) == union
     finally:
         solver.set_tuning(0, 0)
+
+
+# Graphs whose certificate needs the critical set: a member of the densest set
+# whose clique-degree inside it barely exceeds rho*, or outsiders that must
+# take the whole share of their cliques with it (found by a sweep of R-MAT and
+# Chung-Lu seeds; each answer checked against the oracle).
+CRITICAL_GRAPHS = [
+    ("rmat11_s4", lambda: gen.rmat(11, 12, (), seed=4), (4, 5)),
+    ("rmat12_s2", lambda: gen.rmat(12, 10, (), seed=2), (4,)),
+    ("chunglu8k_s5", lambda: gen.chunglu(8000, 2.1, 2.0, 400.0, 40000, 0, 0.95, seed=5), (4, 5)),
+    ("chunglu8k_s6", lambda: gen.chunglu(8000, 2.1, 2.0, 400.0, 40000, 0, 0.95, seed=6), (4,)),
+]
+
+
+def test_dense_critical_certificate(solver):
+    used = 0
+    solver.set_tuning(0, kcds.TUNE_DENSE)
+    try:
+        for name, make, ks in CRITICAL_GRAPHS:
+            g = make()
+            for k in ks:
+                solver.upload(g.offset, g.nbr)
+                res, ins = solver.solve(k)
+                r, _, _, _, ins_r = O.solve(g.offset, g.nbr, k)
+                print(f"  {name} k={k} rho={res.rho_num}/{res.rho_den} core={res.core_vertices} "
+                      f"passes={res.dense_passes} critical={res.dense_critical}", flush=True)
+                assert res.dense_passes > 0
+                assert (res.rho_num, res.rho_den) == (r.rho_num, r.rho_den)
+                np.testing.assert_array_equal(ins, ins_r)
+                used += res.dense_critical > 0
+    finally:
+        solver.set_tuning(0, 0)
+    assert used >= 3  # the critical rule took part in most of these certificates
