@@ -325,6 +325,16 @@ int kcds_count_dag_noalloc(int device, uint32_t n, const uint32_t *doff, const u
  *                        out-list wider than 512 it falls back to the slack
  *                        degeneracy order, then to the exact one. */
 #define KCDS_TUNE_DEGENERACY_ORDER 512u
+/*   KCDS_TUNE_BUILD_FRAMES  let the enumerating peel build every removed
+ *                        vertex's frame from its neighbours' out-lists.  By
+ *                        default the frames of vertices of degree <= 512
+ *                        that lie in a clique are built once, in parallel,
+ *                        before the first level (about 8 bytes per
+ *                        neighbour, up to 8 words per neighbour for degree
+ *                        257..512; skipped when that does not fit the HBM
+ *                        budget), and each step restricts and renumbers the
+ *                        stored rows instead.  Same cores either way. */
+#define KCDS_TUNE_BUILD_FRAMES 1024u
 int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
 
 /* Listing launch shape: waves per root (workgroup size / 64, 1..8) of each

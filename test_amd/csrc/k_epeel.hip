@@ -50,6 +50,10 @@ constexpr uint32_t ECHUNK_LOG = 16;             // a narrow frame takes a work i
 constexpr uint32_t EMAXP = 64;                  //   up to 64 (k >= 4)
 constexpr uint32_t EUNROLL = 8;                 // 64-element chunks in flight per wave
 constexpr uint32_t EMPTY_ID = 0xFFFFFFFFu;
+constexpr uint32_t FMAX = 4096;                 // widest stored frame (members; see k_ep_frames_wide)
+constexpr uint32_t FWORDS = FMAX / 64;
+constexpr uint32_t FW_HB = 13, FW_H = 1u << FW_HB;  // k_ep_frames_wide's table slots (load <= 1/2)
+constexpr uint32_t FW_ROWS = 128;                // rows k_ep_frames_wide builds per pass
 constexpr uint32_t LIVE = 0xFFFFFFFFu;          // erem of a vertex still in the graph
 
 // Counters 64 words apart (same-address device atomics are serialised).
@@ -80,7 +84,12 @@ struct EFrame {
   uint4 item;
   uint32_t cand[EPIECE], crem[EPIECE];
   uint32_t wcnt[ETHREADS / 64];  // members kept by each wave in the current staging chunk
+  ull smask[FWORDS];             // e_load: members of a stored frame that lie in X_v
+  ull fmask[FWORDS];             //   those of them in the current frontier
+  ull cmv[FWORDS * 6];           //   compress round masks of smask's words
+  uint32_t sbase[FWORDS];        //   kept members before each word
 };
+static_assert(ETHREADS >= ED, "e_load takes a stored frame's members one per thread");
 
 __device__ __forceinline__ uint32_t ehslot(uint32_t key) {
   return (uint32_t)__umul24(key ^ (key >> 24), 0x9E3779u) >> (32 - EHB);
@@ -96,6 +105,10 @@ __device__ __forceinline__ ull above_bits(int q, uint32_t c) {
 __device__ __forceinline__ ull erow(const EFrame &f, uint32_t j, int w) { return f.A[(uint32_t)w * ED + j]; }
 
 __device__ __forceinline__ uint32_t words_of(uint32_t d) { return d <= 64 ? 1u : d <= 128 ? 2u : d <= 256 ? 4u : 8u; }
+// 64-bit words per row of a stored frame of d members
+__host__ __device__ __forceinline__ uint32_t stored_words(uint32_t d) {
+  return d <= 64 ? 1u : d <= 128 ? 2u : d <= 256 ? 4u : d <= 512 ? 8u : (d + 63) / 64;
+}
 
 template <int W> __device__ __forceinline__ void ecredit(ull *cr, uint32_t c, ull E) {
   const uint32_t lane = lane_id();
@@ -423,6 +436,75 @@ __device__ __forceinline__ uint32_t e_find(const EFrame &f, uint32_t w) {
   }
 }
 
+// Stages a stored frame (k_ep_frames, k_ep_frames_wide): the rows of the
+// subgraph induced by all of N(v) = src[0, len) (len <= FMAX), word-major,
+// row i's word w at rows[w * len + i], stored_words(len) words.  The members
+// in X_v (at most ED of them: the frontier admits v as narrow only then)
+// are kept in src order, as e_stage keeps them, with their frontier bits;
+// each kept row is restricted to the kept members and compressed to the new
+// numbering, so the frame equals the one e_stage + e_build would build,
+// without reading any out-list.
+__device__ void e_load(EFrame &f, const uint32_t *__restrict__ src, uint32_t len, uint32_t v, uint32_t s,
+                       const uint32_t *__restrict__ erem, const ull *__restrict__ rows) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nt = blockDim.x;
+  const uint32_t Wf = stored_words(len);
+  if (tid < 8) f.fr[tid] = 0;
+  // members in X_v, and those in the frontier: one mask word per wave and chunk
+  for (uint32_t t0 = 0; t0 < len; t0 += nt) {
+    const uint32_t i = t0 + tid;
+    const uint32_t u = i < len ? src[i] : EMPTY_ID;
+    const uint32_t r = u != EMPTY_ID ? erem[u] : 0u;
+    const bool keep = u != EMPTY_ID && (r > s || (r == s && u > v));
+    const ull bal = __ballot(keep), fb = __ballot(keep && r == s);
+    const uint32_t q = (t0 >> 6) + wv;
+    if (lane == 0 && q < Wf) {
+      f.smask[q] = bal;
+      f.fmask[q] = fb;
+    }
+  }
+  lds_sync();
+  if (tid < Wf) compress_masks(f.smask[tid], &f.cmv[tid * 6]);
+  if (wv == 0) {
+    const uint32_t x = lane < Wf ? (uint32_t)__popcll(f.smask[lane]) : 0u;
+    const uint32_t incl = wave_incl_scan_u32(x);
+    f.sbase[lane] = incl - x;
+    if (lane == 63) f.d = min(incl, ED);
+  }
+  lds_sync();
+  for (uint32_t i = tid; i < len; i += nt) {
+    const uint32_t q = i >> 6, bi = i & 63;
+    const ull m = f.smask[q];
+    if (!((m >> bi) & 1ull)) continue;
+    const uint32_t pos = f.sbase[q] + (uint32_t)__popcll(m & ((1ull << bi) - 1ull));
+    if (pos >= ED) continue;
+    f.L[pos] = src[i];
+    if ((f.fmask[q] >> bi) & 1ull) atomicOr(&f.fr[pos >> 6], 1ull << (pos & 63));
+    // the kept bits of each stored word, packed; words fill in order
+    ull acc = 0;
+    uint32_t fill = 0, ow = 0;
+    for (uint32_t w = 0; w < Wf; ++w) {
+      const ull mw = f.smask[w];
+      const uint32_t pcw = (uint32_t)__popcll(mw);
+      if (!pcw) continue;
+      const ull y = compress_apply(rows[(size_t)w * len + i] & mw, &f.cmv[w * 6]);
+      acc |= y << fill;
+      const uint32_t nf = fill + pcw;
+      if (nf >= 64) {
+        if (ow < 8) f.A[ow * ED + pos] = acc;
+        ++ow;
+        acc = fill ? (y >> (64 - fill)) : 0ull;
+        fill = nf - 64;
+      } else {
+        fill = nf;
+      }
+    }
+    if (fill && ow < 8) f.A[ow * ED + pos] = acc;
+    if (fill) ++ow;
+    for (; ow < 8; ++ow) f.A[ow * ED + pos] = 0ull;
+  }
+  lds_sync();
+}
+
 // Rows of the staged members: every member's DAG out-list is read (64-element
 // chunks, EUNROLL in flight per wave, across list boundaries) and looked up
 // in the table; a hit j in member i's list sets bit j of row i and bit i of
@@ -499,6 +581,7 @@ __device__ __forceinline__ ull e_task_total(EFrame &f, ull share) {
 
 struct EArgs {
   const uint32_t *foff, *fnbr, *doff, *dnbr;
+  const ull *fo, *fstore;  // stored frames (k_ep_frames), or null
   ull *cnt;
   uint32_t *erem;
   GraphPeel *gp;
@@ -514,13 +597,15 @@ __device__ void e_narrow(EFrame &f, uint32_t v, uint32_t g, uint32_t j, uint32_t
                          const EAppend &a, ull &dm) {
   const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = blockDim.x >> 6;
   const uint32_t b = x.foff[v], len = x.foff[v + 1] - b;
-  e_stage(f, x.fnbr + b, len, v, x.s, x.erem, x.doff, false);
+  const bool stored = x.fstore && x.fo[v + 1] > x.fo[v];
+  if (stored) e_load(f, x.fnbr + b, len, v, x.s, x.erem, x.fstore + x.fo[v]);
+  else e_stage(f, x.fnbr + b, len, v, x.s, x.erem, x.doff, false);
   const uint32_t d = f.d;
   if (d + 1 < (uint32_t)K) {
     lds_sync();
     return;
   }
-  e_build(f, d, x.dnbr);
+  if (!stored) e_build(f, d, x.dnbr);
   const uint32_t Wd = words_of(d);
   ull S[8], cr[8];
 #pragma unroll
@@ -689,7 +774,8 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
                                                      uint32_t *erem, uint32_t *fv0, uint32_t *fv1, uint4 *fi0,
                                                      uint4 *fi1, uint32_t seg_v, uint32_t seg_i, GraphPeel *gp,
                                                      const uint32_t *__restrict__ gid, uint32_t *tlist, Scatter sc,
-                                                     uint8_t *alive) {
+                                                     uint8_t *alive, const ull *__restrict__ fo,
+                                                     const ull *__restrict__ fstore) {
   __shared__ EFrame f;
   __shared__ ull s_hdr[2];
   __shared__ uint32_t s_hdr2[4], s_f[ESEG + 1], s_i[ESEG + 1], s_last, s_sz[2][2][ESEG];
@@ -768,7 +854,7 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
         if (atomicExch(&gp[g].touch, 1u) == 0u) tlist[atomicAdd(&ctl->ntouch[0], 1u)] = g;
       }
     }
-    const EArgs x{foff, fnbr, doff, dnbr, cnt, erem, gp, sc, L, s};
+    const EArgs x{foff, fnbr, doff, dnbr, fo, fstore, cnt, erem, gp, sc, L, s};
     const EAppend a{foff, fnbr, nv + (size_t)seg * seg_v, &ctl->fsize[cur ^ 1][seg][0], ni + (size_t)seg * seg_i,
                     &ctl->isize[cur ^ 1][seg][0], erem, s + 1, K >= 4};
     ull dm = ~0ull;
@@ -861,6 +947,113 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
   }
 }
 
+// Words of v's stored frame: stored_words(d) * d for a vertex of degree
+// d <= maxd that lies in a clique (its count is positive), else none.
+__global__ void k_ep_fsize(uint32_t n, const uint32_t *foff, const ull *cnt0, uint32_t maxd, ull *words) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const uint32_t d = foff[v + 1] - foff[v];
+    words[v] = d <= maxd && cnt0[v] ? (ull)stored_words(d) * d : 0ull;
+  }
+}
+
+// The stored frames, one workgroup per vertex at a time: N(v) staged whole
+// (every vertex is still live) and its rows built from the out-lists, then
+// written word-major to fstore + fo[v].  Runs after k_ep_init.
+__global__ __launch_bounds__(ETHREADS) void k_ep_frames(uint32_t n, const uint32_t *__restrict__ foff,
+                                                       const uint32_t *__restrict__ fnbr,
+                                                       const uint32_t *__restrict__ doff,
+                                                       const uint32_t *__restrict__ dnbr,
+                                                       const uint32_t *__restrict__ erem, const ull *__restrict__ fo,
+                                                       ull *fstore, uint32_t *ticket) {
+  __shared__ EFrame f;
+  __shared__ uint32_t s_v;
+  while (true) {
+    if (threadIdx.x == 0) s_v = atomicAdd(ticket, 1u);
+    lds_sync();
+    const uint32_t v = s_v;
+    lds_sync();  // every thread has read s_v
+    if (v >= n) break;
+    const uint32_t b = foff[v], d = foff[v + 1] - b;
+    if (fo[v + 1] == fo[v] || d > ED) continue;
+    e_stage(f, fnbr + b, d, v, 0u, erem, doff, false);
+    e_build(f, d, dnbr);
+    const uint32_t W = words_of(d);
+    ull *dst = fstore + fo[v];
+    for (uint32_t t = threadIdx.x; t < W * d; t += blockDim.x) dst[t] = f.A[(t / d) * ED + t % d];
+    lds_sync();
+  }
+}
+
+__device__ __forceinline__ uint32_t fwslot(uint32_t key) {
+  return (uint32_t)__umul24(key ^ (key >> 24), 0x9E3779u) >> (32 - FW_HB);
+}
+
+// The stored frames of vertices of degree ED+1..FMAX, one workgroup per
+// vertex at a time: N(v) in a table (id -> index), then FW_ROWS rows at a
+// time, row i = the members adjacent to member i, found by looking up each
+// entry of member i's full adjacency list (so no row needs another's
+// out-list), written word-major to fstore + fo[v].
+__global__ __launch_bounds__(ETHREADS) void k_ep_frames_wide(uint32_t n, const uint32_t *__restrict__ foff,
+                                                            const uint32_t *__restrict__ fnbr,
+                                                            const ull *__restrict__ fo, ull *fstore,
+                                                            uint32_t *ticket) {
+  __shared__ ull hent[FW_H];
+  __shared__ ull blk[FW_ROWS * FWORDS];
+  __shared__ uint32_t s_v;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, nt = blockDim.x;
+  const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nt >> 6;
+  uint32_t *hk = reinterpret_cast<uint32_t *>(hent);
+  while (true) {
+    if (tid == 0) s_v = atomicAdd(ticket, 1u);
+    lds_sync();
+    const uint32_t v = s_v;
+    lds_sync();  // every thread has read s_v
+    if (v >= n) break;
+    const uint32_t b = foff[v], d = foff[v + 1] - b;
+    if (fo[v + 1] == fo[v] || d <= ED || d > FMAX) continue;
+    for (uint32_t t = tid; t < FW_H; t += nt) hent[t] = ~0ull;
+    lds_sync();
+    for (uint32_t t = tid; t < d; t += nt) {
+      const uint32_t u = fnbr[b + t];
+      uint32_t h = fwslot(u);
+      while (atomicCAS(&hk[2 * h], EMPTY_ID, u) != EMPTY_ID) h = (h + 1) & (FW_H - 1);
+      hk[2 * h + 1] = t;
+    }
+    lds_sync();
+    const uint32_t Wf = stored_words(d);
+    ull *dst = fstore + fo[v];
+    for (uint32_t r0 = 0; r0 < d; r0 += FW_ROWS) {
+      const uint32_t nr = min(FW_ROWS, d - r0);
+      for (uint32_t t = tid; t < nr * Wf; t += nt) blk[t] = 0ull;
+      lds_sync();
+      for (uint32_t row = wv; row < nr; row += nw) {
+        const uint32_t u = fnbr[b + r0 + row], ub = foff[u], ue = foff[u + 1];
+        for (uint32_t e = ub + lane; e < ue; e += KCDS_WAVE) {
+          const uint32_t w = fnbr[e];
+          uint32_t h = fwslot(w);
+          while (true) {
+            const ull he = hent[h];
+            const uint32_t key = (uint32_t)he;
+            if (key == w) {
+              const uint32_t j = (uint32_t)(he >> 32);
+              atomicOr(&blk[row * Wf + (j >> 6)], 1ull << (j & 63));
+              break;
+            }
+            if (key == EMPTY_ID) break;
+            h = (h + 1) & (FW_H - 1);
+          }
+        }
+      }
+      lds_sync();
+      for (uint32_t t = tid; t < nr * Wf; t += nt) {
+        const uint32_t w = t / nr, row = t % nr;
+        dst[(size_t)w * d + r0 + row] = blk[row * Wf + w];
+      }
+      lds_sync();
+    }
+  }
+}
+
 // working counts (scattered layout), core = -1, every vertex live (alive is
 // the slot-ordered live flag that OPEN scans; slots past n stay 0)
 __global__ void k_ep_init(uint32_t n, const ull *cnt0, ull *cnt, long long *core, uint32_t *erem, Scatter sc,
@@ -933,7 +1126,7 @@ __global__ void k_fill_i32(int32_t *p, int32_t v, uint64_t len) {
 
 typedef void (*EpStepFn)(EpCtl *, uint32_t, const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *,
                          ull *, long long *, uint32_t *, uint32_t *, uint32_t *, uint4 *, uint4 *, uint32_t, uint32_t,
-                         GraphPeel *, const uint32_t *, uint32_t *, Scatter, uint8_t *);
+                         GraphPeel *, const uint32_t *, uint32_t *, Scatter, uint8_t *, const ull *, const ull *);
 
 EpStepFn ep_step_fn(uint32_t k) {
   switch (k) {
@@ -1034,6 +1227,45 @@ void epeel_run(Ctx &c) {
   // one workgroup of ETHREADS threads per CU (a 55 KB frame; its 150-250
   // VGPRs per thread leave room for one 512-thread workgroup per CU)
   const uint32_t sgrid = (uint32_t)c.num_cus * (ETHREADS >= 512 ? 1u : 2u);
+  // stored frames: each narrow vertex's rows built once, all vertices in
+  // parallel, so that a step only restricts and renumbers them
+  const ull *fo = nullptr, *fstore = nullptr;
+  if (n && k >= 3 && !(c.tune_flags & KCDS_TUNE_BUILD_FRAMES)) {
+    buf_reserve(c, c.fofs, sizeof(ull) * ((size_t)n + 2));
+    buf_reserve(c, c.tmp1, sizeof(ull) * ((size_t)n + 64));
+    ull *words = c.tmp1.as<ull>();
+    uint32_t *ticket = (uint32_t *)(words + n + 32);
+    // vertices up to degree FMAX; up to ED alone when those do not fit
+    ull total = 0;
+    size_t fbytes = 0;
+    for (uint32_t maxd : {FMAX, ED}) {
+      k_ep_fsize<<<egrid(n), 256, 0, c.stream>>>(n, c.foff.as<uint32_t>(), c.cnt.as<ull>(), maxd, words);
+      KCDS_LAUNCH_CHECK();
+      scan_u64_to_u64(c, words, c.fofs.as<ull>(), n);
+      HIP_CHECK(hipMemcpyAsync(&total, c.fofs.as<ull>() + n, sizeof(ull), hipMemcpyDeviceToHost, c.stream));
+      c.sync();
+      fbytes = (size_t)total * sizeof(ull);
+      if (c.used + fbytes + ((size_t)64 << 20) <= c.budget) break;
+      total = 0;
+    }
+    if (total) {
+      buf_reserve(c, c.fstore, fbytes);
+      HIP_CHECK(hipMemsetAsync(ticket, 0, sizeof(uint32_t), c.stream));
+      k_ep_frames<<<sgrid, ETHREADS, 0, c.stream>>>(n, c.foff.as<uint32_t>(), c.fnbr.as<uint32_t>(),
+                                                    c.doff.as<uint32_t>(), c.dnbr.as<uint32_t>(),
+                                                    c.erem.as<uint32_t>(), c.fofs.as<ull>(), c.fstore.as<ull>(),
+                                                    ticket);
+      KCDS_LAUNCH_CHECK();
+      if (c.maxdeg > ED) {
+        HIP_CHECK(hipMemsetAsync(ticket + 1, 0, sizeof(uint32_t), c.stream));
+        k_ep_frames_wide<<<sgrid, ETHREADS, 0, c.stream>>>(n, c.foff.as<uint32_t>(), c.fnbr.as<uint32_t>(),
+                                                           c.fofs.as<ull>(), c.fstore.as<ull>(), ticket + 1);
+        KCDS_LAUNCH_CHECK();
+      }
+      fo = c.fofs.as<ull>();
+      fstore = c.fstore.as<ull>();
+    }
+  }
   uint32_t batch = c.peel_steps ? (uint32_t)std::min<uint64_t>(c.peel_steps, 1u << 20) : 32;
   uint64_t rounds = 0;
   const bool log = getenv("KCDS_LOG") != nullptr;  // progress lines on stderr
@@ -1043,7 +1275,7 @@ void epeel_run(Ctx &c) {
       hipLaunchKernelGGL(step, dim3(sgrid), dim3(ETHREADS), 0, c.stream, ctl, n, c.foff.as<uint32_t>(),
                          c.fnbr.as<uint32_t>(), c.doff.as<uint32_t>(), c.dnbr.as<uint32_t>(), cnt,
                          c.core.as<long long>(), c.erem.as<uint32_t>(), fv0, fv1, fi0, fi1, seg_v, seg_i, gp,
-                         c.gid.as<uint32_t>(), c.tlist.as<uint32_t>(), sc, alive);
+                         c.gid.as<uint32_t>(), c.tlist.as<uint32_t>(), sc, alive, fo, fstore);
       KCDS_LAUNCH_CHECK();
     }
     rounds += batch;
@@ -1064,6 +1296,10 @@ void epeel_run(Ctx &c) {
   }
   c.peel_steps = h0->steps;
   uint32_t levels = h0->levels;
+  if (fstore) {  // the later stages get the HBM back
+    buf_free(c, c.fstore);
+    buf_free(c, c.fofs);
+  }
   if (log)
     fprintf(stderr, "kcds epeel: step kernels ended after %u steps, %u levels, %.1f ms\n", h0->steps, h0->levels,
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count());

@@ -134,7 +134,7 @@ void kcds_destroy(kcds_ctx *ctx) {
   FlowBufs &f = c.fb;
   DevBuf *bufs[] = {&c.off, &c.nbr, &c.rank, &c.inv, &c.doff, &c.dnbr, &c.roots, &c.qctr, &c.cnt, &c.scal,
                     &c.rootT, &c.lsum, &c.cliques, &c.incoff, &c.inc, &c.core, &c.status, &c.foff, &c.fnbr, &c.erem, &c.kdoff, &c.kdnbr, &c.comp, &c.tmp0,
-                    &c.tmp1, &c.tmp2, &c.tmp3, &c.gvoff, &c.gid, &c.gpeel, &c.tlist, &c.inset, &c.wide,
+                    &c.tmp1, &c.tmp2, &c.tmp3, &c.gvoff, &c.gid, &c.gpeel, &c.tlist, &c.inset, &c.wide, &c.fofs, &c.fstore,
                     &g.par, &g.flag, &g.dense, &g.cid, &g.local, &g.vcount, &g.voff, &g.verts, &g.ccount,
                     &g.coff, &g.cfill, &g.ccl, &f.scal, &f.cside, &f.g_off, &f.g_head, &f.g_rev, &f.g_res,
                     &f.g_excess, &f.g_height, &f.g_fr, &f.g_ctl, &c.gsum, &f.g_nch, &f.g_choff, &f.g_chv,

@@ -89,6 +89,34 @@ __device__ __forceinline__ ull wave_sum_u64(ull v) {
 // no LDS traffic (a shuffle round trips through the LDS crossbar six times).
 // Lanes switched off by divergence would contribute stale values, so callers
 // use it only where control flow has been wave-uniform.
+// Bit compress (generalised extract): the bits of x at the set positions of
+// a mask m, packed to the low end in order.  Warren, Hacker's Delight, 2nd
+// ed., section 7-4: six rounds, each moving the selected bits right by 2^i;
+// the round masks depend on m alone, so they are computed once per mask.
+__host__ __device__ __forceinline__ void compress_masks(ull m, ull *mv) {
+  ull mk = ~m << 1;  // counts the zeros of m to the right of each position
+  for (int i = 0; i < 6; ++i) {
+    ull mp = mk ^ (mk << 1);  // parallel suffix of mk
+    mp ^= mp << 2;
+    mp ^= mp << 4;
+    mp ^= mp << 8;
+    mp ^= mp << 16;
+    mp ^= mp << 32;
+    const ull v = mp & m;  // bits that move 2^i places this round
+    mv[i] = v;
+    m = (m ^ v) | (v >> (1u << i));
+    mk &= ~mp;
+  }
+}
+// x must already be restricted to the mask the round masks came from.
+__host__ __device__ __forceinline__ ull compress_apply(ull x, const ull *mv) {
+  for (int i = 0; i < 6; ++i) {
+    const ull t = x & mv[i];
+    x = (x ^ t) | (t >> (1u << i));
+  }
+  return x;
+}
+
 __device__ __forceinline__ uint32_t wave_sum_u32_full(uint32_t v) {
   v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
   v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2

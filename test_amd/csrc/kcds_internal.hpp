@@ -168,6 +168,7 @@ struct Ctx {
   // enumerating peel: full adjacency over new ids (in-list, then out-list)
   // and the step at which each vertex leaves (0xFFFFFFFF while live)
   DevBuf foff, fnbr, erem;
+  DevBuf fofs, fstore;      // the enumerating peel's stored frames (word offsets per vertex, rows)
   DevBuf kdoff, kdnbr;      // the k'-core's DAG (listing of the core cliques)
   bool enum_peel = false;   // the last peel enumerated cliques (no clique list)
   bool core_counted = false; // ... and the k'-cores' cliques have been counted since

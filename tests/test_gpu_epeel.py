@@ -1,7 +1,8 @@
 """GPU parity of the enumerating peel: the clique-core peel that keeps no clique
 list and enumerates each removed vertex's live cliques from its
-neighbourhood (narrow frames for degree <= 512, candidate-split frames for
-wider vertices), then lists only the k'-core's cliques.
+neighbourhood (narrow frames for degree <= 512, stored before the first
+level or built at the step; candidate-split frames for wider vertices), then
+lists only the k'-core's cliques.
 
 Bit-exact against the oracle's peel over the materialised clique list (core
 numbers, maxDensity as a rational, k', core size, core cliques, largest
@@ -41,15 +42,20 @@ def graphs():
 GRAPHS = list(graphs())
 
 
-@pytest.mark.parametrize("mode", ["dense_mode", "steps_only"])
+MODES = {"dense_mode": kcds.TUNE_DENSE_PEEL, "steps_only": 0, "built_frames": kcds.TUNE_BUILD_FRAMES}
+
+
+@pytest.mark.parametrize("mode", list(MODES))
 @pytest.mark.parametrize("name,off,nbr,ks,_solve_ks", GRAPHS, ids=[g[0] for g in GRAPHS])
 def test_enum_peel_matches_oracle(solver, name, off, nbr, ks, _solve_ks, mode):
     """dense_mode: a single graph's peel goes over to the bitmask-matrix mode
     once at most 1,024 vertices are live (from the start for graphs that
-    small); steps_only: the default, the step kernels to the end."""
+    small); steps_only: the default, the step kernels to the end, narrow
+    frames loaded from the frames stored before the first level;
+    built_frames: every frame built from the out-lists at its step."""
     n = len(off) - 1
     deg = np.diff(off.astype(np.int64))
-    tune = kcds.TUNE_DENSE_PEEL if mode == "dense_mode" else 0
+    tune = MODES[mode]
     for k in ks:
         solver.set_tuning(0, tune)
         solver.upload(off, nbr)
