@@ -286,7 +286,7 @@ int kcds_count_dag_noalloc(int device, uint32_t n, const uint32_t *doff, const u
  *                        only the k'-core's cliques.  Without the flag a solve
  *                        takes this path by itself when the clique list would
  *                        not fit in HBM (T * (8k + 4) bytes, or T >= 2^32),
- *                        and for a single graph from 2^27 cliques up, where
+ *                        and for a single graph from 2^25 cliques up, where
  *                        enumerating is the faster peel. */
 #define KCDS_TUNE_ENUM_PEEL 32u
 /*   KCDS_TUNE_DENSE      take the dense exact search for a single graph whose

@@ -858,9 +858,13 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
     const EAppend a{foff, fnbr, nv + (size_t)seg * seg_v, &ctl->fsize[cur ^ 1][seg][0], ni + (size_t)seg * seg_i,
                     &ctl->isize[cur ^ 1][seg][0], erem, s + 1, K >= 4};
     ull dm = ~0ull;
+    // item b goes to workgroup b; the items past the grid are taken from the
+    // ticket counter by the workgroups that finish first (so a step with
+    // fewer items than workgroups touches no shared counter)
+    bool first = true;
     while (true) {
       if (tid == 0) {
-        const uint32_t it = atomicAdd(&ctl->ticket[0], 1u);
+        const uint32_t it = first ? blockIdx.x : gridDim.x + atomicAdd(&ctl->ticket[0], 1u);
         f.item_idx = it;
         if (it < is) {
           uint32_t sg = 0;
@@ -875,6 +879,9 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
       const uint4 item = f.item;
       const uint32_t skip = f.skip;
       lds_sync();  // every thread has read the item before it is replaced
+      first = false;
+      // no own item (blockIdx.x >= is: then none past the grid either), or
+      // the counter ran past the last item
       if (it >= is) break;
       if (skip) continue;
       const uint32_t v = item.x, g = gid[v];

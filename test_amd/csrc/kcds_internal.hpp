@@ -260,10 +260,11 @@ bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, u
 // most 1,024 live vertices; -1 when not applicable.
 long long dense_peel_finish(Ctx &c, const ull *cnt, Scatter sc, GraphPeel *gp, long long last_level);
 constexpr ull DENSE_MIN_T = 1ull << 24;
-// A single graph with at least this many cliques takes the enumerating peel
-// (measured on MI355X: 6.9e8 4-cliques 0.47 s against 0.79 s listed and
-// peeled, 2.3e9 1.13 s against 2.43 s; even at 9e7).
-constexpr ull ENUM_MIN_T = 1ull << 27;  // k'-cores with this many cliques take the dense search
+// A single graph with at least this many cliques takes the enumerating peel.
+// Measured on MI355X with stored frames, over the R-MAT graphs of a served
+// stream (k = 3..6), per graph: from 3.5e7 cliques up enumerating took 0.3 to
+// 0.8 of the listed-and-peeled time, below 2.5e7 about 1.0 to 1.2.
+constexpr ull ENUM_MIN_T = 1ull << 25;
 constexpr uint32_t DENSE_MAX_ITERS = 12000;  // major cycles of the minimum-norm-point search
 
 // ---- components (k_comp.hip) ----
