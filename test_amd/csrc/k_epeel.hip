@@ -955,11 +955,14 @@ __global__ __launch_bounds__(ETHREADS) void k_ep_step(EpCtl *ctl, uint32_t n, co
 }
 
 // Words of v's stored frame: stored_words(d) * d for a vertex of degree
-// d <= maxd that lies in a clique (its count is positive), else none.
+// d <= maxd that lies in a clique (its count is positive), else none; above
+// degree ED only for a vertex whose frame may be split into pieces (count
+// from 2^ECHUNK_LOG up), each of which would build it again.
 __global__ void k_ep_fsize(uint32_t n, const uint32_t *foff, const ull *cnt0, uint32_t maxd, ull *words) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
     const uint32_t d = foff[v + 1] - foff[v];
-    words[v] = d <= maxd && cnt0[v] ? (ull)stored_words(d) * d : 0ull;
+    const ull c = cnt0[v];
+    words[v] = d <= maxd && c && (d <= ED || (c >> ECHUNK_LOG) > 0) ? (ull)stored_words(d) * d : 0ull;
   }
 }
 
