@@ -337,6 +337,13 @@ int kcds_count_dag_noalloc(int device, uint32_t n, const uint32_t *doff, const u
 #define KCDS_TUNE_BUILD_FRAMES 1024u
 int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
 
+/* Major cycles the dense exact search may run before a solve whose k'-core
+ * clique list fits in HBM lists it and takes the clique max-flow instead
+ * (same result; 0 restores the default, 512).  A core too large to list, or
+ * KCDS_TUNE_DENSE, keeps the search going up to 12,000 cycles.  No
+ * counterpart in the reference (it always builds the clique network). */
+int kcds_set_dense_budget(kcds_ctx *ctx, uint32_t major_cycles);
+
 /* Listing launch shape: waves per root (workgroup size / 64, 1..8) of each
  * width class W = 1, 2, 4, 8 words (roots of out-degree <= 64, 128, 256,
  * 512); NULL restores the defaults {1, 2, 4, 8}.  Replaces the reference's

@@ -375,6 +375,10 @@ int kcds_set_graph_budget(kcds_ctx *ctx, uint64_t bytes) {
   return guarded(ctx, [&](Ctx &c) { c.graph_budget = bytes; });
 }
 
+int kcds_set_dense_budget(kcds_ctx *ctx, uint32_t major_cycles) {
+  return guarded(ctx, [&](Ctx &c) { c.dense_budget = major_cycles; });
+}
+
 int kcds_set_list_shape(kcds_ctx *ctx, const uint32_t *waves_per_root) {
   return guarded(ctx, [&](Ctx &c) {
     for (int i = 0; i < 4; ++i) {

@@ -123,6 +123,7 @@ struct Ctx {
   bool order_replaced = false;  // kcds_orient's listing order was too wide; the degeneracy order is used
   uint64_t peel_steps = 0;   // steps of the last grid peel
   uint64_t graph_budget = 0; // per-graph cap on T * (8k + 4) bytes (0 = none)
+  uint32_t dense_budget = 0; // dense search cycles before a listable core takes the flow (0: DENSE_LISTABLE_ITERS)
 
   // pinned host staging for small readbacks
   void *pinned = nullptr;
@@ -266,6 +267,11 @@ constexpr ull DENSE_MIN_T = 1ull << 24;
 // 0.8 of the listed-and-peeled time, below 2.5e7 about 1.0 to 1.2.
 constexpr ull ENUM_MIN_T = 1ull << 25;
 constexpr uint32_t DENSE_MAX_ITERS = 12000;  // major cycles of the minimum-norm-point search
+// ... when the k'-core's clique list would fit in HBM, so the clique flow
+// network can take over: a certificate usually holds within 100 cycles, and
+// one that needs thousands (an outsider whose best load sits within 1e-6 of
+// rho*) costs more than the flow search on the listed cliques
+constexpr uint32_t DENSE_LISTABLE_ITERS = 512;
 
 // ---- components (k_comp.hip) ----
 void components_run(Ctx &c);

@@ -301,7 +301,12 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
     if ((c.tune_flags & KCDS_TUNE_DENSE) || c.peel.core_cliques >= DENSE_MIN_T) {
       ull num = 0, den = 0;
       uint32_t ncomp = 0, passes = 0, critical = 0;
-      if (dense_core_solve(c, k, num, den, ncomp, passes, critical, DENSE_MAX_ITERS)) {
+      const size_t avail = c.budget > c.used ? c.budget - c.used : 0;
+      const bool listable = c.peel.core_cliques < 0xFFFFFFFFull / (2ull * k) &&
+                            (double)c.peel.core_cliques * (8.0 * k + 4.0) <= 0.85 * (double)avail;
+      const uint32_t iters = (c.tune_flags & KCDS_TUNE_DENSE) || !listable ? DENSE_MAX_ITERS
+                             : c.dense_budget ? c.dense_budget : DENSE_LISTABLE_ITERS;
+      if (dense_core_solve(c, k, num, den, ncomp, passes, critical, iters)) {
         R.ms_components = t.stop();
         Timer tf(c.stream);
         R.ncomp = ncomp;

@@ -26,7 +26,7 @@ U64P = ctypes.POINTER(ctypes.c_uint64)
 EXPORTS = ("kcds_create", "kcds_destroy", "kcds_last_error", "kcds_upload_csr", "kcds_orient", "kcds_count",
            "kcds_list", "kcds_download_cliques", "kcds_peel", "kcds_components", "kcds_maxflow", "kcds_solve",
            "kcds_sync", "kcds_set_tuning", "kcds_get_list_info", "kcds_solve_batch", "kcds_set_list_shape",
-           "kcds_autotune_list", "kcds_set_graph_budget", "kcds_server_create", "kcds_submit", "kcds_poll",
+           "kcds_autotune_list", "kcds_set_graph_budget", "kcds_set_dense_budget", "kcds_server_create", "kcds_submit", "kcds_poll",
            "kcds_server_clock", "kcds_server_cancel_pending", "kcds_server_destroy", "kcds_count_dag_noalloc")
 
 GRAPH_SOLVED, GRAPH_OVER_BUDGET = 0, 1
@@ -101,6 +101,7 @@ def lib():
         L.kcds_set_list_shape.argtypes = [V, U32P]
         L.kcds_autotune_list.argtypes = [V, ctypes.c_uint32, U32P, ctypes.POINTER(ctypes.c_float)]
         L.kcds_set_graph_budget.argtypes = [V, ctypes.c_uint64]
+        L.kcds_set_dense_budget.argtypes = [V, ctypes.c_uint32]
         _LIB = L
     return _LIB
 
@@ -224,6 +225,10 @@ class Solver:
     def set_graph_budget(self, nbytes):
         """Per-graph cap on T*(8k+4) bytes of materialised cliques (0 = none)."""
         self._chk(lib().kcds_set_graph_budget(self._h, int(nbytes)))
+
+    def set_dense_budget(self, major_cycles):
+        """Dense search cycles before a listable k'-core takes the clique max-flow (0 = default)."""
+        self._chk(lib().kcds_set_dense_budget(self._h, int(major_cycles)))
 
     def set_list_shape(self, waves_per_root=None):
         """Waves per root of the four width classes (None = library defaults)."""

@@ -9,6 +9,7 @@ The search is forced (KCDS_TUNE_DENSE) on graphs whose k'-core has at most
 rational and maximal densest set, including ties that span components.
 """
 import itertools
+import math
 import os
 
 import numpy as np
@@ -169,3 +170,41 @@ def test_dense_critical_certificate(solver):
     finally:
         solver.set_tuning(0, 0)
     assert used >= 3  # the critical rule took part in most of these certificates
+
+
+def test_dense_budget_falls_back_to_flow(solver):
+    """A k'-core routed to the dense search by default (a planted K80 less one
+    edge at k = 5: about 2.3e7 cliques, from 2^24 up) whose clique list fits:
+    with a budget of one major cycle the search gives up and the solve lists
+    the core and runs the clique max-flow.  Both exact paths must agree, and
+    the answer lies inside the planted clique."""
+    g = gen.er(3000, 0.002, 80, seed=41)
+    pl = [int(v) for v in g.planted]
+    off = np.asarray(g.offset, np.int64)
+    adj = [set(g.nbr[off[v]:off[v + 1]].tolist()) for v in range(len(off) - 1)]
+    a, b = pl[0], pl[1]
+    adj[a].discard(b)
+    adj[b].discard(a)
+    o = np.zeros(len(adj) + 1, np.int64)
+    o[1:] = np.cumsum([len(s) for s in adj])
+    nbr = np.array([u for v in range(len(adj)) for u in sorted(adj[v])], np.uint32)
+    o = o.astype(np.uint32)
+    solver.set_tuning(0, 0)
+    solver.upload(o, nbr)
+    res_d, ins_d = solver.solve(5)
+    solver.set_dense_budget(1)
+    try:
+        solver.upload(o, nbr)
+        res_f, ins_f = solver.solve(5)
+    finally:
+        solver.set_dense_budget(0)
+    print(f"  K80-e k=5: dense rho={res_d.rho_num}/{res_d.rho_den} passes={res_d.dense_passes}; "
+          f"budget 1: rho={res_f.rho_num}/{res_f.rho_den} flows={res_f.flow_calls}", flush=True)
+    assert res_d.dense_passes > 0 and res_d.flow_calls == 0
+    assert res_f.dense_passes == 0 and res_f.flow_calls > 0
+    assert (res_f.rho_num, res_f.rho_den) == (res_d.rho_num, res_d.rho_den)
+    np.testing.assert_array_equal(ins_f, ins_d)
+    assert set(np.flatnonzero(ins_d).tolist()) <= set(pl)
+    # K80 less an edge keeps both ends: (C(80, 5) - C(78, 3)) / 80 = 299,549.25
+    # beats C(79, 5) / 79 = 285,285 with one end dropped
+    assert res_d.rho_num * 80 == (math.comb(80, 5) - math.comb(78, 3)) * res_d.rho_den
