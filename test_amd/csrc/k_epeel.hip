@@ -1255,7 +1255,9 @@ void epeel_run(Ctx &c) {
       HIP_CHECK(hipMemcpyAsync(&total, c.fofs.as<ull>() + n, sizeof(ull), hipMemcpyDeviceToHost, c.stream));
       c.sync();
       fbytes = (size_t)total * sizeof(ull);
-      if (c.used + fbytes + ((size_t)64 << 20) <= c.budget) break;
+      // a store kept from an earlier solve is reused where it is large enough
+      const size_t grow = fbytes > c.fstore.bytes ? fbytes - c.fstore.bytes : 0;
+      if (c.used + grow + ((size_t)64 << 20) <= c.budget) break;
       total = 0;
     }
     if (total) {
@@ -1306,7 +1308,10 @@ void epeel_run(Ctx &c) {
   }
   c.peel_steps = h0->steps;
   uint32_t levels = h0->levels;
-  if (fstore) {  // the later stages get the HBM back
+  // the store is kept for the next solve (freeing waits for the whole
+  // device, other streams included) unless the context is past half its HBM
+  // budget, when the later stages get it back
+  if (fstore && c.used > c.budget / 2) {
     buf_free(c, c.fstore);
     buf_free(c, c.fofs);
   }
