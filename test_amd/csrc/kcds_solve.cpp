@@ -372,6 +372,16 @@ void solve_graphs(Ctx &c, uint32_t k, kcds_result *agg, kcds_graph_result *per_g
       ad[gi] = nc;
     }
   }
+  // the peel's density record is that of a vertex set too (the vertices
+  // left at some level), often rho* itself: starting there, one max-flow
+  // certifies a component whose densest set the peel already found
+  for (uint32_t gi = 0; gi < G; ++gi) {
+    const GraphPeel &P = c.h_gpeel[gi];
+    if (P.max_den && ratio_gt(P.max_num, P.max_den, an[gi], ad[gi])) {
+      an[gi] = P.max_num;
+      ad[gi] = P.max_den;
+    }
+  }
   for (uint32_t gi = 0; gi < G; ++gi) {
     ull gg = gcd(an[gi], ad[gi]);
     if (gg) {
