@@ -250,9 +250,11 @@ int kcds_count_dag_noalloc(int device, uint32_t n, const uint32_t *doff, const u
 
 /* Listing-kernel launch shape: waves per CU for the persistent listing
  * kernels (0 = library default), and flags:
- *   KCDS_TUNE_GRID_PEEL  peel with the grid-wide step kernels even when every
- *                        graph fits the workgroup-per-graph peel (both give
- *                        the same results; used to test one against the other). */
+ *   KCDS_TUNE_GRID_PEEL  peel a batch with the grid-wide step kernels even
+ *                        when every graph fits the workgroup-per-graph peel
+ *                        (both give the same results; used to test one
+ *                        against the other).  A single graph always takes
+ *                        the grid peel. */
 #define KCDS_TUNE_GRID_PEEL 1u
 /*   KCDS_TUNE_GRID_FLOW  solve every component with the grid-wide max-flow
  *                        (normally only components of >= 49,152 cliques,
@@ -334,6 +336,11 @@ int kcds_count_dag_noalloc(int device, uint32_t n, const uint32_t *doff, const u
  *                        budget), and each step restricts and renumbers the
  *                        stored rows instead.  Same cores either way. */
 #define KCDS_TUNE_BUILD_FRAMES 1024u
+/*   KCDS_TUNE_BLOCK_PEEL  peel a single graph that fits the workgroup-per-graph
+ *                        peel (n <= 32,768, T <= 2^18) with it, as a batch's
+ *                        small graphs are; by default a single graph takes the
+ *                        grid peel.  Same cores either way. */
+#define KCDS_TUNE_BLOCK_PEEL 2048u
 int kcds_set_tuning(kcds_ctx *ctx, uint32_t list_waves_per_cu, uint32_t flags);
 
 /* Major cycles the dense exact search may run before a solve whose k'-core

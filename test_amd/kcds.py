@@ -32,7 +32,7 @@ EXPORTS = ("kcds_create", "kcds_destroy", "kcds_last_error", "kcds_upload_csr", 
 GRAPH_SOLVED, GRAPH_OVER_BUDGET = 0, 1
 TUNE_GRID_PEEL, TUNE_GRID_FLOW, TUNE_EXACT_ORDER, TUNE_NO_GRAPHS, TUNE_BLOCK_FLOW = 1, 2, 4, 8, 16
 TUNE_ENUM_PEEL, TUNE_DENSE, TUNE_DENSE_PEEL, TUNE_FLOW_ALL, TUNE_DEGENERACY_ORDER = 32, 64, 128, 256, 512
-TUNE_BUILD_FRAMES = 1024
+TUNE_BUILD_FRAMES, TUNE_BLOCK_PEEL = 1024, 2048
 E_INVALID, E_HIP, E_NOMEM, E_STATE, E_LIMIT, E_PENDING = -1, -2, -3, -4, -5, -6
 
 

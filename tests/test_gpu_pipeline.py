@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from test_amd import gen
+from test_amd import gen, kcds
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -60,7 +60,7 @@ GRAPHS = list(graphs())
 @pytest.mark.parametrize("name,off,nbr,ks", GRAPHS, ids=[g[0] for g in GRAPHS])
 def test_peel_and_components(solver, name, off, nbr, ks, grid_peel):
     """Both peels (workgroup per graph, and grid-wide step kernels) against the oracle."""
-    solver.set_tuning(0, 1 if grid_peel else 0)
+    solver.set_tuning(0, kcds.TUNE_GRID_PEEL if grid_peel else kcds.TUNE_BLOCK_PEEL)
     try:
         _peel_and_components(solver, name, off, nbr, ks)
     finally:
