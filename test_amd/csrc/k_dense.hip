@@ -992,6 +992,23 @@ bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, u
   }
   // start: each clique to its member of smallest clique-degree
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return deg[a] < deg[b]; });
+  // the critical set starts with the core's members of least slack: those
+  // whose clique-degree is within 10 % of the core's density c(core)/n (the
+  // certificate's usual bottleneck when the answer is the whole core), at
+  // most SEED_CRIT of them, so the atoms carry the rule from the first one
+  {
+    constexpr uint32_t SEED_CRIT = 8;
+    u128 tot = 0;
+    for (uint32_t u = 0; u < n; ++u) tot += deg[u];
+    const u128 cc = tot / k;  // every clique is counted at its k members
+    for (uint32_t i = 0; i < n && D.ncrit < SEED_CRIT; ++i) {
+      const uint32_t u = order[i];
+      // deg[u] <= 1.1 * cc / n, in integers
+      if ((u128)deg[u] * 10 * n > cc * 11) break;
+      D.crit[u] = 1;
+      ++D.ncrit;
+    }
+  }
   Corral Q;
   Q.n = n;
   Q.cap = n + 3;
