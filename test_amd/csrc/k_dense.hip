@@ -1171,10 +1171,11 @@ bool dense_core_solve(Ctx &c, uint32_t k, ull &num, ull &den, uint32_t &ncomp, u
   };
   const auto t_start = std::chrono::steady_clock::now();
   // the certificate has slack, so it often holds well before x has
-  // converged: it is tried from cycle 32 or a gap of 1e-3 on, then each time
-  // the gap has fallen tenfold or the cycle count has grown by half
-  double next_try_gap = 1e-3;
-  uint32_t next_try_major = 32;
+  // converged: it is tried from cycle 16 or a gap of 3e-3 on (C2 19 passes,
+  // C3 19; from cycle 32 or 1e-3, 30 and 28), then each time the gap has
+  // fallen tenfold or the cycle count has grown by half
+  double next_try_gap = 3e-3;
+  uint32_t next_try_major = 16;
   for (uint32_t major = 1; major <= max_iters; ++major) {
     // linear oracle: the greedy split minimising <x, q> (each clique to its
     // member of smallest load)
