@@ -135,6 +135,33 @@ def bench_c2_solve(g, k, kcds):
                             "dense_or_components": res.ms_components, "flow": res.ms_flow}}
 
 
+def bench_c2_edges(np, g, kcds, reps=3):
+    """The headline graph handed over as an edge list: every edge once in a random direction plus 20 % repeats,
+    shuffled; kcds_upload_edges builds the CSR on the GPU (host pairs, so the time includes their PCIe copy)."""
+    rng = np.random.default_rng(7)
+    off = np.asarray(g.offset, np.int64)
+    src = np.repeat(np.arange(g.n), np.diff(off))
+    dst = np.asarray(g.nbr, np.int64)
+    pairs = np.stack([src[src < dst], dst[src < dst]], axis=1)
+    flip = rng.random(len(pairs)) < 0.5
+    pairs[flip] = pairs[flip][:, ::-1]
+    uv = np.concatenate([pairs, pairs[rng.integers(0, len(pairs), len(pairs) // 5)]]).astype(np.uint32)
+    uv = np.ascontiguousarray(uv[rng.permutation(len(uv))])
+    s = kcds.Solver(0)
+    s.upload_edges(g.n, uv)  # warm-up: buffers reserved
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        m = s.upload_edges(g.n, uv)
+        times.append(time.perf_counter() - t0)
+    o2, n2 = s.download_csr()
+    same = bool(np.array_equal(o2, g.offset) and np.array_equal(n2, g.nbr))
+    s.close()
+    t = min(times)
+    return {"C2_edges_pairs": len(uv), "C2_edges_distinct": m, "C2_edges_to_csr_ms": t * 1e3,
+            "C2_edges_pairs_per_s": len(uv) / t, "C2_edges_csr_equal": same}
+
+
 def bench_c3(gen, kcds, reps=3, solve=True, shape=None):
     """C3 (Chung-Lu n = 1M, ~10M edges, near-K80, k = 5): the same step as the
     headline (GPU (degree, id) order + DAG + per-vertex counting pass) with its own roofline
@@ -253,6 +280,7 @@ def main():
         if args.only == "c2solve":
             g, k = gen.config(2)
             out = bench_c2_solve(g, k, kcds)
+            out.update(bench_c2_edges(np, g, kcds))
         elif args.only == "c3":
             out = bench_c3(gen, kcds, solve=not args.no_solves,
                            shape=[int(x) for x in args.shape.split(",")] if args.shape else None)
@@ -358,6 +386,7 @@ def main():
         s1.close()
         if not args.no_solves:
             extras.update(bench_c2_solve(g, k, kcds))
+        extras.update(bench_c2_edges(np, g, kcds))
         extras.update(bench_c3(gen, kcds, solve=not args.no_solves))
         extras.update(bench_c4(np, gen, kcds, args.c4_plain_sample))
         if args.c5_window > 0:

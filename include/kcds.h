@@ -76,6 +76,22 @@ const char *kcds_last_error(const kcds_ctx *ctx);
  * later stages. */
 int kcds_upload_csr(kcds_ctx *ctx, uint32_t n, const uint32_t *offset, const uint32_t *nbr);
 
+/* Builds the device CSR from an edge list on the GPU: uv[2m] holds m pairs
+ * (u, v) with ids < n, in either direction, repeated or self-loops allowed.
+ * Each pair is taken in both directions, self-loops are dropped and every
+ * neighbour list is sorted ascending without duplicates -- the form
+ * kcds_upload_csr takes, and what the CPU loaders make of a `u v` file.
+ * *m_out (may be NULL) receives the number of distinct undirected edges.
+ * KCDS_E_INVALID for an id >= n; KCDS_E_LIMIT above 2^31 - 2^15 pairs.
+ * Resets all later stages.  Replaces the host-side parse-and-build of
+ * Graph::Graph (src/graph.cpp:7-40) for callers that hold the pairs in
+ * memory. */
+int kcds_upload_edges(kcds_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *uv, uint64_t *m_out);
+
+/* Copies the device CSR to the host: offset[n+1] (if not NULL) and the first
+ * min(cap, nnz) neighbour ids (if nbr is not NULL); *nnz receives the total. */
+int kcds_download_csr(kcds_ctx *ctx, uint32_t *offset, uint32_t *nbr, uint64_t cap, uint64_t *nnz);
+
 /* Builds the DAG used for listing.  listing_order == NULL: an order computed
  * on the GPU -- by default the degree order (vertices sorted by degree, ties
  * by id), or the degeneracy order (KCDS_TUNE_DEGENERACY_ORDER, with slack

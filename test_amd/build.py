@@ -23,7 +23,7 @@ HIP_FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 LIB_SOURCES = ["kcds_api.cpp", "kcds_solve.cpp", "kcds_serve.cpp", "k_util.hip", "k_orient.hip", "k_list.hip", "k_peel.hip", "k_epeel.hip", "k_dense.hip",
-               "k_comp.hip", "k_flow.hip", "k_gflow.hip"]
+               "k_comp.hip", "k_flow.hip", "k_gflow.hip", "k_edges.hip"]
 
 
 def _run(cmd):

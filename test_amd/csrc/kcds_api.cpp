@@ -192,6 +192,34 @@ int kcds_upload_csr(kcds_ctx *ctx, uint32_t n, const uint32_t *offset, const uin
   });
 }
 
+int kcds_upload_edges(kcds_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *uv, uint64_t *m_out) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (m && !uv) fail(KCDS_E_INVALID, "null edge array");
+    reset_after_graph(c);
+    c.have_graph = false;
+    build_csr_from_edges(c, n, m, uv);
+    c.n = n;
+    c.ngraphs = 1;
+    c.h_gvoff.assign({0u, n});
+    c.have_graph = true;
+    if (m_out) *m_out = c.nnz / 2;
+  });
+}
+
+int kcds_download_csr(kcds_ctx *ctx, uint32_t *offset, uint32_t *nbr, uint64_t cap, uint64_t *nnz) {
+  return guarded(ctx, [&](Ctx &c) {
+    if (!c.have_graph) fail(KCDS_E_STATE, "kcds_download_csr before a graph upload");
+    if (nnz) *nnz = c.nnz;
+    if (offset)
+      HIP_CHECK(hipMemcpyAsync(offset, c.off.p, sizeof(uint32_t) * ((size_t)c.n + 1), hipMemcpyDeviceToHost,
+                               c.stream));
+    const uint64_t take = std::min<uint64_t>(cap, c.nnz);
+    if (nbr && take)
+      HIP_CHECK(hipMemcpyAsync(nbr, c.nbr.p, sizeof(uint32_t) * take, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+  });
+}
+
 int kcds_orient(kcds_ctx *ctx, const uint32_t *listing_order) {
   return guarded(ctx, [&](Ctx &c) {
     if (!c.have_graph) fail(KCDS_E_STATE, "kcds_orient before kcds_upload_csr");

@@ -225,6 +225,8 @@ bool orient_from_rank(Ctx &c, bool approx = false);
 // fills rank/inv from a GPU k-core peel; thresholds x -> x + (x >> slack_shift)
 // (NO_SLACK: the exact degeneracy order)
 void degeneracy_rank_gpu(Ctx &c, uint32_t slack_shift);
+// pairs uv[2m] (host) -> c.off, c.nbr, c.nnz, c.maxdeg (k_edges.hip)
+void build_csr_from_edges(Ctx &c, uint32_t n, uint64_t m, const uint32_t *uv_host);
 void orient_gpu(Ctx &c);  // degeneracy_rank_gpu + orient_from_rank, slack order first
 
 // ---- listing (k_list.hip) ----

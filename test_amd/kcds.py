@@ -23,7 +23,8 @@ U8P = ctypes.POINTER(ctypes.c_uint8)
 U64P = ctypes.POINTER(ctypes.c_uint64)
 
 # every entry point include/kcds.h declares
-EXPORTS = ("kcds_create", "kcds_destroy", "kcds_last_error", "kcds_upload_csr", "kcds_orient", "kcds_count",
+EXPORTS = ("kcds_create", "kcds_destroy", "kcds_last_error", "kcds_upload_csr", "kcds_upload_edges",
+           "kcds_download_csr", "kcds_orient", "kcds_count",
            "kcds_list", "kcds_download_cliques", "kcds_peel", "kcds_components", "kcds_maxflow", "kcds_solve",
            "kcds_sync", "kcds_set_tuning", "kcds_get_list_info", "kcds_solve_batch", "kcds_set_list_shape",
            "kcds_autotune_list", "kcds_set_graph_budget", "kcds_set_dense_budget", "kcds_server_create", "kcds_submit", "kcds_poll",
@@ -84,6 +85,8 @@ def lib():
         L.kcds_last_error.argtypes = [V]
         L.kcds_last_error.restype = ctypes.c_char_p
         L.kcds_upload_csr.argtypes = [V, ctypes.c_uint32, U32P, U32P]
+        L.kcds_upload_edges.argtypes = [V, ctypes.c_uint32, ctypes.c_uint64, U32P, U64P]
+        L.kcds_download_csr.argtypes = [V, U32P, U32P, ctypes.c_uint64, U64P]
         L.kcds_orient.argtypes = [V, U32P]
         L.kcds_count.argtypes = [V, ctypes.c_uint32, U64P, I64P]
         L.kcds_list.argtypes = [V, ctypes.c_uint32, U64P]
@@ -146,6 +149,25 @@ class Solver:
         self._nbr = np.ascontiguousarray(nbr, dtype=np.uint32)
         self.n = len(self._off) - 1
         self._chk(lib().kcds_upload_csr(self._h, self.n, _p(self._off, U32P), _p(self._nbr, U32P)))
+
+    def upload_edges(self, n, uv):
+        """Edge list -> CSR on the GPU (kcds_upload_edges); uv: m x 2 ids < n.
+        Returns the number of distinct undirected edges."""
+        self._uv = np.ascontiguousarray(uv, dtype=np.uint32).reshape(-1, 2)
+        self._off = self._nbr = None
+        self.n = int(n)
+        m = ctypes.c_uint64()
+        self._chk(lib().kcds_upload_edges(self._h, self.n, len(self._uv), _p(self._uv, U32P), ctypes.byref(m)))
+        return m.value
+
+    def download_csr(self):
+        """The device CSR (offset[n+1], nbr[nnz]) as numpy arrays."""
+        nnz = ctypes.c_uint64()
+        self._chk(lib().kcds_download_csr(self._h, None, None, 0, ctypes.byref(nnz)))
+        off = np.zeros(self.n + 1, np.uint32)
+        nbr = np.zeros(max(1, nnz.value), np.uint32)
+        self._chk(lib().kcds_download_csr(self._h, _p(off, U32P), _p(nbr, U32P), nnz.value, ctypes.byref(nnz)))
+        return off, nbr[:nnz.value]
 
     def orient(self, listing_order=None):
         lo = None if listing_order is None else np.ascontiguousarray(listing_order, dtype=np.uint32)
